@@ -1,0 +1,270 @@
+"""Benchmark of the hot path: stable key-value radix sort (satradix) of 2^28
+pairs, uint64 keys uniform over 64 bits (seed 0xC3A), uint32 payload = index.
+
+One step = one clo_sort_kv_with_device_data call, out of place (the unsorted
+input stays resident in HBM and is re-sorted every step).  Prints one JSON
+line.  `roofline` is for the dominant kernel (satradix_scatter, one per digit
+pass): algorithmic bytes per launch = n * 2 * (8 + 4) divided by its average
+launch duration measured with HIP events on the exec stream over the timed
+region.  `cpu_baseline` times the oracle's reference-structured satradix
+(4-bit digits, 256-element blocks, blocks spread over host threads) on a
+bounded sample of the same workload.
+
+Other modes (not the headline line):
+  --copy-probe     runs only the calibration copy kernel (for memory counters)
+  --suite          also measures the other configurations, one JSON line each
+                   on stderr
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "cl-ops_amd")]
+
+import clops  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "radix sort Gkeys/s (2^28 u64 KV), % of HBM roofline; learned-dispatch regret"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(q, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    q.finish()
+    q.prof_reset()
+    e0 = q.marker()
+    for _ in range(steps):
+        fn()
+    e1 = q.marker()
+    q.finish()
+    return q.elapsed_ms(e0, e1) / steps
+
+
+def cpu_baseline_kv(n_sample, threads):
+    """Reference-structured satradix on the host (the oracle), KV u64/u32."""
+    sys.path[:0] = [os.path.join(ROOT, "oracle")]
+    import oracle
+    keys = oracle.generate(oracle.GEN_U64, 0xC3A, n_sample)
+    vals = np.arange(n_sample, dtype=np.uint32)
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        ko, vo = oracle.satradix(keys, vals, bits=4, lws=256, threads=threads)
+        times.append(time.perf_counter() - t0)
+    t = min(times)
+    ok = bool(np.all(ko[:-1] <= ko[1:]))
+    return {
+        "value": n_sample / t / 1e9,
+        "unit": "Gkeys/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d KV pairs (u64 key U(0xC3A,i), u32 index), oracle satradix restating "
+                  "the reference decomposition (4-bit digits, 16 passes, 256-element blocks: "
+                  "split-sort, histogram, counter scan, scatter), best of 2, %.2f s, sorted=%s"
+                  % (n_sample, t, ok),
+    }
+
+
+def traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "satradix_scatter_kv_u64_2p28_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
+def headline(args, ctx):
+    n = 1 << args.log2n
+    q = clops.Queue(ctx, profiling=True)
+    kin = clops.Buffer(ctx, n * 8)
+    vin = clops.Buffer(ctx, n * 4)
+    kout = clops.Buffer(ctx, n * 8)
+    vout = clops.Buffer(ctx, n * 4)
+    clops.generate(q, kin, n, clops.GEN_U64, 0xC3A)
+    clops.generate(q, vin, n, clops.GEN_IOTA_U32, 0)
+    q.finish()
+    s = clops.Sorter(ctx, "satradix", np.uint64)
+
+    def step():
+        s.sort_kv_device(q, kin, vin, kout, vout, n)
+
+    ms = time_steps(q, step, args.steps, args.warmup)
+    cnt, tot = q.prof("satradix_scatter")
+    hcnt, htot = q.prof("satradix_histogram")
+    passes = s.last_passes()
+    scatter_ms = tot / max(cnt, 1)
+    bytes_per_launch = n * 2 * (8 + 4)
+    achieved = bytes_per_launch / (scatter_ms * 1e-3) / 1e9
+    # verification of the last step's output on the device
+    bad = clops.check_sorted(q, kout, n, 8, vout)
+    same = clops.multiset_hash(q, kin, n, 8) == clops.multiset_hash(q, kout, n, 8)
+    total_bytes = n * 8 + passes * bytes_per_launch
+    log("headline: %.3f ms/step, scatter %.3f ms x %d, histogram %.3f ms, passes %d, "
+        "sorted-violations %d, multiset-equal %s, whole-sort HBM fraction %.3f"
+        % (ms, scatter_ms, cnt // max(args.steps, 1), htot / max(hcnt, 1), passes, bad, same,
+           total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+    if bad != 0 or not same:
+        raise SystemExit("benchmark output failed verification")
+    return {
+        "ms": ms, "n": n, "passes": passes, "scatter_ms": scatter_ms, "achieved": achieved,
+        "bytes_per_launch": bytes_per_launch,
+    }
+
+
+def suite(ctx):
+    """Other configurations, one stderr JSON line each."""
+    q = clops.Queue(ctx, profiling=True)
+    out = []
+    # C1: 2^20 uint32 keys only.
+    n = 1 << 20
+    kin = clops.Buffer(ctx, n * 4)
+    kout = clops.Buffer(ctx, n * 4)
+    clops.generate(q, kin, n, clops.GEN_U32_HI, 0xC1)
+    s = clops.Sorter(ctx, "satradix", np.uint32)
+    ms = time_steps(q, lambda: s.sort_device(q, kin, kout, n), 20, 5)
+    out.append({"config": "satradix u32 2^20 keys", "ms": ms, "Gkeys/s": n / ms / 1e6})
+    # C3 variants: [0, 2^20) keys and Zipf.
+    n = 1 << 28
+    kin = clops.Buffer(ctx, n * 8)
+    vin = clops.Buffer(ctx, n * 4)
+    kout = clops.Buffer(ctx, n * 8)
+    vout = clops.Buffer(ctx, n * 4)
+    clops.generate(q, vin, n, clops.GEN_IOTA_U32, 0)
+    s = clops.Sorter(ctx, "satradix", np.uint64)
+    clops.generate(q, kin, n, clops.GEN_U64_MASK, 0xC3B, (1 << 20) - 1)
+    ms = time_steps(q, lambda: s.sort_kv_device(q, kin, vin, kout, vout, n), 5, 2)
+    p = s.last_passes()
+    out.append({"config": "satradix KV u64 2^28 keys<2^20", "ms": ms, "Gkeys/s": n / ms / 1e6,
+                "passes": p, "hbm_frac": n * (8 + 24 * p) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    cdfh = np.empty(1 << 24, dtype=np.float64)
+    import ctypes
+    clops.lib().clo_util_zipf_cdf(cdfh.ctypes.data_as(ctypes.c_void_p), cdfh.size, 1.1)
+    cdf = clops.Buffer.from_array(ctx, cdfh)
+    clops._call(clops.lib().clo_util_generate_zipf, q.ptr, kin.ptr, n, cdf.ptr, cdfh.size, 0xC3C)
+    ms = time_steps(q, lambda: s.sort_kv_device(q, kin, vin, kout, vout, n), 5, 2)
+    p = s.last_passes()
+    out.append({"config": "satradix KV u64 2^28 zipf1.1", "ms": ms, "Gkeys/s": n / ms / 1e6,
+                "passes": p, "hbm_frac": n * (8 + 24 * p) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    # Scans (C2 shapes).
+    for elem, sumt, n in ((np.uint32, np.uint32, 1 << 28), (np.uint32, np.uint64, 1 << 28),
+                          (np.uint64, np.uint64, 1 << 27)):
+        bi = clops.Buffer(ctx, n * np.dtype(elem).itemsize)
+        bo = clops.Buffer(ctx, n * np.dtype(sumt).itemsize)
+        clops.generate(q, bi, n, clops.GEN_U32_MASK if elem == np.uint32 else clops.GEN_U64,
+                       0x2C, 0xFF)
+        sc = clops.Scanner(ctx, elem, sumt)
+        ms = time_steps(q, lambda: sc.scan_device(q, bi, bo, n), 10, 3)
+        by = n * (np.dtype(elem).itemsize + np.dtype(sumt).itemsize)
+        out.append({"config": "scan %s->%s n=%d" % (np.dtype(elem).name, np.dtype(sumt).name, n),
+                    "ms": ms, "GB/s": by / ms / 1e6, "hbm_frac": by / ms / 1e6 / HBM_PEAK_GBS})
+    # RNG fills (C2).
+    G, draws = 1 << 26, 16
+    ob = clops.Buffer(ctx, G * draws * 4)
+    for gen, ss in (("xorshift128", 16), ("mwc64x", 8)):
+        r = clops.Rng(ctx, gen, clops.SEED_DEV_GID, G, 0, q=q)
+        ms = time_steps(q, lambda: r.fill(q, ob, draws), 10, 3)
+        by = G * (2 * ss + 4 * draws)
+        out.append({"config": "rng fill %s 2^26x16" % gen, "ms": ms, "GB/s": by / ms / 1e6,
+                    "hbm_frac": by / ms / 1e6 / HBM_PEAK_GBS})
+    # Bitonic 2^24 u32.
+    n = 1 << 24
+    kin = clops.Buffer(ctx, n * 4)
+    kout = clops.Buffer(ctx, n * 4)
+    clops.generate(q, kin, n, clops.GEN_U32_HI, 0xC3D)
+    for algo in ("abitonic", "sbitonic"):
+        s = clops.Sorter(ctx, algo, np.uint32)
+        ms = time_steps(q, lambda: s.sort_device(q, kin, kout, n), 5, 2)
+        sw = s.last_sweeps()
+        out.append({"config": "%s u32 2^24" % algo, "ms": ms, "Gkeys/s": n / ms / 1e6,
+                    "sweeps": sw, "hbm_frac": sw * 2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    for o in out:
+        log(json.dumps(o))
+
+
+def copy_probe(ctx, nbytes):
+    q = clops.Queue(ctx)
+    a = clops.Buffer(ctx, nbytes)
+    b = clops.Buffer(ctx, nbytes)
+    for _ in range(3):
+        clops._call(clops.lib().clo_util_copy, q.ptr, b.ptr, a.ptr, nbytes)
+    q.finish()
+    log("copy probe: 3 x util_copy of %d bytes" % nbytes)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--suite", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    ctx = clops.Context(0)
+    if args.copy_probe:
+        copy_probe(ctx, 1 << 31)
+        return
+    h = headline(args, ctx)
+    if args.suite:
+        suite(ctx)
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline_kv(1 << args.cpu_sample_log2, threads)
+    n = h["n"]
+    value = n / (h["ms"] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "satradix stable KV sort, 2^%d pairs, u64 key U(0xC3A,i) uniform over "
+                        "64 bits, u32 value = index, out of place" % args.log2n,
+            "n": n, "key": "u64", "value": "u32", "digit_bits": 8,
+            "passes_executed": h["passes"],
+            "whole_sort_hbm_frac": round((n * 8 + h["passes"] * h["bytes_per_launch"]) /
+                                         (h["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(h["achieved"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(h["achieved"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(),
+            "kernel": "satradix_scatter",
+            "kernel_avg_ms": round(h["scatter_ms"], 4),
+            "bytes_per_launch": h["bytes_per_launch"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

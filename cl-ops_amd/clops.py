@@ -1,0 +1,433 @@
+"""ctypes front end of libcl_ops.so for Python callers, tests and benchmarks.
+
+The library itself is the C API in include/*.h; this module only declares its
+signatures, turns GError results into exceptions and moves numpy arrays in and
+out of device buffers.  It never computes anything on the host: if the HIP
+library is missing it raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcl_ops.so")
+
+CL_QUEUE_PROFILING_ENABLE = 1 << 1
+
+# CloType (include/clo_common.h)
+CLO_CHAR, CLO_UCHAR, CLO_SHORT, CLO_USHORT, CLO_INT, CLO_UINT, CLO_LONG, CLO_ULONG = range(8)
+CLO_HALF, CLO_FLOAT, CLO_DOUBLE = 8, 9, 10
+
+# enum clo_error_codes
+CLO_SUCCESS, CLO_ERROR_OPENFILE, CLO_ERROR_ARGS, CLO_ERROR_STREAM_WRITE = 0, 1, 2, 3
+CLO_ERROR_IMPL_NOT_FOUND, CLO_ERROR_UNKNOWN_TYPE, CLO_ERROR_LIBRARY = 5, 6, 7
+
+# CloRngSeedType
+SEED_DEV_GID, SEED_HOST_MT, SEED_EXT_DEV, SEED_EXT_HOST = 0, 1, 2, 3
+
+# CloGenKind (include/clo_util.h)
+GEN_U64, GEN_U32_HI, GEN_U64_MASK, GEN_U32_MASK, GEN_IOTA_U32, GEN_ZIPF_U64 = range(6)
+GEN_U32_MOD, GEN_U64_MOD = 6, 7
+
+_DTYPE_TO_CLO = {
+    np.dtype(np.int8): CLO_CHAR, np.dtype(np.uint8): CLO_UCHAR,
+    np.dtype(np.int16): CLO_SHORT, np.dtype(np.uint16): CLO_USHORT,
+    np.dtype(np.int32): CLO_INT, np.dtype(np.uint32): CLO_UINT,
+    np.dtype(np.int64): CLO_LONG, np.dtype(np.uint64): CLO_ULONG,
+}
+_CLO_TO_DTYPE = {v: k for k, v in _DTYPE_TO_CLO.items()}
+
+
+class GError(ctypes.Structure):
+    _fields_ = [("domain", ctypes.c_uint32), ("code", ctypes.c_int), ("message", ctypes.c_char_p)]
+
+
+GErrorP = ctypes.POINTER(GError)
+
+
+class ClopsError(RuntimeError):
+    def __init__(self, domain, code, message):
+        super().__init__("[%#x/%d] %s" % (domain, code, message))
+        self.domain = domain
+        self.code = code
+        self.message = message
+
+
+_lib = None
+
+
+def lib():
+    """Loads libcl_ops.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libcl_ops.so not built: run __graft_entry__.build() or "
+                           "make -C cl-ops_amd")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, u32, u64, i32, cp = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
+                                 ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p)
+    ep = ctypes.POINTER(GErrorP)
+    d = ctypes.c_double
+    sig = {
+        # errors and common
+        "clo_error_free": (None, [GErrorP]),
+        "clo_error_quark": (u32, []),
+        "ccl_error_quark": (u32, []),
+        "clo_type_get_name": (cp, [i32]),
+        "clo_type_sizeof": (sz, [i32]),
+        "clo_type_by_name": (i32, [cp, ep]),
+        "clo_nlpo2": (ctypes.c_uint, [ctypes.c_uint]),
+        "clo_ones32": (ctypes.c_uint, [ctypes.c_uint]),
+        "clo_tzc": (ctypes.c_uint, [ctypes.c_int]),
+        "clo_sum": (ctypes.c_uint, [ctypes.c_uint]),
+        "ccl_debug_fail_next_allocs": (None, [i32]),
+        # thin HIP layer
+        "ccl_context_new_from_device_index": (vp, [i32, ep]),
+        "ccl_context_destroy": (None, [vp]),
+        "ccl_queue_new": (vp, [vp, vp, u64, ep]),
+        "ccl_queue_finish": (u32, [vp, ep]),
+        "ccl_queue_destroy": (None, [vp]),
+        "ccl_queue_enqueue_marker": (vp, [vp, ep]),
+        "ccl_queue_get_stream": (vp, [vp]),
+        "ccl_buffer_new": (vp, [vp, u64, sz, vp, ep]),
+        "ccl_buffer_destroy": (None, [vp]),
+        "ccl_buffer_get_device_ptr": (vp, [vp]),
+        "ccl_buffer_get_size": (sz, [vp]),
+        "ccl_buffer_enqueue_write": (vp, [vp, vp, u32, sz, sz, vp, vp, ep]),
+        "ccl_buffer_enqueue_read": (vp, [vp, vp, u32, sz, sz, vp, vp, ep]),
+        "ccl_buffer_enqueue_copy": (vp, [vp, vp, vp, sz, sz, sz, vp, ep]),
+        "ccl_event_elapsed_ms": (d, [vp, vp, ep]),
+        "ccl_event_get_name": (cp, [vp]),
+        "ccl_prof_reset": (None, [vp]),
+        "ccl_prof_get": (sz, [vp, cp, ctypes.POINTER(d), ep]),
+        "ccl_prof_get_name": (cp, [vp, sz]),
+        # sort
+        "clo_sort_new": (vp, [cp, cp, vp, ctypes.POINTER(i32), ctypes.POINTER(i32), cp, cp, cp,
+                              ep]),
+        "clo_sort_destroy": (None, [vp]),
+        "clo_sort_with_device_data": (vp, [vp, vp, vp, vp, vp, sz, sz, ep]),
+        "clo_sort_with_host_data": (u32, [vp, vp, vp, vp, vp, sz, sz, ep]),
+        "clo_sort_kv_with_device_data": (vp, [vp, vp, vp, vp, vp, vp, vp, sz, sz, ep]),
+        "clo_sort_kv_with_host_data": (u32, [vp, vp, vp, vp, vp, sz, sz, ep]),
+        "clo_sort_get_last_passes": (u32, [vp, ep]),
+        "clo_sort_get_element_type": (i32, [vp]),
+        "clo_sort_get_element_size": (sz, [vp]),
+        "clo_sort_get_key_type": (i32, [vp]),
+        "clo_sort_get_key_size": (sz, [vp]),
+        "clo_sort_get_num_kernels": (u32, [vp, ep]),
+        "clo_sort_get_kernel_name": (cp, [vp, u32, ep]),
+        "clo_sort_get_localmem_usage": (sz, [vp, u32, sz, sz, ep]),
+        "clo_sort_get_context": (vp, [vp]),
+        "clo_sort_abitonic_get_last_sweeps": (u32, [vp]),
+        # scan
+        "clo_scan_new": (vp, [cp, cp, vp, i32, i32, cp, ep]),
+        "clo_scan_destroy": (None, [vp]),
+        "clo_scan_with_device_data": (vp, [vp, vp, vp, vp, vp, sz, sz, ep]),
+        "clo_scan_with_host_data": (u32, [vp, vp, vp, vp, vp, sz, sz, ep]),
+        "clo_scan_get_elem_type": (i32, [vp]),
+        "clo_scan_get_sum_type": (i32, [vp]),
+        "clo_scan_get_element_size": (sz, [vp]),
+        "clo_scan_get_sum_size": (sz, [vp]),
+        "clo_scan_get_num_kernels": (u32, [vp, ep]),
+        "clo_scan_get_kernel_name": (cp, [vp, u32, ep]),
+        "clo_scan_get_localmem_usage": (sz, [vp, u32, sz, sz, ep]),
+        # rng
+        "clo_rng_new": (vp, [cp, i32, vp, sz, u64, cp, vp, vp, ep]),
+        "clo_rng_destroy": (None, [vp]),
+        "clo_rng_get_source": (cp, [vp]),
+        "clo_rng_get_device_seeds": (vp, [vp]),
+        "clo_rng_get_size": (sz, [vp]),
+        "clo_rng_fill": (vp, [vp, vp, vp, u32, u32, u32, ep]),
+        # util
+        "clo_util_generate": (vp, [vp, vp, sz, i32, u64, u64, u64, ep]),
+        "clo_util_generate_zipf": (vp, [vp, vp, sz, vp, sz, u64, ep]),
+        "clo_util_zipf_cdf": (None, [vp, sz, d]),
+        "clo_util_copy": (vp, [vp, vp, vp, sz, ep]),
+        "clo_util_multiset_hash": (u64, [vp, vp, sz, sz, ep]),
+        "clo_util_check_sorted": (sz, [vp, vp, vp, sz, sz, ep]),
+        "clo_util_build_info": (cp, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _call(fn, *args):
+    """Calls an API function whose last argument is GError**; raises on error."""
+    err = GErrorP()
+    r = fn(*args, ctypes.byref(err))
+    if err:
+        e = err.contents
+        exc = ClopsError(e.domain, e.code, (e.message or b"").decode())
+        lib().clo_error_free(err)
+        raise exc
+    return r
+
+
+def clo_type(dtype):
+    return _DTYPE_TO_CLO[np.dtype(dtype)]
+
+
+def dtype_of(clo_t):
+    return _CLO_TO_DTYPE[clo_t]
+
+
+class Context:
+    def __init__(self, device=0):
+        self.L = lib()
+        self.ptr = _call(self.L.ccl_context_new_from_device_index, device)
+
+    def close(self):
+        if self.ptr:
+            self.L.ccl_context_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Queue:
+    def __init__(self, ctx, profiling=False):
+        self.L = lib()
+        self.ctx = ctx
+        self.ptr = _call(self.L.ccl_queue_new, ctx.ptr, None,
+                         CL_QUEUE_PROFILING_ENABLE if profiling else 0)
+
+    def finish(self):
+        _call(self.L.ccl_queue_finish, self.ptr)
+
+    def marker(self):
+        return _call(self.L.ccl_queue_enqueue_marker, self.ptr)
+
+    def elapsed_ms(self, e0, e1):
+        return _call(self.L.ccl_event_elapsed_ms, e0, e1)
+
+    def prof_reset(self):
+        self.L.ccl_prof_reset(self.ptr)
+
+    def prof(self, prefix=""):
+        """(count, total_ms) of the profiled kernels whose name starts with prefix."""
+        tot = ctypes.c_double()
+        n = _call(self.L.ccl_prof_get, self.ptr, prefix.encode(), ctypes.byref(tot))
+        return n, tot.value
+
+    def prof_names(self):
+        out, i = [], 0
+        while True:
+            s = self.L.ccl_prof_get_name(self.ptr, i)
+            if s is None:
+                return out
+            out.append(s.decode())
+            i += 1
+
+    def close(self):
+        if self.ptr:
+            self.L.ccl_queue_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Buffer:
+    def __init__(self, ctx, nbytes, host=None):
+        self.L = lib()
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        hp = None
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            assert host.nbytes <= self.nbytes
+            self._keep = host
+            hp = host.ctypes.data_as(ctypes.c_void_p)
+        self.ptr = _call(self.L.ccl_buffer_new, ctx.ptr, 1, self.nbytes, hp)
+
+    @classmethod
+    def from_array(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        return cls(ctx, max(a.nbytes, 1), a)
+
+    def write(self, q, a, offset=0):
+        a = np.ascontiguousarray(a)
+        _call(self.L.ccl_buffer_enqueue_write, self.ptr, q.ptr, 1, offset, a.nbytes,
+              a.ctypes.data_as(ctypes.c_void_p), None)
+
+    def read(self, q, dtype, count, offset=0):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _call(self.L.ccl_buffer_enqueue_read, self.ptr, q.ptr, 1, offset, out.nbytes,
+                  out.ctypes.data_as(ctypes.c_void_p), None)
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.L.ccl_buffer_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+class Sorter:
+    def __init__(self, ctx, algo, dtype, options=None, compare=None, get_key=None,
+                 key_dtype=None):
+        self.L = lib()
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        et = ctypes.c_int(clo_type(dtype))
+        kt = None if key_dtype is None else ctypes.byref(ctypes.c_int(clo_type(key_dtype)))
+        self.ptr = _call(self.L.clo_sort_new, _b(algo), _b(options), ctx.ptr, ctypes.byref(et),
+                         kt, _b(compare), _b(get_key), None)
+
+    def sort_device(self, q, bin_, bout, n, q_comm=None):
+        return _call(self.L.clo_sort_with_device_data, self.ptr, q.ptr,
+                     q_comm.ptr if q_comm else None, bin_.ptr, bout.ptr if bout else None, n, 0)
+
+    def sort_kv_device(self, q, kin, vin, kout=None, vout=None, n=0):
+        return _call(self.L.clo_sort_kv_with_device_data, self.ptr, q.ptr, None, kin.ptr, vin.ptr,
+                     kout.ptr if kout else None, vout.ptr if vout else None, n, 0)
+
+    def sort_host(self, a, q=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        out = np.empty_like(a)
+        _call(self.L.clo_sort_with_host_data, self.ptr, q.ptr if q else None, None,
+              a.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), a.size, 0)
+        return out
+
+    def sort_kv_host(self, keys, vals, q=None):
+        k = np.array(keys, dtype=self.dtype, copy=True)
+        v = np.array(vals, dtype=np.uint32, copy=True)
+        _call(self.L.clo_sort_kv_with_host_data, self.ptr, q.ptr if q else None, None,
+              k.ctypes.data_as(ctypes.c_void_p), v.ctypes.data_as(ctypes.c_void_p), k.size, 0)
+        return k, v
+
+    def last_passes(self):
+        return _call(self.L.clo_sort_get_last_passes, self.ptr)
+
+    def last_sweeps(self):
+        return self.L.clo_sort_abitonic_get_last_sweeps(self.ptr)
+
+    def kernel_names(self):
+        n = _call(self.L.clo_sort_get_num_kernels, self.ptr)
+        return [_call(self.L.clo_sort_get_kernel_name, self.ptr, i).decode() for i in range(n)]
+
+    def close(self):
+        if self.ptr:
+            self.L.clo_sort_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scanner:
+    def __init__(self, ctx, elem_dtype, sum_dtype, algo="blelloch", options=None):
+        self.L = lib()
+        self.ctx = ctx
+        self.elem = np.dtype(elem_dtype)
+        self.sum = np.dtype(sum_dtype)
+        self.ptr = _call(self.L.clo_scan_new, _b(algo), _b(options), ctx.ptr,
+                         clo_type(elem_dtype), clo_type(sum_dtype), None)
+
+    def scan_device(self, q, bin_, bout, n):
+        return _call(self.L.clo_scan_with_device_data, self.ptr, q.ptr, None, bin_.ptr, bout.ptr,
+                     n, 0)
+
+    def scan_host(self, a, q=None):
+        a = np.ascontiguousarray(a, dtype=self.elem)
+        out = np.empty(a.size, dtype=self.sum)
+        _call(self.L.clo_scan_with_host_data, self.ptr, q.ptr if q else None, None,
+              a.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), a.size, 0)
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.L.clo_scan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Rng:
+    def __init__(self, ctx, name, seed_type, count, main_seed=0, hash=None, seeds=None, q=None):
+        self.L = lib()
+        self.ctx = ctx
+        self.name = name
+        self.count = count
+        sp = None
+        if seed_type == SEED_EXT_HOST:
+            self._keep = np.ascontiguousarray(seeds)
+            sp = self._keep.ctypes.data_as(ctypes.c_void_p)
+        elif seed_type == SEED_EXT_DEV:
+            sp = seeds.ptr
+        self.ptr = _call(self.L.clo_rng_new, _b(name), seed_type, sp, count, main_seed, _b(hash),
+                         ctx.ptr, q.ptr if q else None)
+
+    def source(self):
+        return self.L.clo_rng_get_source(self.ptr).decode()
+
+    def size(self):
+        return self.L.clo_rng_get_size(self.ptr)
+
+    def seeds_ptr(self):
+        return self.L.clo_rng_get_device_seeds(self.ptr)
+
+    def read_states(self, q):
+        buf = self.L.clo_rng_get_device_seeds(self.ptr)
+        n = self.size()
+        out = np.empty(n, dtype=np.uint8)
+        _call(self.L.ccl_buffer_enqueue_read, buf, q.ptr, 1, 0, n,
+              out.ctypes.data_as(ctypes.c_void_p), None)
+        return out
+
+    def fill(self, q, out_buf, draws, bits=32, maxint=0):
+        return _call(self.L.clo_rng_fill, self.ptr, q.ptr, out_buf.ptr, draws, bits, maxint)
+
+    def close(self):
+        if self.ptr:
+            self.L.clo_rng_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generate(q, buf, n, kind, seed, param=0, start=0):
+    return _call(lib().clo_util_generate, q.ptr, buf.ptr, n, kind, seed, param, start)
+
+
+def multiset_hash(q, buf, n, width):
+    return _call(lib().clo_util_multiset_hash, q.ptr, buf.ptr, n, width)
+
+
+def check_sorted(q, keys, n, width, vals=None):
+    return _call(lib().clo_util_check_sorted, q.ptr, keys.ptr, vals.ptr if vals else None, n,
+                 width)

@@ -1,0 +1,388 @@
+// sbitonic and abitonic (CL_Ops' src/cl_ops/sort/clo_sort_sbitonic.{c,cl} and
+// clo_sort_abitonic.{c,cl}) for gfx950.
+//
+// Both run one bitonic sorting network over N = nlpo2(n) positions; they
+// differ only in how the network's steps are grouped into kernel launches.
+// The network is the all-ascending form: stage s (blocks of B = 2^s) starts
+// with a "flip" step pairing i with i ^ (B-1) inside the block, followed by
+// half-cleaner steps pairing i with i ^ 2^(t-1), t = s-1..1; every comparator
+// puts the element that sorts first at the lower index.  Positions >= n hold
+// virtual elements that sort after everything: a comparator whose upper
+// position is >= n does nothing, so real elements never leave [0, n) and the
+// caller's buffer needs no padding.  For keys-only sorts the output equals
+// the reference's (a sorted permutation is unique up to equal, identical keys).
+//
+// Order and signedness are folded into an XOR mask applied on load and store:
+// signed keys flip the sign bit, descending order complements the key, and the
+// kernels then sort unsigned words ascending.
+
+#include <cstring>
+
+#include "clo_internal.hpp"
+#include "dev_common.hpp"
+
+using namespace clo;
+using namespace clo::dev;
+
+namespace {
+
+constexpr int kBitBlock = 256;
+
+template <typename T>
+struct TileMax {
+	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : 13;  // 32 or 64 KiB of LDS
+};
+
+__device__ __forceinline__ u64 insert_zero_bits(u64 v, u32 pos, u32 q) {
+	const u64 low = v & ((1ull << pos) - 1ull);
+	return ((v >> pos) << (pos + q)) | low;
+}
+
+// Q consecutive steps of stage s starting at step t (t == s: the first is the
+// flip step), each thread holding the 2^Q positions one comparator group
+// touches.  Q = 1 is one network step (the sbitonic kernel).
+template <typename T, int Q>
+__global__ __launch_bounds__(kBitBlock) void bitonic_group(T* __restrict__ data, u64 n, u64 npo2,
+	u32 s, u32 t, T xmask) {
+	constexpr int V = 1 << Q;
+	const u64 tid = static_cast<u64>(blockIdx.x) * kBitBlock + threadIdx.x;
+	if (tid >= (npo2 >> Q)) return;
+	const u32 pos = t - Q;  // lowest free bit
+	const u64 x = insert_zero_bits(tid, pos, Q);
+	u64 masks[Q];
+#pragma unroll
+	for (int r = 0; r < Q; ++r) masks[r] = 1ull << (t - 1 - r);
+	if (t == s) masks[0] = (1ull << s) - 1ull;
+	u64 idx[V];
+	T val[V];
+#pragma unroll
+	for (int j = 0; j < V; ++j) {
+		u64 e = x;
+#pragma unroll
+		for (int r = 0; r < Q; ++r)
+			if (j & (1 << r)) e ^= masks[r];
+		idx[j] = e;
+		val[j] = e < n ? static_cast<T>(data[e] ^ xmask) : T(0);
+	}
+#pragma unroll
+	for (int r = 0; r < Q; ++r) {
+#pragma unroll
+		for (int j = 0; j < V; ++j) {
+			if (j & (1 << r)) continue;
+			const int k = j | (1 << r);
+			const bool jlo = idx[j] < idx[k];
+			const u64 hi = jlo ? idx[k] : idx[j];
+			if (hi >= n) continue;
+			T a = val[j], b = val[k];
+			T mn = a < b ? a : b, mx = a < b ? b : a;
+			val[j] = jlo ? mn : mx;
+			val[k] = jlo ? mx : mn;
+		}
+	}
+#pragma unroll
+	for (int j = 0; j < V; ++j)
+		if (idx[j] < n) data[idx[j]] = static_cast<T>(val[j] ^ xmask);
+}
+
+// One comparator of the step (s, t) inside an LDS tile; tile position p of the
+// pair index i.  Returns lo/hi tile positions.
+__device__ __forceinline__ void pair_pos(u32 i, u32 s, u32 t, u32* lo, u32* hi) {
+	if (t == s) {
+		const u32 half = 1u << (s - 1);
+		const u32 blk = i >> (s - 1);
+		const u32 off = i & (half - 1u);
+		*lo = (blk << s) + off;
+		*hi = (blk << s) + (2u * half - 1u - off);
+	} else {
+		const u32 h = 1u << (t - 1);
+		*lo = ((i >> (t - 1)) << t) + (i & (h - 1u));
+		*hi = *lo + h;
+	}
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_steps(T* s_t, u64 tile_base, u64 n, u32 tile_n, u32 s_first,
+	u32 s_last, u32 t_top_limit) {
+	for (u32 s = s_first; s <= s_last; ++s) {
+		const u32 t0 = s < t_top_limit ? s : t_top_limit;
+		for (u32 t = t0; t >= 1; --t) {
+			for (u32 i = threadIdx.x; i < tile_n / 2; i += kBitBlock) {
+				u32 lo, hi;
+				pair_pos(i, s, t, &lo, &hi);
+				if (tile_base + hi < n) {
+					T a = s_t[lo], b = s_t[hi];
+					if (b < a) {
+						s_t[lo] = b;
+						s_t[hi] = a;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	}
+}
+
+// Stages 1..k of every tile of 2^k positions, in LDS.
+template <typename T>
+__global__ __launch_bounds__(kBitBlock) void bitonic_tilesort(T* __restrict__ data, u64 n, u32 k,
+	T xmask) {
+	__shared__ T s_t[1u << TileMax<T>::kLog];
+	const u32 tile_n = 1u << k;
+	const u64 base = static_cast<u64>(blockIdx.x) << k;
+	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
+		const u64 g = base + i;
+		s_t[i] = g < n ? static_cast<T>(data[g] ^ xmask) : T(0);
+	}
+	__syncthreads();
+	tile_steps(s_t, base, n, tile_n, 1, k, k);
+	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
+		const u64 g = base + i;
+		if (g < n) data[g] = static_cast<T>(s_t[i] ^ xmask);
+	}
+}
+
+// Steps k..1 of stage s > k (half-cleaners inside each tile), in LDS.
+template <typename T>
+__global__ __launch_bounds__(kBitBlock) void bitonic_local(T* __restrict__ data, u64 n, u32 k,
+	u32 s, T xmask) {
+	__shared__ T s_t[1u << TileMax<T>::kLog];
+	const u32 tile_n = 1u << k;
+	const u64 base = static_cast<u64>(blockIdx.x) << k;
+	if (base >= n) return;
+	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
+		const u64 g = base + i;
+		s_t[i] = g < n ? static_cast<T>(data[g] ^ xmask) : T(0);
+	}
+	__syncthreads();
+	tile_steps(s_t, base, n, tile_n, s, s, k);
+	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
+		const u64 g = base + i;
+		if (g < n) data[g] = static_cast<T>(s_t[i] ^ xmask);
+	}
+}
+
+struct BitonicData {
+	bool advanced;
+	u32 minps = 1, maxps = 4, maxsfs = 0xFFFFFFFFu;
+	u32 last_sweeps = 0;
+};
+
+template <typename T>
+T make_xmask(const CloSort* s) {
+	T m = 0;
+	if (type_is_signed(s->elem_type)) m ^= static_cast<T>(T(1) << (sizeof(T) * 8 - 1));
+	if (s->descending) m ^= static_cast<T>(~T(0));
+	return m;
+}
+
+template <typename T>
+bool launch_group(CCLQueue* q, T* d, u64 n, u64 npo2, u32 s, u32 t, u32 Qn, T xm, const char* nm,
+	GError** err) {
+	const u64 threads = npo2 >> Qn;
+	const u32 grid = static_cast<u32>((threads + kBitBlock - 1) / kBitBlock);
+	Launch L(q, nm);
+	switch (Qn) {
+	case 1: bitonic_group<T, 1><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
+	case 2: bitonic_group<T, 2><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
+	case 3: bitonic_group<T, 3><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
+	case 4: bitonic_group<T, 4><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
+	}
+	return L.done(err);
+}
+
+template <typename T>
+bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GError** err) {
+	const T xm = make_xmask<T>(sorter);
+	u64 npo2 = 1;
+	u32 S = 0;
+	while (npo2 < n) {
+		npo2 <<= 1;
+		++S;
+	}
+	u32 sweeps = 0;
+	if (!bd->advanced) {
+		// sbitonic: one launch per (stage, step), clo_sort_sbitonic.c:102-118.
+		for (u32 s = 1; s <= S; ++s)
+			for (u32 t = s; t >= 1; --t) {
+				if (!launch_group<T>(q, d, n, npo2, s, t, 1, xm, CLO_SORT_SBITONIC_KNAME, err))
+					return false;
+				++sweeps;
+			}
+		bd->last_sweeps = sweeps;
+		return true;
+	}
+	// abitonic strategy: stages 1..k in one LDS tile kernel, then per stage the
+	// global steps in register groups of up to maxps steps (at least minps when
+	// that many remain) and the last k steps in one LDS kernel.
+	u32 k = TileMax<T>::kLog;
+	if (bd->maxsfs < k) k = bd->maxsfs;
+	if (k > S) k = S;
+	if (k >= 1) {
+		const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
+		Launch L(q, CLO_SORT_ABITONIC_KNAME_TILESORT);
+		bitonic_tilesort<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, xm);
+		if (!L.done(err)) return false;
+		++sweeps;
+	}
+	static const char* gname[5] = {"", CLO_SORT_ABITONIC_KNAME_ANY,
+		CLO_SORT_ABITONIC_KNAME_PRIV_2S4V, CLO_SORT_ABITONIC_KNAME_PRIV_3S8V,
+		CLO_SORT_ABITONIC_KNAME_PRIV_4S16V};
+	for (u32 s = k + 1; s <= S; ++s) {
+		u32 t = s;
+		const u32 t_stop = k;  // steps t > t_stop are global
+		while (t > t_stop) {
+			u32 rem = t - t_stop;
+			u32 Qn = rem < bd->maxps ? rem : bd->maxps;
+			if (Qn < bd->minps && rem >= bd->minps) Qn = bd->minps;
+			if (Qn > t) Qn = t;
+			if (!launch_group<T>(q, d, n, npo2, s, t, Qn, xm, gname[Qn], err)) return false;
+			++sweeps;
+			t -= Qn;
+		}
+		if (k >= 1) {
+			const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
+			Launch L(q, CLO_SORT_ABITONIC_KNAME_LOCAL);
+			bitonic_local<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, s, xm);
+			if (!L.done(err)) return false;
+			++sweeps;
+		}
+	}
+	bd->last_sweeps = sweeps;
+	return true;
+}
+
+CCLEvent* bitonic_sort(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm, CCLBuffer* in,
+	CCLBuffer* out, size_t numel, size_t lws_max, GError** err) {
+	(void)lws_max;
+	(void)cq_comm;
+	BitonicData* bd = static_cast<BitonicData*>(sorter->data);
+	const size_t es = clo_type_sizeof(sorter->elem_type);
+	void* d = in->ptr;
+	if (out && out->ptr != in->ptr) {
+		// Sort a copy in data_out; data_in stays unchanged.
+		if (numel && hip_failed(hipMemcpyAsync(out->ptr, in->ptr, numel * es,
+				hipMemcpyDeviceToDevice, cq_exec->stream), err, "hipMemcpyAsync"))
+			return nullptr;
+		d = out->ptr;
+	}
+	bool ok = true;
+	bd->last_sweeps = 0;
+	if (numel > 1) {
+		switch (es) {
+		case 1: ok = run_network<u8>(sorter, bd, cq_exec, static_cast<u8*>(d), numel, err); break;
+		case 2: ok = run_network<u16>(sorter, bd, cq_exec, static_cast<u16*>(d), numel, err); break;
+		case 4: ok = run_network<u32>(sorter, bd, cq_exec, static_cast<u32*>(d), numel, err); break;
+		case 8: ok = run_network<u64>(sorter, bd, cq_exec, static_cast<u64*>(d), numel, err); break;
+		default: ok = false;
+		}
+	}
+	if (!ok) return nullptr;
+	return record_event(cq_exec, bd->advanced ? "abitonic_done" : "sbitonic_done", err);
+}
+
+const char* sbitonic_init(CloSort* sorter, const char* options, GError** err) {
+	(void)options;  // clo_sort_sbitonic.c ignores options
+	(void)err;
+	BitonicData* bd = new BitonicData();
+	bd->advanced = false;
+	sorter->data = bd;
+	return "sbitonic (gfx950: one launch per network step)";
+}
+
+const char* abitonic_init(CloSort* sorter, const char* options, GError** err) {
+	// clo_sort_abitonic.c:458-567.
+	BitonicData* bd = new BitonicData();
+	bd->advanced = true;
+	sorter->data = bd;
+	std::vector<Opt> opts;
+	if (!parse_options(options, opts, "abitonic sort", err)) return nullptr;
+	for (auto& o : opts) {
+		const int v = atoi(o.value.c_str());
+		if (o.key == "minps") {
+			if (v < 1 || v > 4) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "Option 'minps' must be between 1 and 4.");
+				return nullptr;
+			}
+			bd->minps = static_cast<u32>(v);
+		} else if (o.key == "maxps") {
+			if (v < 1 || v > 4) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "Option 'maxps' must be between 1 and 4.");
+				return nullptr;
+			}
+			bd->maxps = static_cast<u32>(v);
+		} else if (o.key == "maxsfs") {
+			bd->maxsfs = static_cast<u32>(v);
+		} else {
+			set_error(err, kQuarkClo, CLO_ERROR_ARGS, "Invalid option key '%s' for abitonic sort.",
+				o.key.c_str());
+			return nullptr;
+		}
+	}
+	if (bd->maxps < bd->minps) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"'minps' (%u) must be less or equal than 'maxps' (%u).", bd->minps, bd->maxps);
+		return nullptr;
+	}
+	return "abitonic (gfx950: LDS tiles + register-fused global steps)";
+}
+
+void bitonic_finalize(CloSort* sorter) {
+	delete static_cast<BitonicData*>(sorter->data);
+	sorter->data = nullptr;
+}
+
+cl_uint sbitonic_num_kernels(CloSort*, GError**) { return 1; }
+const char* sbitonic_kernel_name(CloSort*, cl_uint i, GError** err) {
+	if (i != 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "kernel index %u out of range", i);
+		return nullptr;
+	}
+	return CLO_SORT_SBITONIC_KNAME;
+}
+size_t sbitonic_localmem(CloSort*, cl_uint, size_t, size_t, GError**) { return 0; }
+
+cl_uint abitonic_num_kernels(CloSort*, GError**) { return CLO_SORT_ABITONIC_NUM_KERNELS; }
+const char* abitonic_kernel_name(CloSort*, cl_uint i, GError** err) {
+	static const char* names[] = CLO_SORT_ABITONIC_KERNELNAMES;
+	if (i >= CLO_SORT_ABITONIC_NUM_KERNELS) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "kernel index %u out of range", i);
+		return nullptr;
+	}
+	return names[i];
+}
+size_t abitonic_localmem(CloSort* sorter, cl_uint i, size_t, size_t, GError** err) {
+	if (i >= CLO_SORT_ABITONIC_NUM_KERNELS) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "kernel index %u out of range", i);
+		return 0;
+	}
+	const size_t es = clo_type_sizeof(sorter->elem_type);
+	if (i < 4) return 0;  // register-only kernels
+	return (es >= 8 ? (1u << 12) : (1u << 13)) * es;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Host-pass only: the vtable holds host function addresses.
+#if !defined(__HIP_DEVICE_COMPILE__)
+const CloSortImplDef clo_sort_sbitonic_def = {"sbitonic", CL_TRUE, sbitonic_init,
+	bitonic_finalize, bitonic_sort, sbitonic_num_kernels, sbitonic_kernel_name, sbitonic_localmem};
+#endif
+
+
+// Host-pass only: the vtable holds host function addresses.
+#if !defined(__HIP_DEVICE_COMPILE__)
+const CloSortImplDef clo_sort_abitonic_def = {"abitonic", CL_TRUE, abitonic_init,
+	bitonic_finalize, bitonic_sort, abitonic_num_kernels, abitonic_kernel_name, abitonic_localmem};
+#endif
+
+
+cl_uint clo_sort_abitonic_get_last_sweeps(CloSort* sorter) {
+	if (!sorter || !sorter->data) return 0;
+	if (strcmp(sorter->impl.name, "abitonic") != 0 && strcmp(sorter->impl.name, "sbitonic") != 0)
+		return 0;
+	return static_cast<BitonicData*>(sorter->data)->last_sweeps;
+}
+
+}  // extern "C"

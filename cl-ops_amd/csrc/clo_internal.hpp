@@ -1,0 +1,148 @@
+// Internal definitions shared by the library's translation units: the
+// layouts behind the opaque CCL* handles, error helpers, the launch wrapper
+// that names and (optionally) times every kernel, and device scratch caches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cl_ops.h"
+
+struct ccl_context {
+	int dev;
+	int refc;
+};
+
+struct ccl_event {
+	hipEvent_t ev;
+	std::string name;
+};
+
+struct ccl_prof_rec {
+	std::string name;
+	hipEvent_t start;
+	hipEvent_t end;
+};
+
+struct ccl_queue {
+	CCLContext* ctx;
+	hipStream_t stream;
+	bool owns_stream;
+	bool prof;
+	std::vector<ccl_event*> events;       // every event this queue recorded
+	std::vector<ccl_prof_rec> prof_recs;  // used records since last reset
+	std::vector<hipEvent_t> prof_pool;    // spare events for prof records
+	std::vector<std::string> prof_names;  // scratch for ccl_prof_get_name
+};
+
+struct ccl_buffer {
+	CCLContext* ctx;
+	void* ptr;
+	size_t size;
+	bool owns;
+	int refc;
+};
+
+struct ccl_event_wait_list {
+	std::vector<CCLEvent*> evts;
+};
+
+struct ccl_program {
+	int unused;
+};
+
+namespace clo {
+
+enum : GQuark { kQuarkClo = 0x636c6f01u, kQuarkCcl = 0x63636c01u };
+
+void set_error(GError** err, GQuark domain, int code, const char* fmt, ...)
+	__attribute__((format(printf, 4, 5)));
+
+// Sets a CCL_ERROR (code = hipError_t) and returns true if `e` is an error.
+bool hip_failed(hipError_t e, GError** err, const char* what);
+
+// Device allocation honouring the allocation-failure test hook.
+hipError_t dev_malloc(void** p, size_t bytes);
+
+// Records a new event on the queue (owned by the queue).
+CCLEvent* record_event(CCLQueue* q, const char* name, GError** err);
+
+// Activates the context's device for the calling thread.
+bool use_device(CCLContext* ctx, GError** err);
+
+// Kernel launch wrapper: records start/end events when the queue profiles,
+// and converts a launch failure into a GError.
+struct Launch {
+	CCLQueue* q;
+	const char* name;
+	ccl_prof_rec* rec;
+	Launch(CCLQueue* q_, const char* name_);
+	bool done(GError** err);  // after the launch; false on error
+};
+
+// A growable device scratch buffer owned by a sorter/scanner.  Growing frees
+// the previous allocation (hipFree synchronises with the device, so kernels
+// still reading it have finished).
+struct Scratch {
+	void* ptr = nullptr;
+	size_t size = 0;
+	bool zero_on_alloc = false;
+	bool ensure(size_t bytes, GError** err);
+	void release();
+	~Scratch() { release(); }
+};
+
+inline size_t div_ceil(size_t a, size_t b) { return (a + b - 1) / b; }
+
+// Option string "k=v,k=v".  Returns false (and sets CLO_ERROR_ARGS) on a
+// token without exactly one '='.
+struct Opt {
+	std::string key, value;
+};
+bool parse_options(const char* options, std::vector<Opt>& out,
+	const char* algo, GError** err);
+
+// Normalises a compare snippet: 0 = ascending ("((a) > (b))" or NULL),
+// 1 = descending ("((a) < (b))"), -1 = unsupported.
+int compare_kind(const char* compare);
+// True for NULL, "", "(x)", "x".
+bool get_key_is_identity(const char* get_key);
+
+bool type_is_integer(CloType t);
+bool type_is_signed(CloType t);
+
+}  // namespace clo
+
+// Internal accessors of the sorter shared by the algorithm files.
+struct clo_sort {
+	CloSortImplDef impl;
+	CCLContext* ctx;
+	CloType elem_type;
+	CloType key_type;
+	int descending;
+	void* data;
+	// Set by satradix: device word with the executed pass count of the last
+	// call, and the queue it ran on.
+	CCLQueue* last_queue;
+};
+
+struct clo_scan {
+	CloScanImplDef impl;
+	CCLContext* ctx;
+	CloType elem_type;
+	CloType sum_type;
+	void* data;
+};
+
+// KV entry points of the algorithms that support them.
+CCLEvent* clo_satradix_sort_kv(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in,
+	CCLBuffer* keys_out, CCLBuffer* vals_out, size_t numel, GError** err);
+CCLEvent* clo_gselect_sort_kv(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in,
+	CCLBuffer* keys_out, CCLBuffer* vals_out, size_t numel, GError** err);
+cl_uint clo_satradix_last_passes(CloSort* sorter, GError** err);

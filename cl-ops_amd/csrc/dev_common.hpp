@@ -1,0 +1,93 @@
+// Device helpers shared by the gfx950 kernels: wave64 lane arithmetic,
+// block-wide scans, and the look-back status words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace clo {
+namespace dev {
+
+typedef uint8_t u8;
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ u32 lane_id() { return threadIdx.x & (kWave - 1); }
+__device__ __forceinline__ u32 wave_id() { return threadIdx.x / kWave; }
+
+// Number of set bits of `mask` in lanes below the calling lane.
+__device__ __forceinline__ u32 count_below(u64 mask) {
+	return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(mask >> 32),
+		__builtin_amdgcn_mbcnt_lo(static_cast<u32>(mask), 0u));
+}
+
+__device__ __forceinline__ u64 ballot(bool p) { return __ballot(p); }
+
+// Inclusive wave-wide prefix sum.
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T x) {
+	const u32 lane = lane_id();
+#pragma unroll
+	for (int off = 1; off < kWave; off <<= 1) {
+		T y = __shfl_up(x, off, kWave);
+		if (lane >= static_cast<u32>(off)) x += y;
+	}
+	return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+	for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+	return x;
+}
+
+// Exclusive block-wide prefix sum for a block of NT threads (NT multiple of
+// 64).  `s_wsum` holds NT/64 elements of shared scratch.  Ends with a
+// barrier, so s_wsum may be reused afterwards.
+template <int NT, typename T>
+__device__ __forceinline__ T block_exclusive_sum(T v, T* s_wsum, T* total) {
+	constexpr int W = NT / kWave;
+	T inc = wave_inclusive_sum(v);
+	if (lane_id() == kWave - 1) s_wsum[wave_id()] = inc;
+	__syncthreads();
+	T pre = 0, tot = 0;
+	const u32 w = wave_id();
+#pragma unroll
+	for (int i = 0; i < W; ++i) {
+		T s = s_wsum[i];
+		if (static_cast<u32>(i) < w) pre += s;
+		tot += s;
+	}
+	if (total) *total = tot;
+	__syncthreads();
+	return pre + inc - v;
+}
+
+// ---- look-back status words -----------------------------------------------
+// 64-bit word: high half = (epoch << 2) | flag, low half = 32-bit value.
+// The epoch (30 bits) is unique per kernel launch, so words left over from
+// earlier launches read as "not ready" and the array never needs clearing.
+constexpr u32 kFlagAggregate = 1;
+constexpr u32 kFlagPrefix = 2;
+
+__device__ __forceinline__ void status_store(u64* p, u32 epoch, u32 flag, u32 value) {
+	u64 w = (static_cast<u64>((epoch << 2) | flag) << 32) | value;
+	__hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ u64 status_load(const u64* p) {
+	return __hip_atomic_load(const_cast<u64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ u32 status_flag(u64 w, u32 epoch) {
+	u32 hi = static_cast<u32>(w >> 32);
+	return (hi >> 2) == epoch ? (hi & 3u) : 0u;
+}
+
+}  // namespace dev
+}  // namespace clo

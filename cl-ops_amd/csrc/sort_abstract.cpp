@@ -1,0 +1,261 @@
+// Sorter factory, dispatch and host-data paths
+// (CL_Ops' src/cl_ops/sort/clo_sort_abstract.c:91-629).
+
+#include <cstring>
+
+#include "clo_internal.hpp"
+
+using namespace clo;
+
+extern "C" {
+
+CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
+	CloType* elem_type, CloType* key_type, const char* compare, const char* get_key,
+	const char* compiler_opts, GError** err) {
+	(void)compiler_opts;  // kernels are compiled ahead of time
+	if (err && *err) return nullptr;
+	if (!ctx || !elem_type) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "clo_sort_new: NULL context or element type");
+		return nullptr;
+	}
+	// clo_sort_abstract.c:102-121: search the implementation by name.
+	const CloSortImplDef* defs[] = {&clo_sort_sbitonic_def, &clo_sort_abitonic_def,
+		&clo_sort_gselect_def, &clo_sort_satradix_def};
+	const CloSortImplDef* impl = nullptr;
+	for (const CloSortImplDef* d : defs)
+		if (type && strcmp(type, d->name) == 0) impl = d;
+	if (!impl) {
+		set_error(err, kQuarkClo, CLO_ERROR_IMPL_NOT_FOUND,
+			"The requested sort implementation, '%s', was not found.", type ? type : "(null)");
+		return nullptr;
+	}
+	CloType et = *elem_type;
+	CloType kt = key_type ? *key_type : et;
+	if (!type_is_integer(et)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"Element type '%s' is not supported: sorts take integer types (char..ulong).",
+			clo_type_get_name(et) ? clo_type_get_name(et) : "?");
+		return nullptr;
+	}
+	if (kt != et || !get_key_is_identity(get_key)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"Custom keys (key type or get_key snippet) are not supported: "
+			"the key is the element.");
+		return nullptr;
+	}
+	int ck = compare_kind(compare);
+	if (ck < 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"Unsupported compare snippet '%s' (use ((a) > (b)) or ((a) < (b))).", compare);
+		return nullptr;
+	}
+	CloSort* s = new clo_sort();
+	s->impl = *impl;
+	s->ctx = ctx;
+	ccl_context_ref(ctx);
+	s->elem_type = et;
+	s->key_type = kt;
+	s->descending = ck;
+	s->data = nullptr;
+	s->last_queue = nullptr;
+	GError* e = nullptr;
+	const char* src = s->impl.init(s, options, &e);
+	if (!src || e) {
+		if (e) {
+			if (err && !*err) *err = e;
+			else clo_error_free(e);
+		} else {
+			set_error(err, kQuarkClo, CLO_ERROR_ARGS, "initialisation of %s failed", type);
+		}
+		clo_sort_destroy(s);
+		return nullptr;
+	}
+	return s;
+}
+
+void clo_sort_destroy(CloSort* sorter) {
+	if (!sorter) return;
+	if (sorter->impl.finalize) sorter->impl.finalize(sorter);
+	ccl_context_destroy(sorter->ctx);
+	delete sorter;
+}
+
+static bool check_device_args(CloSort* s, CCLQueue* cq, CCLBuffer* in, CCLBuffer* out,
+	size_t numel, size_t elem, GError** err) {
+	if (!s || !cq || !in) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL sorter, queue or input buffer");
+		return false;
+	}
+	// 32-bit element indices, as the reference's kernels (clo_nlpo2 caps n at 2^31).
+	if (numel > 0x80000000ull) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "numel %zu exceeds 2^31", numel);
+		return false;
+	}
+	if (in->size < numel * elem || (out && out->size < numel * elem)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"buffer smaller than numel (%zu) elements of %zu bytes", numel, elem);
+		return false;
+	}
+	return use_device(cq->ctx, err);
+}
+
+CCLEvent* clo_sort_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
+	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, size_t lws_max, GError** err) {
+	if (err && *err) return nullptr;
+	if (!check_device_args(sorter, cq_exec, data_in, data_out, numel,
+			sorter ? clo_type_sizeof(sorter->elem_type) : 0, err))
+		return nullptr;
+	return sorter->impl.sort_with_device_data(sorter, cq_exec, cq_comm, data_in, data_out,
+		numel, lws_max, err);
+}
+
+cl_bool clo_sort_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
+	void* data_in, void* data_out, size_t numel, size_t lws_max, GError** err) {
+	if (err && *err) return CL_FALSE;
+	if (!sorter || !data_in || !data_out) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL sorter or host pointer");
+		return CL_FALSE;
+	}
+	size_t bytes = numel * clo_type_sizeof(sorter->elem_type);
+	CCLQueue* own = nullptr;
+	CCLBuffer* din = nullptr;
+	CCLBuffer* dout = nullptr;
+	cl_bool ok = CL_FALSE;
+	CCLEvent* evt = nullptr;
+	CCLEventWaitList ewl = nullptr;
+	// clo_sort_abstract.c:333-341: internal queue when cq_exec is NULL.
+	if (!cq_exec) {
+		own = ccl_queue_new(sorter->ctx, nullptr, 0, err);
+		if (!own) return CL_FALSE;
+		cq_exec = own;
+	}
+	if (!cq_comm) cq_comm = cq_exec;
+	din = ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, bytes, nullptr, err);
+	if (!din) goto done;
+	if (!sorter->impl.in_place) {
+		dout = ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, bytes, nullptr, err);
+		if (!dout) goto done;
+	}
+	evt = ccl_buffer_enqueue_write(din, cq_comm, CL_FALSE, 0, bytes, data_in, nullptr, err);
+	if (!evt) goto done;
+	// The copy on cq_comm must finish before the sort on cq_exec starts.
+	ccl_event_wait_list_add(&ewl, evt, nullptr);
+	if (!ccl_event_wait(&ewl, err)) goto done;
+	evt = clo_sort_with_device_data(sorter, cq_exec, cq_comm, din, dout, numel, lws_max, err);
+	if (!evt) goto done;
+	ccl_event_wait_list_add(&ewl, evt, nullptr);
+	evt = ccl_buffer_enqueue_read(dout ? dout : din, cq_comm, CL_TRUE, 0, bytes, data_out,
+		&ewl, err);
+	if (!evt) goto done;
+	ok = CL_TRUE;
+done:
+	ccl_event_wait_list_clear(&ewl);
+	if (din) ccl_buffer_destroy(din);
+	if (dout) ccl_buffer_destroy(dout);
+	if (own) ccl_queue_destroy(own);
+	return ok;
+}
+
+CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
+	CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out, CCLBuffer* vals_out,
+	size_t numel, size_t lws_max, GError** err) {
+	(void)lws_max;
+	if (err && *err) return nullptr;
+	if (!check_device_args(sorter, cq_exec, keys_in, keys_out, numel,
+			sorter ? clo_type_sizeof(sorter->elem_type) : 0, err))
+		return nullptr;
+	if (!vals_in || vals_in->size < numel * 4 || (vals_out && vals_out->size < numel * 4) ||
+		((keys_out == nullptr) != (vals_out == nullptr))) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"KV sort needs uint32 value buffers of numel elements; keys_out and vals_out "
+			"must both be NULL (in place) or both set");
+		return nullptr;
+	}
+	if (strcmp(sorter->impl.name, "satradix") == 0)
+		return clo_satradix_sort_kv(sorter, cq_exec, cq_comm, keys_in, vals_in, keys_out,
+			vals_out, numel, err);
+	if (strcmp(sorter->impl.name, "gselect") == 0)
+		return clo_gselect_sort_kv(sorter, cq_exec, cq_comm, keys_in, vals_in, keys_out,
+			vals_out, numel, err);
+	set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+		"'%s' is not a stable sort; key-value sorting needs satradix or gselect",
+		sorter->impl.name);
+	return nullptr;
+}
+
+cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
+	void* keys, void* vals, size_t numel, size_t lws_max, GError** err) {
+	if (err && *err) return CL_FALSE;
+	if (!sorter || !keys || !vals) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL sorter or host pointer");
+		return CL_FALSE;
+	}
+	size_t kb = numel * clo_type_sizeof(sorter->elem_type), vb = numel * 4;
+	CCLQueue* own = nullptr;
+	CCLBuffer *dk = nullptr, *dv = nullptr;
+	CCLEvent* evt = nullptr;
+	CCLEventWaitList ewl = nullptr;
+	cl_bool ok = CL_FALSE;
+	if (!cq_exec) {
+		own = ccl_queue_new(sorter->ctx, nullptr, 0, err);
+		if (!own) return CL_FALSE;
+		cq_exec = own;
+	}
+	if (!cq_comm) cq_comm = cq_exec;
+	dk = ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, kb, nullptr, err);
+	dv = dk ? ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, vb, nullptr, err) : nullptr;
+	if (!dk || !dv) goto done;
+	if (!ccl_buffer_enqueue_write(dk, cq_comm, CL_TRUE, 0, kb, keys, nullptr, err)) goto done;
+	if (!ccl_buffer_enqueue_write(dv, cq_comm, CL_TRUE, 0, vb, vals, nullptr, err)) goto done;
+	evt = clo_sort_kv_with_device_data(sorter, cq_exec, cq_comm, dk, dv, nullptr, nullptr,
+		numel, lws_max, err);
+	if (!evt) goto done;
+	ccl_event_wait_list_add(&ewl, evt, nullptr);
+	if (!ccl_buffer_enqueue_read(dk, cq_comm, CL_TRUE, 0, kb, keys, &ewl, err)) goto done;
+	if (!ccl_buffer_enqueue_read(dv, cq_comm, CL_TRUE, 0, vb, vals, nullptr, err)) goto done;
+	ok = CL_TRUE;
+done:
+	ccl_event_wait_list_clear(&ewl);
+	if (dk) ccl_buffer_destroy(dk);
+	if (dv) ccl_buffer_destroy(dv);
+	if (own) ccl_queue_destroy(own);
+	return ok;
+}
+
+cl_uint clo_sort_get_last_passes(CloSort* sorter, GError** err) {
+	if (!sorter) return 0;
+	if (strcmp(sorter->impl.name, "satradix") != 0) return 0;
+	return clo_satradix_last_passes(sorter, err);
+}
+
+CCLContext* clo_sort_get_context(CloSort* sorter) { return sorter ? sorter->ctx : nullptr; }
+CCLProgram* clo_sort_get_program(CloSort* sorter) {
+	(void)sorter;
+	return nullptr;  // no runtime-compiled program: kernels are built ahead of time
+}
+CloType clo_sort_get_element_type(CloSort* sorter) { return sorter->elem_type; }
+size_t clo_sort_get_element_size(CloSort* sorter) { return clo_type_sizeof(sorter->elem_type); }
+CloType clo_sort_get_key_type(CloSort* sorter) { return sorter->key_type; }
+size_t clo_sort_get_key_size(CloSort* sorter) { return clo_type_sizeof(sorter->key_type); }
+void* clo_sort_get_data(CloSort* sorter) { return sorter ? sorter->data : nullptr; }
+void clo_sort_set_data(CloSort* sorter, void* data) {
+	if (sorter) sorter->data = data;
+}
+
+cl_uint clo_sort_get_num_kernels(CloSort* sorter, GError** err) {
+	if (!sorter) return 0;
+	return sorter->impl.get_num_kernels(sorter, err);
+}
+
+const char* clo_sort_get_kernel_name(CloSort* sorter, cl_uint i, GError** err) {
+	if (!sorter) return nullptr;
+	return sorter->impl.get_kernel_name(sorter, i, err);
+}
+
+size_t clo_sort_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, size_t numel,
+	GError** err) {
+	if (!sorter) return 0;
+	return sorter->impl.get_localmem_usage(sorter, i, lws_max, numel, err);
+}
+
+}  // extern "C"

@@ -1,0 +1,276 @@
+// Seeded device input generators, a calibration copy kernel and device-side
+// result checks (additions with no counterpart in CL_Ops).
+
+#include <cmath>
+#include <cstring>
+
+#include "clo_internal.hpp"
+#include "dev_common.hpp"
+
+using namespace clo;
+using namespace clo::dev;
+
+namespace {
+
+constexpr u64 kGolden = 0x9E3779B97F4A7C15ull;
+
+// splitmix64 finalizer (Steele, Lea & Flood, OOPSLA 2014).
+__host__ __device__ __forceinline__ u64 mix64(u64 z) {
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ u64 U(u64 seed, u64 i) { return mix64(seed + (i + 1) * kGolden); }
+
+template <int KIND>
+__global__ __launch_bounds__(256) void util_generate(void* out, u64 n, u64 seed, u64 param,
+	u64 start) {
+	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+	for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const u64 c = i + start;
+		switch (KIND) {
+		case CLO_GEN_U64:
+			static_cast<u64*>(out)[i] = U(seed, c);
+			break;
+		case CLO_GEN_U32_HI:
+			static_cast<u32*>(out)[i] = static_cast<u32>(U(seed, c) >> 32);
+			break;
+		case CLO_GEN_U64_MASK:
+			static_cast<u64*>(out)[i] = U(seed, c) & param;
+			break;
+		case CLO_GEN_U32_MASK:
+			static_cast<u32*>(out)[i] = static_cast<u32>(U(seed, c) >> 32) & static_cast<u32>(param);
+			break;
+		case CLO_GEN_IOTA_U32:
+			static_cast<u32*>(out)[i] = static_cast<u32>(c);
+			break;
+		case CLO_GEN_U32_MOD:
+			static_cast<u32*>(out)[i] = static_cast<u32>(U(seed, c) >> 32) % static_cast<u32>(param);
+			break;
+		case CLO_GEN_U64_MOD:
+			static_cast<u64*>(out)[i] = U(seed, c) % param;
+			break;
+		}
+	}
+}
+
+__global__ __launch_bounds__(256) void util_generate_zipf(u64* out, u64 n, const double* cdf,
+	u64 ranks, u64 seed) {
+	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+	for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const double u = static_cast<double>(U(seed, i) >> 11) * 0x1.0p-53;
+		// Smallest r with cdf[r] > u.
+		u64 lo = 0, hi = ranks - 1;
+		while (lo < hi) {
+			u64 mid = (lo + hi) >> 1;
+			if (cdf[mid] > u) hi = mid;
+			else lo = mid + 1;
+		}
+		out[i] = mix64(lo);
+	}
+}
+
+__global__ __launch_bounds__(256) void util_copy(uint4* dst, const uint4* src, u64 n16,
+	unsigned char* dtail, const unsigned char* stail, u32 ntail) {
+	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+	u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+	for (; i < n16; i += stride) dst[i] = src[i];
+	if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void util_hash(const T* x, u64 n, u64* acc) {
+	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+	u64 s = 0;
+	for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+		s += mix64(static_cast<u64>(x[i]));
+	s = wave_sum(s);
+	if (lane_id() == 0) atomicAdd(reinterpret_cast<unsigned long long*>(acc),
+		static_cast<unsigned long long>(s));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void util_check_sorted(const T* k, const u32* v, u64 n,
+	u64* bad) {
+	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+	u64 c = 0;
+	for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i + 1 < n; i += stride) {
+		T a = k[i], b = k[i + 1];
+		if (a > b) ++c;
+		else if (v && a == b && !(v[i] < v[i + 1])) ++c;
+	}
+	c = wave_sum(c);
+	if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(bad),
+		static_cast<unsigned long long>(c));
+}
+
+u32 grid_for(u64 n) {
+	u64 g = (n + 255) / 256;
+	if (g > 8192) g = 8192;
+	if (g == 0) g = 1;
+	return static_cast<u32>(g);
+}
+
+}  // namespace
+
+extern "C" {
+
+CCLEvent* clo_util_generate(CCLQueue* cq, CCLBuffer* out, size_t n, CloGenKind kind,
+	cl_ulong seed, cl_ulong param, cl_ulong start, GError** err) {
+	if (!cq || !out) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL queue or buffer");
+		return nullptr;
+	}
+	size_t w = (kind == CLO_GEN_U64 || kind == CLO_GEN_U64_MASK || kind == CLO_GEN_ZIPF_U64 ||
+		kind == CLO_GEN_U64_MOD) ? 8 : 4;
+	if (kind == CLO_GEN_ZIPF_U64 || static_cast<int>(kind) < 0 || kind > CLO_GEN_U64_MOD) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "generator kind %d not valid here", kind);
+		return nullptr;
+	}
+	if ((kind == CLO_GEN_U32_MOD || kind == CLO_GEN_U64_MOD) && param == 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "modulus must be non-zero");
+		return nullptr;
+	}
+	if (out->size < n * w) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "buffer too small");
+		return nullptr;
+	}
+	if (!use_device(cq->ctx, err)) return nullptr;
+	Launch L(cq, "util_generate");
+	u32 g = grid_for(n);
+	switch (kind) {
+	case CLO_GEN_U64: util_generate<CLO_GEN_U64><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_U32_HI: util_generate<CLO_GEN_U32_HI><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_U64_MASK: util_generate<CLO_GEN_U64_MASK><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_U32_MASK: util_generate<CLO_GEN_U32_MASK><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_IOTA_U32: util_generate<CLO_GEN_IOTA_U32><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_U32_MOD: util_generate<CLO_GEN_U32_MOD><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_U64_MOD: util_generate<CLO_GEN_U64_MOD><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	default: break;
+	}
+	if (!L.done(err)) return nullptr;
+	return record_event(cq, "util_generate", err);
+}
+
+void clo_util_zipf_cdf(double* cdf, size_t ranks, double s) {
+	// Normalised partial sums of (r+1)^-s, accumulated in long double.
+	long double tot = 0.0L;
+	for (size_t r = 0; r < ranks; ++r) tot += powl(static_cast<long double>(r + 1), -s);
+	long double acc = 0.0L;
+	for (size_t r = 0; r < ranks; ++r) {
+		acc += powl(static_cast<long double>(r + 1), -s);
+		cdf[r] = static_cast<double>(acc / tot);
+	}
+	if (ranks) cdf[ranks - 1] = 1.0;
+}
+
+CCLEvent* clo_util_generate_zipf(CCLQueue* cq, CCLBuffer* out, size_t n, CCLBuffer* cdf,
+	size_t ranks, cl_ulong seed, GError** err) {
+	if (!cq || !out || !cdf || ranks == 0 || out->size < n * 8 || cdf->size < ranks * 8) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "invalid zipf generator arguments");
+		return nullptr;
+	}
+	if (!use_device(cq->ctx, err)) return nullptr;
+	Launch L(cq, "util_generate_zipf");
+	util_generate_zipf<<<grid_for(n), 256, 0, cq->stream>>>(static_cast<u64*>(out->ptr), n,
+		static_cast<const double*>(cdf->ptr), ranks, seed);
+	if (!L.done(err)) return nullptr;
+	return record_event(cq, "util_generate_zipf", err);
+}
+
+CCLEvent* clo_util_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src, size_t bytes,
+	GError** err) {
+	if (!cq || !dst || !src || dst->size < bytes || src->size < bytes ||
+		(reinterpret_cast<uintptr_t>(dst->ptr) | reinterpret_cast<uintptr_t>(src->ptr)) % 16) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "invalid copy arguments");
+		return nullptr;
+	}
+	if (!use_device(cq->ctx, err)) return nullptr;
+	const u64 n16 = bytes / 16;
+	const u32 ntail = static_cast<u32>(bytes % 16);
+	Launch L(cq, "util_copy");
+	u64 g = (n16 + 255) / 256;
+	if (g > 4096) g = 4096;
+	if (g == 0) g = 1;
+	util_copy<<<static_cast<u32>(g), 256, 0, cq->stream>>>(static_cast<uint4*>(dst->ptr),
+		static_cast<const uint4*>(src->ptr), n16,
+		static_cast<unsigned char*>(dst->ptr) + n16 * 16,
+		static_cast<const unsigned char*>(src->ptr) + n16 * 16, ntail);
+	if (!L.done(err)) return nullptr;
+	return record_event(cq, "util_copy", err);
+}
+
+static bool scratch_u64(CCLQueue* cq, u64** p, GError** err) {
+	if (hip_failed(hipMallocAsync(reinterpret_cast<void**>(p), 8, cq->stream), err,
+			"hipMallocAsync"))
+		return false;
+	return !hip_failed(hipMemsetAsync(*p, 0, 8, cq->stream), err, "hipMemsetAsync");
+}
+
+static bool read_u64(CCLQueue* cq, u64* p, u64* out, GError** err) {
+	bool ok = !hip_failed(hipMemcpyAsync(out, p, 8, hipMemcpyDeviceToHost, cq->stream), err,
+		"hipMemcpyAsync");
+	ok = ok && !hip_failed(hipStreamSynchronize(cq->stream), err, "hipStreamSynchronize");
+	(void)hipFreeAsync(p, cq->stream);
+	return ok;
+}
+
+cl_ulong clo_util_multiset_hash(CCLQueue* cq, CCLBuffer* buf, size_t n, size_t width,
+	GError** err) {
+	if (!cq || !buf || buf->size < n * width) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "invalid hash arguments");
+		return 0;
+	}
+	if (!use_device(cq->ctx, err)) return 0;
+	u64* acc = nullptr;
+	if (!scratch_u64(cq, &acc, err)) return 0;
+	u32 g = grid_for(n);
+	if (g > 2048) g = 2048;
+	switch (width) {
+	case 1: util_hash<u8><<<g, 256, 0, cq->stream>>>(static_cast<const u8*>(buf->ptr), n, acc); break;
+	case 2: util_hash<u16><<<g, 256, 0, cq->stream>>>(static_cast<const u16*>(buf->ptr), n, acc); break;
+	case 4: util_hash<u32><<<g, 256, 0, cq->stream>>>(static_cast<const u32*>(buf->ptr), n, acc); break;
+	case 8: util_hash<u64><<<g, 256, 0, cq->stream>>>(static_cast<const u64*>(buf->ptr), n, acc); break;
+	default:
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "width must be 1, 2, 4 or 8");
+		(void)hipFreeAsync(acc, cq->stream);
+		return 0;
+	}
+	u64 r = 0;
+	if (!read_u64(cq, acc, &r, err)) return 0;
+	return r;
+}
+
+size_t clo_util_check_sorted(CCLQueue* cq, CCLBuffer* keys, CCLBuffer* vals, size_t n,
+	size_t width, GError** err) {
+	if (!cq || !keys || keys->size < n * width || (vals && vals->size < n * 4)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "invalid check arguments");
+		return static_cast<size_t>(-1);
+	}
+	if (!use_device(cq->ctx, err)) return static_cast<size_t>(-1);
+	u64* bad = nullptr;
+	if (!scratch_u64(cq, &bad, err)) return static_cast<size_t>(-1);
+	u32 g = grid_for(n);
+	if (g > 2048) g = 2048;
+	const u32* v = vals ? static_cast<const u32*>(vals->ptr) : nullptr;
+	switch (width) {
+	case 1: util_check_sorted<u8><<<g, 256, 0, cq->stream>>>(static_cast<const u8*>(keys->ptr), v, n, bad); break;
+	case 2: util_check_sorted<u16><<<g, 256, 0, cq->stream>>>(static_cast<const u16*>(keys->ptr), v, n, bad); break;
+	case 4: util_check_sorted<u32><<<g, 256, 0, cq->stream>>>(static_cast<const u32*>(keys->ptr), v, n, bad); break;
+	case 8: util_check_sorted<u64><<<g, 256, 0, cq->stream>>>(static_cast<const u64*>(keys->ptr), v, n, bad); break;
+	default:
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "width must be 1, 2, 4 or 8");
+		(void)hipFreeAsync(bad, cq->stream);
+		return static_cast<size_t>(-1);
+	}
+	u64 r = 0;
+	if (!read_u64(cq, bad, &r, err)) return static_cast<size_t>(-1);
+	return static_cast<size_t>(r);
+}
+
+const char* clo_util_build_info(void) {
+	return "cl-ops MI355X: gfx950, clang " __clang_version__ ", wave64, satradix onesweep 256x16";
+}
+
+}  // extern "C"

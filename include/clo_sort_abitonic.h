@@ -1,0 +1,40 @@
+/*
+ * clo_sort_abitonic.h - advanced bitonic sort: the bitonic network with its
+ * steps fused into register-resident and LDS-resident kernels, as CL_Ops'
+ * src/cl_ops/sort/clo_sort_abitonic.{in.h:33-116,c:58-567}.
+ *
+ * Options: minps=<1..4>, maxps=<1..4> (register steps fused per global
+ * sweep; minps <= maxps), maxsfs=<n> (largest stage-finish fused in LDS, as
+ * log2 of the span).  They change the launch strategy, never the result.
+ */
+#ifndef CLO_SORT_ABITONIC_H_
+#define CLO_SORT_ABITONIC_H_
+
+#include "clo_sort_abstract.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLO_SORT_ABITONIC_NUM_KERNELS 6
+#define CLO_SORT_ABITONIC_KNAME_ANY "abit_any"
+#define CLO_SORT_ABITONIC_KNAME_PRIV_2S4V "abit_priv_2s4v"
+#define CLO_SORT_ABITONIC_KNAME_PRIV_3S8V "abit_priv_3s8v"
+#define CLO_SORT_ABITONIC_KNAME_PRIV_4S16V "abit_priv_4s16v"
+#define CLO_SORT_ABITONIC_KNAME_LOCAL "abit_local"
+#define CLO_SORT_ABITONIC_KNAME_TILESORT "abit_tilesort"
+#define CLO_SORT_ABITONIC_KERNELNAMES { CLO_SORT_ABITONIC_KNAME_ANY, \
+	CLO_SORT_ABITONIC_KNAME_PRIV_2S4V, CLO_SORT_ABITONIC_KNAME_PRIV_3S8V, \
+	CLO_SORT_ABITONIC_KNAME_PRIV_4S16V, CLO_SORT_ABITONIC_KNAME_LOCAL, \
+	CLO_SORT_ABITONIC_KNAME_TILESORT }
+
+extern const CloSortImplDef clo_sort_abitonic_def;
+
+/* Number of global sweeps (kernel launches that read and write the whole
+ * array) of the last abitonic call. */
+cl_uint clo_sort_abitonic_get_last_sweeps(CloSort* sorter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

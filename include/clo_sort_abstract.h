@@ -1,0 +1,100 @@
+/*
+ * clo_sort_abstract.h - sorter factory and vtable.
+ *
+ * Mirrors CL_Ops' src/cl_ops/sort/clo_sort_abstract.in.h:30-170: same
+ * struct names, vtable fields and function signatures.  The vtable's `init`
+ * returns a description string instead of OpenCL source (kernels are
+ * compiled ahead of time for gfx950).
+ */
+#ifndef CLO_SORT_ABSTRACT_H_
+#define CLO_SORT_ABSTRACT_H_
+
+#include "clo_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* clo_sort_abstract.in.h:30 */
+#define CLO_SORT_IMPLS "sbitonic, abitonic, gselect, satradix"
+
+/* clo_sort_abstract.in.h:43-110 */
+typedef struct clo_sort_impl_def {
+	const char* name;
+	cl_bool in_place;
+	const char* (*init)(CloSort* sorter, const char* options, GError** err);
+	void (*finalize)(CloSort* sorter);
+	CCLEvent* (*sort_with_device_data)(CloSort* sorter, CCLQueue* cq_exec,
+		CCLQueue* cq_comm, CCLBuffer* data_in, CCLBuffer* data_out,
+		size_t numel, size_t lws_max, GError** err);
+	cl_uint (*get_num_kernels)(CloSort* sorter, GError** err);
+	const char* (*get_kernel_name)(CloSort* sorter, cl_uint i, GError** err);
+	size_t (*get_localmem_usage)(CloSort* sorter, cl_uint i, size_t lws_max,
+		size_t numel, GError** err);
+} CloSortImplDef;
+
+/* clo_sort_abstract.c:91-207.  `compare` and `get_key` accept NULL (the
+ * defaults "((a) > (b))" ascending and "(x)"); "((a) < (b))" selects
+ * descending order for the comparison sorts.  Other snippets, and a key type
+ * different from the element type, fail with CLO_ERROR_ARGS: kernels are
+ * compiled ahead of time.  `compiler_opts` is accepted and ignored. */
+CloSort* clo_sort_new(const char* type, const char* options,
+	CCLContext* ctx, CloType* elem_type, CloType* key_type,
+	const char* compare, const char* get_key, const char* compiler_opts,
+	GError** err);
+
+/* clo_sort_abstract.c:214-246 */
+void clo_sort_destroy(CloSort* sorter);
+
+/* clo_sort_abstract.c:256-273.  Asynchronous: returns the event recorded
+ * after the last kernel on cq_exec.  data_out == NULL sorts data_in in
+ * place; otherwise the sorted data goes to data_out and data_in is left
+ * unchanged.  Any numel (not only powers of two). */
+CCLEvent* clo_sort_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* data_in, CCLBuffer* data_out,
+	size_t numel, size_t lws_max, GError** err);
+
+/* clo_sort_abstract.c:296-418.  Blocking host-to-host sort. */
+cl_bool clo_sort_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, void* data_in, void* data_out, size_t numel,
+	size_t lws_max, GError** err);
+
+/* clo_sort_abstract.c:428-629 */
+CCLContext* clo_sort_get_context(CloSort* sorter);
+CCLProgram* clo_sort_get_program(CloSort* sorter);
+CloType clo_sort_get_element_type(CloSort* sorter);
+size_t clo_sort_get_element_size(CloSort* sorter);
+CloType clo_sort_get_key_type(CloSort* sorter);
+size_t clo_sort_get_key_size(CloSort* sorter);
+void* clo_sort_get_data(CloSort* sorter);
+void clo_sort_set_data(CloSort* sorter, void* data);
+cl_uint clo_sort_get_num_kernels(CloSort* sorter, GError** err);
+const char* clo_sort_get_kernel_name(CloSort* sorter, cl_uint i, GError** err);
+size_t clo_sort_get_localmem_usage(CloSort* sorter, cl_uint i,
+	size_t lws_max, size_t numel, GError** err);
+
+/* ---- additions (no counterpart in CL_Ops) --------------------------------
+ * Key-value sort: keys of the sorter's element type, 32-bit payloads.
+ * The result is the stable sort of the pairs by key (payloads of equal keys
+ * keep their input order).  keys_out == NULL sorts in place (then vals_out
+ * must be NULL too).  Supported by "satradix" (radix), "gselect" (rank) and,
+ * through a stable rank, none other: bitonic sorts reject KV with
+ * CLO_ERROR_ARGS because they are not stable. */
+CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in,
+	CCLBuffer* keys_out, CCLBuffer* vals_out, size_t numel, size_t lws_max,
+	GError** err);
+cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, void* keys, void* vals, size_t numel, size_t lws_max,
+	GError** err);
+
+/* Number of digit passes the last satradix call executed (passes whose digit
+ * was the same for every key are skipped).  Blocks on the queue of that
+ * call.  0 for the other algorithms. */
+cl_uint clo_sort_get_last_passes(CloSort* sorter, GError** err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLO_SORT_ABSTRACT_H_ */

@@ -1,0 +1,67 @@
+/*
+ * clo_util.h - seeded input generators and a copy kernel (additions with no
+ * counterpart in CL_Ops).  Benchmarks and tests generate their inputs on the
+ * device with these, from the counter-based generator
+ *   U(seed, i) = mix64(seed + (i + 1) * 0x9E3779B97F4A7C15)
+ * where mix64 is the splitmix64 finalizer (Steele, Lea & Flood 2014), so a
+ * CPU restatement produces the same words.
+ */
+#ifndef CLO_UTIL_H_
+#define CLO_UTIL_H_
+
+#include "clo_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+	CLO_GEN_U64 = 0,       /* U(seed, i) */
+	CLO_GEN_U32_HI = 1,    /* U(seed, i) >> 32 */
+	CLO_GEN_U64_MASK = 2,  /* U(seed, i) & param */
+	CLO_GEN_U32_MASK = 3,  /* (U(seed, i) >> 32) & param */
+	CLO_GEN_IOTA_U32 = 4,  /* (uint32) i */
+	CLO_GEN_ZIPF_U64 = 5,  /* mix64(rank), rank from the CDF table, see below */
+	CLO_GEN_U32_MOD = 6,   /* (U(seed, i) >> 32) % param */
+	CLO_GEN_U64_MOD = 7    /* U(seed, i) % param */
+} CloGenKind;
+
+/* Writes n words of the kind's width (4 or 8 bytes) to out at element
+ * offset 0, element i from counter i + start. */
+CCLEvent* clo_util_generate(CCLQueue* cq, CCLBuffer* out, size_t n,
+	CloGenKind kind, cl_ulong seed, cl_ulong param, cl_ulong start,
+	GError** err);
+
+/* Zipf(s) keys over `ranks` ranks: rank r (0-based) has probability
+ * proportional to (r+1)^-s.  cdf is a device buffer of `ranks` doubles,
+ * cdf[r] = P(rank <= r) (built on the host by clo_util_zipf_cdf).  Element
+ * i = mix64(rank) where rank is the smallest r with cdf[r] > u,
+ * u = (U(seed, i) >> 11) * 2^-53. */
+CCLEvent* clo_util_generate_zipf(CCLQueue* cq, CCLBuffer* out, size_t n,
+	CCLBuffer* cdf, size_t ranks, cl_ulong seed, GError** err);
+void clo_util_zipf_cdf(double* cdf, size_t ranks, double s);
+
+/* Plain device copy kernel of `bytes` bytes (dst and src 16-byte aligned).
+ * Used to calibrate memory-traffic counters against a known byte count. */
+CCLEvent* clo_util_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src,
+	size_t bytes, GError** err);
+
+/* Order-independent 64-bit multiset hash of n elements of `width` bytes
+ * (sum of mix64(x) mod 2^64), computed on the device; blocks. */
+cl_ulong clo_util_multiset_hash(CCLQueue* cq, CCLBuffer* buf, size_t n,
+	size_t width, GError** err);
+
+/* Checks on the device that n elements of `width` bytes (unsigned) are
+ * non-decreasing; if vals is non-NULL also that vals (uint32) increase
+ * within runs of equal keys (stability when vals held the input index).
+ * Returns the number of violations; blocks. */
+size_t clo_util_check_sorted(CCLQueue* cq, CCLBuffer* keys, CCLBuffer* vals,
+	size_t n, size_t width, GError** err);
+
+/* Library build description (compiler, target, kernel variants). */
+const char* clo_util_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

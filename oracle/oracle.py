@@ -1,0 +1,173 @@
+"""numpy front end of the CPU oracle (liboracle.so, built from oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by the tests, the smoke check and the
+benchmark's CPU-baseline leg as the checker / timed CPU baseline.  The product
+path (libcl_ops.so) never loads it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+GEN_U64, GEN_U32_HI, GEN_U64_MASK, GEN_U32_MASK, GEN_IOTA_U32 = 0, 1, 2, 3, 4
+GEN_U32_MOD, GEN_U64_MOD = 6, 7
+RNG_NAMES = ["lcg", "xorshift64", "xorshift128", "mwc64x", "parkmiller"]
+STATE_SIZE = {"lcg": 8, "xorshift64": 8, "xorshift128": 16, "mwc64x": 8, "parkmiller": 4}
+
+_U = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+_S = {1: np.int8, 2: np.int16, 4: np.int32, 8: np.int64}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        vp, sz, u64, u32, i32 = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
+                                 ctypes.c_uint32, ctypes.c_int)
+        sig = {
+            "orc_mix64": (u64, [u64]),
+            "orc_generate": (None, [i32, u64, u64, u64, sz, vp]),
+            "orc_zipf_cdf": (None, [vp, sz, ctypes.c_double]),
+            "orc_generate_zipf": (None, [vp, sz, u64, sz, vp]),
+            "orc_satradix": (None, [vp, vp, sz, i32, i32, sz, i32]),
+            "orc_stable_sort": (None, [vp, vp, sz, i32]),
+            "orc_sbitonic": (None, [vp, sz, i32, i32, i32]),
+            "orc_gselect": (None, [vp, vp, vp, vp, sz, i32, i32, i32]),
+            "orc_scan": (None, [vp, i32, i32, vp, i32, sz]),
+            "orc_scan_blelloch": (None, [vp, vp, i32, sz, sz, i32]),
+            "orc_rng_state_size": (sz, [i32]),
+            "orc_rng_from_seed": (None, [i32, u64, vp]),
+            "orc_rng_next": (u32, [i32, vp]),
+            "orc_rng_init_dev_gid": (None, [i32, sz, u64, i32, vp]),
+            "orc_rng_init_host_mt": (None, [i32, sz, u64, vp]),
+            "orc_mt19937": (None, [u32, sz, vp]),
+            "orc_rng_fill": (None, [i32, vp, sz, u32, u32, u32, vp, i32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def mix64(z):
+    return lib().orc_mix64(z & 0xFFFFFFFFFFFFFFFF)
+
+
+def generate(kind, seed, n, param=0, start=0):
+    dt = np.uint64 if kind in (GEN_U64, GEN_U64_MASK, GEN_U64_MOD) else np.uint32
+    out = np.empty(n, dtype=dt)
+    lib().orc_generate(kind, seed, param, start, n, _p(out))
+    return out
+
+
+def zipf_cdf(ranks, s):
+    cdf = np.empty(ranks, dtype=np.float64)
+    lib().orc_zipf_cdf(_p(cdf), ranks, s)
+    return cdf
+
+
+def generate_zipf(cdf, seed, n):
+    out = np.empty(n, dtype=np.uint64)
+    lib().orc_generate_zipf(_p(cdf), len(cdf), seed, n, _p(out))
+    return out
+
+
+def satradix(keys, vals=None, bits=4, lws=256, threads=1):
+    """Reference-structured satradix; returns sorted copies."""
+    k = np.array(keys, copy=True)
+    v = None if vals is None else np.array(vals, dtype=np.uint32, copy=True)
+    lib().orc_satradix(_p(k), _p(v), k.size, k.dtype.itemsize, bits, lws, threads)
+    return k if v is None else (k, v)
+
+
+def stable_sort(keys, vals=None):
+    k = np.array(keys, copy=True)
+    v = None if vals is None else np.array(vals, dtype=np.uint32, copy=True)
+    lib().orc_stable_sort(_p(k), _p(v), k.size, k.dtype.itemsize)
+    return k if v is None else (k, v)
+
+
+def sbitonic(keys, desc=False):
+    k = np.array(keys, copy=True)
+    if k.size & (k.size - 1):
+        raise ValueError("the reference network needs a power-of-two size")
+    lib().orc_sbitonic(_p(k), k.size, k.dtype.itemsize, int(k.dtype.kind == "i"), int(desc))
+    return k
+
+
+def gselect(keys, vals=None, desc=False):
+    k = np.ascontiguousarray(keys)
+    ko = np.empty_like(k)
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32)
+    vo = None if v is None else np.empty_like(v)
+    lib().orc_gselect(_p(k), _p(v), _p(ko), _p(vo), k.size, k.dtype.itemsize,
+                      int(k.dtype.kind == "i"), int(desc))
+    return ko if v is None else (ko, vo)
+
+
+def scan(values, sum_dtype):
+    a = np.ascontiguousarray(values)
+    out = np.empty(a.size, dtype=sum_dtype)
+    lib().orc_scan(_p(a), a.dtype.itemsize, int(a.dtype.kind == "i"), _p(out),
+                   out.dtype.itemsize, a.size)
+    return out
+
+
+def scan_blelloch(values_u32, sum_dtype=np.uint32, lws=256, threads=1):
+    a = np.ascontiguousarray(values_u32, dtype=np.uint32)
+    out = np.empty(a.size, dtype=sum_dtype)
+    lib().orc_scan_blelloch(_p(a), _p(out), out.dtype.itemsize, a.size, lws, threads)
+    return out
+
+
+def _gen_id(name):
+    return RNG_NAMES.index(name)
+
+
+def rng_init_dev_gid(name, count, main_seed=0, hash_id=0):
+    st = np.zeros(count * STATE_SIZE[name], dtype=np.uint8)
+    lib().orc_rng_init_dev_gid(_gen_id(name), count, main_seed, hash_id, _p(st))
+    return st
+
+
+def rng_init_host_mt(name, count, main_seed=0):
+    st = np.zeros(count * STATE_SIZE[name], dtype=np.uint8)
+    lib().orc_rng_init_host_mt(_gen_id(name), count, main_seed, _p(st))
+    return st
+
+
+def rng_from_seed(name, seed):
+    st = np.zeros(STATE_SIZE[name], dtype=np.uint8)
+    lib().orc_rng_from_seed(_gen_id(name), seed & 0xFFFFFFFFFFFFFFFF, _p(st))
+    return st
+
+
+def rng_next(name, state):
+    """One draw; `state` (uint8 array of one state) is advanced in place."""
+    return lib().orc_rng_next(_gen_id(name), _p(state))
+
+
+def mt19937(seed, n):
+    out = np.empty(n, dtype=np.uint32)
+    lib().orc_mt19937(seed & 0xFFFFFFFF, n, _p(out))
+    return out
+
+
+def rng_fill(name, states, draws, bits=32, maxint=0, threads=1):
+    """Runs the bench kernel `draws` times; states (uint8 bytes) advance in place."""
+    G = states.size // STATE_SIZE[name]
+    out = np.empty(draws * G, dtype=np.uint32)
+    lib().orc_rng_fill(_gen_id(name), _p(states), G, draws, bits, maxint, _p(out), threads)
+    return out

@@ -1,0 +1,96 @@
+"""sbitonic, abitonic and gselect on the GPU vs the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INT_TYPES = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64]
+
+
+def _keys(oracle, dtype, n, seed):
+    u = oracle.generate(oracle.GEN_U64, seed, n)
+    return u.astype(np.dtype("u%d" % np.dtype(dtype).itemsize)).view(dtype)
+
+
+@pytest.mark.parametrize("algo", ["sbitonic", "abitonic"])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 8, 100, 1000, 1024, 4097, 8192, 8193, 70000])
+def test_bitonic_sizes(clops, oracle, ctx, algo, n):
+    s = clops.Sorter(ctx, algo, np.uint32)
+    k = _keys(oracle, np.uint32, n, 0xB1 + n)
+    assert np.array_equal(s.sort_host(k), np.sort(k))
+
+
+@pytest.mark.parametrize("algo", ["sbitonic", "abitonic"])
+@pytest.mark.parametrize("dtype", INT_TYPES)
+@pytest.mark.parametrize("desc", [False, True])
+def test_bitonic_types_order(clops, oracle, ctx, algo, dtype, desc):
+    n = 5000
+    s = clops.Sorter(ctx, algo, dtype, compare="((a) < (b))" if desc else None)
+    k = _keys(oracle, dtype, n, 0xB7)
+    ref = np.sort(k)[::-1] if desc else np.sort(k)
+    assert np.array_equal(s.sort_host(k), ref)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int64])
+def test_bitonic_po2_matches_reference_network(clops, oracle, ctx, dtype):
+    n = 1 << 14
+    k = _keys(oracle, dtype, n, 0xB9)
+    for algo in ("sbitonic", "abitonic"):
+        s = clops.Sorter(ctx, algo, dtype)
+        assert np.array_equal(s.sort_host(k), oracle.sbitonic(k))
+
+
+@pytest.mark.parametrize("opts", ["minps=1,maxps=1", "minps=2,maxps=2", "minps=3,maxps=3",
+                                  "maxps=4", "maxsfs=0", "maxsfs=5", "minps=1,maxps=3,maxsfs=9"])
+def test_abitonic_options(clops, oracle, ctx, opts):
+    n = 1 << 17
+    s = clops.Sorter(ctx, "abitonic", np.uint32, options=opts)
+    k = _keys(oracle, np.uint32, n, 0xBA)
+    assert np.array_equal(s.sort_host(k), np.sort(k))
+
+
+def test_abitonic_bad_options(clops, ctx):
+    for bad in ("minps=0", "maxps=5", "minps=3,maxps=2", "foo=1", "minps"):
+        with pytest.raises(clops.ClopsError) as e:
+            clops.Sorter(ctx, "abitonic", np.uint32, options=bad)
+        assert e.value.code == clops.CLO_ERROR_ARGS
+
+
+def test_abitonic_2p24_sweeps(clops, oracle, ctx, queue):
+    n = 1 << 24
+    k = oracle.generate(oracle.GEN_U32_HI, 0xC3D, n)
+    b = clops.Buffer.from_array(ctx, k)
+    s = clops.Sorter(ctx, "abitonic", np.uint32)
+    s.sort_device(queue, b, None, n)
+    queue.finish()
+    assert np.array_equal(b.read(queue, np.uint32, n), np.sort(k))
+    assert s.last_sweeps() > 0
+
+
+@pytest.mark.parametrize("dtype", INT_TYPES)
+@pytest.mark.parametrize("desc", [False, True])
+def test_gselect_kv_stable(clops, oracle, ctx, dtype, desc):
+    n = 3000
+    k = (_keys(oracle, dtype, n, 0x95).view("u%d" % np.dtype(dtype).itemsize) % 50).astype(
+        "u%d" % np.dtype(dtype).itemsize).view(dtype)
+    v = np.arange(n, dtype=np.uint32)
+    s = clops.Sorter(ctx, "gselect", dtype, compare="((a) < (b))" if desc else None)
+    ko, vo = s.sort_kv_host(k, v)
+    rk, rv = oracle.gselect(k, v, desc=desc)
+    assert np.array_equal(ko, rk) and np.array_equal(vo, rv)
+    assert np.array_equal(s.sort_host(k), rk)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 255, 256, 1023, 1024, 1025, 5000])
+def test_gselect_sizes(clops, oracle, ctx, n):
+    s = clops.Sorter(ctx, "gselect", np.uint64)
+    k = _keys(oracle, np.uint64, n, 0x96 + n)
+    assert np.array_equal(s.sort_host(k), np.sort(k))
+
+
+def test_bitonic_rejects_kv(clops, ctx, queue):
+    s = clops.Sorter(ctx, "abitonic", np.uint32)
+    b = clops.Buffer(ctx, 64)
+    with pytest.raises(clops.ClopsError) as e:
+        s.sort_kv_device(queue, b, b, None, None, 4)
+    assert e.value.code == clops.CLO_ERROR_ARGS

@@ -1,0 +1,95 @@
+"""RNG seeds and fills on the GPU vs the oracle, every stream word."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GENS = ["lcg", "xorshift64", "xorshift128", "mwc64x", "parkmiller"]
+
+
+@pytest.mark.parametrize("gen", GENS)
+@pytest.mark.parametrize("hash_name,hash_id", [(None, 0), ("KNUTH(x)", 1), ("XS1(x)", 2)])
+def test_dev_gid_seeds(clops, oracle, ctx, queue, gen, hash_name, hash_id):
+    count = 10000
+    r = clops.Rng(ctx, gen, clops.SEED_DEV_GID, count, main_seed=1234, hash=hash_name, q=queue)
+    queue.finish()
+    st = r.read_states(queue)
+    assert np.array_equal(st, oracle.rng_init_dev_gid(gen, count, 1234, hash_id))
+
+
+@pytest.mark.parametrize("gen", GENS)
+@pytest.mark.parametrize("seed", [0, 1234, (1 << 40) + 7])
+def test_host_mt_seeds(clops, oracle, ctx, queue, gen, seed):
+    count = 3000
+    r = clops.Rng(ctx, gen, clops.SEED_HOST_MT, count, main_seed=seed)
+    assert np.array_equal(r.read_states(queue), oracle.rng_init_host_mt(gen, count, seed))
+
+
+@pytest.mark.parametrize("gen", GENS)
+@pytest.mark.parametrize("bits,maxint", [(32, 0), (17, 0), (1, 0), (32, 1000), (32, 7)])
+def test_fill_matches_oracle(clops, oracle, ctx, queue, gen, bits, maxint):
+    G, draws = 4099, 16
+    r = clops.Rng(ctx, gen, clops.SEED_HOST_MT, G, main_seed=99)
+    st = r.read_states(queue)
+    out = clops.Buffer(ctx, G * draws * 4)
+    r.fill(queue, out, draws, bits, maxint)
+    got = out.read(queue, np.uint32, G * draws)
+    ref = oracle.rng_fill(gen, st, draws, bits, maxint)
+    assert np.array_equal(got, ref)
+    # the device states advanced exactly `draws` steps
+    assert np.array_equal(r.read_states(queue), st)
+
+
+@pytest.mark.parametrize("gen", GENS)
+def test_fill_resume_ext_host(clops, oracle, ctx, queue, gen):
+    """States read back and re-seeded with EXT_HOST continue the stream."""
+    G = 2048
+    r = clops.Rng(ctx, gen, clops.SEED_DEV_GID, G, main_seed=5, q=queue)
+    out = clops.Buffer(ctx, G * 8 * 4)
+    r.fill(queue, out, 4)
+    st = r.read_states(queue)
+    r2 = clops.Rng(ctx, gen, clops.SEED_EXT_HOST, G, seeds=st)
+    r.fill(queue, out, 4)
+    a = out.read(queue, np.uint32, G * 4)
+    r2.fill(queue, out, 4)
+    b = out.read(queue, np.uint32, G * 4)
+    assert np.array_equal(a, b)
+
+
+def test_ext_dev_shares_buffer(clops, oracle, ctx, queue):
+    G = 1000
+    st = oracle.rng_init_dev_gid("xorshift128", G, 42, 0)
+    buf = clops.Buffer.from_array(ctx, st)
+    r = clops.Rng(ctx, "xorshift128", clops.SEED_EXT_DEV, G, seeds=buf)
+    out = clops.Buffer(ctx, G * 4 * 3)
+    r.fill(queue, out, 3)
+    assert np.array_equal(out.read(queue, np.uint32, 3 * G),
+                          oracle.rng_fill("xorshift128", st.copy(), 3))
+
+
+def test_config2_fills_prefix(clops, oracle, ctx, queue):
+    """Configuration 2 layout (run-major) at 2^20 work-items x 16 draws."""
+    G, draws = 1 << 20, 16
+    for gen in ("xorshift128", "mwc64x"):
+        r = clops.Rng(ctx, gen, clops.SEED_HOST_MT, G, main_seed=0)
+        st = r.read_states(queue)
+        out = clops.Buffer(ctx, G * draws * 4)
+        r.fill(queue, out, draws)
+        got = out.read(queue, np.uint32, G * draws)
+        assert np.array_equal(got, oracle.rng_fill(gen, st, draws, threads=8))
+
+
+def test_errors(clops, ctx):
+    with pytest.raises(clops.ClopsError) as e:
+        clops.Rng(ctx, "tauslcg", clops.SEED_DEV_GID, 10)
+    assert e.value.code == clops.CLO_ERROR_IMPL_NOT_FOUND
+    with pytest.raises(clops.ClopsError) as e:
+        clops.Rng(ctx, "lcg", clops.SEED_DEV_GID, 10, hash="(x * 3 + 1)")
+    assert e.value.code == clops.CLO_ERROR_ARGS
+
+
+def test_source_text(clops, ctx):
+    r = clops.Rng(ctx, "mwc64x", clops.SEED_HOST_MT, 4)
+    src = r.source()
+    assert "namespace clo_rng_mwc64x" in src and "4294883355U" in src
+    assert "clo_rng_next_int" in src

@@ -1,0 +1,330 @@
+"""Pins the CPU oracle (CPU only).
+
+The reference's own tests hold no value checks (src/tests/test_rng.c asserts
+only "no error"), so every oracle function is pinned here by something it
+does not share code with: published known answers, numpy's independent
+implementations, exhaustive enumeration on tiny inputs, and algebraic
+properties of the generators (periods, skip-ahead identities).
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+M64 = (1 << 64) - 1
+
+
+# --------------------------------------------------------------------------
+# Seeded input generator (splitmix64)
+
+
+def test_splitmix64_known_outputs(oracle):
+    # splitmix64 from state 0 (Vigna's reference C code): the first outputs.
+    u = oracle.generate(oracle.GEN_U64, 0, 3)
+    assert [int(x) for x in u] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_generator_kinds(oracle):
+    u = oracle.generate(oracle.GEN_U64, 77, 1000, start=5)
+    assert np.array_equal(oracle.generate(oracle.GEN_U32_HI, 77, 1000, start=5),
+                          (u >> np.uint64(32)).astype(np.uint32))
+    assert np.array_equal(oracle.generate(oracle.GEN_U64_MASK, 77, 1000, param=0xFFFFF, start=5),
+                          u & np.uint64(0xFFFFF))
+    assert np.array_equal(oracle.generate(oracle.GEN_U32_MOD, 77, 1000, param=1000, start=5),
+                          (u >> np.uint64(32)).astype(np.uint32) % np.uint32(1000))
+    assert np.array_equal(oracle.generate(oracle.GEN_IOTA_U32, 0, 10, start=3),
+                          np.arange(3, 13, dtype=np.uint32))
+
+
+def test_zipf_distribution(oracle):
+    # H(2^16, 1.1) by direct summation; the top rank takes 1/H of the mass.
+    ranks, s = 1 << 16, 1.1
+    H = math.fsum((r + 1) ** -s for r in range(ranks))
+    cdf = oracle.zipf_cdf(ranks, s)
+    assert abs(cdf[0] - 1.0 / H) < 1e-12 and cdf[-1] == 1.0 and np.all(np.diff(cdf) > 0)
+    k = oracle.generate_zipf(cdf, 0xC3C, 200000)
+    top = oracle.mix64(0)
+    frac = float(np.mean(k == np.uint64(top)))
+    assert abs(frac - 1.0 / H) < 0.01
+
+
+# --------------------------------------------------------------------------
+# Sorts
+
+
+def _stable_ref(k, v):
+    idx = np.argsort(k, kind="stable")
+    return k[idx], v[idx]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("n", [0, 1, 2, 17, 256, 257, 5000])
+def test_stable_sort_matches_numpy(oracle, dtype, n):
+    k = oracle.generate(oracle.GEN_U64, n + 1, n).astype(dtype)
+    k[: n // 3] = k[0] if n else k[: n // 3]
+    v = np.arange(n, dtype=np.uint32)
+    rk, rv = oracle.stable_sort(k, v)
+    ek, ev = _stable_ref(k, v)
+    assert np.array_equal(rk, ek) and np.array_equal(rv, ev)
+
+
+@pytest.mark.parametrize("bits", [1, 2, 3, 4, 5, 7, 8])
+@pytest.mark.parametrize("lws", [16, 256, 1000])
+@pytest.mark.parametrize("dtype", [np.uint16, np.uint32, np.uint64])
+def test_satradix_restatement(oracle, bits, lws, dtype):
+    n = 3001
+    k = (oracle.generate(oracle.GEN_U64, bits * 7 + lws, n) % np.uint64(5000)).astype(dtype)
+    v = np.arange(n, dtype=np.uint32)
+    rk, rv = oracle.satradix(k, v, bits=bits, lws=lws, threads=3)
+    ek, ev = _stable_ref(k, v)
+    assert np.array_equal(rk, ek) and np.array_equal(rv, ev)
+
+
+def test_sbitonic_exhaustive_tiny(oracle):
+    # every permutation (with repeats) of small power-of-two arrays
+    for n in (2, 4, 8):
+        vals = list(range(n // 2)) * 2
+        for perm in set(itertools.permutations(vals)):
+            a = np.array(perm, dtype=np.int32)
+            assert np.array_equal(oracle.sbitonic(a), np.sort(a))
+            assert np.array_equal(oracle.sbitonic(a, desc=True), np.sort(a)[::-1])
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.uint16, np.int32, np.uint64, np.int64])
+def test_sbitonic_po2(oracle, dtype):
+    n = 1 << 12
+    k = oracle.generate(oracle.GEN_U64, 3, n).astype(np.dtype(dtype).str.replace("i", "u")).view(
+        dtype)
+    assert np.array_equal(oracle.sbitonic(k), np.sort(k))
+    assert np.array_equal(oracle.sbitonic(k, desc=True), np.sort(k)[::-1])
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.uint32, np.int64])
+@pytest.mark.parametrize("desc", [False, True])
+def test_gselect_restatement(oracle, dtype, desc):
+    n = 700
+    k = (oracle.generate(oracle.GEN_U64, 9, n) % np.uint64(40)).astype(np.int64) - 20
+    k = k.astype(dtype)
+    v = np.arange(n, dtype=np.uint32)
+    rk, rv = oracle.gselect(k, v, desc=desc)
+    idx = np.argsort(-k.astype(np.float64) if desc else k, kind="stable")
+    assert np.array_equal(rk, k[idx]) and np.array_equal(rv, v[idx])
+
+
+# --------------------------------------------------------------------------
+# Scan
+
+
+@pytest.mark.parametrize("elem", [np.int8, np.uint8, np.int16, np.uint32, np.int64, np.uint64])
+@pytest.mark.parametrize("sumt", [np.uint8, np.int16, np.uint32, np.int32, np.uint64, np.int64])
+def test_scan_matches_numpy(oracle, elem, sumt):
+    n = 2000
+    a = oracle.generate(oracle.GEN_U64, 4, n).astype(np.dtype(elem).str.replace("i", "u")).view(
+        elem)
+    want = np.zeros(n, dtype=sumt)
+    # numpy converts as C does (sign extension), and cumsum wraps
+    want[1:] = np.cumsum(a[:-1].astype(sumt), dtype=sumt)
+    assert np.array_equal(oracle.scan(a, sumt), want)
+
+
+@pytest.mark.parametrize("n", [1, 511, 512, 4096, 100000])
+@pytest.mark.parametrize("lws", [4, 64, 256])
+def test_blelloch_restatement(oracle, n, lws):
+    a = oracle.generate(oracle.GEN_U32_HI, 11, n)
+    for st in (np.uint32, np.uint64):
+        assert np.array_equal(oracle.scan_blelloch(a, st, lws=lws, threads=4), oracle.scan(a, st))
+
+
+# --------------------------------------------------------------------------
+# RNG: known answers
+
+
+def test_parkmiller_known_answer(oracle):
+    # Park & Miller (CACM 31(10), 1988): from seed 1, the 10000th state is 1043618065.
+    st = oracle.rng_from_seed("parkmiller", 1)
+    for _ in range(10000):
+        out = oracle.rng_next("parkmiller", st)
+    assert int(st.view(np.int32)[0]) == 1043618065
+    assert out == (1043618065 << 1) & 0xFFFFFFFF
+
+
+def test_parkmiller_negative_states_truncate(oracle):
+    # HOST_MT seeds make negative int states; OpenCL's % truncates toward zero.
+    for s0 in (-1, -2, -123456789, -(1 << 31), 2147483646, 0, 2147483647):
+        st = oracle.rng_from_seed("parkmiller", s0 & 0xFFFFFFFF)
+        s = s0
+        for _ in range(50):
+            r = oracle.rng_next("parkmiller", st)
+            s = int(math.fmod(s * 16807, 2147483647))  # exact: |s*16807| < 2^53
+            assert int(st.view(np.int32)[0]) == s
+            assert r == (s & 0xFFFFFFFF) << 1 & 0xFFFFFFFF
+
+
+def test_lcg_java_random(oracle):
+    # java.util.Random(0): scrambled seed 0 ^ 0x5DEECE66D; nextInt() = -1155484576.
+    st = oracle.rng_from_seed("lcg", 0x5DEECE66D)
+    assert oracle.rng_next("lcg", st) == (-1155484576) & 0xFFFFFFFF
+    # java.util.Random(42).nextInt() = -1170105035
+    st = oracle.rng_from_seed("lcg", 42 ^ 0x5DEECE66D)
+    assert oracle.rng_next("lcg", st) == (-1170105035) & 0xFFFFFFFF
+
+
+def test_lcg_skip_ahead(oracle):
+    a, c, m = 0x5DEECE66D, 0xB, 1 << 48
+    st = oracle.rng_from_seed("lcg", 12345)
+    for _ in range(1000):
+        oracle.rng_next("lcg", st)
+    # x_n = a^n x_0 + c (a^n - 1)/(a - 1)  (mod m), computed with big ints
+    an = pow(a, 1000, m * (a - 1))
+    xn = (an * 12345 + c * ((an - 1) // (a - 1))) % m
+    assert int(st.view(np.uint64)[0]) == xn
+
+
+def test_xorshift128_marsaglia(oracle):
+    # Marsaglia (2003) xor128 with x,y,z,w = 123456789, 362436069, 521288629, 88675123
+    st = np.array([123456789, 362436069, 521288629, 88675123], dtype=np.uint32).view(np.uint8)
+    got = [oracle.rng_next("xorshift128", st) for _ in range(3)]
+    assert got == [3701687786, 458299110, 2500872618]
+
+
+def test_mwc64x_skip_ahead(oracle):
+    # MWC: S' = A*x + c with S = c*2^32 + x, hence S_n = A^n S_0 mod (A*2^32 - 1).
+    A = 4294883355
+    M = A * (1 << 32) - 1
+    seed = 0x0123456789ABCDEF % M
+    st = oracle.rng_from_seed("mwc64x", seed)
+    outs = []
+    for _ in range(777):
+        outs.append(oracle.rng_next("mwc64x", st))
+    x, c = (int(w) for w in st.view(np.uint32))
+    assert c * (1 << 32) + x == pow(A, 777, M) * seed % M
+    s1 = A * seed % M
+    assert outs[1] == (s1 & 0xFFFFFFFF) ^ (s1 >> 32)
+
+
+def _gf2_matrix(step, bits):
+    cols = []
+    for i in range(bits):
+        cols.append(step(1 << i))
+    return cols  # column i = image of basis vector i
+
+
+def _gf2_apply(cols, v):
+    r = 0
+    i = 0
+    while v:
+        if v & 1:
+            r ^= cols[i]
+        v >>= 1
+        i += 1
+    return r
+
+
+def _gf2_mul(A, B):
+    return [_gf2_apply(A, b) for b in B]
+
+
+def _gf2_pow(A, e, bits):
+    R = [1 << i for i in range(bits)]
+    while e:
+        if e & 1:
+            R = _gf2_mul(A, R)
+        A = _gf2_mul(A, A)
+        e >>= 1
+    return R
+
+
+def _full_period(cols, bits, factors):
+    ident = [1 << i for i in range(bits)]
+    order = (1 << bits) - 1
+    if _gf2_pow(cols, order, bits) != ident:
+        return False
+    return all(_gf2_pow(cols, order // p, bits) != ident for p in factors)
+
+
+def test_xorshift64_full_period(oracle):
+    # The (21, 35, 4) triple has period 2^64 - 1 (Marsaglia 2003); a wrong
+    # shift or direction would not.
+    def step(v):
+        st = np.array([v], dtype=np.uint64).view(np.uint8)
+        oracle.rng_next("xorshift64", st)
+        return int(st.view(np.uint64)[0])
+
+    cols = _gf2_matrix(step, 64)
+    f64 = [3, 5, 17, 257, 641, 65537, 6700417]
+    assert math.prod(f64) == (1 << 64) - 1
+    assert _full_period(cols, 64, f64)
+    # linearity of the oracle step
+    a, b = 0x123456789ABCDEF0, 0x0FEDCBA987654321
+    assert step(a ^ b) == step(a) ^ step(b)
+
+
+def test_xorshift128_full_period(oracle):
+    def step(v):
+        st = np.array([v & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF, (v >> 64) & 0xFFFFFFFF,
+                       (v >> 96) & 0xFFFFFFFF], dtype=np.uint32).view(np.uint8)
+        oracle.rng_next("xorshift128", st)
+        w = [int(x) for x in st.view(np.uint32)]
+        return w[0] | w[1] << 32 | w[2] << 64 | w[3] << 96
+
+    cols = _gf2_matrix(step, 128)
+    f128 = [3, 5, 17, 257, 641, 65537, 274177, 6700417, 67280421310721]
+    assert math.prod(f128) == (1 << 128) - 1
+    assert _full_period(cols, 128, f128)
+
+
+def test_mt19937_known_answer(oracle):
+    # C++ [rand.predef]: the 10000th output of mt19937 with default seed 5489.
+    assert int(oracle.mt19937(5489, 10000)[-1]) == 4123659995
+
+
+def test_seed_conversions(oracle):
+    s = 0x0123456789ABCDEF
+    xs = oracle.rng_from_seed("xorshift128", s).view(np.uint32)
+    assert [int(x) for x in xs] == [s & M64 & 0xFFFFFFFF, (s >> 16) & 0xFFFFFFFF,
+                                    (s >> 32) & 0xFFFFFFFF, (s >> 46) & 0xFFFFFFFF]
+    mw = oracle.rng_from_seed("mwc64x", s).view(np.uint32)
+    assert [int(x) for x in mw] == [s & 0xFFFFFFFF, s >> 32]
+    pm = oracle.rng_from_seed("parkmiller", 0xFFFFFFFF80000001)
+    assert int(pm.view(np.int32)[0]) == -(1 << 31) + 1
+
+
+def test_dev_gid_hashes(oracle):
+    count, seed = 64, 1234
+    for h, fn in ((0, lambda x: x),
+                  (1, lambda x: (x * 2654435761) % (1 << 32)),
+                  (2, None)):
+        st = oracle.rng_init_dev_gid("lcg", count, seed, h).view(np.uint64)
+        for gid in range(count):
+            x = gid + seed
+            if h == 2:
+                x = (((x >> 16) ^ x) * 0x45D9F3B) & M64
+                x = (((x >> 16) ^ x) * 0x45D9F3B) & M64
+                x = (x >> 16) ^ x
+            else:
+                x = fn(x) & M64
+            assert int(st[gid]) == x
+
+
+def test_host_mt_layout(oracle):
+    words = oracle.mt19937(1234, 8)
+    st = oracle.rng_init_host_mt("xorshift128", 2, 1234)
+    assert np.array_equal(st.view(np.uint32), words)
+    st = oracle.rng_init_host_mt("lcg", 4, (1 << 40) + 1234)  # seed truncated to 32 bits
+    assert np.array_equal(st.view(np.uint32), words)
+
+
+def test_fill_layout(oracle):
+    # out[r*G + gid] is the r-th draw of work-item gid
+    G, draws = 37, 5
+    for gen in oracle.RNG_NAMES:
+        st = oracle.rng_init_dev_gid(gen, G, 9, 0)
+        per = st.copy().reshape(G, -1)
+        want = np.zeros((draws, G), dtype=np.uint32)
+        for gid in range(G):
+            s = per[gid].copy()
+            for r in range(draws):
+                want[r, gid] = oracle.rng_next(gen, s) >> 7
+        got = oracle.rng_fill(gen, st, draws, bits=25, threads=3)
+        assert np.array_equal(got, want.reshape(-1))
