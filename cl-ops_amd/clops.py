@@ -112,6 +112,7 @@ def lib():
         "clo_sort_kv_with_device_data": (vp, [vp, vp, vp, vp, vp, vp, vp, sz, sz, ep]),
         "clo_sort_kv_with_host_data": (u32, [vp, vp, vp, vp, vp, sz, sz, ep]),
         "clo_sort_get_last_passes": (u32, [vp, ep]),
+        "clo_sort_segmented_with_device_data": (vp, [vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, ep]),
         "clo_sort_get_element_type": (i32, [vp]),
         "clo_sort_get_element_size": (sz, [vp]),
         "clo_sort_get_key_type": (i32, [vp]),
@@ -126,6 +127,7 @@ def lib():
         "clo_scan_destroy": (None, [vp]),
         "clo_scan_with_device_data": (vp, [vp, vp, vp, vp, vp, sz, sz, ep]),
         "clo_scan_with_host_data": (u32, [vp, vp, vp, vp, vp, sz, sz, ep]),
+        "clo_scan_segmented_with_device_data": (vp, [vp, vp, vp, vp, vp, vp, sz, sz, ep]),
         "clo_scan_get_elem_type": (i32, [vp]),
         "clo_scan_get_sum_type": (i32, [vp]),
         "clo_scan_get_element_size": (sz, [vp]),
@@ -319,6 +321,11 @@ class Sorter:
               k.ctypes.data_as(ctypes.c_void_p), v.ctypes.data_as(ctypes.c_void_p), k.size, 0)
         return k, v
 
+    def sort_segmented_device(self, q, kin, offsets, nseg, n, vin=None, kout=None, vout=None):
+        return _call(self.L.clo_sort_segmented_with_device_data, self.ptr, q.ptr, None, kin.ptr,
+                     vin.ptr if vin else None, kout.ptr if kout else None,
+                     vout.ptr if vout else None, offsets.ptr, nseg, n)
+
     def last_passes(self):
         return _call(self.L.clo_sort_get_last_passes, self.ptr)
 
@@ -353,6 +360,10 @@ class Scanner:
     def scan_device(self, q, bin_, bout, n):
         return _call(self.L.clo_scan_with_device_data, self.ptr, q.ptr, None, bin_.ptr, bout.ptr,
                      n, 0)
+
+    def scan_segmented_device(self, q, bin_, bout, offsets, nseg, n):
+        return _call(self.L.clo_scan_segmented_with_device_data, self.ptr, q.ptr, None, bin_.ptr,
+                     bout.ptr, offsets.ptr, nseg, n)
 
     def scan_host(self, a, q=None):
         a = np.ascontiguousarray(a, dtype=self.elem)

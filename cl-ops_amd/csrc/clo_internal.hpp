@@ -128,6 +128,10 @@ struct clo_sort {
 	// Set by satradix: device word with the executed pass count of the last
 	// call, and the queue it ran on.
 	CCLQueue* last_queue;
+	// Segmented sort: tuning (segtile, seggrid options) and its scratch.
+	uint32_t seg_tile;
+	uint32_t seg_grid;
+	void* seg_cache;
 };
 
 struct clo_scan {
@@ -146,3 +150,7 @@ CCLEvent* clo_gselect_sort_kv(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in,
 	CCLBuffer* keys_out, CCLBuffer* vals_out, size_t numel, GError** err);
 cl_uint clo_satradix_last_passes(CloSort* sorter, GError** err);
+CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* kin,
+	CCLBuffer* vin, CCLBuffer* kout, CCLBuffer* vout, CCLBuffer* offsets, size_t nseg,
+	size_t numel, uint32_t tile_items, uint32_t grid, void** cache, GError** err);
+void clo_satradix_segsort_cache_free(void* cache);

@@ -485,6 +485,22 @@ const char* satradix_init(CloSort* sorter, const char* options, GError** err) {
 				return nullptr;
 			}
 			radix = static_cast<u32>(r);
+		} else if (o.key == "segtile") {
+			const long t = atol(o.value.c_str());
+			const size_t ks = clo_type_sizeof(sorter->elem_type);
+			if (!(t == 8 || t == 16 || (t == 32 && ks == 4))) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+					"segtile must be 8, 16 or 32 (8 or 16 for 64-bit keys).");
+				return nullptr;
+			}
+			sorter->seg_tile = static_cast<u32>(t);
+		} else if (o.key == "seggrid") {
+			const long g = atol(o.value.c_str());
+			if (g < 0 || g > 65536) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "seggrid must be in 0..65536.");
+				return nullptr;
+			}
+			sorter->seg_grid = static_cast<u32>(g);
 		} else if (o.key.size() >= 4 && strncasecmp(o.key.c_str(), "scan", 4) == 0) {
 			if (o.key.size() == 4) sd->scan_type = o.value;
 		} else {

@@ -90,19 +90,91 @@ __device__ Acc lookback(const u64* status, u32 tile, u32 epoch) {
 	}
 }
 
-template <typename In, typename Acc, int OUTW>
+// (flag, value) pair of the segmented scan: flag = a segment starts inside,
+// value = sum since the last segment start (or of everything without one).
+template <typename Acc>
+struct SegPair {
+	u32 f;
+	Acc v;
+};
+
+template <typename Acc>
+__device__ __forceinline__ SegPair<Acc> seg_combine(SegPair<Acc> a, SegPair<Acc> b) {
+	return {a.f | b.f, b.f ? b.v : a.v + b.v};
+}
+
+// Exclusive block scan of (flag, value) pairs; *total = all threads combined.
+template <typename Acc>
+__device__ SegPair<Acc> block_exclusive_segscan(SegPair<Acc> mine, SegPair<Acc>* s_w,
+	SegPair<Acc>* total) {
+	constexpr int W = kScanBlock / kWave;
+	const u32 lane = lane_id();
+	SegPair<Acc> inc = mine;
+#pragma unroll
+	for (int off = 1; off < kWave; off <<= 1) {
+		SegPair<Acc> o;
+		o.f = __shfl_up(inc.f, off, kWave);
+		o.v = __shfl_up(inc.v, off, kWave);
+		if (lane >= static_cast<u32>(off)) inc = seg_combine(o, inc);
+	}
+	if (lane == kWave - 1) s_w[wave_id()] = inc;
+	__syncthreads();
+	SegPair<Acc> pre = {0u, Acc(0)}, tot = {0u, Acc(0)};
+#pragma unroll
+	for (int i = 0; i < W; ++i) {
+		if (static_cast<u32>(i) < wave_id()) pre = seg_combine(pre, s_w[i]);
+		tot = seg_combine(tot, s_w[i]);
+	}
+	*total = tot;
+	__syncthreads();
+	SegPair<Acc> ex;
+	ex.f = __shfl_up(inc.f, 1, kWave);
+	ex.v = __shfl_up(inc.v, 1, kWave);
+	if (lane == 0) ex = {0u, Acc(0)};
+	return seg_combine(pre, ex);
+}
+
+// First index j with off[j] >= x (off non-decreasing).
+__device__ __forceinline__ u32 first_at_or_after(const u32* off, u32 nseg, u64 x) {
+	u32 lo = 0, hi = nseg;
+	while (lo < hi) {
+		const u32 mid = (lo + hi) >> 1;
+		if (off[mid] < x) lo = mid + 1;
+		else hi = mid;
+	}
+	return lo;
+}
+
+// SEG = false: plain exclusive scan.  SEG = true: the sum restarts at every
+// segment start off[j] (the per-segment exclusive scans of a batch).
+template <typename In, typename Acc, int OUTW, bool SEG>
 __global__ __launch_bounds__(kScanBlock) void blelloch_lookback_scan(const In* __restrict__ in,
 	void* __restrict__ out, u64 n, u64* __restrict__ status, u32* __restrict__ ticket,
-	u32 ticket_base, u32 epoch) {
+	u32 ticket_base, u32 epoch, const u32* __restrict__ off, u32 nseg) {
 	__shared__ Acc s_t[pad(kScanTile) + 1];
 	__shared__ Acc s_wsum[kScanBlock / kWave];
+	__shared__ SegPair<Acc> s_pw[kScanBlock / kWave];
+	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
 	__shared__ Acc s_prefix;
 	__shared__ u32 s_tile;
+	__shared__ u32 s_j0;
 	const u32 tid = threadIdx.x;
 	if (tid == 0) s_tile = atomicAdd(ticket, 1u) - ticket_base;
+	if (SEG)
+		for (u32 i = tid; i < kScanTile / 32; i += kScanBlock) s_head[i] = 0;
 	__syncthreads();
 	const u32 tile = s_tile;
 	const u64 base = static_cast<u64>(tile) * kScanTile;
+	if (SEG) {
+		if (tid == 0) s_j0 = first_at_or_after(off, nseg, base);
+		__syncthreads();
+		for (u32 j = s_j0 + tid; j < nseg; j += kScanBlock) {
+			const u64 o = off[j];
+			if (o >= base + kScanTile) break;
+			const u32 p = static_cast<u32>(o - base);
+			atomicOr(&s_head[p >> 5], 1u << (p & 31));
+		}
+	}
 
 	// Striped (coalesced) load, converted to the sum type.
 #pragma unroll
@@ -115,14 +187,36 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_lookback_scan(const In* _
 
 	// Blocked: thread tid owns tile elements [tid*ITEMS, (tid+1)*ITEMS).
 	Acc x[kScanItems];
+	u32 heads = 0;  // bit j: item j starts a segment
 	Acc run = 0;
+	u32 any_head = 0;
 #pragma unroll
 	for (int j = 0; j < kScanItems; ++j) {
 		x[j] = s_t[pad(tid * kScanItems + j)];
+		if (SEG) {
+			const u32 p = tid * kScanItems + j;
+			if ((s_head[p >> 5] >> (p & 31)) & 1u) {
+				heads |= 1u << j;
+				run = 0;
+				any_head = 1;
+			}
+		}
 		run += x[j];
 	}
 	Acc tile_total;
-	const Acc tpre = block_exclusive_sum<kScanBlock>(run, s_wsum, &tile_total);
+	Acc tpre;
+	u32 carry_known = 0;  // SEG: a segment starts before this thread's items in the tile
+	u32 tile_head = 0;
+	if (SEG) {
+		SegPair<Acc> tot;
+		const SegPair<Acc> ex = block_exclusive_segscan<Acc>({any_head, run}, s_pw, &tot);
+		tpre = ex.v;
+		carry_known = ex.f;
+		tile_total = tot.v;
+		tile_head = tot.f;
+	} else {
+		tpre = block_exclusive_sum<kScanBlock>(run, s_wsum, &tile_total);
+	}
 
 	if (tile == 0) {
 		if (tid == 0) {
@@ -130,18 +224,23 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_lookback_scan(const In* _
 			s_prefix = 0;
 		}
 	} else if (tid < kWave) {
-		if (tid == 0) Status<Acc>::store(status, tile, epoch, kFlagAggregate, tile_total);
+		// A tile containing a segment start knows its carry-out (the sum after
+		// its last start) without looking back: publish it as a prefix.
+		if (tid == 0)
+			Status<Acc>::store(status, tile, epoch, tile_head ? kFlagPrefix : kFlagAggregate,
+				tile_total);
 		const Acc excl = lookback<Acc>(status, tile, epoch);
 		if (tid == 0) {
-			Status<Acc>::store(status, tile, epoch, kFlagPrefix, excl + tile_total);
+			if (!tile_head) Status<Acc>::store(status, tile, epoch, kFlagPrefix, excl + tile_total);
 			s_prefix = excl;
 		}
 	}
 	__syncthreads();
 
-	Acc acc = s_prefix + tpre;
+	Acc acc = carry_known ? tpre : s_prefix + tpre;
 #pragma unroll
 	for (int j = 0; j < kScanItems; ++j) {
+		if (SEG && ((heads >> j) & 1u)) acc = 0;
 		s_t[pad(tid * kScanItems + j)] = acc;
 		acc += x[j];
 	}
@@ -175,23 +274,39 @@ struct BlellochData {
 	}
 };
 
-typedef void (*ScanKernel)(const void*, void*, u64, u64*, u32*, u32, u32);
+struct ScanLaunch {
+	const void* in;
+	void* out;
+	u64 n;
+	u64* status;
+	u32* ticket;
+	u32 base;
+	u32 epoch;
+	u32 tiles;
+	const u32* off;  // segment starts (segmented scan) or nullptr
+	u32 nseg;
+	hipStream_t st;
+};
 
 template <typename In, typename Acc, int OUTW>
-void launch_scan(const void* in, void* out, u64 n, u64* status, u32* ticket, u32 base, u32 epoch,
-	u32 tiles, hipStream_t st) {
-	blelloch_lookback_scan<In, Acc, OUTW><<<tiles, kScanBlock, 0, st>>>(
-		static_cast<const In*>(in), out, n, status, ticket, base, epoch);
+void launch_scan(const ScanLaunch& a) {
+	if (a.off)
+		blelloch_lookback_scan<In, Acc, OUTW, true><<<a.tiles, kScanBlock, 0, a.st>>>(
+			static_cast<const In*>(a.in), a.out, a.n, a.status, a.ticket, a.base, a.epoch, a.off,
+			a.nseg);
+	else
+		blelloch_lookback_scan<In, Acc, OUTW, false><<<a.tiles, kScanBlock, 0, a.st>>>(
+			static_cast<const In*>(a.in), a.out, a.n, a.status, a.ticket, a.base, a.epoch, nullptr,
+			0);
 }
 
 template <typename In>
-bool launch_for_in(size_t sum_size, const void* in, void* out, u64 n, u64* status, u32* ticket,
-	u32 base, u32 epoch, u32 tiles, hipStream_t st) {
+bool launch_for_in(size_t sum_size, const ScanLaunch& a) {
 	switch (sum_size) {
-	case 1: launch_scan<In, u32, 1>(in, out, n, status, ticket, base, epoch, tiles, st); return true;
-	case 2: launch_scan<In, u32, 2>(in, out, n, status, ticket, base, epoch, tiles, st); return true;
-	case 4: launch_scan<In, u32, 4>(in, out, n, status, ticket, base, epoch, tiles, st); return true;
-	case 8: launch_scan<In, u64, 8>(in, out, n, status, ticket, base, epoch, tiles, st); return true;
+	case 1: launch_scan<In, u32, 1>(a); return true;
+	case 2: launch_scan<In, u32, 2>(a); return true;
+	case 4: launch_scan<In, u32, 4>(a); return true;
+	case 8: launch_scan<In, u64, 8>(a); return true;
 	}
 	return false;
 }
@@ -211,10 +326,8 @@ void blelloch_finalize(CloScan* scanner) {
 	scanner->data = nullptr;
 }
 
-CCLEvent* blelloch_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueue* cq_comm,
-	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, size_t lws_max, GError** err) {
-	(void)cq_comm;
-	(void)lws_max;
+CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
+	CCLBuffer* data_out, size_t numel, const u32* off, u32 nseg, GError** err) {
 	BlellochData* bd = static_cast<BlellochData*>(scanner->data);
 	if (numel == 0) return record_event(cq_exec, "blelloch_done", err);
 	const size_t tiles = div_ceil(numel, kScanTile);
@@ -238,30 +351,35 @@ CCLEvent* blelloch_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CC
 			return nullptr;
 		bd->epoch = 1;
 	}
-	const u32 base = bd->issued;
+	ScanLaunch a;
+	a.base = bd->issued;
 	bd->issued += static_cast<u32>(tiles);
-	u64* status = static_cast<u64*>(bd->status.ptr);
-	u32* ticket = static_cast<u32*>(bd->meta.ptr);
+	a.status = static_cast<u64*>(bd->status.ptr);
+	a.ticket = static_cast<u32*>(bd->meta.ptr);
+	a.in = data_in->ptr;
+	a.out = data_out->ptr;
+	a.n = numel;
+	a.epoch = bd->epoch;
+	a.tiles = static_cast<u32>(tiles);
+	a.off = off;
+	a.nseg = nseg;
+	a.st = cq_exec->stream;
 	const size_t ss = clo_type_sizeof(scanner->sum_type);
-	const void* in = data_in->ptr;
-	void* out = data_out->ptr;
-	const u32 t = static_cast<u32>(tiles);
-	hipStream_t st = cq_exec->stream;
-	Launch L(cq_exec, CLO_SCAN_BLELLOCH_KNAME_LOOKBACK);
+	Launch L(cq_exec, off ? CLO_SCAN_BLELLOCH_KNAME_SEGMENTED : CLO_SCAN_BLELLOCH_KNAME_LOOKBACK);
 	bool ok = false;
 	switch (scanner->elem_type) {
-	case CLO_CHAR: ok = launch_for_in<int8_t>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
-	case CLO_UCHAR: ok = launch_for_in<u8>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
-	case CLO_SHORT: ok = launch_for_in<int16_t>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
-	case CLO_USHORT: ok = launch_for_in<u16>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
-	case CLO_INT: ok = launch_for_in<int32_t>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
-	case CLO_UINT: ok = launch_for_in<u32>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
-	case CLO_LONG: ok = launch_for_in<int64_t>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
-	case CLO_ULONG: ok = launch_for_in<u64>(ss, in, out, numel, status, ticket, base, bd->epoch, t, st); break;
+	case CLO_CHAR: ok = launch_for_in<int8_t>(ss, a); break;
+	case CLO_UCHAR: ok = launch_for_in<u8>(ss, a); break;
+	case CLO_SHORT: ok = launch_for_in<int16_t>(ss, a); break;
+	case CLO_USHORT: ok = launch_for_in<u16>(ss, a); break;
+	case CLO_INT: ok = launch_for_in<int32_t>(ss, a); break;
+	case CLO_UINT: ok = launch_for_in<u32>(ss, a); break;
+	case CLO_LONG: ok = launch_for_in<int64_t>(ss, a); break;
+	case CLO_ULONG: ok = launch_for_in<u64>(ss, a); break;
 	default: break;
 	}
 	if (!ok) {
-		bd->issued = base;
+		bd->issued = a.base;
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "unsupported scan types");
 		return nullptr;
 	}
@@ -270,6 +388,13 @@ CCLEvent* blelloch_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CC
 		return nullptr;
 	}
 	return record_event(cq_exec, "blelloch_done", err);
+}
+
+CCLEvent* blelloch_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueue* cq_comm,
+	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, size_t lws_max, GError** err) {
+	(void)cq_comm;
+	(void)lws_max;
+	return scan_common(scanner, cq_exec, data_in, data_out, numel, nullptr, 0, err);
 }
 
 cl_uint blelloch_get_num_kernels(CloScan* scanner, GError** err) {
@@ -284,7 +409,7 @@ const char* blelloch_get_kernel_name(CloScan* scanner, cl_uint i, GError** err) 
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "kernel index %u out of range", i);
 		return nullptr;
 	}
-	return CLO_SCAN_BLELLOCH_KNAME_LOOKBACK;
+	return i == 0 ? CLO_SCAN_BLELLOCH_KNAME_LOOKBACK : CLO_SCAN_BLELLOCH_KNAME_SEGMENTED;
 }
 
 size_t blelloch_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, size_t numel,
@@ -374,6 +499,31 @@ CCLEvent* clo_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueu
 	if (!use_device(cq_exec->ctx, err)) return nullptr;
 	return scanner->impl.scan_with_device_data(scanner, cq_exec, cq_comm, data_in, data_out,
 		numel, lws_max, err);
+}
+
+CCLEvent* clo_scan_segmented_with_device_data(CloScan* scanner, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* data_in, CCLBuffer* data_out, CCLBuffer* offsets, size_t nseg,
+	size_t numel, GError** err) {
+	(void)cq_comm;
+	if (err && *err) return nullptr;
+	if (!scanner || !cq_exec || !data_in || !data_out || !offsets) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL scanner, queue or buffer");
+		return nullptr;
+	}
+	if (data_in->size < numel * clo_type_sizeof(scanner->elem_type) ||
+		data_out->size < numel * clo_type_sizeof(scanner->sum_type) || offsets->size < nseg * 4 ||
+		nseg > 0xFFFFFFFFull || numel > 0xFFFFFFFFull) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"buffers smaller than numel elements / nseg offsets, or sizes beyond 2^32-1");
+		return nullptr;
+	}
+	if (!use_device(cq_exec->ctx, err)) return nullptr;
+	if (strcmp(scanner->impl.name, "blelloch") != 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmented scan needs a blelloch scanner");
+		return nullptr;
+	}
+	return scan_common(scanner, cq_exec, data_in, data_out, numel,
+		static_cast<const u32*>(offsets->ptr), static_cast<u32>(nseg), err);
 }
 
 cl_bool clo_scan_with_host_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueue* cq_comm,

@@ -1,0 +1,526 @@
+// Batched segmented sort (an addition with no counterpart in CL_Ops): every
+// segment [offsets[j], offsets[j+1]) of one array is stable-sorted by key in
+// one persistent launch.
+//
+// Work items, taken largest first from one atomic queue:
+//   large  (len > CAP)      one block, LSD digit passes through global memory
+//                           (chunks of CAP staged in LDS, coalesced runs)
+//   medium (W < len <= CAP) one block, all digit passes inside LDS
+//   chunk  (len <= W)       every small segment starting in the element window
+//                           [c*W, (c+1)*W) is packed into one LDS tile and
+//                           sorted by (segment, key): key passes, then passes
+//                           over the local segment index
+// with CAP = 256 * IPT elements and W = CAP / 2 (so a chunk never exceeds
+// CAP).  Digit passes whose digit is equal over the whole item are skipped.
+
+#include <cstring>
+
+#include "clo_internal.hpp"
+#include "radix_block.hpp"
+
+using namespace clo;
+using namespace clo::dev;
+
+namespace {
+
+struct SegMeta {
+	u32 n_medium;
+	u32 n_large;
+	u32 head;
+	u32 pad;
+};
+
+__global__ void seg_reset(SegMeta* m) {
+	if (threadIdx.x == 0) {
+		m->n_medium = 0;
+		m->n_large = 0;
+		m->head = 0;
+	}
+}
+
+__device__ __forceinline__ u32 seg_end(const u32* off, u32 j, u32 nseg, u32 numel) {
+	return j + 1 < nseg ? off[j + 1] : numel;
+}
+
+// Appends medium and large segment indices (one atomic per wave and list).
+__global__ __launch_bounds__(256) void seg_classify(const u32* __restrict__ off, u32 nseg,
+	u32 numel, u32 W, u32 CAP, u32* __restrict__ medium, u32* __restrict__ large, SegMeta* m) {
+	const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+	u32 len = 0;
+	if (j < nseg) len = seg_end(off, j, nseg, numel) - off[j];
+	const bool is_med = j < nseg && len > W && len <= CAP;
+	const bool is_lrg = j < nseg && len > CAP;
+	const u64 bm = ballot(is_med), bl = ballot(is_lrg);
+	u32 basem = 0, basel = 0;
+	if (lane_id() == 0) {
+		if (bm) basem = atomicAdd(&m->n_medium, static_cast<u32>(__popcll(bm)));
+		if (bl) basel = atomicAdd(&m->n_large, static_cast<u32>(__popcll(bl)));
+	}
+	basem = __shfl(basem, 0, kWave);
+	basel = __shfl(basel, 0, kWave);
+	if (is_med) medium[basem + count_below(bm)] = j;
+	if (is_lrg) large[basel + count_below(bl)] = j;
+}
+
+template <typename K>
+struct SegArgs {
+	const K* kin;
+	const u32* vin;
+	K* kout;   // final destination (== kin for in-place)
+	u32* vout;
+	K* kscr;   // numel-sized scratch for large segments
+	u32* vscr;
+	const u32* off;
+	const u32* medium;
+	const u32* large;
+	SegMeta* meta;
+	u32 nseg;
+	u32 numel;
+	u32 nchunks;
+};
+
+template <typename K, bool KV, int IPT>
+struct SegSmem {
+	static constexpr u32 kCap = kRB * IPT;
+	static constexpr u32 kW = kCap / 2;
+	K k[kCap];
+	u32 v[KV ? kCap : 1];
+	u16 sid[kCap];
+	u32 lpos[kW + 1];
+	u32 hist[sizeof(K)][256];
+	u32 base[256];
+	u32 red[2 * kRBWaves];
+	u32 misc[4];
+	RankSmem rs;
+};
+
+template <typename K>
+__device__ __forceinline__ u32 digit_of(K k, u32 shift) {
+	return static_cast<u32>(k >> shift) & 0xFFu;
+}
+
+// Block-wide AND and OR of the keys in s.k[0, n).
+template <typename K, bool KV, int IPT>
+__device__ void and_or_lds(SegSmem<K, KV, IPT>& s, u32 n, K* a_out, K* o_out) {
+	K a = ~K(0), o = K(0);
+	for (u32 i = threadIdx.x; i < n; i += kRB) {
+		a &= s.k[i];
+		o |= s.k[i];
+	}
+#pragma unroll
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		a &= __shfl_xor(a, off, kWave);
+		o |= __shfl_xor(o, off, kWave);
+	}
+	K* red = reinterpret_cast<K*>(s.hist);  // scratch: 2 * waves values
+	if (lane_id() == 0) {
+		red[wave_id()] = a;
+		red[kRBWaves + wave_id()] = o;
+	}
+	__syncthreads();
+	a = ~K(0);
+	o = K(0);
+#pragma unroll
+	for (int w = 0; w < kRBWaves; ++w) {
+		a &= red[w];
+		o |= red[kRBWaves + w];
+	}
+	__syncthreads();
+	*a_out = a;
+	*o_out = o;
+}
+
+// One stable LDS pass over s.k/s.v/s.sid[0, n): digit from the key (shift)
+// or, when by_sid, from the local segment index.
+template <typename K, bool KV, int IPT>
+__device__ void lds_pass(SegSmem<K, KV, IPT>& s, u32 n, u32 shift, bool by_sid, bool carry_sid) {
+	const u32 lane = lane_id();
+	const u32 w = wave_id();
+	K k[IPT];
+	u32 v[IPT];
+	u32 sd[IPT];
+	u32 dig[IPT];
+	bool valid[IPT];
+	u32 pos[IPT];
+#pragma unroll
+	for (int j = 0; j < IPT; ++j) {
+		const u32 e = w * kWave * IPT + j * kWave + lane;
+		valid[j] = e < n;
+		k[j] = valid[j] ? s.k[e] : K(0);
+		if (KV) v[j] = valid[j] ? s.v[e] : 0u;
+		sd[j] = (carry_sid && valid[j]) ? s.sid[e] : 0u;
+		dig[j] = by_sid ? ((sd[j] >> shift) & 0xFFu) : digit_of(k[j], shift);
+	}
+	block_rank<IPT>(dig, valid, pos, 8, s.rs);
+#pragma unroll
+	for (int j = 0; j < IPT; ++j) {
+		if (!valid[j]) continue;
+		s.k[pos[j]] = k[j];
+		if (KV) s.v[pos[j]] = v[j];
+		if (carry_sid) s.sid[pos[j]] = static_cast<u16>(sd[j]);
+	}
+	__syncthreads();
+}
+
+template <typename K, bool KV, int IPT>
+__device__ void sort_key_passes(SegSmem<K, KV, IPT>& s, u32 n, bool carry_sid) {
+	K a, o;
+	and_or_lds<K, KV, IPT>(s, n, &a, &o);
+	const K diff = a ^ o;
+	for (u32 p = 0; p < sizeof(K); ++p) {
+		if (((diff >> (8 * p)) & 0xFF) == 0) continue;  // uniform: every key shares the digit
+		lds_pass<K, KV, IPT>(s, n, 8 * p, false, carry_sid);
+	}
+}
+
+// Medium segment: whole segment in LDS.
+template <typename K, bool KV, int IPT>
+__device__ void sort_medium(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
+	const u32 start = a.off[j];
+	const u32 len = seg_end(a.off, j, a.nseg, a.numel) - start;
+	for (u32 i = threadIdx.x; i < len; i += kRB) {
+		s.k[i] = a.kin[start + i];
+		if (KV) s.v[i] = a.vin[start + i];
+	}
+	__syncthreads();
+	sort_key_passes<K, KV, IPT>(s, len, false);
+	for (u32 i = threadIdx.x; i < len; i += kRB) {
+		a.kout[start + i] = s.k[i];
+		if (KV) a.vout[start + i] = s.v[i];
+	}
+	__syncthreads();
+}
+
+// First segment index with offsets[j] >= x.
+__device__ __forceinline__ u32 lower_bound(const u32* off, u32 nseg, u32 x) {
+	u32 lo = 0, hi = nseg;
+	while (lo < hi) {
+		const u32 mid = (lo + hi) >> 1;
+		if (off[mid] < x) lo = mid + 1;
+		else hi = mid;
+	}
+	return lo;
+}
+
+// Chunk of small segments starting in [c*W, (c+1)*W).
+template <typename K, bool KV, int IPT>
+__device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
+	constexpr u32 W = SegSmem<K, KV, IPT>::kW;
+	const u32 lo = c * W;
+	const u32 hi = lo + W;
+	if (threadIdx.x == 0) {
+		const u32 j0 = lower_bound(a.off, a.nseg, lo);
+		const u32 j1 = lower_bound(a.off, a.nseg, hi);
+		s.misc[0] = j0;
+		s.misc[1] = j1 - j0;
+	}
+	__syncthreads();
+	const u32 j0 = s.misc[0];
+	const u32 m = s.misc[1];  // <= W segments start in the window
+	// LDS start of each included segment (excluded ones take no room).
+	u32 carry = 0;
+	for (u32 r0 = 0; r0 < m; r0 += kRB) {
+		const u32 r = r0 + threadIdx.x;
+		u32 len = 0;
+		if (r < m) {
+			const u32 j = j0 + r;
+			len = seg_end(a.off, j, a.nseg, a.numel) - a.off[j];
+			if (len > W) len = 0;
+		}
+		u32 tot;
+		const u32 ex = block_exclusive_sum<kRB>(len, s.rs.wsum, &tot);
+		if (r < m) s.lpos[r] = carry + ex;
+		carry += tot;
+	}
+	if (threadIdx.x == 0) s.lpos[m] = carry;
+	__syncthreads();
+	const u32 n = carry;  // <= CAP
+	if (n == 0) return;
+	// Gather: element i belongs to the included segment r with
+	// lpos[r] <= i < lpos[r+1].
+	for (u32 i = threadIdx.x; i < n; i += kRB) {
+		u32 l = 0, h = m;  // last r with lpos[r] <= i
+		while (h - l > 1) {
+			const u32 mid = (l + h) >> 1;
+			if (s.lpos[mid] <= i) l = mid;
+			else h = mid;
+		}
+		while (l + 1 < m && s.lpos[l + 1] <= i) ++l;
+		const u32 g = a.off[j0 + l] + (i - s.lpos[l]);
+		s.k[i] = a.kin[g];
+		if (KV) s.v[i] = a.vin[g];
+		s.sid[i] = static_cast<u16>(l);
+	}
+	__syncthreads();
+	const bool multi = m > 1;
+	sort_key_passes<K, KV, IPT>(s, n, multi);
+	if (multi) {
+		// Stable regroup by segment: LSD passes over the local index.
+		lds_pass<K, KV, IPT>(s, n, 0, true, true);
+		if (m > 256) lds_pass<K, KV, IPT>(s, n, 8, true, true);
+	}
+	for (u32 i = threadIdx.x; i < n; i += kRB) {
+		const u32 r = multi ? s.sid[i] : 0u;
+		const u32 g = a.off[j0 + r] + (i - s.lpos[r]);
+		a.kout[g] = s.k[i];
+		if (KV) a.vout[g] = s.v[i];
+	}
+	__syncthreads();
+}
+
+// Large segment: LSD passes through global memory.
+template <typename K, bool KV, int IPT>
+__device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
+	constexpr u32 CAP = SegSmem<K, KV, IPT>::kCap;
+	constexpr u32 P = sizeof(K);
+	const u32 start = a.off[j];
+	const u32 len = seg_end(a.off, j, a.nseg, a.numel) - start;
+	const u32 tid = threadIdx.x;
+	for (u32 i = tid; i < P * 256; i += kRB) (&s.hist[0][0])[i] = 0;
+	__syncthreads();
+	// One read: every digit histogram plus AND/OR of the keys.
+	K ka = ~K(0), ko = K(0);
+	for (u32 i = tid; i < len; i += kRB) {
+		const K k = a.kin[start + i];
+		ka &= k;
+		ko |= k;
+#pragma unroll
+		for (u32 p = 0; p < P; ++p) atomicAdd(&s.hist[p][digit_of(k, 8 * p)], 1u);
+	}
+#pragma unroll
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		ka &= __shfl_xor(ka, off, kWave);
+		ko |= __shfl_xor(ko, off, kWave);
+	}
+	K* red = reinterpret_cast<K*>(s.k);
+	if (lane_id() == 0) {
+		red[wave_id()] = ka;
+		red[kRBWaves + wave_id()] = ko;
+	}
+	__syncthreads();
+	K diff = 0;
+	{
+		K aa = ~K(0), oo = K(0);
+#pragma unroll
+		for (int w = 0; w < kRBWaves; ++w) {
+			aa &= red[w];
+			oo |= red[kRBWaves + w];
+		}
+		diff = aa ^ oo;
+	}
+	u32 executed = 0;
+#pragma unroll
+	for (u32 p = 0; p < P; ++p) executed += ((diff >> (8 * p)) & 0xFF) ? 1u : 0u;
+	__syncthreads();
+	const bool in_place = a.kout == a.kin;
+	const K* src = a.kin + start;
+	const u32* vsrc = KV ? a.vin + start : nullptr;
+	u32 k_ex = 0;
+	for (u32 p = 0; p < P; ++p) {
+		if (((diff >> (8 * p)) & 0xFF) == 0) continue;
+		// destination of executed pass k_ex (the last one lands in kout when possible)
+		bool to_out;
+		if (in_place) to_out = (k_ex % 2) == 1;
+		else to_out = ((executed - 1 - k_ex) % 2) == 0;
+		K* dst = (to_out ? a.kout : a.kscr) + start;
+		u32* vdst = KV ? (to_out ? a.vout : a.vscr) + start : nullptr;
+		const u32 shift = 8 * p;
+		{
+			const u32 c = s.hist[p][tid];
+			const u32 ex = block_exclusive_sum<kRB>(c, s.rs.wsum, static_cast<u32*>(nullptr));
+			s.base[tid] = ex;
+		}
+		__syncthreads();
+		for (u32 c0 = 0; c0 < len; c0 += CAP) {
+			const u32 cnt = (len - c0) < CAP ? (len - c0) : CAP;
+			const u32 lane = lane_id();
+			const u32 w = wave_id();
+			K k[IPT];
+			u32 v[IPT];
+			u32 dig[IPT];
+			bool valid[IPT];
+			u32 pos[IPT];
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj) {
+				const u32 e = w * kWave * IPT + jj * kWave + lane;
+				valid[jj] = e < cnt;
+				k[jj] = valid[jj] ? src[c0 + e] : K(0);
+				if (KV) v[jj] = valid[jj] ? vsrc[c0 + e] : 0u;
+				dig[jj] = digit_of(k[jj], shift);
+			}
+			block_rank<IPT>(dig, valid, pos, 8, s.rs);
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj)
+				if (valid[jj]) s.k[pos[jj]] = k[jj];
+			__syncthreads();
+			u32 gidx[IPT];
+#pragma unroll
+			for (int r = 0; r < IPT; ++r) {
+				const u32 i = r * kRB + tid;
+				if (i < cnt) {
+					const K key = s.k[i];
+					const u32 d = digit_of(key, shift);
+					gidx[r] = s.base[d] + (i - s.rs.tstart[d]);
+					dst[gidx[r]] = key;
+				}
+			}
+			if (KV) {
+				__syncthreads();
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj)
+					if (valid[jj]) s.v[pos[jj]] = v[jj];
+				__syncthreads();
+#pragma unroll
+				for (int r = 0; r < IPT; ++r) {
+					const u32 i = r * kRB + tid;
+					if (i < cnt) vdst[gidx[r]] = s.v[i];
+				}
+			}
+			__syncthreads();
+			s.base[tid] += s.rs.total[tid];
+			__syncthreads();
+		}
+		src = dst;
+		vsrc = vdst;
+		++k_ex;
+	}
+	// Result not yet in kout: in place with an odd pass count, or out of place
+	// with nothing to do.
+	const K* fin = src;
+	if (fin != a.kout + start) {
+		for (u32 i = tid; i < len; i += kRB) {
+			a.kout[start + i] = fin[i];
+			if (KV) a.vout[start + i] = vsrc[i];
+		}
+	}
+	__syncthreads();
+}
+
+template <typename K, bool KV, int IPT>
+__global__ __launch_bounds__(kRB) void seg_sort_persistent(SegArgs<K> a) {
+	__shared__ SegSmem<K, KV, IPT> s;
+	const u32 nl = a.meta->n_large;
+	const u32 nm = a.meta->n_medium;
+	const u32 total = nl + nm + a.nchunks;
+	for (;;) {
+		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head, 1u);
+		__syncthreads();
+		const u32 item = s.misc[3];
+		__syncthreads();
+		if (item >= total) break;
+		if (item < nl) sort_large<K, KV, IPT>(a, s, a.large[item]);
+		else if (item < nl + nm) sort_medium<K, KV, IPT>(a, s, a.medium[item - nl]);
+		else sort_chunk<K, KV, IPT>(a, s, item - nl - nm);
+	}
+}
+
+struct SegData {
+	Scratch lists;  // medium + large index lists
+	Scratch meta;
+	Scratch kscr;
+	Scratch vscr;
+};
+
+template <typename K, bool KV, int IPT>
+bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, GError** err) {
+	constexpr u32 CAP = kRB * IPT;
+	constexpr u32 W = CAP / 2;
+	SegArgs<K> a = a0;
+	a.nchunks = static_cast<u32>(div_ceil(a.numel, W));
+	hipStream_t st = q->stream;
+	{
+		Launch L(q, "segsort_reset");
+		seg_reset<<<1, 64, 0, st>>>(a.meta);
+		if (!L.done(err)) return false;
+	}
+	{
+		Launch L(q, "segsort_classify");
+		const u32 g = static_cast<u32>(div_ceil(a.nseg, 256));
+		seg_classify<<<g ? g : 1, 256, 0, st>>>(a.off, a.nseg, a.numel, W, CAP,
+			const_cast<u32*>(a.medium), const_cast<u32*>(a.large), a.meta);
+		if (!L.done(err)) return false;
+	}
+	{
+		Launch L(q, "segsort_persistent");
+		seg_sort_persistent<K, KV, IPT><<<grid, kRB, 0, st>>>(a);
+		if (!L.done(err)) return false;
+	}
+	return true;
+}
+
+}  // namespace
+
+struct clo_segsort_cache {
+	SegData d;
+};
+
+// Entry point used by clo_sort_segmented_with_device_data (satradix sorter).
+CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* kin,
+	CCLBuffer* vin, CCLBuffer* kout, CCLBuffer* vout, CCLBuffer* offsets, size_t nseg,
+	size_t numel, u32 tile_items, u32 grid, void** cache, GError** err) {
+	(void)sorter;
+	if (!*cache) *cache = new clo_segsort_cache();
+	SegData* sd = &static_cast<clo_segsort_cache*>(*cache)->d;
+	const size_t ks = clo_type_sizeof(sorter->elem_type);
+	if (numel == 0 || nseg == 0) return record_event(cq, "segsort_done", err);
+	if (!sd->lists.ensure(2 * nseg * sizeof(u32), err) || !sd->meta.ensure(sizeof(SegMeta), err) ||
+		!sd->kscr.ensure(numel * ks, err) || (vin && !sd->vscr.ensure(numel * 4, err)))
+		return nullptr;
+	if (grid == 0) {
+		int dev = cq->ctx->dev, cus = 0;
+		if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+			cus <= 0)
+			cus = 256;
+		grid = static_cast<u32>(cus) * 2;
+	}
+	u32* lists = static_cast<u32*>(sd->lists.ptr);
+	const bool kv = vin != nullptr;
+	const bool inplace = kout == nullptr || kout->ptr == kin->ptr;
+#define SEG_ARGS(K)                                                                          \
+	SegArgs<K> a;                                                                            \
+	a.kin = static_cast<const K*>(kin->ptr);                                                 \
+	a.vin = kv ? static_cast<const u32*>(vin->ptr) : nullptr;                                \
+	a.kout = inplace ? static_cast<K*>(kin->ptr) : static_cast<K*>(kout->ptr);                \
+	a.vout = kv ? (inplace ? static_cast<u32*>(vin->ptr) : static_cast<u32*>(vout->ptr))     \
+		: nullptr;                                                                           \
+	a.kscr = static_cast<K*>(sd->kscr.ptr);                                                  \
+	a.vscr = kv ? static_cast<u32*>(sd->vscr.ptr) : nullptr;                                  \
+	a.off = static_cast<const u32*>(offsets->ptr);                                           \
+	a.medium = lists;                                                                        \
+	a.large = lists + nseg;                                                                  \
+	a.meta = static_cast<SegMeta*>(sd->meta.ptr);                                            \
+	a.nseg = static_cast<u32>(nseg);                                                         \
+	a.numel = static_cast<u32>(numel);                                                       \
+	a.nchunks = 0;
+	bool ok = false;
+	if (ks == 4) {
+		SEG_ARGS(u32)
+		switch (tile_items) {
+		case 8: ok = kv ? seg_launch<u32, true, 8>(cq, sd, a, grid, err) : seg_launch<u32, false, 8>(cq, sd, a, grid, err); break;
+		case 16: ok = kv ? seg_launch<u32, true, 16>(cq, sd, a, grid, err) : seg_launch<u32, false, 16>(cq, sd, a, grid, err); break;
+		case 32: ok = kv ? seg_launch<u32, true, 32>(cq, sd, a, grid, err) : seg_launch<u32, false, 32>(cq, sd, a, grid, err); break;
+		default:
+			set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmented tile items must be 8, 16 or 32");
+			return nullptr;
+		}
+	} else if (ks == 8) {
+		SEG_ARGS(u64)
+		switch (tile_items) {
+		case 8: ok = kv ? seg_launch<u64, true, 8>(cq, sd, a, grid, err) : seg_launch<u64, false, 8>(cq, sd, a, grid, err); break;
+		case 16: ok = kv ? seg_launch<u64, true, 16>(cq, sd, a, grid, err) : seg_launch<u64, false, 16>(cq, sd, a, grid, err); break;
+		default:
+			set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmented tile items for 64-bit keys must be 8 or 16");
+			return nullptr;
+		}
+	} else {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmented sort takes 32- or 64-bit keys");
+		return nullptr;
+	}
+#undef SEG_ARGS
+	if (!ok) return nullptr;
+	return record_event(cq, "segsort_done", err);
+}
+
+void clo_satradix_segsort_cache_free(void* cache) {
+	delete static_cast<clo_segsort_cache*>(cache);
+}

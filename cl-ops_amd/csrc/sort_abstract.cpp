@@ -58,6 +58,9 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 	s->descending = ck;
 	s->data = nullptr;
 	s->last_queue = nullptr;
+	s->seg_tile = 0;
+	s->seg_grid = 0;
+	s->seg_cache = nullptr;
 	GError* e = nullptr;
 	const char* src = s->impl.init(s, options, &e);
 	if (!src || e) {
@@ -76,6 +79,7 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 void clo_sort_destroy(CloSort* sorter) {
 	if (!sorter) return;
 	if (sorter->impl.finalize) sorter->impl.finalize(sorter);
+	if (sorter->seg_cache) clo_satradix_segsort_cache_free(sorter->seg_cache);
 	ccl_context_destroy(sorter->ctx);
 	delete sorter;
 }
@@ -220,6 +224,34 @@ done:
 	if (dv) ccl_buffer_destroy(dv);
 	if (own) ccl_queue_destroy(own);
 	return ok;
+}
+
+CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
+	CCLBuffer* vals_out, CCLBuffer* offsets, size_t nseg, size_t numel, GError** err) {
+	(void)cq_comm;
+	if (err && *err) return nullptr;
+	if (!check_device_args(sorter, cq_exec, keys_in, keys_out, numel,
+			sorter ? clo_type_sizeof(sorter->elem_type) : 0, err))
+		return nullptr;
+	if (strcmp(sorter->impl.name, "satradix") != 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"segmented sorting is provided by satradix sorters, not '%s'", sorter->impl.name);
+		return nullptr;
+	}
+	if (!offsets || offsets->size < nseg * 4 || nseg > 0xFFFFFFFFull ||
+		(vals_in && (vals_in->size < numel * 4 || (vals_out && vals_out->size < numel * 4))) ||
+		(vals_in && ((keys_out == nullptr) != (vals_out == nullptr))) ||
+		(!vals_in && vals_out)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"segmented sort needs nseg uint32 offsets and, for key-value data, uint32 value "
+			"buffers of numel elements (keys_out/vals_out both NULL or both set)");
+		return nullptr;
+	}
+	const size_t ks = clo_type_sizeof(sorter->elem_type);
+	uint32_t tile = sorter->seg_tile ? sorter->seg_tile : (ks == 8 ? 16u : 32u);
+	return clo_satradix_sort_segmented(sorter, cq_exec, keys_in, vals_in, keys_out, vals_out,
+		offsets, nseg, numel, tile, sorter->seg_grid, &sorter->seg_cache, err);
 }
 
 cl_uint clo_sort_get_last_passes(CloSort* sorter, GError** err) {

@@ -40,6 +40,14 @@ void clo_scan_destroy(CloScan* scan);
 CCLEvent* clo_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* data_in, CCLBuffer* data_out,
 	size_t numel, size_t lws_max, GError** err);
+/* Batched segmented exclusive scan (addition): the sum restarts at every
+ * segment start offsets[j] (device buffer of nseg non-decreasing uint32
+ * starts), giving the exclusive scan of each segment, all in one launch.
+ * Blelloch scanners only. */
+CCLEvent* clo_scan_segmented_with_device_data(CloScan* scanner, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* data_in, CCLBuffer* data_out, CCLBuffer* offsets, size_t nseg,
+	size_t numel, GError** err);
+
 cl_bool clo_scan_with_host_data(CloScan* scanner, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, void* data_in, void* data_out, size_t numel,
 	size_t lws_max, GError** err);

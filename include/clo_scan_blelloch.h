@@ -14,8 +14,9 @@
 extern "C" {
 #endif
 
-#define CLO_SCAN_BLELLOCH_NUM_KERNELS 1
+#define CLO_SCAN_BLELLOCH_NUM_KERNELS 2
 #define CLO_SCAN_BLELLOCH_KNAME_LOOKBACK "blelloch_lookback_scan"
+#define CLO_SCAN_BLELLOCH_KNAME_SEGMENTED "blelloch_segmented_scan"
 
 extern const CloScanImplDef clo_scan_blelloch_def;
 

@@ -88,6 +88,17 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, void* keys, void* vals, size_t numel, size_t lws_max,
 	GError** err);
 
+/* Batched segmented sort (satradix sorters, 32- or 64-bit keys): every
+ * segment [offsets[j], offsets[j+1]) -- the last one ends at numel -- is
+ * stable-sorted by key, all segments in one persistent launch.  `offsets` is a
+ * device buffer of nseg uint32 segment starts, non-decreasing, offsets[0] = 0.
+ * vals_in may be NULL (keys only).  keys_out == NULL sorts in place.
+ * Sorter options tune it: segtile=8|16|32 (LDS tile = 256*segtile elements;
+ * 8|16 for 64-bit keys), seggrid=<blocks> (0 = two per compute unit). */
+CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
+	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
+	CCLBuffer* vals_out, CCLBuffer* offsets, size_t nseg, size_t numel, GError** err);
+
 /* Number of digit passes the last satradix call executed (passes whose digit
  * was the same for every key are skipped).  Blocks on the queue of that
  * call.  0 for the other algorithms. */

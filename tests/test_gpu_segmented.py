@@ -1,0 +1,137 @@
+"""Batched segmented sort and scan on the GPU vs the oracle applied per segment."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _lengths(oracle, nseg, max_len, seed):
+    # log-uniform lengths in [1, max_len] (configuration 4's distribution)
+    u = oracle.generate(oracle.GEN_U64, seed, nseg)
+    x = (u >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return np.clip(np.floor(np.exp(x * np.log(max_len))), 1, max_len).astype(np.uint32)
+
+
+def _offsets(lens):
+    off = np.zeros(lens.size, dtype=np.uint32)
+    off[1:] = np.cumsum(lens[:-1], dtype=np.uint64).astype(np.uint32)
+    return off
+
+
+def _ref_sort(oracle, k, v, off, n):
+    rk, rv = k.copy(), None if v is None else v.copy()
+    ends = np.append(off[1:], n)
+    for s, e in zip(off, ends):
+        if e - s > 1:
+            if v is None:
+                rk[s:e] = oracle.stable_sort(k[s:e])
+            else:
+                rk[s:e], rv[s:e] = oracle.stable_sort(k[s:e], v[s:e])
+    return rk, rv
+
+
+CASES = [
+    # (nseg, max_len) covering chunks, medium and large segments
+    (1, 100000), (7, 3), (300, 64), (500, 5000), (200, 70000), (2000, 20000),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64])
+@pytest.mark.parametrize("kv", [False, True])
+@pytest.mark.parametrize("nseg,max_len", CASES)
+def test_segmented_sort(clops, oracle, ctx, queue, dtype, kv, nseg, max_len):
+    lens = _lengths(oracle, nseg, max_len, 0xC4A + nseg)
+    off = _offsets(lens)
+    n = int(lens.sum(dtype=np.uint64))
+    k = oracle.generate(oracle.GEN_U64, 0xC4B, n).astype(dtype)
+    v = np.arange(n, dtype=np.uint32) if kv else None
+    s = clops.Sorter(ctx, "satradix", dtype)
+    kb = clops.Buffer.from_array(ctx, k)
+    vb = clops.Buffer.from_array(ctx, v) if kv else None
+    ob = clops.Buffer.from_array(ctx, off)
+    s.sort_segmented_device(queue, kb, ob, nseg, n, vin=vb)
+    queue.finish()
+    rk, rv = _ref_sort(oracle, k, v, off, n)
+    assert np.array_equal(kb.read(queue, dtype, n), rk)
+    if kv:
+        assert np.array_equal(vb.read(queue, np.uint32, n), rv)
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_segmented_sort_tiles_out_of_place(clops, oracle, ctx, queue, tile):
+    nseg = 3000
+    lens = _lengths(oracle, nseg, 30000, 0x51)
+    lens[::7] = 0  # empty segments
+    off = _offsets(lens)
+    n = int(lens.sum(dtype=np.uint64))
+    k = (oracle.generate(oracle.GEN_U64, 0x52, n) % np.uint64(97)).astype(np.uint32)
+    v = np.arange(n, dtype=np.uint32)
+    s = clops.Sorter(ctx, "satradix", np.uint32, options="segtile=%d,seggrid=300" % tile)
+    kb, vb = clops.Buffer.from_array(ctx, k), clops.Buffer.from_array(ctx, v)
+    ko, vo = clops.Buffer(ctx, n * 4), clops.Buffer(ctx, n * 4)
+    ob = clops.Buffer.from_array(ctx, off)
+    s.sort_segmented_device(queue, kb, ob, nseg, n, vin=vb, kout=ko, vout=vo)
+    queue.finish()
+    rk, rv = _ref_sort(oracle, k, v, off, n)
+    assert np.array_equal(ko.read(queue, np.uint32, n), rk)
+    assert np.array_equal(vo.read(queue, np.uint32, n), rv)
+    assert np.array_equal(kb.read(queue, np.uint32, n), k)  # input untouched
+
+
+def _ref_segscan(oracle, a, off, n, sumt):
+    out = np.empty(n, dtype=sumt)
+    starts = np.append(off, n)
+    if off.size and off[0] > 0:
+        out[:off[0]] = oracle.scan(a[:off[0]], sumt)
+    for s, e in zip(starts[:-1], starts[1:]):
+        out[s:e] = oracle.scan(a[s:e], sumt)
+    return out
+
+
+@pytest.mark.parametrize("elem,sumt", [(np.uint32, np.uint32), (np.uint32, np.uint64),
+                                       (np.int16, np.int64), (np.uint64, np.uint64)])
+@pytest.mark.parametrize("nseg,max_len", [(1, 50000), (1000, 10), (3000, 2000), (100, 70000)])
+def test_segmented_scan(clops, oracle, ctx, queue, elem, sumt, nseg, max_len):
+    lens = _lengths(oracle, nseg, max_len, 0xC4C + nseg)
+    lens[3::11] = 0
+    off = _offsets(lens)
+    n = int(lens.sum(dtype=np.uint64))
+    a = oracle.generate(oracle.GEN_U64, 0x5E, n).astype(
+        np.dtype(elem).str.replace("i", "u")).view(elem)
+    sc = clops.Scanner(ctx, elem, sumt)
+    bi = clops.Buffer.from_array(ctx, a)
+    bo = clops.Buffer(ctx, max(n, 1) * np.dtype(sumt).itemsize)
+    ob = clops.Buffer.from_array(ctx, off)
+    sc.scan_segmented_device(queue, bi, bo, ob, nseg, n)
+    queue.finish()
+    assert np.array_equal(bo.read(queue, sumt, n), _ref_segscan(oracle, a, off, n, sumt))
+
+
+def test_config4_shape(clops, oracle, ctx, queue):
+    """32768 log-uniform segments in [1, 65536]: device checks + a bit-exact sample."""
+    nseg = 32768
+    lens = _lengths(oracle, nseg, 65536, 0xC4A)
+    off = _offsets(lens)
+    n = int(lens.sum(dtype=np.uint64))
+    kb = clops.Buffer(ctx, n * 4)
+    clops.generate(queue, kb, n, clops.GEN_U32_HI, 0xC4B)
+    ob = clops.Buffer.from_array(ctx, off)
+    h0 = clops.multiset_hash(queue, kb, n, 4)
+    s = clops.Sorter(ctx, "satradix", np.uint32)
+    s.sort_segmented_device(queue, kb, ob, nseg, n)
+    queue.finish()
+    assert clops.multiset_hash(queue, kb, n, 4) == h0
+    got = kb.read(queue, np.uint32, n)
+    k = oracle.generate(oracle.GEN_U32_HI, 0xC4B, n)
+    ends = np.append(off[1:], n)
+    for j in list(range(0, nseg, 97)) + [int(np.argmax(lens))]:
+        s0, e0 = int(off[j]), int(ends[j])
+        assert np.array_equal(got[s0:e0], np.sort(k[s0:e0])), j
+    sc = clops.Scanner(ctx, np.uint32, np.uint32)
+    so = clops.Buffer(ctx, n * 4)
+    sc.scan_segmented_device(queue, kb, so, ob, nseg, n)
+    queue.finish()
+    res = so.read(queue, np.uint32, n)
+    for j in range(0, nseg, 89):
+        s0, e0 = int(off[j]), int(ends[j])
+        assert np.array_equal(res[s0:e0], oracle.scan(got[s0:e0], np.uint32)), j
