@@ -3,10 +3,10 @@ pairs, uint64 keys uniform over 64 bits (seed 0xC3A), uint32 payload = index.
 
 One step = one clo_sort_kv_with_device_data call, out of place (the unsorted
 input stays resident in HBM and is re-sorted every step).  Prints one JSON
-line.  `roofline` is for the dominant kernel (satradix_scatter, one per digit
-pass): algorithmic bytes per launch = n * 2 * (8 + 4) divided by its average
-launch duration measured with HIP events on the exec stream over the timed
-region.  `cpu_baseline` times the oracle's reference-structured satradix
+line.  `roofline` is for the dominant kernel (satradix_downsweep, one per
+digit pass): algorithmic bytes per launch = n * 2 * (8 + 4) divided by its
+average launch duration measured with HIP events on the exec stream over the
+timed region.  `cpu_baseline` times the oracle's reference-structured satradix
 (4-bit digits, 256-element blocks, blocks spread over host threads) on a
 bounded sample of the same workload.
 
@@ -75,7 +75,7 @@ def cpu_baseline_kv(n_sample, threads):
 
 
 def traffic_from_profiles():
-    p = os.path.join(ROOT, "profiles", "satradix_scatter_kv_u64_2p28_traffic.json")
+    p = os.path.join(ROOT, "profiles", "satradix_downsweep_kv_u64_2p28_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -98,8 +98,12 @@ def headline(args, ctx):
         s.sort_kv_device(q, kin, vin, kout, vout, n)
 
     ms = time_steps(q, step, args.steps, args.warmup)
-    cnt, tot = q.prof("satradix_scatter")
-    hcnt, htot = q.prof("satradix_histogram")
+    for name in q.prof_names():
+        c, t = q.prof(name)
+        log("  kernel %-22s launches/step %5.1f  avg %.4f ms  total/step %.4f ms"
+            % (name, c / args.steps, t / max(c, 1), t / args.steps))
+    cnt, tot = q.prof("satradix_downsweep")
+    hcnt, htot = q.prof("satradix_upsweep")
     passes = s.last_passes()
     scatter_ms = tot / max(cnt, 1)
     bytes_per_launch = n * 2 * (8 + 4)
@@ -108,7 +112,7 @@ def headline(args, ctx):
     bad = clops.check_sorted(q, kout, n, 8, vout)
     same = clops.multiset_hash(q, kin, n, 8) == clops.multiset_hash(q, kout, n, 8)
     total_bytes = n * 8 + passes * bytes_per_launch
-    log("headline: %.3f ms/step, scatter %.3f ms x %d, histogram %.3f ms, passes %d, "
+    log("headline: %.3f ms/step, downsweep %.3f ms x %d, upsweep %.3f ms, passes %d, "
         "sorted-violations %d, multiset-equal %s, whole-sort HBM fraction %.3f"
         % (ms, scatter_ms, cnt // max(args.steps, 1), htot / max(hcnt, 1), passes, bad, same,
            total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
@@ -257,7 +261,7 @@ def main():
             "unit": "GB/s",
             "frac": round(h["achieved"] / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(),
-            "kernel": "satradix_scatter",
+            "kernel": "satradix_downsweep",
             "kernel_avg_ms": round(h["scatter_ms"], 4),
             "bytes_per_launch": h["bytes_per_launch"],
         },

@@ -6,16 +6,23 @@
 // kernels never use CLO_SORT_COMPARE).  Key-value pairs keep the payloads of
 // equal keys in input order.
 //
-// Kernel plan per call (all on cq_exec, no host synchronisation):
-//   satradix_histogram  one read of the keys -> digit counts of every pass
-//   satradix_offsets    one block: exclusive digit starts per pass, which
-//                       passes are trivial (one digit holds every key), and
-//                       the source/destination buffer of every executed pass
-//   satradix_scatter    per digit pass: tile load, wave64 ballot ranking,
-//                       decoupled look-back of per-digit tile prefixes,
-//                       LDS-staged scatter (one read + one write of the data)
+// Kernel plan per call (all on cq_exec, no host synchronisation, no
+// inter-block waiting):
+//   satradix_upsweep    per tile of 8192 keys: digit counts (the first pass
+//                       also takes the keys' bitwise AND/OR)
+//   satradix_plan       one block: which digits vary between keys (the other
+//                       passes are skipped) and the buffers of each pass
+//   satradix_reduce     per 64 tiles: sum of their digit counts
+//   satradix_scan       (two kernels) tile counts -> global output offsets
+//   satradix_downsweep  per tile: wave64 ballot ranking, LDS-staged scatter
+//                       to the offsets (contiguous runs per digit)
 //   satradix_copy       only when an in-place sort executed an odd number of
 //                       passes (the result sits in the scratch buffer)
+// The reduce-then-scan structure reads the keys twice per pass (upsweep and
+// downsweep).  A single-pass decoupled look-back variant was measured on
+// MI355X: the per-block ticket atomic serialises (~11 ns per block) and the
+// look-back walks over ~30 predecessors per tile in 256-digit status rows,
+// which made the pass 2.3x slower than the scatter alone.
 
 #include <cstring>
 #include <string>
@@ -32,10 +39,23 @@ constexpr int kRadixMax = 256;
 constexpr int kBlock = 256;  // == kRadixMax: one thread per digit bin
 constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxPasses = 64;
-constexpr int kHistBlock = 256;
-constexpr int kHistLds = 2048;  // max passes * bins over all digit widths
+constexpr u32 kScanTiles = 64;  // tiles per block of the counter reduce/scan
+constexpr u32 kXcds = 8;        // MI355X: blocks go round-robin to 8 XCDs (L2 domains)
 
 enum BufSel : u32 { kIn = 0, kAux = 1, kOut = 2 };
+
+// Tile of a block.  Consecutive block indices run on different XCDs, so
+// mapping them to consecutive tiles would make neighbouring tiles write
+// partial cache lines of the same digit runs from 8 different L2 caches.
+// Giving each XCD a contiguous range of tiles keeps a digit run's writes in
+// one L2.  Any mapping is correct (tiles never wait on each other); this one
+// is the fast one.  Grid = kXcds * per; blocks with tile >= tiles exit.
+__device__ __forceinline__ u32 xcd_tile(u32 tiles) {
+	const u32 per = (tiles + kXcds - 1) / kXcds;
+	return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+inline u32 xcd_grid(u32 tiles) { return kXcds * ((tiles + kXcds - 1) / kXcds); }
 
 struct PassDesc {
 	u32 active;
@@ -52,83 +72,123 @@ struct SortPlan {
 	u32 executed;
 };
 
-// Device-side metadata block of one sorter.
-struct Meta {
-	SortPlan plan;
-	u32 tickets[kMaxPasses];
-	u32 hist[kMaxPasses * kRadixMax];  // zero between calls (offsets clears it)
-	u32 base[kMaxPasses * kRadixMax];
-};
-
 template <typename K>
 struct Bufs {
 	K* k[3];
 	u32* v[3];
 };
 
-// ---------------------------------------------------------------------------
-// Histogram of every digit position, one read of the keys.
-template <typename K>
-__global__ __launch_bounds__(kHistBlock) void satradix_histogram(const K* __restrict__ keys,
-	u32 n, u32* __restrict__ hist, u32 passes, u32 bits) {
-	__shared__ u32 s_h[kHistLds];
-	const u32 nbins = 1u << bits;
-	const u32 mask = nbins - 1u;
-	const u32 total = passes * nbins;
-	for (u32 i = threadIdx.x; i < total; i += kHistBlock) s_h[i] = 0;
+// Digit counts of one tile of kBlock*ITEMS keys.  The first pass reads the
+// input and also records the tile's bitwise AND and OR of the keys, from which
+// the plan kernel finds the digits that are equal in every key.
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(kBlock) void satradix_upsweep(Bufs<K> b,
+	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits,
+	u32* __restrict__ counts, K* __restrict__ andor, u32 first) {
+	constexpr u32 kTile = kBlock * ITEMS;
+	__shared__ u32 s_h[kWaves][kRadixMax];
+	__shared__ K s_red[2 * kWaves];
+	u32 shift = 0;
+	const K* __restrict__ src = b.k[kIn];
+	if (!first) {
+		const PassDesc d = plan->pass[pass];
+		if (!d.active) return;
+		shift = d.shift;
+		src = b.k[d.src];
+	}
+	const u32 tile = xcd_tile(tiles);
+	if (tile >= tiles) return;
+	const u32 tid = threadIdx.x;
+	const u32 lane = lane_id();
+	const u32 w = wave_id();
+	for (u32 i = lane; i < kRadixMax; i += kWave) s_h[w][i] = 0;
 	__syncthreads();
-	const u32 stride = gridDim.x * kHistBlock;
-	for (u32 i = blockIdx.x * kHistBlock + threadIdx.x; i < n; i += stride) {
-		K k = keys[i];
-		for (u32 p = 0; p < passes; ++p) {
-			u32 d = static_cast<u32>(k >> (p * bits)) & mask;
-			atomicAdd(&s_h[p * nbins + d], 1u);
+	const u32 mask = (1u << bits) - 1u;
+	const u32 wbase = tile * kTile + w * kWave * ITEMS;
+	K k[ITEMS];
+#pragma unroll
+	for (int j = 0; j < ITEMS; ++j) {
+		const u32 idx = wbase + j * kWave + lane;
+		k[j] = idx < n ? src[idx] : K(0);
+	}
+	K ka = ~K(0), ko = K(0);
+#pragma unroll
+	for (int j = 0; j < ITEMS; ++j) {
+		if (wbase + j * kWave + lane < n) {
+			atomicAdd(&s_h[w][static_cast<u32>(k[j] >> shift) & mask], 1u);
+			ka &= k[j];
+			ko |= k[j];
+		}
+	}
+	if (first) {
+#pragma unroll
+		for (int off = kWave / 2; off > 0; off >>= 1) {
+			ka &= __shfl_xor(ka, off, kWave);
+			ko |= __shfl_xor(ko, off, kWave);
+		}
+		if (lane == 0) {
+			s_red[w] = ka;
+			s_red[kWaves + w] = ko;
 		}
 	}
 	__syncthreads();
-	for (u32 i = threadIdx.x; i < total; i += kHistBlock) {
-		u32 c = s_h[i];
-		if (c) atomicAdd(&hist[(i / nbins) * kRadixMax + (i % nbins)], c);
+	u32 c = 0;
+#pragma unroll
+	for (int ww = 0; ww < kWaves; ++ww) c += s_h[ww][tid];
+	counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
+	if (first && tid == 0) {
+		K a = ~K(0), o = K(0);
+#pragma unroll
+		for (int ww = 0; ww < kWaves; ++ww) {
+			a &= s_red[ww];
+			o |= s_red[kWaves + ww];
+		}
+		andor[2 * static_cast<size_t>(tile)] = a;
+		andor[2 * static_cast<size_t>(tile) + 1] = o;
 	}
 }
 
-// ---------------------------------------------------------------------------
-// Digit starts per pass + the pass plan.  One block of kBlock threads.
-__global__ __launch_bounds__(kBlock) void satradix_offsets(Meta* __restrict__ meta, u32 n,
-	u32 passes, u32 bits, u32 in_place) {
-	__shared__ u32 s_wsum[kWaves];
-	__shared__ u32 s_trivial[kMaxPasses];
-	const u32 t = threadIdx.x;
-	const u32 nbins = 1u << bits;
-	for (u32 p = 0; p < passes; ++p) {
-		u32 c = 0;
-		if (t < nbins) {
-			c = meta->hist[p * kRadixMax + t];
-			meta->hist[p * kRadixMax + t] = 0;  // ready for the next call
-		}
-		u32 excl = block_exclusive_sum<kBlock>(c, s_wsum, static_cast<u32*>(nullptr));
-		if (t < nbins) meta->base[p * kRadixMax + t] = excl;
-		int triv = __syncthreads_or(c == n ? 1 : 0);
-		if (t == 0) s_trivial[p] = triv;
-		if (t == 0) meta->tickets[p] = 0;
+// Which digit passes run (a digit equal in every key needs no pass), and the
+// source and destination buffer of each executed pass.  One block.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void satradix_plan(SortPlan* __restrict__ plan,
+	const K* __restrict__ andor, u32 tiles, u32 passes, u32 bits, u32 in_place) {
+	__shared__ K s_red[2 * kWaves];
+	const u32 tid = threadIdx.x;
+	K a = ~K(0), o = K(0);
+	for (u32 t = tid; t < tiles; t += kBlock) {
+		a &= andor[2 * static_cast<size_t>(t)];
+		o |= andor[2 * static_cast<size_t>(t) + 1];
+	}
+#pragma unroll
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		a &= __shfl_xor(a, off, kWave);
+		o |= __shfl_xor(o, off, kWave);
+	}
+	if (lane_id() == 0) {
+		s_red[wave_id()] = a;
+		s_red[kWaves + wave_id()] = o;
 	}
 	__syncthreads();
-	if (t != 0) return;
+	if (tid != 0) return;
+	a = ~K(0);
+	o = K(0);
+	for (int ww = 0; ww < kWaves; ++ww) {
+		a &= s_red[ww];
+		o |= s_red[kWaves + ww];
+	}
+	const K diff = a ^ o;  // bits that differ between keys
+	const u32 mask = (1u << bits) - 1u;
 	u32 executed = 0;
-	for (u32 p = 0; p < passes; ++p) executed += s_trivial[p] ? 0u : 1u;
-	SortPlan& pl = meta->plan;
+	for (u32 p = 0; p < passes; ++p) executed += (static_cast<u32>(diff >> (p * bits)) & mask) ? 1u : 0u;
 	u32 k = 0;
 	u32 cur = kIn;
 	for (u32 p = 0; p < kMaxPasses; ++p) {
 		PassDesc d = {0u, 0u, 0u, 0u};
-		if (p < passes && !s_trivial[p]) {
+		if (p < passes && (static_cast<u32>(diff >> (p * bits)) & mask)) {
 			u32 dst;
-			if (in_place) {
-				dst = (k % 2 == 0) ? kAux : kIn;
-			} else {
-				// The last executed pass must land in kOut.
-				dst = ((executed - 1 - k) % 2 == 0) ? kOut : kAux;
-			}
+			if (in_place) dst = (k % 2 == 0) ? kAux : kIn;
+			else dst = ((executed - 1 - k) % 2 == 0) ? kOut : kAux;  // last one lands in kOut
 			d.active = 1;
 			d.shift = p * bits;
 			d.src = cur;
@@ -136,36 +196,106 @@ __global__ __launch_bounds__(kBlock) void satradix_offsets(Meta* __restrict__ me
 			cur = dst;
 			++k;
 		}
-		pl.pass[p] = d;
+		plan->pass[p] = d;
 	}
-	pl.executed = executed;
-	pl.copy_active = 0;
-	pl.copy_src = 0;
-	pl.copy_dst = 0;
+	plan->executed = executed;
+	plan->copy_active = 0;
+	plan->copy_src = 0;
+	plan->copy_dst = 0;
 	if (in_place && (executed % 2 == 1)) {
-		pl.copy_active = 1;
-		pl.copy_src = kAux;
-		pl.copy_dst = kIn;
+		plan->copy_active = 1;
+		plan->copy_src = kAux;
+		plan->copy_dst = kIn;
 	} else if (!in_place && executed == 0) {
-		pl.copy_active = 1;
-		pl.copy_src = kIn;
-		pl.copy_dst = kOut;
+		plan->copy_active = 1;
+		plan->copy_src = kIn;
+		plan->copy_dst = kOut;
 	}
 }
 
-// ---------------------------------------------------------------------------
-// One digit pass: stable ranking of a tile of kBlock*ITEMS elements with wave
-// ballots, decoupled look-back for the tile's per-digit global offsets, and a
+// Loads the kScanTiles count rows of block `blk` (digit d) into registers.
+__device__ __forceinline__ void load_rows(const u32* __restrict__ counts, u32 blk, u32 tiles,
+	u32 d, u32 (&c)[kScanTiles]) {
+	const u32 t0 = blk * kScanTiles;
+#pragma unroll
+	for (u32 i = 0; i < kScanTiles; ++i)
+		c[i] = (t0 + i < tiles) ? counts[static_cast<size_t>(t0 + i) * kRadixMax + d] : 0u;
+}
+
+// partial[blk][d] = sum of counts[t][d] over the kScanTiles tiles of block blk.
+__global__ __launch_bounds__(kBlock) void satradix_reduce(const SortPlan* __restrict__ plan,
+	u32 pass, const u32* __restrict__ counts, u32 tiles, u32* __restrict__ partial) {
+	if (!plan->pass[pass].active) return;
+	const u32 d = threadIdx.x;
+	u32 c[kScanTiles];
+	load_rows(counts, blockIdx.x, tiles, d, c);
+	u32 s = 0;
+#pragma unroll
+	for (u32 i = 0; i < kScanTiles; ++i) s += c[i];
+	partial[blockIdx.x * kRadixMax + d] = s;
+}
+
+// One block: partial[blk][d] <- (#keys with a smaller digit) +
+// (#keys with digit d in blocks < blk), i.e. the offset of block blk's first
+// key of digit d.
+__global__ __launch_bounds__(kBlock) void satradix_scan_partials(const SortPlan* __restrict__ plan,
+	u32 pass, u32* __restrict__ partial, u32 nblk) {
+	__shared__ u32 s_wsum[kWaves];
+	if (!plan->pass[pass].active) return;
+	constexpr u32 kChunk = 32;  // rows loaded per round (independent loads in flight)
+	const u32 d = threadIdx.x;
+	u32 total = 0;
+	for (u32 q0 = 0; q0 < nblk; q0 += kChunk) {
+		u32 v[kChunk];
+#pragma unroll
+		for (u32 i = 0; i < kChunk; ++i) v[i] = q0 + i < nblk ? partial[(q0 + i) * kRadixMax + d] : 0u;
+#pragma unroll
+		for (u32 i = 0; i < kChunk; ++i) total += v[i];
+	}
+	u32 run = block_exclusive_sum<kBlock>(total, s_wsum, static_cast<u32*>(nullptr));
+	for (u32 q0 = 0; q0 < nblk; q0 += kChunk) {
+		u32 v[kChunk];
+#pragma unroll
+		for (u32 i = 0; i < kChunk; ++i) v[i] = q0 + i < nblk ? partial[(q0 + i) * kRadixMax + d] : 0u;
+#pragma unroll
+		for (u32 i = 0; i < kChunk; ++i) {
+			if (q0 + i < nblk) partial[(q0 + i) * kRadixMax + d] = run;
+			run += v[i];
+		}
+	}
+}
+
+// Turns the tile counts into global output offsets, in place:
+// counts[t][d] <- (#keys with a smaller digit) + (#keys with digit d in tiles < t).
+__global__ __launch_bounds__(kBlock) void satradix_scan(const SortPlan* __restrict__ plan,
+	u32 pass, u32* __restrict__ counts, u32 tiles, const u32* __restrict__ partial) {
+	if (!plan->pass[pass].active) return;
+	const u32 d = threadIdx.x;
+	u32 c[kScanTiles];
+	load_rows(counts, blockIdx.x, tiles, d, c);
+	u32 run = partial[blockIdx.x * kRadixMax + d];
+	const u32 t0 = blockIdx.x * kScanTiles;
+#pragma unroll
+	for (u32 i = 0; i < kScanTiles; ++i) {
+		if (t0 + i < tiles) counts[static_cast<size_t>(t0 + i) * kRadixMax + d] = run;
+		run += c[i];
+	}
+}
+
+// One digit pass over one tile: stable ranking with wave ballots, then a
 // scatter staged through LDS so that global writes are contiguous runs.
 //
 // Element order inside a tile is wave-striped: wave w owns elements
 // [w*64*ITEMS, (w+1)*64*ITEMS), item j of lane l is element w*64*ITEMS+j*64+l.
 // Processing items j = 0.. in order with per-wave running digit counts gives
 // each element the number of equal-digit elements before it (stability).
-template <typename K, bool KV, int ITEMS>
-__global__ __launch_bounds__(kBlock) void satradix_scatter(Bufs<K> b,
-	const Meta* __restrict__ meta, u32* __restrict__ tickets, u64* __restrict__ status,
-	u32 pass, u32 n, u32 bits, u32 epoch) {
+// FIXED_BITS = 8 compiles the digit-match loop for the default 8-bit digits;
+// 0 takes the digit width from `bits` (radix option).
+template <typename K, bool KV, int ITEMS, int FIXED_BITS>
+__global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
+	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits_arg,
+	const u32* __restrict__ offsets) {
+	const u32 bits = FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits_arg;
 	constexpr u32 kTile = kBlock * ITEMS;
 	constexpr u32 kStageBytes = kTile * (sizeof(K) > 4 ? sizeof(K) : (KV ? 4 : sizeof(K)));
 	__shared__ __attribute__((aligned(16))) unsigned char s_stage[kStageBytes];
@@ -173,23 +303,21 @@ __global__ __launch_bounds__(kBlock) void satradix_scatter(Bufs<K> b,
 	__shared__ u32 s_tstart[kRadixMax];
 	__shared__ u32 s_gofs[kRadixMax];
 	__shared__ u32 s_wsum[kWaves];
-	__shared__ u32 s_ticket;
 
-	const PassDesc d = meta->plan.pass[pass];
-	if (!d.active) return;  // uniform: trivial digit, nothing moves
+	const PassDesc d = plan->pass[pass];
+	if (!d.active) return;  // uniform: the digit is equal in every key
+	const u32 tile = xcd_tile(tiles);
+	if (tile >= tiles) return;
 	const u32 tid = threadIdx.x;
-	if (tid == 0) s_ticket = atomicAdd(&tickets[pass], 1u);
 	const u32 lane = lane_id();
 	const u32 w = wave_id();
 	for (u32 i = lane; i < kRadixMax; i += kWave) s_whist[w][i] = 0;
-	__syncthreads();
-	const u32 tile = s_ticket;
 	const u32 tile_base = tile * kTile;
-	const u32 nbins = 1u << bits;
-	const u32 mask = nbins - 1u;
+	const u32 mask = (1u << bits) - 1u;
 	const u32 shift = d.shift;
 	const K* __restrict__ src = b.k[d.src];
 	K* __restrict__ dst = b.k[d.dst];
+	const u32 goff = offsets[static_cast<size_t>(tile) * kRadixMax + tid];
 
 	// Load (coalesced: 64 consecutive elements per wave instruction).
 	const u32 wbase = tile_base + w * kWave * ITEMS;
@@ -197,17 +325,18 @@ __global__ __launch_bounds__(kBlock) void satradix_scatter(Bufs<K> b,
 	u32 v[ITEMS];
 #pragma unroll
 	for (int j = 0; j < ITEMS; ++j) {
-		u32 idx = wbase + j * kWave + lane;
+		const u32 idx = wbase + j * kWave + lane;
 		k[j] = idx < n ? src[idx] : K(0);
 	}
 	if (KV) {
 		const u32* __restrict__ vsrc = b.v[d.src];
 #pragma unroll
 		for (int j = 0; j < ITEMS; ++j) {
-			u32 idx = wbase + j * kWave + lane;
+			const u32 idx = wbase + j * kWave + lane;
 			v[j] = idx < n ? vsrc[idx] : 0u;
 		}
 	}
+	__syncthreads();
 
 	// Rank inside the wave.
 	u32 rank[ITEMS];
@@ -216,7 +345,9 @@ __global__ __launch_bounds__(kBlock) void satradix_scatter(Bufs<K> b,
 		const bool valid = (wbase + j * kWave + lane) < n;
 		const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
 		u64 peers = ballot(valid);
-		for (u32 bt = 0; bt < bits; ++bt) {
+#pragma unroll
+		for (u32 bt = 0; bt < (FIXED_BITS ? FIXED_BITS : 8); ++bt) {
+			if (!FIXED_BITS && bt >= bits) break;
 			const bool bit = (dig >> bt) & 1u;
 			const u64 bb = ballot(bit);
 			peers &= bit ? bb : ~bb;
@@ -231,43 +362,17 @@ __global__ __launch_bounds__(kBlock) void satradix_scatter(Bufs<K> b,
 	}
 	__syncthreads();
 
-	// Per-digit tile totals, per-wave exclusive offsets, aggregate publish.
+	// Per-digit tile totals and per-wave exclusive offsets.
 	u32 total = 0;
-	{
-		u32 run = 0;
 #pragma unroll
-		for (int ww = 0; ww < kWaves; ++ww) {
-			u32 c = s_whist[ww][tid];
-			s_whist[ww][tid] = run;
-			run += c;
-		}
-		total = run;
-		if (tid < nbins)
-			status_store(&status[static_cast<size_t>(tile) * kRadixMax + tid], epoch,
-				tile == 0 ? kFlagPrefix : kFlagAggregate, total);
+	for (int ww = 0; ww < kWaves; ++ww) {
+		const u32 c = s_whist[ww][tid];
+		s_whist[ww][tid] = total;
+		total += c;
 	}
 	const u32 tstart = block_exclusive_sum<kBlock>(total, s_wsum, static_cast<u32*>(nullptr));
 	s_tstart[tid] = tstart;
-
-	// Decoupled look-back: exclusive prefix of this digit over earlier tiles.
-	u32 excl = 0;
-	if (tile > 0 && tid < nbins) {
-		u32 pred = tile - 1;
-		for (;;) {
-			const u64 word = status_load(&status[static_cast<size_t>(pred) * kRadixMax + tid]);
-			const u32 f = status_flag(word, epoch);
-			if (f == 0) {
-				__builtin_amdgcn_s_sleep(1);
-				continue;
-			}
-			excl += static_cast<u32>(word);
-			if (f == kFlagPrefix) break;
-			--pred;
-		}
-		status_store(&status[static_cast<size_t>(tile) * kRadixMax + tid], epoch, kFlagPrefix,
-			excl + total);
-	}
-	if (tid < nbins) s_gofs[tid] = meta->base[pass * kRadixMax + tid] + excl - tstart;
+	s_gofs[tid] = goff - tstart;
 	__syncthreads();
 
 	// Stage keys in digit order in LDS.
@@ -316,14 +421,13 @@ __global__ __launch_bounds__(kBlock) void satradix_scatter(Bufs<K> b,
 
 // ---------------------------------------------------------------------------
 template <typename K, bool KV>
-__global__ __launch_bounds__(256) void satradix_copy(Bufs<K> b, const Meta* __restrict__ meta,
+__global__ __launch_bounds__(256) void satradix_copy(Bufs<K> b, const SortPlan* __restrict__ plan,
 	u32 n) {
-	const SortPlan& pl = meta->plan;
-	if (!pl.copy_active) return;
-	const K* __restrict__ ks = b.k[pl.copy_src];
-	K* __restrict__ kd = b.k[pl.copy_dst];
-	const u32* __restrict__ vs = b.v[pl.copy_src];
-	u32* __restrict__ vd = b.v[pl.copy_dst];
+	if (!plan->copy_active) return;
+	const K* __restrict__ ks = b.k[plan->copy_src];
+	K* __restrict__ kd = b.k[plan->copy_dst];
+	const u32* __restrict__ vs = b.v[plan->copy_src];
+	u32* __restrict__ vd = b.v[plan->copy_dst];
 	const u32 stride = gridDim.x * blockDim.x;
 	for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		kd[i] = ks[i];
@@ -334,21 +438,16 @@ __global__ __launch_bounds__(256) void satradix_copy(Bufs<K> b, const Meta* __re
 // ---------------------------------------------------------------------------
 // Host side.
 
-constexpr int kItemsDefault = 16;
+constexpr int kItemsDefault = 32;
 
 struct SatradixData {
 	u32 bits = 8;
 	std::string scan_type = "blelloch";
 	Scratch aux_keys;
 	Scratch aux_vals;
-	Scratch status;
-	Scratch meta;
-	u32 epoch = 0;
+	Scratch counts;  // tiles x 256 digit counts, then offsets
+	Scratch meta;    // plan + per-tile AND/OR + partial sums
 	CCLQueue* last_queue = nullptr;
-	SatradixData() {
-		status.zero_on_alloc = true;
-		meta.zero_on_alloc = true;
-	}
 };
 
 inline SatradixData* sdata(CloSort* s) { return static_cast<SatradixData*>(s->data); }
@@ -361,13 +460,20 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	const u32 passes = (keybits + bits - 1) / bits;
 	const bool in_place = (kout == nullptr || kout == kin);
 	const u32 tile = kBlock * ITEMS;
-	const size_t tiles = div_ceil(n, tile);
+	const u32 tiles = static_cast<u32>(div_ceil(n, tile));
+	const u32 nblk = static_cast<u32>(div_ceil(tiles, kScanTiles));
+	const size_t plan_bytes = (sizeof(SortPlan) + 255) / 256 * 256;
+	const size_t andor_bytes = (2 * static_cast<size_t>(tiles) * sizeof(K) + 255) / 256 * 256;
 	if (!sd->aux_keys.ensure(n * sizeof(K), err)) return false;
 	if (KV && !sd->aux_vals.ensure(n * sizeof(u32), err)) return false;
-	if (!sd->status.ensure(tiles * kRadixMax * sizeof(u64), err)) return false;
-	if (!sd->meta.ensure(sizeof(Meta), err)) return false;
-	Meta* meta = static_cast<Meta*>(sd->meta.ptr);
-	u64* status = static_cast<u64*>(sd->status.ptr);
+	if (!sd->counts.ensure(static_cast<size_t>(tiles) * kRadixMax * sizeof(u32), err)) return false;
+	if (!sd->meta.ensure(plan_bytes + andor_bytes + static_cast<size_t>(nblk) * kRadixMax * 4, err))
+		return false;
+	char* mb = static_cast<char*>(sd->meta.ptr);
+	SortPlan* plan = reinterpret_cast<SortPlan*>(mb);
+	K* andor = reinterpret_cast<K*>(mb + plan_bytes);
+	u32* partial = reinterpret_cast<u32*>(mb + plan_bytes + andor_bytes);
+	u32* counts = static_cast<u32*>(sd->counts.ptr);
 
 	Bufs<K> b;
 	b.k[kIn] = kin;
@@ -379,37 +485,56 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	const u32 nn = static_cast<u32>(n);
 	hipStream_t st = q->stream;
 
+	const u32 grid = xcd_grid(tiles);
 	{
-		Launch L(q, CLO_SORT_SATRADIX_KNAME_HISTOGRAM);
-		u32 grid = static_cast<u32>(div_ceil(n, kHistBlock * 16));
-		if (grid > 2048) grid = 2048;
-		if (grid == 0) grid = 1;
-		satradix_histogram<K><<<grid, kHistBlock, 0, st>>>(kin, nn, meta->hist, passes, bits);
+		Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
+		satradix_upsweep<K, ITEMS><<<grid, kBlock, 0, st>>>(b, plan, 0, nn, tiles, bits, counts,
+			andor, 1);
 		if (!L.done(err)) return false;
 	}
 	{
-		Launch L(q, CLO_SORT_SATRADIX_KNAME_OFFSETS);
-		satradix_offsets<<<1, kBlock, 0, st>>>(meta, nn, passes, bits, in_place ? 1u : 0u);
+		Launch L(q, CLO_SORT_SATRADIX_KNAME_PLAN);
+		satradix_plan<K><<<1, kBlock, 0, st>>>(plan, andor, tiles, passes, bits, in_place ? 1u : 0u);
 		if (!L.done(err)) return false;
 	}
 	for (u32 p = 0; p < passes; ++p) {
-		if (++sd->epoch >= (1u << 30)) {
-			// Epoch space exhausted: clear the status words once and restart.
-			if (hip_failed(hipStreamSynchronize(st), err, "hipStreamSynchronize") ||
-				hip_failed(hipMemset(sd->status.ptr, 0, sd->status.size), err, "hipMemset"))
-				return false;
-			sd->epoch = 1;
+		if (p > 0) {
+			Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
+			satradix_upsweep<K, ITEMS><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles, bits,
+				counts, andor, 0);
+			if (!L.done(err)) return false;
 		}
-		Launch L(q, CLO_SORT_SATRADIX_KNAME_SCATTER);
-		satradix_scatter<K, KV, ITEMS><<<static_cast<u32>(tiles), kBlock, 0, st>>>(b, meta,
-			meta->tickets, status, p, nn, bits, sd->epoch);
-		if (!L.done(err)) return false;
+		{
+			Launch L(q, CLO_SORT_SATRADIX_KNAME_REDUCE);
+			satradix_reduce<<<nblk, kBlock, 0, st>>>(plan, p, counts, tiles, partial);
+			if (!L.done(err)) return false;
+		}
+		{
+			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
+			satradix_scan_partials<<<1, kBlock, 0, st>>>(plan, p, partial, nblk);
+			if (!L.done(err)) return false;
+		}
+		{
+			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
+			satradix_scan<<<nblk, kBlock, 0, st>>>(plan, p, counts, tiles, partial);
+			if (!L.done(err)) return false;
+		}
+		{
+			Launch L(q, CLO_SORT_SATRADIX_KNAME_DOWNSWEEP);
+			if (bits == 8)
+				satradix_downsweep<K, KV, ITEMS, 8><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles,
+					bits, counts);
+			else
+				satradix_downsweep<K, KV, ITEMS, 0><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles,
+					bits, counts);
+			if (!L.done(err)) return false;
+		}
 	}
 	{
 		Launch L(q, CLO_SORT_SATRADIX_KNAME_COPY);
 		u32 grid = static_cast<u32>(div_ceil(n, 256 * 8));
 		if (grid > 4096) grid = 4096;
-		satradix_copy<K, KV><<<grid, 256, 0, st>>>(b, meta, nn);
+		satradix_copy<K, KV><<<grid, 256, 0, st>>>(b, plan, nn);
 		if (!L.done(err)) return false;
 	}
 	sd->last_queue = q;
@@ -515,7 +640,7 @@ const char* satradix_init(CloSort* sorter, const char* options, GError** err) {
 		return nullptr;
 	}
 	sd->bits = clo_tzc(static_cast<int>(radix));
-	return "satradix (gfx950: histogram + offsets + onesweep scatter)";
+	return "satradix (gfx950: upsweep + counter scan + downsweep per digit pass)";
 }
 
 void satradix_finalize(CloSort* sorter) {
@@ -556,13 +681,17 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	(void)numel;
 	const size_t ks = clo_type_sizeof(sorter->elem_type);
 	switch (i) {
-	case CLO_SORT_SATRADIX_KIDX_HISTOGRAM:
-		return kHistLds * 4;
-	case CLO_SORT_SATRADIX_KIDX_OFFSETS:
-		return (kWaves + kMaxPasses) * 4;
-	case CLO_SORT_SATRADIX_KIDX_SCATTER:
+	case CLO_SORT_SATRADIX_KIDX_UPSWEEP:
+		return kWaves * kRadixMax * 4 + 2 * kWaves * ks;
+	case CLO_SORT_SATRADIX_KIDX_PLAN:
+		return 2 * kWaves * ks;
+	case CLO_SORT_SATRADIX_KIDX_REDUCE:
+		return 0;
+	case CLO_SORT_SATRADIX_KIDX_SCAN:
+		return kWaves * 4;
+	case CLO_SORT_SATRADIX_KIDX_DOWNSWEEP:
 		return kBlock * kItemsDefault * (ks > 4 ? ks : 4) + kWaves * kRadixMax * 4 +
-			2 * kRadixMax * 4 + kWaves * 4 + 4;
+			2 * kRadixMax * 4 + kWaves * 4;
 	case CLO_SORT_SATRADIX_KIDX_COPY:
 		return 0;
 	}
@@ -583,8 +712,8 @@ cl_uint clo_satradix_last_passes(CloSort* sorter, GError** err) {
 	SatradixData* sd = sdata(sorter);
 	if (!sd || !sd->meta.ptr || !sd->last_queue) return 0;
 	u32 executed = 0;
-	Meta* meta = static_cast<Meta*>(sd->meta.ptr);
-	if (hip_failed(hipMemcpyAsync(&executed, &meta->plan.executed, sizeof(u32),
+	SortPlan* plan = static_cast<SortPlan*>(sd->meta.ptr);
+	if (hip_failed(hipMemcpyAsync(&executed, &plan->executed, sizeof(u32),
 			hipMemcpyDeviceToHost, sd->last_queue->stream), err, "hipMemcpyAsync") ||
 		hip_failed(hipStreamSynchronize(sd->last_queue->stream), err, "hipStreamSynchronize"))
 		return 0;

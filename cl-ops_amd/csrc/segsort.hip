@@ -87,6 +87,7 @@ struct SegSmem {
 	u32 v[KV ? kCap : 1];
 	u16 sid[kCap];
 	u32 lpos[kW + 1];
+	u32 segj[kW];
 	u32 hist[sizeof(K)][256];
 	u32 base[256];
 	u32 red[2 * kRBWaves];
@@ -216,21 +217,28 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	}
 	__syncthreads();
 	const u32 j0 = s.misc[0];
-	const u32 m = s.misc[1];  // <= W segments start in the window
-	// LDS start of each included segment (excluded ones take no room).
-	u32 carry = 0;
-	for (u32 r0 = 0; r0 < m; r0 += kRB) {
+	const u32 mall = s.misc[1];  // segments starting in the window (empty ones included)
+	// Compact the non-empty small segments (distinct starts inside a window of
+	// W elements, so at most W of them): segj[r] = segment index, lpos[r] =
+	// its start in the LDS tile.
+	u32 carry = 0, m = 0;
+	for (u32 r0 = 0; r0 < mall; r0 += kRB) {
 		const u32 r = r0 + threadIdx.x;
 		u32 len = 0;
-		if (r < m) {
-			const u32 j = j0 + r;
+		const u32 j = j0 + r;
+		if (r < mall) {
 			len = seg_end(a.off, j, a.nseg, a.numel) - a.off[j];
 			if (len > W) len = 0;
 		}
-		u32 tot;
+		u32 tot, cnt;
 		const u32 ex = block_exclusive_sum<kRB>(len, s.rs.wsum, &tot);
-		if (r < m) s.lpos[r] = carry + ex;
+		const u32 exc = block_exclusive_sum<kRB>(len ? 1u : 0u, s.rs.wsum, &cnt);
+		if (len) {
+			s.segj[m + exc] = j;
+			s.lpos[m + exc] = carry + ex;
+		}
 		carry += tot;
+		m += cnt;
 	}
 	if (threadIdx.x == 0) s.lpos[m] = carry;
 	__syncthreads();
@@ -246,7 +254,7 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 			else h = mid;
 		}
 		while (l + 1 < m && s.lpos[l + 1] <= i) ++l;
-		const u32 g = a.off[j0 + l] + (i - s.lpos[l]);
+		const u32 g = a.off[s.segj[l]] + (i - s.lpos[l]);
 		s.k[i] = a.kin[g];
 		if (KV) s.v[i] = a.vin[g];
 		s.sid[i] = static_cast<u16>(l);
@@ -261,7 +269,7 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	}
 	for (u32 i = threadIdx.x; i < n; i += kRB) {
 		const u32 r = multi ? s.sid[i] : 0u;
-		const u32 g = a.off[j0 + r] + (i - s.lpos[r]);
+		const u32 g = a.off[s.segj[r]] + (i - s.lpos[r]);
 		a.kout[g] = s.k[i];
 		if (KV) a.vout[g] = s.v[i];
 	}

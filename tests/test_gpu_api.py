@@ -27,9 +27,9 @@ def test_introspection(clops, ctx):
     L = clops.lib()
     assert L.clo_sort_get_element_size(s.ptr) == 8
     assert L.clo_sort_get_key_type(s.ptr) == clops.CLO_ULONG
-    assert s.kernel_names() == ["satradix_histogram", "satradix_offsets", "satradix_scatter",
-                                "satradix_copy"]
-    assert clops._call(L.clo_sort_get_localmem_usage, s.ptr, 2, 256, 1000) > 0
+    assert s.kernel_names() == ["satradix_upsweep", "satradix_plan", "satradix_reduce",
+                                "satradix_scan", "satradix_downsweep", "satradix_copy"]
+    assert clops._call(L.clo_sort_get_localmem_usage, s.ptr, 4, 256, 1000) > 0
     sc = clops.Scanner(ctx, np.uint32, np.uint64)
     assert L.clo_scan_get_sum_size(sc.ptr) == 8
     assert clops._call(L.clo_scan_get_kernel_name, sc.ptr, 0) == b"blelloch_lookback_scan"
@@ -43,9 +43,9 @@ def test_profiling_records_kernels(clops, oracle, ctx):
     s = clops.Sorter(ctx, "satradix", np.uint64)
     q.prof_reset()
     s.sort_device(q, b, None, n)
-    cnt, ms = q.prof("satradix_scatter")
+    cnt, ms = q.prof("satradix_downsweep")
     assert cnt == 8 and ms > 0
-    assert "satradix_histogram" in q.prof_names()
+    assert "satradix_upsweep" in q.prof_names()
 
 
 def test_alloc_failure_reported(clops, oracle, ctx, queue):
