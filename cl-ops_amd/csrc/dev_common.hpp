@@ -27,6 +27,23 @@ __device__ __forceinline__ u32 count_below(u64 mask) {
 
 __device__ __forceinline__ u64 ballot(bool p) { return __ballot(p); }
 
+// ---- XCD-aware tile mapping -------------------------------------------------
+// MI355X dispatches consecutive blocks round-robin over its 8 XCDs, each with
+// its own L2.  Kernels whose tiles are independent map block b to tile
+// (b % 8) * per + b / 8 so that each XCD works through a contiguous range of
+// tiles: neighbouring tiles then share an L2 (their partial cache lines of
+// the same output runs merge there instead of being written back from 8
+// caches).  Any mapping is correct; this one is the fast one.  Launch
+// xcd_grid(tiles) blocks; blocks whose tile is >= tiles return.
+constexpr u32 kXcds = 8;
+
+__device__ __forceinline__ u32 xcd_tile(u32 tiles) {
+	const u32 per = (tiles + kXcds - 1) / kXcds;
+	return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+inline u32 xcd_grid(u32 tiles) { return kXcds * ((tiles + kXcds - 1) / kXcds); }
+
 // Inclusive wave-wide prefix sum.
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_sum(T x) {

@@ -40,22 +40,8 @@ constexpr int kBlock = 256;  // == kRadixMax: one thread per digit bin
 constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxPasses = 64;
 constexpr u32 kScanTiles = 64;  // tiles per block of the counter reduce/scan
-constexpr u32 kXcds = 8;        // MI355X: blocks go round-robin to 8 XCDs (L2 domains)
 
 enum BufSel : u32 { kIn = 0, kAux = 1, kOut = 2 };
-
-// Tile of a block.  Consecutive block indices run on different XCDs, so
-// mapping them to consecutive tiles would make neighbouring tiles write
-// partial cache lines of the same digit runs from 8 different L2 caches.
-// Giving each XCD a contiguous range of tiles keeps a digit run's writes in
-// one L2.  Any mapping is correct (tiles never wait on each other); this one
-// is the fast one.  Grid = kXcds * per; blocks with tile >= tiles exit.
-__device__ __forceinline__ u32 xcd_tile(u32 tiles) {
-	const u32 per = (tiles + kXcds - 1) / kXcds;
-	return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-}
-
-inline u32 xcd_grid(u32 tiles) { return kXcds * ((tiles + kXcds - 1) / kXcds); }
 
 struct PassDesc {
 	u32 active;

@@ -1,16 +1,22 @@
 // Scanner factory (CL_Ops' src/cl_ops/scan/clo_scan_abstract.c:74-567) and
-// the "blelloch" exclusive scan as one single-pass decoupled look-back kernel
-// (Merrill & Garland, "Single-pass Parallel Prefix Scan with Decoupled
-// Look-back", NVIDIA TR 2016) for gfx950.
+// the "blelloch" exclusive scan for gfx950.
 //
 // Result contract (clo_scan_blelloch.cl:49-211): out[i] = sum_{j<i} (SUM)in[j]
 // with the element converted to the sum type as C does (sign extension for
 // signed elements) and the sum taken modulo 2^(8*sizeof(SUM)).
 //
-// One launch reads the input once and writes the output once.  Tiles take a
-// ticket from an atomic counter, so a tile's predecessors are always running
-// or done; the tile's aggregate and inclusive prefix are published in 64-bit
-// status words tagged with a per-launch epoch (no clearing between launches).
+// Three kernels, the reference's three-phase structure (workgroupScan,
+// workgroupSumsScan, addWorkgroupSums: clo_scan_blelloch.c:78-214) reordered
+// as reduce-then-scan so that the data is written once:
+//   blelloch_tile_reduce  per tile of 4096 elements: its sum (segmented: its
+//                         (segment-start flag, sum after the last start))
+//   blelloch_tiles_scan   one block: exclusive scan of the tile sums
+//   blelloch_tile_scan    per tile: scan in LDS, plus the tile's carry-in
+// The data is read twice and written once; no block ever waits for another.
+// (A single-pass decoupled look-back scan was measured on MI355X at 20-30% of
+// HBM peak: its cross-tile look-back chain, not the data, set the time.)
+// Segmented scans take segment starts from a sorted offsets array: the sum
+// restarts at every start.
 
 #include <cstring>
 
@@ -25,70 +31,9 @@ namespace {
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr u32 kScanTile = kScanBlock * kScanItems;
+constexpr int kTilesBlock = 1024;  // threads of the tile-sum scan
 
 __host__ __device__ constexpr u32 pad(u32 i) { return i + (i >> 5); }
-
-// Status of 32-bit sums: one word per tile.  64-bit sums: two words (low and
-// high halves) with the same flag; a reader accepts them only when both carry
-// the current epoch and the same flag.
-template <typename Acc>
-struct Status;
-
-template <>
-struct Status<u32> {
-	static constexpr int kWords = 1;
-	__device__ static void store(u64* s, u32 tile, u32 epoch, u32 flag, u32 v) {
-		status_store(&s[tile], epoch, flag, v);
-	}
-	__device__ static u32 load(const u64* s, u32 tile, u32 epoch, u32* v) {
-		u64 w = status_load(&s[tile]);
-		*v = static_cast<u32>(w);
-		return status_flag(w, epoch);
-	}
-};
-
-template <>
-struct Status<u64> {
-	static constexpr int kWords = 2;
-	__device__ static void store(u64* s, u32 tile, u32 epoch, u32 flag, u64 v) {
-		status_store(&s[2 * static_cast<size_t>(tile)], epoch, flag, static_cast<u32>(v));
-		status_store(&s[2 * static_cast<size_t>(tile) + 1], epoch, flag,
-			static_cast<u32>(v >> 32));
-	}
-	__device__ static u32 load(const u64* s, u32 tile, u32 epoch, u64* v) {
-		u64 lo = status_load(&s[2 * static_cast<size_t>(tile)]);
-		u64 hi = status_load(&s[2 * static_cast<size_t>(tile) + 1]);
-		u32 f = status_flag(lo, epoch);
-		if (f != status_flag(hi, epoch)) return 0;
-		*v = (hi << 32) | (lo & 0xFFFFFFFFull);
-		return f;
-	}
-};
-
-// Exclusive prefix of this tile over all earlier tiles, computed by wave 0:
-// each round inspects the 64 predecessors below `hi` at once.
-template <typename Acc>
-__device__ Acc lookback(const u64* status, u32 tile, u32 epoch) {
-	const u32 lane = lane_id();
-	Acc excl = 0;
-	int64_t top = static_cast<int64_t>(tile) - 1;
-	for (;;) {
-		const int64_t idx = top - lane;
-		Acc v = 0;
-		u32 f = kFlagPrefix;  // below tile 0: an empty prefix
-		if (idx >= 0) f = Status<Acc>::load(status, static_cast<u32>(idx), epoch, &v);
-		const u64 pmask = ballot(f == kFlagPrefix);
-		const u32 lim = pmask ? static_cast<u32>(__builtin_ctzll(pmask)) : 63u;
-		const u64 need = lim >= 63 ? ~0ull : ((2ull << lim) - 1ull);
-		if (ballot(f == 0) & need) {
-			__builtin_amdgcn_s_sleep(1);
-			continue;
-		}
-		excl += wave_sum(lane <= lim ? v : Acc(0));
-		if (pmask) return excl;
-		top -= kWave;
-	}
-}
 
 // (flag, value) pair of the segmented scan: flag = a segment starts inside,
 // value = sum since the last segment start (or of everything without one).
@@ -103,11 +48,12 @@ __device__ __forceinline__ SegPair<Acc> seg_combine(SegPair<Acc> a, SegPair<Acc>
 	return {a.f | b.f, b.f ? b.v : a.v + b.v};
 }
 
-// Exclusive block scan of (flag, value) pairs; *total = all threads combined.
-template <typename Acc>
+// Exclusive block scan of (flag, value) pairs over NT threads; *total = all
+// threads combined.  Ends with a barrier.
+template <int NT, typename Acc>
 __device__ SegPair<Acc> block_exclusive_segscan(SegPair<Acc> mine, SegPair<Acc>* s_w,
 	SegPair<Acc>* total) {
-	constexpr int W = kScanBlock / kWave;
+	constexpr int W = NT / kWave;
 	const u32 lane = lane_id();
 	SegPair<Acc> inc = mine;
 #pragma unroll
@@ -145,38 +91,27 @@ __device__ __forceinline__ u32 first_at_or_after(const u32* off, u32 nseg, u64 x
 	return lo;
 }
 
-// SEG = false: plain exclusive scan.  SEG = true: the sum restarts at every
-// segment start off[j] (the per-segment exclusive scans of a batch).
-template <typename In, typename Acc, int OUTW, bool SEG>
-__global__ __launch_bounds__(kScanBlock) void blelloch_lookback_scan(const In* __restrict__ in,
-	void* __restrict__ out, u64 n, u64* __restrict__ status, u32* __restrict__ ticket,
-	u32 ticket_base, u32 epoch, const u32* __restrict__ off, u32 nseg) {
-	__shared__ Acc s_t[pad(kScanTile) + 1];
-	__shared__ Acc s_wsum[kScanBlock / kWave];
-	__shared__ SegPair<Acc> s_pw[kScanBlock / kWave];
-	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
-	__shared__ Acc s_prefix;
-	__shared__ u32 s_tile;
-	__shared__ u32 s_j0;
+// Bitmap (in LDS) of the segment starts inside the tile at `base`.  Contains
+// barriers.
+__device__ void mark_heads(u32* s_head, u32* s_j0, const u32* __restrict__ off, u32 nseg,
+	u64 base) {
 	const u32 tid = threadIdx.x;
-	if (tid == 0) s_tile = atomicAdd(ticket, 1u) - ticket_base;
-	if (SEG)
-		for (u32 i = tid; i < kScanTile / 32; i += kScanBlock) s_head[i] = 0;
+	for (u32 i = tid; i < kScanTile / 32; i += kScanBlock) s_head[i] = 0;
+	if (tid == 0) *s_j0 = first_at_or_after(off, nseg, base);
 	__syncthreads();
-	const u32 tile = s_tile;
-	const u64 base = static_cast<u64>(tile) * kScanTile;
-	if (SEG) {
-		if (tid == 0) s_j0 = first_at_or_after(off, nseg, base);
-		__syncthreads();
-		for (u32 j = s_j0 + tid; j < nseg; j += kScanBlock) {
-			const u64 o = off[j];
-			if (o >= base + kScanTile) break;
-			const u32 p = static_cast<u32>(o - base);
-			atomicOr(&s_head[p >> 5], 1u << (p & 31));
-		}
+	for (u32 j = *s_j0 + tid; j < nseg; j += kScanBlock) {
+		const u64 o = off[j];
+		if (o >= base + kScanTile) break;
+		const u32 p = static_cast<u32>(o - base);
+		atomicOr(&s_head[p >> 5], 1u << (p & 31));
 	}
+	__syncthreads();
+}
 
-	// Striped (coalesced) load, converted to the sum type.
+// Stages the tile (striped, coalesced loads) in LDS converted to the sum type.
+template <typename In, typename Acc>
+__device__ __forceinline__ void stage_tile(Acc* s_t, const In* __restrict__ in, u64 base, u64 n) {
+	const u32 tid = threadIdx.x;
 #pragma unroll
 	for (int j = 0; j < kScanItems; ++j) {
 		const u32 i = j * kScanBlock + tid;
@@ -184,68 +119,130 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_lookback_scan(const In* _
 		s_t[pad(i)] = g < n ? static_cast<Acc>(in[g]) : Acc(0);
 	}
 	__syncthreads();
+}
 
-	// Blocked: thread tid owns tile elements [tid*ITEMS, (tid+1)*ITEMS).
-	Acc x[kScanItems];
-	u32 heads = 0;  // bit j: item j starts a segment
+// Thread's blocked items: (any segment start, sum after the last start).
+template <typename Acc, bool SEG>
+__device__ __forceinline__ SegPair<Acc> thread_items(const Acc* s_t, const u32* s_head,
+	u64* heads_out) {
+	const u32 tid = threadIdx.x;
+	u64 heads = 0;
 	Acc run = 0;
-	u32 any_head = 0;
+	u32 any = 0;
 #pragma unroll
 	for (int j = 0; j < kScanItems; ++j) {
-		x[j] = s_t[pad(tid * kScanItems + j)];
-		if (SEG) {
-			const u32 p = tid * kScanItems + j;
-			if ((s_head[p >> 5] >> (p & 31)) & 1u) {
-				heads |= 1u << j;
-				run = 0;
-				any_head = 1;
-			}
+		const u32 p = tid * kScanItems + j;
+		const Acc x = s_t[pad(p)];
+		if (SEG && ((s_head[p >> 5] >> (p & 31)) & 1u)) {
+			heads |= 1ull << j;
+			run = 0;
+			any = 1;
 		}
-		run += x[j];
+		run += x;
 	}
-	Acc tile_total;
-	Acc tpre;
-	u32 carry_known = 0;  // SEG: a segment starts before this thread's items in the tile
-	u32 tile_head = 0;
+	*heads_out = heads;
+	return {any, run};
+}
+
+template <typename In, typename Acc, bool SEG>
+__global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce(const In* __restrict__ in, u64 n,
+	u32 tiles, const u32* __restrict__ off, u32 nseg, Acc* __restrict__ tsum,
+	u32* __restrict__ tflag) {
+	__shared__ Acc s_t[SEG ? pad(kScanTile) + 1 : 1];
+	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
+	__shared__ Acc s_wsum[kScanBlock / kWave];
+	__shared__ SegPair<Acc> s_pw[kScanBlock / kWave];
+	__shared__ u32 s_j0;
+	const u32 tile = xcd_tile(tiles);
+	if (tile >= tiles) return;
+	const u64 base = static_cast<u64>(tile) * kScanTile;
+	const u32 tid = threadIdx.x;
+	if (!SEG) {
+		// Order does not matter for a plain sum: striped loads, block reduce.
+		Acc s = 0;
+#pragma unroll
+		for (int j = 0; j < kScanItems; ++j) {
+			const u64 g = base + j * kScanBlock + tid;
+			if (g < n) s += static_cast<Acc>(in[g]);
+		}
+		Acc tot;
+		block_exclusive_sum<kScanBlock>(s, s_wsum, &tot);
+		if (tid == 0) tsum[tile] = tot;
+		return;
+	}
+	mark_heads(s_head, &s_j0, off, nseg, base);
+	stage_tile<In, Acc>(s_t, in, base, n);
+	u64 heads;
+	const SegPair<Acc> mine = thread_items<Acc, true>(s_t, s_head, &heads);
+	SegPair<Acc> tot;
+	block_exclusive_segscan<kScanBlock, Acc>(mine, s_pw, &tot);
+	if (tid == 0) {
+		tsum[tile] = tot.v;
+		tflag[tile] = tot.f;
+	}
+}
+
+// One block: tsum[t] <- carry into tile t (exclusive, segmented when SEG).
+template <typename Acc, bool SEG>
+__global__ __launch_bounds__(kTilesBlock) void blelloch_tiles_scan(Acc* __restrict__ tsum,
+	const u32* __restrict__ tflag, u32 tiles) {
+	__shared__ Acc s_wsum[kTilesBlock / kWave];
+	__shared__ SegPair<Acc> s_pw[kTilesBlock / kWave];
+	const u32 per = (tiles + kTilesBlock - 1) / kTilesBlock;
+	const u32 t0 = threadIdx.x * per;
+	const u32 t1 = t0 + per < tiles ? t0 + per : tiles;
+	SegPair<Acc> mine = {0u, Acc(0)};
+	for (u32 t = t0; t < t1; ++t) mine = seg_combine(mine, SegPair<Acc>{SEG ? tflag[t] : 0u, tsum[t]});
+	SegPair<Acc> carry, tot;
+	if (SEG) {
+		carry = block_exclusive_segscan<kTilesBlock, Acc>(mine, s_pw, &tot);
+	} else {
+		Acc total;
+		carry = {0u, block_exclusive_sum<kTilesBlock>(mine.v, s_wsum, &total)};
+	}
+	for (u32 t = t0; t < t1; ++t) {
+		const SegPair<Acc> x = {SEG ? tflag[t] : 0u, tsum[t]};
+		tsum[t] = carry.v;
+		carry = seg_combine(carry, x);
+	}
+}
+
+template <typename In, typename Acc, int OUTW, bool SEG>
+__global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan(const In* __restrict__ in,
+	void* __restrict__ out, u64 n, u32 tiles, const Acc* __restrict__ carry_in,
+	const u32* __restrict__ off, u32 nseg) {
+	__shared__ Acc s_t[pad(kScanTile) + 1];
+	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
+	__shared__ Acc s_wsum[kScanBlock / kWave];
+	__shared__ SegPair<Acc> s_pw[kScanBlock / kWave];
+	__shared__ u32 s_j0;
+	const u32 tile = xcd_tile(tiles);
+	if (tile >= tiles) return;
+	const u64 base = static_cast<u64>(tile) * kScanTile;
+	const u32 tid = threadIdx.x;
+	const Acc carry = carry_in[tile];
+	if (SEG) mark_heads(s_head, &s_j0, off, nseg, base);
+	stage_tile<In, Acc>(s_t, in, base, n);
+	u64 heads;
+	const SegPair<Acc> mine = thread_items<Acc, SEG>(s_t, s_head, &heads);
+	Acc acc;
 	if (SEG) {
 		SegPair<Acc> tot;
-		const SegPair<Acc> ex = block_exclusive_segscan<Acc>({any_head, run}, s_pw, &tot);
-		tpre = ex.v;
-		carry_known = ex.f;
-		tile_total = tot.v;
-		tile_head = tot.f;
+		const SegPair<Acc> ex = block_exclusive_segscan<kScanBlock, Acc>(mine, s_pw, &tot);
+		acc = ex.f ? ex.v : carry + ex.v;
 	} else {
-		tpre = block_exclusive_sum<kScanBlock>(run, s_wsum, &tile_total);
+		Acc tot;
+		acc = carry + block_exclusive_sum<kScanBlock>(mine.v, s_wsum, &tot);
 	}
-
-	if (tile == 0) {
-		if (tid == 0) {
-			Status<Acc>::store(status, 0, epoch, kFlagPrefix, tile_total);
-			s_prefix = 0;
-		}
-	} else if (tid < kWave) {
-		// A tile containing a segment start knows its carry-out (the sum after
-		// its last start) without looking back: publish it as a prefix.
-		if (tid == 0)
-			Status<Acc>::store(status, tile, epoch, tile_head ? kFlagPrefix : kFlagAggregate,
-				tile_total);
-		const Acc excl = lookback<Acc>(status, tile, epoch);
-		if (tid == 0) {
-			if (!tile_head) Status<Acc>::store(status, tile, epoch, kFlagPrefix, excl + tile_total);
-			s_prefix = excl;
-		}
-	}
-	__syncthreads();
-
-	Acc acc = carry_known ? tpre : s_prefix + tpre;
 #pragma unroll
 	for (int j = 0; j < kScanItems; ++j) {
-		if (SEG && ((heads >> j) & 1u)) acc = 0;
-		s_t[pad(tid * kScanItems + j)] = acc;
-		acc += x[j];
+		if (SEG && ((heads >> j) & 1ull)) acc = 0;
+		const u32 p = pad(tid * kScanItems + j);
+		const Acc x = s_t[p];
+		s_t[p] = acc;
+		acc += x;
 	}
 	__syncthreads();
-
 #pragma unroll
 	for (int j = 0; j < kScanItems; ++j) {
 		const u32 i = j * kScanBlock + tid;
@@ -261,53 +258,61 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_lookback_scan(const In* _
 }
 
 struct BlellochData {
-	Scratch status;
-	Scratch meta;  // the ticket counter; never reallocated after creation
-	u32 epoch = 0;
-	u32 issued = 0;
-	// Set when a launch failed after tickets were reserved: the counter no
-	// longer matches `issued` and is cleared before the next launch.
-	bool dirty = false;
-	BlellochData() {
-		status.zero_on_alloc = true;
-		meta.zero_on_alloc = true;
-	}
+	Scratch tiles;  // per-tile sums (then carries) and segment flags
 };
 
 struct ScanLaunch {
 	const void* in;
 	void* out;
 	u64 n;
-	u64* status;
-	u32* ticket;
-	u32 base;
-	u32 epoch;
 	u32 tiles;
+	void* tsum;
+	u32* tflag;
 	const u32* off;  // segment starts (segmented scan) or nullptr
 	u32 nseg;
-	hipStream_t st;
+	CCLQueue* q;
 };
 
+template <typename In, typename Acc, int OUTW, bool SEG>
+bool launch_scan3(const ScanLaunch& a, GError** err) {
+	hipStream_t st = a.q->stream;
+	const u32 grid = xcd_grid(a.tiles);
+	const In* in = static_cast<const In*>(a.in);
+	Acc* tsum = static_cast<Acc*>(a.tsum);
+	{
+		Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_REDUCE);
+		blelloch_tile_reduce<In, Acc, SEG><<<grid, kScanBlock, 0, st>>>(in, a.n, a.tiles, a.off,
+			a.nseg, tsum, a.tflag);
+		if (!L.done(err)) return false;
+	}
+	{
+		Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_TILES);
+		blelloch_tiles_scan<Acc, SEG><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles);
+		if (!L.done(err)) return false;
+	}
+	{
+		Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_SCAN);
+		blelloch_tile_scan<In, Acc, OUTW, SEG><<<grid, kScanBlock, 0, st>>>(in, a.out, a.n,
+			a.tiles, tsum, a.off, a.nseg);
+		if (!L.done(err)) return false;
+	}
+	return true;
+}
+
 template <typename In, typename Acc, int OUTW>
-void launch_scan(const ScanLaunch& a) {
-	if (a.off)
-		blelloch_lookback_scan<In, Acc, OUTW, true><<<a.tiles, kScanBlock, 0, a.st>>>(
-			static_cast<const In*>(a.in), a.out, a.n, a.status, a.ticket, a.base, a.epoch, a.off,
-			a.nseg);
-	else
-		blelloch_lookback_scan<In, Acc, OUTW, false><<<a.tiles, kScanBlock, 0, a.st>>>(
-			static_cast<const In*>(a.in), a.out, a.n, a.status, a.ticket, a.base, a.epoch, nullptr,
-			0);
+bool launch_scan(const ScanLaunch& a, GError** err) {
+	return a.off ? launch_scan3<In, Acc, OUTW, true>(a, err) : launch_scan3<In, Acc, OUTW, false>(a, err);
 }
 
 template <typename In>
-bool launch_for_in(size_t sum_size, const ScanLaunch& a) {
+bool launch_for_in(size_t sum_size, const ScanLaunch& a, GError** err) {
 	switch (sum_size) {
-	case 1: launch_scan<In, u32, 1>(a); return true;
-	case 2: launch_scan<In, u32, 2>(a); return true;
-	case 4: launch_scan<In, u32, 4>(a); return true;
-	case 8: launch_scan<In, u64, 8>(a); return true;
+	case 1: return launch_scan<In, u32, 1>(a, err);
+	case 2: return launch_scan<In, u32, 2>(a, err);
+	case 4: return launch_scan<In, u32, 4>(a, err);
+	case 8: return launch_scan<In, u64, 8>(a, err);
 	}
+	set_error(err, kQuarkClo, CLO_ERROR_ARGS, "unsupported sum size %zu", sum_size);
 	return false;
 }
 
@@ -318,7 +323,7 @@ const char* blelloch_init(CloScan* scanner, const char* options, GError** err) {
 		return nullptr;
 	}
 	scanner->data = new BlellochData();
-	return "blelloch (gfx950: single-pass decoupled look-back)";
+	return "blelloch (gfx950: tile reduce + tile-sum scan + tile scan)";
 }
 
 void blelloch_finalize(CloScan* scanner) {
@@ -335,58 +340,35 @@ CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "numel too large");
 		return nullptr;
 	}
-	const size_t words = Status<u64>::kWords;
-	if (!bd->status.ensure(tiles * words * sizeof(u64), err)) return nullptr;
-	if (!bd->meta.ensure(256, err)) return nullptr;
-	if (bd->dirty) {
-		if (hip_failed(hipStreamSynchronize(cq_exec->stream), err, "hipStreamSynchronize") ||
-			hip_failed(hipMemset(bd->meta.ptr, 0, bd->meta.size), err, "hipMemset"))
-			return nullptr;
-		bd->issued = 0;
-		bd->dirty = false;
-	}
-	if (++bd->epoch >= (1u << 30)) {
-		if (hip_failed(hipStreamSynchronize(cq_exec->stream), err, "hipStreamSynchronize") ||
-			hip_failed(hipMemset(bd->status.ptr, 0, bd->status.size), err, "hipMemset"))
-			return nullptr;
-		bd->epoch = 1;
-	}
+	const size_t ss = clo_type_sizeof(scanner->sum_type);
+	const size_t acc = ss == 8 ? 8 : 4;
+	const size_t sum_bytes = (tiles * acc + 255) / 256 * 256;
+	if (!bd->tiles.ensure(sum_bytes + tiles * 4, err)) return nullptr;
 	ScanLaunch a;
-	a.base = bd->issued;
-	bd->issued += static_cast<u32>(tiles);
-	a.status = static_cast<u64*>(bd->status.ptr);
-	a.ticket = static_cast<u32*>(bd->meta.ptr);
 	a.in = data_in->ptr;
 	a.out = data_out->ptr;
 	a.n = numel;
-	a.epoch = bd->epoch;
 	a.tiles = static_cast<u32>(tiles);
+	a.tsum = bd->tiles.ptr;
+	a.tflag = reinterpret_cast<u32*>(static_cast<char*>(bd->tiles.ptr) + sum_bytes);
 	a.off = off;
 	a.nseg = nseg;
-	a.st = cq_exec->stream;
-	const size_t ss = clo_type_sizeof(scanner->sum_type);
-	Launch L(cq_exec, off ? CLO_SCAN_BLELLOCH_KNAME_SEGMENTED : CLO_SCAN_BLELLOCH_KNAME_LOOKBACK);
+	a.q = cq_exec;
 	bool ok = false;
 	switch (scanner->elem_type) {
-	case CLO_CHAR: ok = launch_for_in<int8_t>(ss, a); break;
-	case CLO_UCHAR: ok = launch_for_in<u8>(ss, a); break;
-	case CLO_SHORT: ok = launch_for_in<int16_t>(ss, a); break;
-	case CLO_USHORT: ok = launch_for_in<u16>(ss, a); break;
-	case CLO_INT: ok = launch_for_in<int32_t>(ss, a); break;
-	case CLO_UINT: ok = launch_for_in<u32>(ss, a); break;
-	case CLO_LONG: ok = launch_for_in<int64_t>(ss, a); break;
-	case CLO_ULONG: ok = launch_for_in<u64>(ss, a); break;
-	default: break;
-	}
-	if (!ok) {
-		bd->issued = a.base;
+	case CLO_CHAR: ok = launch_for_in<int8_t>(ss, a, err); break;
+	case CLO_UCHAR: ok = launch_for_in<u8>(ss, a, err); break;
+	case CLO_SHORT: ok = launch_for_in<int16_t>(ss, a, err); break;
+	case CLO_USHORT: ok = launch_for_in<u16>(ss, a, err); break;
+	case CLO_INT: ok = launch_for_in<int32_t>(ss, a, err); break;
+	case CLO_UINT: ok = launch_for_in<u32>(ss, a, err); break;
+	case CLO_LONG: ok = launch_for_in<int64_t>(ss, a, err); break;
+	case CLO_ULONG: ok = launch_for_in<u64>(ss, a, err); break;
+	default:
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "unsupported scan types");
-		return nullptr;
+		break;
 	}
-	if (!L.done(err)) {
-		bd->dirty = true;
-		return nullptr;
-	}
+	if (!ok) return nullptr;
 	return record_event(cq_exec, "blelloch_done", err);
 }
 
@@ -409,7 +391,8 @@ const char* blelloch_get_kernel_name(CloScan* scanner, cl_uint i, GError** err) 
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "kernel index %u out of range", i);
 		return nullptr;
 	}
-	return i == 0 ? CLO_SCAN_BLELLOCH_KNAME_LOOKBACK : CLO_SCAN_BLELLOCH_KNAME_SEGMENTED;
+	static const char* names[] = CLO_SCAN_BLELLOCH_KERNELNAMES;
+	return names[i];
 }
 
 size_t blelloch_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, size_t numel,
@@ -421,7 +404,8 @@ size_t blelloch_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, 
 		return 0;
 	}
 	const size_t acc = clo_type_sizeof(scanner->sum_type) > 4 ? 8 : 4;
-	return (kScanTile + kScanTile / 32 + 1 + kScanBlock / kWave + 1) * acc + 4;
+	if (i == 1) return (kTilesBlock / kWave) * (acc + 8);
+	return (kScanTile + kScanTile / 32 + 1 + 2 * (kScanBlock / kWave)) * acc + kScanTile / 8 + 4;
 }
 
 }  // namespace

@@ -1,9 +1,10 @@
 /*
  * clo_scan_blelloch.h - exclusive scan registered as "blelloch"
- * (src/cl_ops/scan/clo_scan_blelloch.in.h:30-46).  On MI355X it is a
- * single-pass decoupled look-back scan (one read and one write of the data)
- * instead of the reference's three-kernel up/down-sweep.  Any option fails
- * with CLO_ERROR_ARGS, as in the reference (clo_scan_blelloch.c:43-45).
+ * (src/cl_ops/scan/clo_scan_blelloch.in.h:30-46).  On MI355X it is three
+ * kernels like the reference's (per-tile sums, a scan of the tile sums, the
+ * tile scans with their carry), ordered reduce-then-scan: the data is read
+ * twice and written once, and no tile ever waits for another.  Any option
+ * fails with CLO_ERROR_ARGS, as in the reference (clo_scan_blelloch.c:43-45).
  */
 #ifndef CLO_SCAN_BLELLOCH_H_
 #define CLO_SCAN_BLELLOCH_H_
@@ -14,9 +15,12 @@
 extern "C" {
 #endif
 
-#define CLO_SCAN_BLELLOCH_NUM_KERNELS 2
-#define CLO_SCAN_BLELLOCH_KNAME_LOOKBACK "blelloch_lookback_scan"
-#define CLO_SCAN_BLELLOCH_KNAME_SEGMENTED "blelloch_segmented_scan"
+#define CLO_SCAN_BLELLOCH_NUM_KERNELS 3
+#define CLO_SCAN_BLELLOCH_KNAME_REDUCE "blelloch_tile_reduce"
+#define CLO_SCAN_BLELLOCH_KNAME_TILES "blelloch_tiles_scan"
+#define CLO_SCAN_BLELLOCH_KNAME_SCAN "blelloch_tile_scan"
+#define CLO_SCAN_BLELLOCH_KERNELNAMES { CLO_SCAN_BLELLOCH_KNAME_REDUCE, \
+	CLO_SCAN_BLELLOCH_KNAME_TILES, CLO_SCAN_BLELLOCH_KNAME_SCAN }
 
 extern const CloScanImplDef clo_scan_blelloch_def;
 
