@@ -98,6 +98,10 @@ def headline(args, ctx):
         s.sort_kv_device(q, kin, vin, kout, vout, n)
 
     ms = time_steps(q, step, args.steps, args.warmup)
+    q_plain = clops.Queue(ctx)
+    ms_plain = time_steps(q_plain, lambda: s.sort_kv_device(q_plain, kin, vin, kout, vout, n),
+                          args.steps, 1)
+    log("  same steps without per-kernel events: %.3f ms/step" % ms_plain)
     for name in q.prof_names():
         c, t = q.prof(name)
         log("  kernel %-22s launches/step %5.1f  avg %.4f ms  total/step %.4f ms"

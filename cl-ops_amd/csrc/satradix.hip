@@ -39,15 +39,23 @@ constexpr int kRadixMax = 256;
 constexpr int kBlock = 256;  // == kRadixMax: one thread per digit bin
 constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxPasses = 64;
-constexpr u32 kScanTiles = 64;  // tiles per block of the counter reduce/scan
+constexpr u32 kScanTiles = 256;  // tiles per group of the counter reduce/scan
 
 enum BufSel : u32 { kIn = 0, kAux = 1, kOut = 2 };
 
+// One digit pass.  `dsrc` = 1: the pass's upsweep counts the digit bytes the
+// previous executed pass's downsweep wrote (1 byte per key) instead of reading
+// the keys again; `nact` = 1: this pass's downsweep writes those bytes for the
+// next executed pass, whose digit starts at bit `nshift`.
 struct PassDesc {
 	u32 active;
 	u32 shift;
 	u32 src;
 	u32 dst;
+	u32 dsrc;
+	u32 nact;
+	u32 nshift;
+	u32 pad;
 };
 
 struct SortPlan {
@@ -66,21 +74,26 @@ struct Bufs {
 
 // Digit counts of one tile of kBlock*ITEMS keys.  The first pass reads the
 // input and also records the tile's bitwise AND and OR of the keys, from which
-// the plan kernel finds the digits that are equal in every key.
+// the plan kernel finds the digits that are equal in every key.  Later passes
+// count the digit bytes of their keys (`digits`, written by the previous
+// executed pass's downsweep): 1 byte per key read instead of sizeof(K).
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(kBlock) void satradix_upsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits,
-	u32* __restrict__ counts, K* __restrict__ andor, u32 first) {
+	u32* __restrict__ counts, K* __restrict__ andor, u32 first, const u8* __restrict__ digits) {
 	constexpr u32 kTile = kBlock * ITEMS;
+	static_assert(ITEMS % 4 == 0, "digit bytes are read as words");
 	__shared__ u32 s_h[kWaves][kRadixMax];
 	__shared__ K s_red[2 * kWaves];
 	u32 shift = 0;
+	u32 dsrc = 0;
 	const K* __restrict__ src = b.k[kIn];
 	if (!first) {
 		const PassDesc d = plan->pass[pass];
 		if (!d.active) return;
 		shift = d.shift;
 		src = b.k[d.src];
+		dsrc = d.dsrc;
 	}
 	const u32 tile = xcd_tile(tiles);
 	if (tile >= tiles) return;
@@ -89,6 +102,29 @@ __global__ __launch_bounds__(kBlock) void satradix_upsweep(Bufs<K> b,
 	const u32 w = wave_id();
 	for (u32 i = lane; i < kRadixMax; i += kWave) s_h[w][i] = 0;
 	__syncthreads();
+	if (dsrc) {
+		const u32 base = tile * kTile;
+		if (base + kTile <= n) {
+			// kTile is a multiple of 4 and the array is 16-byte aligned.
+			const u32* __restrict__ dw = reinterpret_cast<const u32*>(digits + base);
+			u32 x[ITEMS / 4];
+#pragma unroll
+			for (int j = 0; j < ITEMS / 4; ++j) x[j] = dw[j * kBlock + tid];
+#pragma unroll
+			for (int j = 0; j < ITEMS / 4; ++j) {
+#pragma unroll
+				for (int q = 0; q < 4; ++q) atomicAdd(&s_h[w][(x[j] >> (8 * q)) & 0xFFu], 1u);
+			}
+		} else {
+			for (u32 i = base + tid; i < n; i += kBlock) atomicAdd(&s_h[w][digits[i]], 1u);
+		}
+		__syncthreads();
+		u32 c = 0;
+#pragma unroll
+		for (int ww = 0; ww < kWaves; ++ww) c += s_h[ww][tid];
+		counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
+		return;
+	}
 	const u32 mask = (1u << bits) - 1u;
 	const u32 wbase = tile * kTile + w * kWave * ITEMS;
 	K k[ITEMS];
@@ -135,16 +171,31 @@ __global__ __launch_bounds__(kBlock) void satradix_upsweep(Bufs<K> b,
 }
 
 // Which digit passes run (a digit equal in every key needs no pass), and the
-// source and destination buffer of each executed pass.  One block.
+// source and destination buffer of each executed pass.  One block of
+// kPlanThreads; each thread keeps kPlanUnroll tile loads in flight.
+constexpr int kPlanThreads = 1024;
+constexpr u32 kPlanUnroll = 8;
+
 template <typename K>
-__global__ __launch_bounds__(kBlock) void satradix_plan(SortPlan* __restrict__ plan,
+__global__ __launch_bounds__(kPlanThreads) void satradix_plan(SortPlan* __restrict__ plan,
 	const K* __restrict__ andor, u32 tiles, u32 passes, u32 bits, u32 in_place) {
-	__shared__ K s_red[2 * kWaves];
+	constexpr int kPW = kPlanThreads / kWave;
+	__shared__ K s_red[2 * kPW];
 	const u32 tid = threadIdx.x;
 	K a = ~K(0), o = K(0);
-	for (u32 t = tid; t < tiles; t += kBlock) {
-		a &= andor[2 * static_cast<size_t>(t)];
-		o |= andor[2 * static_cast<size_t>(t) + 1];
+	for (u32 t0 = 0; t0 < tiles; t0 += kPlanThreads * kPlanUnroll) {
+		K xa[kPlanUnroll], xo[kPlanUnroll];
+#pragma unroll
+		for (u32 j = 0; j < kPlanUnroll; ++j) {
+			const u32 t = t0 + j * kPlanThreads + tid;
+			xa[j] = t < tiles ? andor[2 * static_cast<size_t>(t)] : ~K(0);
+			xo[j] = t < tiles ? andor[2 * static_cast<size_t>(t) + 1] : K(0);
+		}
+#pragma unroll
+		for (u32 j = 0; j < kPlanUnroll; ++j) {
+			a &= xa[j];
+			o |= xo[j];
+		}
 	}
 #pragma unroll
 	for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -153,15 +204,15 @@ __global__ __launch_bounds__(kBlock) void satradix_plan(SortPlan* __restrict__ p
 	}
 	if (lane_id() == 0) {
 		s_red[wave_id()] = a;
-		s_red[kWaves + wave_id()] = o;
+		s_red[kPW + wave_id()] = o;
 	}
 	__syncthreads();
 	if (tid != 0) return;
 	a = ~K(0);
 	o = K(0);
-	for (int ww = 0; ww < kWaves; ++ww) {
+	for (int ww = 0; ww < kPW; ++ww) {
 		a &= s_red[ww];
-		o |= s_red[kWaves + ww];
+		o |= s_red[kPW + ww];
 	}
 	const K diff = a ^ o;  // bits that differ between keys
 	const u32 mask = (1u << bits) - 1u;
@@ -169,8 +220,9 @@ __global__ __launch_bounds__(kBlock) void satradix_plan(SortPlan* __restrict__ p
 	for (u32 p = 0; p < passes; ++p) executed += (static_cast<u32>(diff >> (p * bits)) & mask) ? 1u : 0u;
 	u32 k = 0;
 	u32 cur = kIn;
+	u32 prev = kMaxPasses;  // previous executed pass
 	for (u32 p = 0; p < kMaxPasses; ++p) {
-		PassDesc d = {0u, 0u, 0u, 0u};
+		PassDesc d = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 		if (p < passes && (static_cast<u32>(diff >> (p * bits)) & mask)) {
 			u32 dst;
 			if (in_place) dst = (k % 2 == 0) ? kAux : kIn;
@@ -179,7 +231,13 @@ __global__ __launch_bounds__(kBlock) void satradix_plan(SortPlan* __restrict__ p
 			d.shift = p * bits;
 			d.src = cur;
 			d.dst = dst;
+			d.dsrc = prev < kMaxPasses ? 1u : 0u;
 			cur = dst;
+			if (prev < kMaxPasses) {
+				plan->pass[prev].nact = 1;
+				plan->pass[prev].nshift = p * bits;
+			}
+			prev = p;
 			++k;
 		}
 		plan->pass[p] = d;
@@ -199,70 +257,106 @@ __global__ __launch_bounds__(kBlock) void satradix_plan(SortPlan* __restrict__ p
 	}
 }
 
-// Loads the kScanTiles count rows of block `blk` (digit d) into registers.
-__device__ __forceinline__ void load_rows(const u32* __restrict__ counts, u32 blk, u32 tiles,
-	u32 d, u32 (&c)[kScanTiles]) {
-	const u32 t0 = blk * kScanTiles;
+// Counter scan.  Tiles form groups of kScanTiles; a group block has
+// kGroupThreads = 4 x 256 threads: thread (q, d) handles digit d of the
+// group's quarter q (kQuarter consecutive tiles).
+constexpr int kGroupThreads = 4 * kRadixMax;
+constexpr u32 kQuarter = kScanTiles / 4;
+
+// Loads this thread's kQuarter count rows (digit d) into registers.
+__device__ __forceinline__ void load_quarter(const u32* __restrict__ counts, u32 t0, u32 tiles,
+	u32 d, u32 (&c)[kQuarter]) {
 #pragma unroll
-	for (u32 i = 0; i < kScanTiles; ++i)
+	for (u32 i = 0; i < kQuarter; ++i)
 		c[i] = (t0 + i < tiles) ? counts[static_cast<size_t>(t0 + i) * kRadixMax + d] : 0u;
 }
 
-// partial[blk][d] = sum of counts[t][d] over the kScanTiles tiles of block blk.
-__global__ __launch_bounds__(kBlock) void satradix_reduce(const SortPlan* __restrict__ plan,
+// partial[d][g] = sum of counts[t][d] over the tiles of group g (digit-major,
+// so the next kernel scans it as one flat array).
+__global__ __launch_bounds__(kGroupThreads) void satradix_reduce(const SortPlan* __restrict__ plan,
 	u32 pass, const u32* __restrict__ counts, u32 tiles, u32* __restrict__ partial) {
+	__shared__ u32 s_q[4][kRadixMax];
 	if (!plan->pass[pass].active) return;
-	const u32 d = threadIdx.x;
-	u32 c[kScanTiles];
-	load_rows(counts, blockIdx.x, tiles, d, c);
+	const u32 d = threadIdx.x % kRadixMax, q = threadIdx.x / kRadixMax;
+	u32 c[kQuarter];
+	load_quarter(counts, blockIdx.x * kScanTiles + q * kQuarter, tiles, d, c);
 	u32 s = 0;
 #pragma unroll
-	for (u32 i = 0; i < kScanTiles; ++i) s += c[i];
-	partial[blockIdx.x * kRadixMax + d] = s;
+	for (u32 i = 0; i < kQuarter; ++i) s += c[i];
+	s_q[q][d] = s;
+	__syncthreads();
+	if (q == 0) partial[d * gridDim.x + blockIdx.x] = s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d];
 }
 
-// One block: partial[blk][d] <- (#keys with a smaller digit) +
-// (#keys with digit d in blocks < blk), i.e. the offset of block blk's first
-// key of digit d.
-__global__ __launch_bounds__(kBlock) void satradix_scan_partials(const SortPlan* __restrict__ plan,
-	u32 pass, u32* __restrict__ partial, u32 nblk) {
-	__shared__ u32 s_wsum[kWaves];
+// One block: flat exclusive scan of the digit-major partials, so that
+// partial[d][g] <- (#keys with a smaller digit) + (#keys with digit d in
+// groups < g), the offset of group g's first key of digit d.  Rounds of
+// kPartThreads * kPartItems values are staged in LDS with coalesced loads,
+// then each thread scans kPartItems consecutive values.
+constexpr int kPartThreads = 1024;
+constexpr u32 kPartItems = 16;
+constexpr u32 kPartRound = kPartThreads * kPartItems;
+
+__host__ __device__ constexpr u32 ppad(u32 i) { return i + (i >> 5); }
+
+__global__ __launch_bounds__(kPartThreads) void satradix_scan_partials(
+	const SortPlan* __restrict__ plan, u32 pass, u32* __restrict__ partial, u32 total) {
+	__shared__ u32 s_v[ppad(kPartRound)];
+	__shared__ u32 s_wsum[kPartThreads / kWave];
 	if (!plan->pass[pass].active) return;
-	constexpr u32 kChunk = 32;  // rows loaded per round (independent loads in flight)
-	const u32 d = threadIdx.x;
-	u32 total = 0;
-	for (u32 q0 = 0; q0 < nblk; q0 += kChunk) {
-		u32 v[kChunk];
+	const u32 tid = threadIdx.x;
+	u32 carry = 0;
+	for (u32 base = 0; base < total; base += kPartRound) {
 #pragma unroll
-		for (u32 i = 0; i < kChunk; ++i) v[i] = q0 + i < nblk ? partial[(q0 + i) * kRadixMax + d] : 0u;
-#pragma unroll
-		for (u32 i = 0; i < kChunk; ++i) total += v[i];
-	}
-	u32 run = block_exclusive_sum<kBlock>(total, s_wsum, static_cast<u32*>(nullptr));
-	for (u32 q0 = 0; q0 < nblk; q0 += kChunk) {
-		u32 v[kChunk];
-#pragma unroll
-		for (u32 i = 0; i < kChunk; ++i) v[i] = q0 + i < nblk ? partial[(q0 + i) * kRadixMax + d] : 0u;
-#pragma unroll
-		for (u32 i = 0; i < kChunk; ++i) {
-			if (q0 + i < nblk) partial[(q0 + i) * kRadixMax + d] = run;
-			run += v[i];
+		for (u32 j = 0; j < kPartItems; ++j) {
+			const u32 i = j * kPartThreads + tid;
+			s_v[ppad(i)] = base + i < total ? partial[base + i] : 0u;
 		}
+		__syncthreads();
+		u32 v[kPartItems];
+		u32 sum = 0;
+#pragma unroll
+		for (u32 k = 0; k < kPartItems; ++k) {
+			v[k] = s_v[ppad(tid * kPartItems + k)];
+			sum += v[k];
+		}
+		u32 round_total;
+		u32 run = carry + block_exclusive_sum<kPartThreads>(sum, s_wsum, &round_total);
+#pragma unroll
+		for (u32 k = 0; k < kPartItems; ++k) {
+			s_v[ppad(tid * kPartItems + k)] = run;
+			run += v[k];
+		}
+		__syncthreads();
+#pragma unroll
+		for (u32 j = 0; j < kPartItems; ++j) {
+			const u32 i = j * kPartThreads + tid;
+			if (base + i < total) partial[base + i] = s_v[ppad(i)];
+		}
+		carry += round_total;
+		__syncthreads();
 	}
 }
 
 // Turns the tile counts into global output offsets, in place:
 // counts[t][d] <- (#keys with a smaller digit) + (#keys with digit d in tiles < t).
-__global__ __launch_bounds__(kBlock) void satradix_scan(const SortPlan* __restrict__ plan,
+__global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* __restrict__ plan,
 	u32 pass, u32* __restrict__ counts, u32 tiles, const u32* __restrict__ partial) {
+	__shared__ u32 s_q[4][kRadixMax];
 	if (!plan->pass[pass].active) return;
-	const u32 d = threadIdx.x;
-	u32 c[kScanTiles];
-	load_rows(counts, blockIdx.x, tiles, d, c);
-	u32 run = partial[blockIdx.x * kRadixMax + d];
-	const u32 t0 = blockIdx.x * kScanTiles;
+	const u32 d = threadIdx.x % kRadixMax, q = threadIdx.x / kRadixMax;
+	const u32 t0 = blockIdx.x * kScanTiles + q * kQuarter;
+	u32 c[kQuarter];
+	load_quarter(counts, t0, tiles, d, c);
+	u32 s = 0;
 #pragma unroll
-	for (u32 i = 0; i < kScanTiles; ++i) {
+	for (u32 i = 0; i < kQuarter; ++i) s += c[i];
+	s_q[q][d] = s;
+	__syncthreads();
+	u32 run = partial[d * gridDim.x + blockIdx.x];
+	for (u32 qq = 0; qq < q; ++qq) run += s_q[qq][d];
+#pragma unroll
+	for (u32 i = 0; i < kQuarter; ++i) {
 		if (t0 + i < tiles) counts[static_cast<size_t>(t0 + i) * kRadixMax + d] = run;
 		run += c[i];
 	}
@@ -280,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void satradix_scan(const SortPlan* __restri
 template <typename K, bool KV, int ITEMS, int FIXED_BITS>
 __global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits_arg,
-	const u32* __restrict__ offsets) {
+	const u32* __restrict__ offsets, u8* __restrict__ ndig) {
 	const u32 bits = FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits_arg;
 	constexpr u32 kTile = kBlock * ITEMS;
 	constexpr u32 kStageBytes = kTile * (sizeof(K) > 4 ? sizeof(K) : (KV ? 4 : sizeof(K)));
@@ -376,15 +470,31 @@ __global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
 
 	const u32 count = (n - tile_base) < kTile ? (n - tile_base) : kTile;
 	u32 oidx[ITEMS];
+	if (d.nact) {
+		const u32 nshift = d.nshift;
 #pragma unroll
-	for (int r = 0; r < ITEMS; ++r) {
-		const u32 i = r * kBlock + tid;
-		if (i < count) {
-			const K key = s_keys[i];
-			const u32 dig = static_cast<u32>(key >> shift) & mask;
-			const u32 o = s_gofs[dig] + i;
-			dst[o] = key;
-			oidx[r] = o;
+		for (int r = 0; r < ITEMS; ++r) {
+			const u32 i = r * kBlock + tid;
+			if (i < count) {
+				const K key = s_keys[i];
+				const u32 dig = static_cast<u32>(key >> shift) & mask;
+				const u32 o = s_gofs[dig] + i;
+				dst[o] = key;
+				ndig[o] = static_cast<u8>(static_cast<u32>(key >> nshift) & mask);
+				oidx[r] = o;
+			}
+		}
+	} else {
+#pragma unroll
+		for (int r = 0; r < ITEMS; ++r) {
+			const u32 i = r * kBlock + tid;
+			if (i < count) {
+				const K key = s_keys[i];
+				const u32 dig = static_cast<u32>(key >> shift) & mask;
+				const u32 o = s_gofs[dig] + i;
+				dst[o] = key;
+				oidx[r] = o;
+			}
 		}
 	}
 	if (KV) {
@@ -433,6 +543,7 @@ struct SatradixData {
 	Scratch aux_vals;
 	Scratch counts;  // tiles x 256 digit counts, then offsets
 	Scratch meta;    // plan + per-tile AND/OR + partial sums
+	Scratch digits;  // next pass's digit byte per key
 	CCLQueue* last_queue = nullptr;
 };
 
@@ -455,6 +566,8 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	if (!sd->counts.ensure(static_cast<size_t>(tiles) * kRadixMax * sizeof(u32), err)) return false;
 	if (!sd->meta.ensure(plan_bytes + andor_bytes + static_cast<size_t>(nblk) * kRadixMax * 4, err))
 		return false;
+	if (passes > 1 && !sd->digits.ensure((n + 15) / 16 * 16, err)) return false;
+	u8* digits = passes > 1 ? static_cast<u8*>(sd->digits.ptr) : nullptr;
 	char* mb = static_cast<char*>(sd->meta.ptr);
 	SortPlan* plan = reinterpret_cast<SortPlan*>(mb);
 	K* andor = reinterpret_cast<K*>(mb + plan_bytes);
@@ -475,44 +588,45 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	{
 		Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
 		satradix_upsweep<K, ITEMS><<<grid, kBlock, 0, st>>>(b, plan, 0, nn, tiles, bits, counts,
-			andor, 1);
+			andor, 1, digits);
 		if (!L.done(err)) return false;
 	}
 	{
 		Launch L(q, CLO_SORT_SATRADIX_KNAME_PLAN);
-		satradix_plan<K><<<1, kBlock, 0, st>>>(plan, andor, tiles, passes, bits, in_place ? 1u : 0u);
+		satradix_plan<K><<<1, kPlanThreads, 0, st>>>(plan, andor, tiles, passes, bits, in_place ? 1u : 0u);
 		if (!L.done(err)) return false;
 	}
 	for (u32 p = 0; p < passes; ++p) {
 		if (p > 0) {
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
 			satradix_upsweep<K, ITEMS><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles, bits,
-				counts, andor, 0);
+				counts, andor, 0, digits);
 			if (!L.done(err)) return false;
 		}
 		{
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_REDUCE);
-			satradix_reduce<<<nblk, kBlock, 0, st>>>(plan, p, counts, tiles, partial);
+			satradix_reduce<<<nblk, kGroupThreads, 0, st>>>(plan, p, counts, tiles, partial);
 			if (!L.done(err)) return false;
 		}
 		{
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
-			satradix_scan_partials<<<1, kBlock, 0, st>>>(plan, p, partial, nblk);
+			satradix_scan_partials<<<1, kPartThreads, 0, st>>>(plan, p, partial,
+				nblk * kRadixMax);
 			if (!L.done(err)) return false;
 		}
 		{
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
-			satradix_scan<<<nblk, kBlock, 0, st>>>(plan, p, counts, tiles, partial);
+			satradix_scan<<<nblk, kGroupThreads, 0, st>>>(plan, p, counts, tiles, partial);
 			if (!L.done(err)) return false;
 		}
 		{
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_DOWNSWEEP);
 			if (bits == 8)
 				satradix_downsweep<K, KV, ITEMS, 8><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles,
-					bits, counts);
+					bits, counts, digits);
 			else
 				satradix_downsweep<K, KV, ITEMS, 0><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles,
-					bits, counts);
+					bits, counts, digits);
 			if (!L.done(err)) return false;
 		}
 	}
@@ -670,11 +784,11 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	case CLO_SORT_SATRADIX_KIDX_UPSWEEP:
 		return kWaves * kRadixMax * 4 + 2 * kWaves * ks;
 	case CLO_SORT_SATRADIX_KIDX_PLAN:
-		return 2 * kWaves * ks;
+		return 2 * (kPlanThreads / kWave) * ks;
 	case CLO_SORT_SATRADIX_KIDX_REDUCE:
-		return 0;
+		return 4 * kRadixMax * 4;
 	case CLO_SORT_SATRADIX_KIDX_SCAN:
-		return kWaves * 4;
+		return (ppad(kPartRound) + kPartThreads / kWave) * 4;
 	case CLO_SORT_SATRADIX_KIDX_DOWNSWEEP:
 		return kBlock * kItemsDefault * (ks > 4 ? ks : 4) + kWaves * kRadixMax * 4 +
 			2 * kRadixMax * 4 + kWaves * 4;

@@ -4,12 +4,12 @@
  * Names follow CL_Ops' src/cl_ops/sort/clo_sort_satradix.in.h:31-55.  The
  * MI355X kernels differ from the reference's three (localsort, histogram,
  * scatter + a Blelloch scan per pass): per digit pass an upsweep (tile digit
- * counts), a two-kernel counter scan and a downsweep that ranks each tile with
- * wave64 ballots and scatters it through LDS.
+ * counts), a reduce and a two-kernel counter scan, and a downsweep that ranks
+ * each tile with wave64 ballots and scatters it through LDS.
  *
  * Options ("key=value,..."): radix=<power of two, 2..256> (default 256),
- * scan=<impl> and scan<opt>=<value> (accepted for compatibility; the digit
- * offsets are computed inside the scatter pass).
+ * scan=<impl> and scan<opt>=<value> (accepted for compatibility: "blelloch"
+ * is the only scan, and the counter scan is built into the sort).
  */
 #ifndef CLO_SORT_SATRADIX_H_
 #define CLO_SORT_SATRADIX_H_

@@ -328,3 +328,31 @@ def test_fill_layout(oracle):
                 want[r, gid] = oracle.rng_next(gen, s) >> 7
         got = oracle.rng_fill(gen, st, draws, bits=25, threads=3)
         assert np.array_equal(got, want.reshape(-1))
+
+
+def _glib_fixture():
+    import json
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "glib_grand.json")
+    with open(p) as f:
+        return json.load(f)
+
+
+def test_mt19937_matches_glib_grand_fixture(oracle):
+    # GLib's GRand words (fixture written by tests/golden/make_glib_grand.py
+    # from the system libglib-2.0) are what the reference's HOST_MT seeding
+    # copies into the seed buffer (src/cl_ops/rng/clo_rng.c:196-203).
+    fx = _glib_fixture()
+    for seed, rec in fx["seeds"].items():
+        words = oracle.mt19937(int(seed), 1000)
+        assert [int(x) for x in words[:32]] == rec["first32"], seed
+        assert int(words[999]) == rec["word1000"], seed
+
+
+def test_host_mt_seeds_match_glib_grand_fixture(oracle):
+    fx = _glib_fixture()
+    for seed, rec in fx["seeds"].items():
+        st = oracle.rng_init_host_mt("xorshift128", 8, int(seed))  # 8 states x 4 words
+        assert [int(x) for x in st.view(np.uint32)] == rec["first32"], seed
+        st = oracle.rng_init_host_mt("mwc64x", 16, int(seed))  # 16 states x 2 words
+        assert [int(x) for x in st.view(np.uint32)] == rec["first32"], seed
