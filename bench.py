@@ -75,11 +75,15 @@ def cpu_baseline_kv(n_sample, threads):
 
 
 def traffic_from_profiles():
-    p = os.path.join(ROOT, "profiles", "satradix_downsweep_kv_u64_2p28_traffic.json")
+    """HBM bytes per satradix_downsweep launch from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel's)."""
+    import re
+    p = os.path.join(ROOT, "profiles", "satradix_downsweep_kv_u64_2p28_traffic.md")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def headline(args, ctx):

@@ -8,18 +8,20 @@
 //
 // Kernel plan per call (all on cq_exec, no host synchronisation, no
 // inter-block waiting):
-//   satradix_upsweep    per tile of 8192 keys: digit counts (the first pass
-//                       also takes the keys' bitwise AND/OR)
+//   satradix_upsweep    per tile (4096 or 8192 keys): digit counts; the first
+//                       pass reads the keys and also takes their bitwise
+//                       AND/OR, later passes read one digit byte per key
 //   satradix_plan       one block: which digits vary between keys (the other
 //                       passes are skipped) and the buffers of each pass
-//   satradix_reduce     per 64 tiles: sum of their digit counts
+//   satradix_reduce     per 256 tiles: sum of their digit counts
 //   satradix_scan       (two kernels) tile counts -> global output offsets
 //   satradix_downsweep  per tile: wave64 ballot ranking, LDS-staged scatter
-//                       to the offsets (contiguous runs per digit)
+//                       to the offsets (contiguous runs per digit), and the
+//                       next executed pass's digit byte of every key
 //   satradix_copy       only when an in-place sort executed an odd number of
 //                       passes (the result sits in the scratch buffer)
-// The reduce-then-scan structure reads the keys twice per pass (upsweep and
-// downsweep).  A single-pass decoupled look-back variant was measured on
+// The reduce-then-scan structure reads the keys once per pass, plus one byte
+// per key for the upsweep.  A single-pass decoupled look-back variant was measured on
 // MI355X: the per-block ticket atomic serialises (~11 ns per block) and the
 // look-back walks over ~30 predecessors per tile in 256-digit status rows,
 // which made the pass 2.3x slower than the scatter alone.
@@ -36,8 +38,6 @@ using namespace clo::dev;
 namespace {
 
 constexpr int kRadixMax = 256;
-constexpr int kBlock = 256;  // == kRadixMax: one thread per digit bin
-constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxPasses = 64;
 constexpr u32 kScanTiles = 256;  // tiles per group of the counter reduce/scan
 
@@ -72,19 +72,21 @@ struct Bufs {
 	u32* v[3];
 };
 
-// Digit counts of one tile of kBlock*ITEMS keys.  The first pass reads the
+// Digit counts of one tile of NT*ITEMS keys.  The first pass reads the
 // input and also records the tile's bitwise AND and OR of the keys, from which
 // the plan kernel finds the digits that are equal in every key.  Later passes
 // count the digit bytes of their keys (`digits`, written by the previous
 // executed pass's downsweep): 1 byte per key read instead of sizeof(K).
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(kBlock) void satradix_upsweep(Bufs<K> b,
+template <typename K, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits,
 	u32* __restrict__ counts, K* __restrict__ andor, u32 first, const u8* __restrict__ digits) {
-	constexpr u32 kTile = kBlock * ITEMS;
+	constexpr u32 kTile = NT * ITEMS;
+	constexpr int NW = NT / kWave;
 	static_assert(ITEMS % 4 == 0, "digit bytes are read as words");
-	__shared__ u32 s_h[kWaves][kRadixMax];
-	__shared__ K s_red[2 * kWaves];
+	static_assert(NT >= kRadixMax, "one thread per digit bin");
+	__shared__ u32 s_h[NW][kRadixMax];
+	__shared__ K s_red[2 * NW];
 	u32 shift = 0;
 	u32 dsrc = 0;
 	const K* __restrict__ src = b.k[kIn];
@@ -109,20 +111,22 @@ __global__ __launch_bounds__(kBlock) void satradix_upsweep(Bufs<K> b,
 			const u32* __restrict__ dw = reinterpret_cast<const u32*>(digits + base);
 			u32 x[ITEMS / 4];
 #pragma unroll
-			for (int j = 0; j < ITEMS / 4; ++j) x[j] = dw[j * kBlock + tid];
+			for (int j = 0; j < ITEMS / 4; ++j) x[j] = dw[j * NT + tid];
 #pragma unroll
 			for (int j = 0; j < ITEMS / 4; ++j) {
 #pragma unroll
 				for (int q = 0; q < 4; ++q) atomicAdd(&s_h[w][(x[j] >> (8 * q)) & 0xFFu], 1u);
 			}
 		} else {
-			for (u32 i = base + tid; i < n; i += kBlock) atomicAdd(&s_h[w][digits[i]], 1u);
+			for (u32 i = base + tid; i < n; i += NT) atomicAdd(&s_h[w][digits[i]], 1u);
 		}
 		__syncthreads();
-		u32 c = 0;
+		if (tid < kRadixMax) {
+			u32 c = 0;
 #pragma unroll
-		for (int ww = 0; ww < kWaves; ++ww) c += s_h[ww][tid];
-		counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
+			for (int ww = 0; ww < NW; ++ww) c += s_h[ww][tid];
+			counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
+		}
 		return;
 	}
 	const u32 mask = (1u << bits) - 1u;
@@ -150,20 +154,22 @@ __global__ __launch_bounds__(kBlock) void satradix_upsweep(Bufs<K> b,
 		}
 		if (lane == 0) {
 			s_red[w] = ka;
-			s_red[kWaves + w] = ko;
+			s_red[NW + w] = ko;
 		}
 	}
 	__syncthreads();
-	u32 c = 0;
+	if (tid < kRadixMax) {
+		u32 c = 0;
 #pragma unroll
-	for (int ww = 0; ww < kWaves; ++ww) c += s_h[ww][tid];
-	counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
+		for (int ww = 0; ww < NW; ++ww) c += s_h[ww][tid];
+		counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
+	}
 	if (first && tid == 0) {
 		K a = ~K(0), o = K(0);
 #pragma unroll
-		for (int ww = 0; ww < kWaves; ++ww) {
+		for (int ww = 0; ww < NW; ++ww) {
 			a &= s_red[ww];
-			o |= s_red[kWaves + ww];
+			o |= s_red[NW + ww];
 		}
 		andor[2 * static_cast<size_t>(tile)] = a;
 		andor[2 * static_cast<size_t>(tile) + 1] = o;
@@ -371,18 +377,20 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 // each element the number of equal-digit elements before it (stability).
 // FIXED_BITS = 8 compiles the digit-match loop for the default 8-bit digits;
 // 0 takes the digit width from `bits` (radix option).
-template <typename K, bool KV, int ITEMS, int FIXED_BITS>
-__global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
+template <typename K, bool KV, int NT, int ITEMS, int FIXED_BITS>
+__global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits_arg,
 	const u32* __restrict__ offsets, u8* __restrict__ ndig) {
 	const u32 bits = FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits_arg;
-	constexpr u32 kTile = kBlock * ITEMS;
+	constexpr u32 kTile = NT * ITEMS;
+	constexpr int NW = NT / kWave;
+	static_assert(NT >= kRadixMax, "one thread per digit bin");
 	constexpr u32 kStageBytes = kTile * (sizeof(K) > 4 ? sizeof(K) : (KV ? 4 : sizeof(K)));
 	__shared__ __attribute__((aligned(16))) unsigned char s_stage[kStageBytes];
-	__shared__ u32 s_whist[kWaves][kRadixMax];
+	__shared__ u32 s_whist[NW][kRadixMax];
 	__shared__ u32 s_tstart[kRadixMax];
 	__shared__ u32 s_gofs[kRadixMax];
-	__shared__ u32 s_wsum[kWaves];
+	__shared__ u32 s_wsum[NW];
 
 	const PassDesc d = plan->pass[pass];
 	if (!d.active) return;  // uniform: the digit is equal in every key
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
 	const u32 shift = d.shift;
 	const K* __restrict__ src = b.k[d.src];
 	K* __restrict__ dst = b.k[d.dst];
-	const u32 goff = offsets[static_cast<size_t>(tile) * kRadixMax + tid];
+	const u32 goff = tid < kRadixMax ? offsets[static_cast<size_t>(tile) * kRadixMax + tid] : 0u;
 
 	// Load (coalesced: 64 consecutive elements per wave instruction).
 	const u32 wbase = tile_base + w * kWave * ITEMS;
@@ -444,15 +452,19 @@ __global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
 
 	// Per-digit tile totals and per-wave exclusive offsets.
 	u32 total = 0;
+	if (tid < kRadixMax) {
 #pragma unroll
-	for (int ww = 0; ww < kWaves; ++ww) {
-		const u32 c = s_whist[ww][tid];
-		s_whist[ww][tid] = total;
-		total += c;
+		for (int ww = 0; ww < NW; ++ww) {
+			const u32 c = s_whist[ww][tid];
+			s_whist[ww][tid] = total;
+			total += c;
+		}
 	}
-	const u32 tstart = block_exclusive_sum<kBlock>(total, s_wsum, static_cast<u32*>(nullptr));
-	s_tstart[tid] = tstart;
-	s_gofs[tid] = goff - tstart;
+	const u32 tstart = block_exclusive_sum<NT>(total, s_wsum, static_cast<u32*>(nullptr));
+	if (tid < kRadixMax) {
+		s_tstart[tid] = tstart;
+		s_gofs[tid] = goff - tstart;
+	}
 	__syncthreads();
 
 	// Stage keys in digit order in LDS.
@@ -474,7 +486,7 @@ __global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
 		const u32 nshift = d.nshift;
 #pragma unroll
 		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * kBlock + tid;
+			const u32 i = r * NT + tid;
 			if (i < count) {
 				const K key = s_keys[i];
 				const u32 dig = static_cast<u32>(key >> shift) & mask;
@@ -487,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
 	} else {
 #pragma unroll
 		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * kBlock + tid;
+			const u32 i = r * NT + tid;
 			if (i < count) {
 				const K key = s_keys[i];
 				const u32 dig = static_cast<u32>(key >> shift) & mask;
@@ -509,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void satradix_downsweep(Bufs<K> b,
 		__syncthreads();
 #pragma unroll
 		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * kBlock + tid;
+			const u32 i = r * NT + tid;
 			if (i < count) vdst[oidx[r]] = s_vals[i];
 		}
 	}
@@ -534,10 +546,19 @@ __global__ __launch_bounds__(256) void satradix_copy(Bufs<K> b, const SortPlan* 
 // ---------------------------------------------------------------------------
 // Host side.
 
-constexpr int kItemsDefault = 32;
+// Tile shapes (threads per block x items per thread):
+//   0: 256 x 32 (8192 keys)   1: 512 x 16 (8192)   2: 256 x 16 (4096)
+//   3: 512 x 32 (16384)
+// The "wg" and "ipt" options pick one; otherwise auto_shape() does.
+struct TileShape {
+	u32 nt, ipt;
+};
+constexpr TileShape kShapes[] = {{256, 32}, {512, 16}, {256, 16}, {512, 32}};
+constexpr int kNumShapes = 4;
 
 struct SatradixData {
 	u32 bits = 8;
+	int shape = -1;  // index into kShapes; -1 = automatic
 	std::string scan_type = "blelloch";
 	Scratch aux_keys;
 	Scratch aux_vals;
@@ -549,14 +570,14 @@ struct SatradixData {
 
 inline SatradixData* sdata(CloSort* s) { return static_cast<SatradixData*>(s->data); }
 
-template <typename K, bool KV, int ITEMS>
+template <typename K, bool KV, int NT, int ITEMS>
 bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* vout, size_t n,
 	GError** err) {
 	const u32 keybits = sizeof(K) * 8;
 	const u32 bits = sd->bits;
 	const u32 passes = (keybits + bits - 1) / bits;
 	const bool in_place = (kout == nullptr || kout == kin);
-	const u32 tile = kBlock * ITEMS;
+	const u32 tile = NT * ITEMS;
 	const u32 tiles = static_cast<u32>(div_ceil(n, tile));
 	const u32 nblk = static_cast<u32>(div_ceil(tiles, kScanTiles));
 	const size_t plan_bytes = (sizeof(SortPlan) + 255) / 256 * 256;
@@ -587,7 +608,7 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	const u32 grid = xcd_grid(tiles);
 	{
 		Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
-		satradix_upsweep<K, ITEMS><<<grid, kBlock, 0, st>>>(b, plan, 0, nn, tiles, bits, counts,
+		satradix_upsweep<K, NT, ITEMS><<<grid, NT, 0, st>>>(b, plan, 0, nn, tiles, bits, counts,
 			andor, 1, digits);
 		if (!L.done(err)) return false;
 	}
@@ -599,7 +620,7 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	for (u32 p = 0; p < passes; ++p) {
 		if (p > 0) {
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
-			satradix_upsweep<K, ITEMS><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles, bits,
+			satradix_upsweep<K, NT, ITEMS><<<grid, NT, 0, st>>>(b, plan, p, nn, tiles, bits,
 				counts, andor, 0, digits);
 			if (!L.done(err)) return false;
 		}
@@ -621,12 +642,16 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 		}
 		{
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_DOWNSWEEP);
-			if (bits == 8)
-				satradix_downsweep<K, KV, ITEMS, 8><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles,
-					bits, counts, digits);
-			else
-				satradix_downsweep<K, KV, ITEMS, 0><<<grid, kBlock, 0, st>>>(b, plan, p, nn, tiles,
-					bits, counts, digits);
+			if constexpr (NT == 256 && ITEMS == 32) {
+				if (bits != 8) {
+					satradix_downsweep<K, KV, NT, ITEMS, 0><<<grid, NT, 0, st>>>(b, plan, p, nn,
+						tiles, bits, counts, digits);
+					if (!L.done(err)) return false;
+					continue;
+				}
+			}
+			satradix_downsweep<K, KV, NT, ITEMS, 8><<<grid, NT, 0, st>>>(b, plan, p, nn, tiles,
+				bits, counts, digits);
 			if (!L.done(err)) return false;
 		}
 	}
@@ -641,22 +666,38 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	return true;
 }
 
+// Default tile shape per key width (measured on MI355X at 2^28 keys: 512x16
+// is 7% faster for u64 keys alone and 12% for u32 keys with values; 256x32
+// wins or ties elsewhere; 16384-key tiles run one block per CU and lose
+// 20-40%).
+inline int auto_shape(size_t ksize, bool kv) {
+	if ((ksize == 8 && !kv) || (ksize == 4 && kv)) return 1;
+	return 0;
+}
+
+template <typename K, bool KV>
+bool launch_shape(SatradixData* sd, CCLQueue* q, void* kin, u32* vin, void* kout, u32* vout,
+	size_t n, GError** err) {
+	int sh = sd->shape >= 0 ? sd->shape : auto_shape(sizeof(K), KV);
+	if (sd->bits != 8) sh = 0;  // other digit widths: the generic-width kernels
+	K* ki = static_cast<K*>(kin);
+	K* ko = static_cast<K*>(kout);
+	switch (sh) {
+	case 1: return launch_all<K, KV, 512, 16>(sd, q, ki, vin, ko, vout, n, err);
+	case 2: return launch_all<K, KV, 256, 16>(sd, q, ki, vin, ko, vout, n, err);
+	case 3: return launch_all<K, KV, 512, 32>(sd, q, ki, vin, ko, vout, n, err);
+	default: return launch_all<K, KV, 256, 32>(sd, q, ki, vin, ko, vout, n, err);
+	}
+}
+
 template <bool KV>
 bool dispatch(SatradixData* sd, CCLQueue* q, size_t ksize, void* kin, u32* vin, void* kout,
 	u32* vout, size_t n, GError** err) {
 	switch (ksize) {
-	case 1:
-		return launch_all<u8, KV, kItemsDefault>(sd, q, static_cast<u8*>(kin), vin,
-			static_cast<u8*>(kout), vout, n, err);
-	case 2:
-		return launch_all<u16, KV, kItemsDefault>(sd, q, static_cast<u16*>(kin), vin,
-			static_cast<u16*>(kout), vout, n, err);
-	case 4:
-		return launch_all<u32, KV, kItemsDefault>(sd, q, static_cast<u32*>(kin), vin,
-			static_cast<u32*>(kout), vout, n, err);
-	case 8:
-		return launch_all<u64, KV, kItemsDefault>(sd, q, static_cast<u64*>(kin), vin,
-			static_cast<u64*>(kout), vout, n, err);
+	case 1: return launch_shape<u8, KV>(sd, q, kin, vin, kout, vout, n, err);
+	case 2: return launch_shape<u16, KV>(sd, q, kin, vin, kout, vout, n, err);
+	case 4: return launch_shape<u32, KV>(sd, q, kin, vin, kout, vout, n, err);
+	case 8: return launch_shape<u64, KV>(sd, q, kin, vin, kout, vout, n, err);
 	}
 	set_error(err, kQuarkClo, CLO_ERROR_ARGS, "unsupported element size %zu", ksize);
 	return false;
@@ -697,8 +738,21 @@ const char* satradix_init(CloSort* sorter, const char* options, GError** err) {
 	SatradixData* sd = new SatradixData();
 	sorter->data = sd;
 	u32 radix = 256;
+	long wg = 0, ipt = 0;
 	for (auto& o : opts) {
-		if (o.key == "radix") {
+		if (o.key == "wg") {
+			wg = atol(o.value.c_str());
+			if (wg != 256 && wg != 512) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "wg must be 256 or 512.");
+				return nullptr;
+			}
+		} else if (o.key == "ipt") {
+			ipt = atol(o.value.c_str());
+			if (ipt != 16 && ipt != 32) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "ipt must be 16 or 32.");
+				return nullptr;
+			}
+		} else if (o.key == "radix") {
 			long r = atol(o.value.c_str());
 			if (r <= 0 || clo_ones32(static_cast<unsigned>(r)) != 1) {
 				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "Radix must be a power of 2.");
@@ -740,6 +794,24 @@ const char* satradix_init(CloSort* sorter, const char* options, GError** err) {
 		return nullptr;
 	}
 	sd->bits = clo_tzc(static_cast<int>(radix));
+	if (wg || ipt) {
+		if (!wg) wg = 256;
+		if (!ipt) ipt = wg == 512 ? 16 : 32;
+		sd->shape = -1;
+		for (int i = 0; i < kNumShapes; ++i)
+			if (kShapes[i].nt == static_cast<u32>(wg) && kShapes[i].ipt == static_cast<u32>(ipt))
+				sd->shape = i;
+		if (sd->shape < 0) {
+			set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+				"Unsupported tile shape wg=%ld ipt=%ld.", wg, ipt);
+			return nullptr;
+		}
+		if (sd->bits != 8 && sd->shape != 0) {
+			set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+				"Digit widths other than 8 bits run the 256x32 tile shape only.");
+			return nullptr;
+		}
+	}
 	return "satradix (gfx950: upsweep + counter scan + downsweep per digit pass)";
 }
 
@@ -780,9 +852,12 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	(void)lws_max;
 	(void)numel;
 	const size_t ks = clo_type_sizeof(sorter->elem_type);
+	const SatradixData* sd = sdata(sorter);
+	const int sh = sd && sd->shape >= 0 ? sd->shape : auto_shape(ks, true);
+	const size_t nt = kShapes[sh].nt, ipt = kShapes[sh].ipt;
 	switch (i) {
 	case CLO_SORT_SATRADIX_KIDX_UPSWEEP:
-		return kWaves * kRadixMax * 4 + 2 * kWaves * ks;
+		return (nt / kWave) * kRadixMax * 4 + 2 * (nt / kWave) * ks;
 	case CLO_SORT_SATRADIX_KIDX_PLAN:
 		return 2 * (kPlanThreads / kWave) * ks;
 	case CLO_SORT_SATRADIX_KIDX_REDUCE:
@@ -790,8 +865,8 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	case CLO_SORT_SATRADIX_KIDX_SCAN:
 		return (ppad(kPartRound) + kPartThreads / kWave) * 4;
 	case CLO_SORT_SATRADIX_KIDX_DOWNSWEEP:
-		return kBlock * kItemsDefault * (ks > 4 ? ks : 4) + kWaves * kRadixMax * 4 +
-			2 * kRadixMax * 4 + kWaves * 4;
+		return nt * ipt * (ks > 4 ? ks : 4) + (nt / kWave) * kRadixMax * 4 +
+			2 * kRadixMax * 4 + (nt / kWave) * 4;
 	case CLO_SORT_SATRADIX_KIDX_COPY:
 		return 0;
 	}

@@ -61,6 +61,27 @@ def test_radix_option(clops, oracle, ctx, radix):
     assert np.array_equal(ko, rk) and np.array_equal(vo, rv)
 
 
+@pytest.mark.parametrize("opt", ["wg=256,ipt=32", "wg=512,ipt=16", "wg=256,ipt=16",
+                                 "wg=512,ipt=32"])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("n", [1, 4095, 4097, 8193, 50001, 300007])
+def test_tile_shapes(clops, oracle, ctx, opt, dtype, n):
+    s = clops.Sorter(ctx, "satradix", dtype, options=opt)
+    k = _keys(oracle, dtype, n, 0xE7 + n, "uniform" if n % 2 else "few")
+    v = np.arange(n, dtype=np.uint32)
+    ko, vo = s.sort_kv_host(k, v)
+    rk, rv = oracle.stable_sort(k, v)
+    assert np.array_equal(ko, rk) and np.array_equal(vo, rv)
+    assert np.array_equal(s.sort_host(k), rk)
+
+
+def test_tile_shape_option_errors(clops, ctx):
+    for bad in ("wg=128", "ipt=8", "ipt=64", "radix=16,wg=512"):
+        with pytest.raises(clops.ClopsError) as e:
+            clops.Sorter(ctx, "satradix", np.uint32, options=bad)
+        assert e.value.code == clops.CLO_ERROR_ARGS
+
+
 def test_config1_2p20_u32(clops, oracle, ctx, queue):
     """2^20 uint32 keys over the full 32-bit range (seed 0xC1), keys only."""
     n = 1 << 20
