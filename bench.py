@@ -198,8 +198,86 @@ def suite(ctx):
         sw = s.last_sweeps()
         out.append({"config": "%s u32 2^24" % algo, "ms": ms, "Gkeys/s": n / ms / 1e6,
                     "sweeps": sw, "hbm_frac": sw * 2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    out.extend(suite_segmented(ctx, q))
     for o in out:
         log(json.dumps(o))
+
+
+def c4_offsets(ctx, q, nseg=32768, max_len=65536, seed=0xC4A):
+    """Segment lengths floor(exp(u ln 65536)) clamped to [1, 65536], u from the
+    seeded generator (U(seed, j) >> 11) * 2^-53; offsets = exclusive sum."""
+    ub = clops.Buffer(ctx, nseg * 8)
+    clops.generate(q, ub, nseg, clops.GEN_U64, seed)
+    u = ub.read(q, np.uint64, nseg)
+    x = (u >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    lens = np.clip(np.floor(np.exp(x * np.log(max_len))), 1, max_len).astype(np.uint32)
+    off = np.zeros(nseg, dtype=np.uint32)
+    off[1:] = np.cumsum(lens[:-1], dtype=np.uint64).astype(np.uint32)
+    return lens, off
+
+
+def suite_segmented(ctx, q, baseline_stride=16):
+    """Batched segmented sort and scan (32768 segments, log-uniform lengths),
+    eager and as one captured graph, against one library call per segment
+    (timed on every baseline_stride-th segment and scaled by the segment
+    count)."""
+    res = []
+    lens, off = c4_offsets(ctx, q)
+    nseg = lens.size
+    n = int(lens.sum(dtype=np.uint64))
+    kin = clops.Buffer(ctx, n * 4)
+    kout = clops.Buffer(ctx, n * 4)
+    sout = clops.Buffer(ctx, n * 4)
+    clops.generate(q, kin, n, clops.GEN_U32_HI, 0xC4B)
+    offb = clops.Buffer.from_array(ctx, off)
+    s = clops.Sorter(ctx, "satradix", np.uint32)
+    sc = clops.Scanner(ctx, np.uint32, np.uint32)
+    ms_sort = time_steps(q, lambda: s.sort_segmented_device(q, kin, offb, nseg, n, kout=kout), 5, 2)
+    ko = kout.read(q, np.uint32, n)
+    inner = np.ones(n - 1, dtype=bool)
+    inner[off[1:] - 1] = False  # pairs that straddle a segment start
+    bad = int(np.count_nonzero((ko[1:] < ko[:-1]) & inner))
+    del ko
+    ms_scan = time_steps(q, lambda: sc.scan_segmented_device(q, kout, sout, offb, nseg, n), 10, 3)
+
+    def both():
+        s.sort_segmented_device(q, kin, offb, nseg, n, kout=kout)
+        sc.scan_segmented_device(q, kout, sout, offb, nseg, n)
+
+    both()
+    q.finish()
+    g = clops.Graph(q)
+    g.begin()
+    try:
+        both()
+    finally:
+        g.end()
+    ms_graph = time_steps(q, g.launch, 10, 3)
+    by = n * 8
+    res.append({"config": "segmented sort u32, %d segments, %d keys" % (nseg, n), "ms": ms_sort,
+                "Gkeys/s": n / ms_sort / 1e6, "hbm_frac": by / ms_sort / 1e6 / HBM_PEAK_GBS,
+                "unsorted_pairs": bad})
+    res.append({"config": "segmented scan u32->u32, same offsets", "ms": ms_scan,
+                "hbm_frac": by / ms_scan / 1e6 / HBM_PEAK_GBS})
+    res.append({"config": "segmented sort + scan as one graph (%d nodes)" % g.num_nodes(),
+                "ms": ms_graph, "eager_ms": ms_sort + ms_scan})
+    # Baseline: one sort call and one scan call per segment (sampled).
+    idx = np.arange(0, nseg, baseline_stride)
+    views = [(kin.view(int(off[j]) * 4, int(lens[j]) * 4), kout.view(int(off[j]) * 4,
+              int(lens[j]) * 4), sout.view(int(off[j]) * 4, int(lens[j]) * 4), int(lens[j]))
+             for j in idx]
+
+    def per_segment():
+        for a, b, c, m in views:
+            s.sort_device(q, a, b, m)
+            sc.scan_device(q, b, c, m)
+
+    ms_base = time_steps(q, per_segment, 2, 1) * nseg / idx.size
+    res.append({"config": "baseline: one sort + one scan call per segment (every %dth segment "
+                          "timed, scaled by %d/%d)" % (baseline_stride, nseg, idx.size),
+                "ms": ms_base, "speedup_graph": ms_base / ms_graph})
+    g.close()
+    return res
 
 
 def copy_probe(ctx, nbytes):

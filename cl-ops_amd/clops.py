@@ -95,6 +95,7 @@ def lib():
         "ccl_buffer_destroy": (None, [vp]),
         "ccl_buffer_get_device_ptr": (vp, [vp]),
         "ccl_buffer_get_size": (sz, [vp]),
+        "ccl_buffer_new_wrap": (vp, [vp, vp, sz]),
         "ccl_buffer_enqueue_write": (vp, [vp, vp, u32, sz, sz, vp, vp, ep]),
         "ccl_buffer_enqueue_read": (vp, [vp, vp, u32, sz, sz, vp, vp, ep]),
         "ccl_buffer_enqueue_copy": (vp, [vp, vp, vp, sz, sz, sz, vp, ep]),
@@ -150,6 +151,12 @@ def lib():
         "clo_util_multiset_hash": (u64, [vp, vp, sz, sz, ep]),
         "clo_util_check_sorted": (sz, [vp, vp, vp, sz, sz, ep]),
         "clo_util_build_info": (cp, []),
+        # graphs
+        "clo_graph_begin_capture": (u32, [vp, ep]),
+        "clo_graph_end_capture": (vp, [vp, ep]),
+        "clo_graph_launch": (vp, [vp, vp, ep]),
+        "clo_graph_get_num_nodes": (sz, [vp]),
+        "clo_graph_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -242,6 +249,39 @@ class Queue:
             pass
 
 
+class Graph:
+    """Work enqueued on `q` between begin() and end() (clo_graph_*)."""
+
+    def __init__(self, q):
+        self.L = lib()
+        self.q = q
+        self.ptr = None
+
+    def begin(self):
+        _call(self.L.clo_graph_begin_capture, self.q.ptr)
+
+    def end(self):
+        self.ptr = _call(self.L.clo_graph_end_capture, self.q.ptr)
+        return self
+
+    def launch(self, q=None):
+        return _call(self.L.clo_graph_launch, self.ptr, (q or self.q).ptr)
+
+    def num_nodes(self):
+        return self.L.clo_graph_get_num_nodes(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            self.L.clo_graph_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Buffer:
     def __init__(self, ctx, nbytes, host=None):
         self.L = lib()
@@ -259,6 +299,18 @@ class Buffer:
     def from_array(cls, ctx, a):
         a = np.ascontiguousarray(a)
         return cls(ctx, max(a.nbytes, 1), a)
+
+    def view(self, offset, nbytes):
+        """A buffer over bytes [offset, offset+nbytes) of this one (no copy;
+        the view must not outlive this buffer)."""
+        v = Buffer.__new__(Buffer)
+        v.L, v.ctx, v.nbytes, v._parent = self.L, self.ctx, int(nbytes), self
+        base = self.L.ccl_buffer_get_device_ptr(self.ptr)
+        v.ptr = self.L.ccl_buffer_new_wrap(self.ctx.ptr, ctypes.c_void_p(base + int(offset)),
+                                           max(int(nbytes), 1))
+        if not v.ptr:
+            raise ClopsError(0, 0, "ccl_buffer_new_wrap failed")
+        return v
 
     def write(self, q, a, offset=0):
         a = np.ascontiguousarray(a)

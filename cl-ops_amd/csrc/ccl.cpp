@@ -52,7 +52,17 @@ bool use_device(CCLContext* ctx, GError** err) {
 	return !hip_failed(hipSetDevice(ctx->dev), err, "hipSetDevice");
 }
 
+thread_local int g_capture_depth = 0;
+
 CCLEvent* record_event(CCLQueue* q, const char* name, GError** err) {
+	if (q->capturing) {
+		if (!q->capture_event) {
+			q->capture_event = new ccl_event();
+			q->capture_event->ev = nullptr;
+			q->capture_event->name = "captured";
+		}
+		return q->capture_event;
+	}
 	CCLEvent* e = new ccl_event();
 	if (hip_failed(hipEventCreate(&e->ev), err, "hipEventCreate")) {
 		delete e;
@@ -70,7 +80,7 @@ CCLEvent* record_event(CCLQueue* q, const char* name, GError** err) {
 
 Launch::Launch(CCLQueue* q_, const char* name_) : q(q_), name(name_), rec(nullptr) {
 	(void)hipGetLastError();  // clear stale launch errors
-	if (!q->prof) return;
+	if (!q->prof || q->capturing) return;
 	ccl_prof_rec r;
 	r.name = name;
 	bool ok = true;
@@ -104,6 +114,12 @@ bool Launch::done(GError** err) {
 
 bool Scratch::ensure(size_t bytes, GError** err) {
 	if (bytes <= size && ptr) return true;
+	if (g_capture_depth > 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"device scratch of %zu bytes needed during graph capture: run the same call once "
+			"before capturing it", bytes);
+		return false;
+	}
 	release();
 	size_t want = bytes + bytes / 8 + 256;  // headroom against regrowth
 	void* p = nullptr;
@@ -299,6 +315,7 @@ void ccl_queue_destroy(CCLQueue* cq) {
 		hipEventDestroy(r.end);
 	}
 	for (hipEvent_t e : cq->prof_pool) hipEventDestroy(e);
+	delete cq->capture_event;
 	if (cq->owns_stream) hipStreamDestroy(cq->stream);
 	ccl_context_destroy(cq->ctx);
 	delete cq;
@@ -373,7 +390,7 @@ size_t ccl_buffer_get_size(CCLBuffer* buf) { return buf ? buf->size : 0; }
 static bool wait_list_on_stream(CCLEventWaitList* ewl, hipStream_t s, GError** err) {
 	if (!ewl || !*ewl) return true;
 	for (CCLEvent* e : (*ewl)->evts)
-		if (e && hip_failed(hipStreamWaitEvent(s, e->ev, 0), err, "hipStreamWaitEvent"))
+		if (e && e->ev && hip_failed(hipStreamWaitEvent(s, e->ev, 0), err, "hipStreamWaitEvent"))
 			return false;
 	ccl_event_wait_list_clear(ewl);
 	return true;
@@ -391,7 +408,7 @@ static CCLEvent* copy_common(CCLQueue* cq, void* dst, const void* src, size_t si
 	if (size && hip_failed(hipMemcpyAsync(dst, src, size, kind, cq->stream), err, name))
 		return nullptr;
 	CCLEvent* e = record_event(cq, name, err);
-	if (e && blocking && hip_failed(hipEventSynchronize(e->ev), err, name)) return nullptr;
+	if (e && e->ev && blocking && hip_failed(hipEventSynchronize(e->ev), err, name)) return nullptr;
 	return e;
 }
 
@@ -457,7 +474,7 @@ cl_bool ccl_event_wait(CCLEventWaitList* ewl, GError** err) {
 	if (!ewl || !*ewl) return CL_TRUE;
 	bool ok = true;
 	for (CCLEvent* e : (*ewl)->evts)
-		if (e && hip_failed(hipEventSynchronize(e->ev), err, "hipEventSynchronize")) {
+		if (e && e->ev && hip_failed(hipEventSynchronize(e->ev), err, "hipEventSynchronize")) {
 			ok = false;
 			break;
 		}
@@ -466,8 +483,8 @@ cl_bool ccl_event_wait(CCLEventWaitList* ewl, GError** err) {
 }
 
 double ccl_event_elapsed_ms(CCLEvent* start, CCLEvent* end, GError** err) {
-	if (!start || !end) {
-		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL event");
+	if (!start || !end || !start->ev || !end->ev) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL event (or one recorded during capture)");
 		return -1.0;
 	}
 	if (hip_failed(hipEventSynchronize(end->ev), err, "hipEventSynchronize")) return -1.0;

@@ -33,6 +33,8 @@ struct ccl_queue {
 	hipStream_t stream;
 	bool owns_stream;
 	bool prof;
+	bool capturing = false;                // between clo_graph_begin/end_capture
+	ccl_event* capture_event = nullptr;   // returned for work enqueued while capturing
 	std::vector<ccl_event*> events;       // every event this queue recorded
 	std::vector<ccl_prof_rec> prof_recs;  // used records since last reset
 	std::vector<hipEvent_t> prof_pool;    // spare events for prof records
@@ -68,8 +70,14 @@ bool hip_failed(hipError_t e, GError** err, const char* what);
 // Device allocation honouring the allocation-failure test hook.
 hipError_t dev_malloc(void** p, size_t bytes);
 
-// Records a new event on the queue (owned by the queue).
+// Records a new event on the queue (owned by the queue).  While the queue is
+// being captured into a graph no event is recorded: the queue's placeholder
+// event (no hipEvent behind it) is returned instead.
 CCLEvent* record_event(CCLQueue* q, const char* name, GError** err);
+
+// Number of graph captures open on the calling thread.  Device scratch is
+// never allocated while one is open (Scratch::ensure fails instead).
+extern thread_local int g_capture_depth;
 
 // Activates the context's device for the calling thread.
 bool use_device(CCLContext* ctx, GError** err);

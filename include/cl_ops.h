@@ -14,5 +14,6 @@
 #include "clo_scan_abstract.h"
 #include "clo_scan_blelloch.h"
 #include "clo_util.h"
+#include "clo_graph.h"
 
 #endif
