@@ -151,6 +151,20 @@ def lib():
         "clo_util_multiset_hash": (u64, [vp, vp, sz, sz, ep]),
         "clo_util_check_sorted": (sz, [vp, vp, vp, sz, sz, ep]),
         "clo_util_build_info": (cp, []),
+        # cost-model MLP
+        "clo_mlp_new": (vp, [vp, u32, u64, ep]),
+        "clo_mlp_destroy": (None, [vp]),
+        "clo_mlp_num_params": (u32, [vp]),
+        "clo_mlp_num_inputs": (u32, [vp]),
+        "clo_mlp_num_hidden": (u32, [vp]),
+        "clo_mlp_get_params": (u32, [vp, vp, ep]),
+        "clo_mlp_set_params": (u32, [vp, vp, ep]),
+        "clo_mlp_gradient": (u32, [vp, vp, vp, vp, sz, vp, vp, ep]),
+        "clo_mlp_train": (u32, [vp, vp, vp, vp, sz, u32, ctypes.c_float, vp, ep]),
+        "clo_mlp_predict": (u32, [vp, vp, vp, sz, vp, ep]),
+        "clo_mlp_predict_device": (vp, [vp, vp, vp, sz, vp, ep]),
+        "clo_mlp_save": (u32, [vp, cp, ep]),
+        "clo_mlp_load": (vp, [vp, cp, ep]),
         # graphs
         "clo_graph_begin_capture": (u32, [vp, ep]),
         "clo_graph_end_capture": (vp, [vp, ep]),
@@ -240,6 +254,74 @@ class Queue:
     def close(self):
         if self.ptr:
             self.L.ccl_queue_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if cols is not None:
+        a = a.reshape(-1, cols)
+    return a
+
+
+class Mlp:
+    """FP32 cost model n_in -> 32 -> 32 -> 1 trained and run on the GPU."""
+
+    def __init__(self, ctx, n_in=None, seed=0, path=None):
+        self.L = lib()
+        self.ctx = ctx
+        if path is not None:
+            self.ptr = _call(self.L.clo_mlp_load, ctx.ptr, _b(path))
+        else:
+            self.ptr = _call(self.L.clo_mlp_new, ctx.ptr, n_in, seed)
+        self.n_in = self.L.clo_mlp_num_inputs(self.ptr)
+        self.count = self.L.clo_mlp_num_params(self.ptr)
+
+    def params(self):
+        p = np.empty(self.count, dtype=np.float32)
+        _call(self.L.clo_mlp_get_params, self.ptr, p.ctypes.data_as(ctypes.c_void_p))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.count
+        _call(self.L.clo_mlp_set_params, self.ptr, p.ctypes.data_as(ctypes.c_void_p))
+
+    def gradient(self, q, X, t):
+        X, t = _f32(X, self.n_in), _f32(t)
+        g = np.empty(self.count, dtype=np.float32)
+        loss = ctypes.c_float()
+        _call(self.L.clo_mlp_gradient, self.ptr, q.ptr, X.ctypes.data_as(ctypes.c_void_p),
+              t.ctypes.data_as(ctypes.c_void_p), t.size, g.ctypes.data_as(ctypes.c_void_p),
+              ctypes.byref(loss))
+        return float(loss.value), g
+
+    def train(self, q, X, t, steps, lr):
+        X, t = _f32(X, self.n_in), _f32(t)
+        loss = ctypes.c_float()
+        _call(self.L.clo_mlp_train, self.ptr, q.ptr, X.ctypes.data_as(ctypes.c_void_p),
+              t.ctypes.data_as(ctypes.c_void_p), t.size, steps, lr, ctypes.byref(loss))
+        return float(loss.value)
+
+    def predict(self, q, X):
+        X = _f32(X, self.n_in)
+        out = np.empty(X.shape[0], dtype=np.float32)
+        _call(self.L.clo_mlp_predict, self.ptr, q.ptr, X.ctypes.data_as(ctypes.c_void_p),
+              X.shape[0], out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def save(self, path):
+        _call(self.L.clo_mlp_save, self.ptr, _b(path))
+
+    def close(self):
+        if self.ptr:
+            self.L.clo_mlp_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

@@ -15,5 +15,6 @@
 #include "clo_scan_blelloch.h"
 #include "clo_util.h"
 #include "clo_graph.h"
+#include "clo_mlp.h"
 
 #endif

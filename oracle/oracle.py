@@ -171,3 +171,82 @@ def rng_fill(name, states, draws, bits=32, maxint=0, threads=1):
     out = np.empty(draws * G, dtype=np.uint32)
     lib().orc_rng_fill(_gen_id(name), _p(states), G, draws, bits, maxint, _p(out), threads)
     return out
+
+
+# --------------------------------------------------------------------------
+# FP64 restatement of the dispatcher's cost model (n_in -> 32 -> 32 -> 1,
+# ReLU), its mean-squared-error gradient and Adam, for checking the GPU
+# kernels.  No counterpart in the reference (the model is an addition); pinned
+# in tests/test_oracle.py by central finite differences and closed forms.
+
+MLP_H = 32
+
+
+def mlp_layout(n_in):
+    """Offsets of W1[32][n_in], b1, W2[32][32], b2, w3, b3 in the flat array."""
+    h = MLP_H
+    o = {"w1": 0, "b1": h * n_in}
+    o["w2"] = o["b1"] + h
+    o["b2"] = o["w2"] + h * h
+    o["w3"] = o["b2"] + h
+    o["b3"] = o["w3"] + h
+    o["count"] = o["b3"] + 1
+    return o
+
+
+def _mlp_unpack(p, n_in):
+    o, h = mlp_layout(n_in), MLP_H
+    p = np.asarray(p, dtype=np.float64)
+    return (p[o["w1"]:o["b1"]].reshape(h, n_in), p[o["b1"]:o["w2"]],
+            p[o["w2"]:o["b2"]].reshape(h, h), p[o["b2"]:o["w3"]], p[o["w3"]:o["b3"]],
+            p[o["b3"]])
+
+
+def mlp_forward(p, X):
+    X = np.asarray(X, dtype=np.float64)
+    W1, b1, W2, b2, w3, b3 = _mlp_unpack(p, X.shape[1])
+    z1 = X @ W1.T + b1
+    h1 = np.maximum(z1, 0.0)
+    z2 = h1 @ W2.T + b2
+    h2 = np.maximum(z2, 0.0)
+    return h2 @ w3 + b3
+
+
+def mlp_loss(p, X, t):
+    e = mlp_forward(p, X) - np.asarray(t, dtype=np.float64)
+    return 0.5 * float(np.mean(e * e))
+
+
+def mlp_grad(p, X, t):
+    """(loss, dloss/dp) of the mean 0.5 (y - t)^2, backpropagated in FP64."""
+    X = np.asarray(X, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    n_in, rows = X.shape[1], X.shape[0]
+    W1, b1, W2, b2, w3, b3 = _mlp_unpack(p, n_in)
+    z1 = X @ W1.T + b1
+    h1 = np.maximum(z1, 0.0)
+    z2 = h1 @ W2.T + b2
+    h2 = np.maximum(z2, 0.0)
+    y = h2 @ w3 + b3
+    e = y - t
+    dy = e / rows
+    d2 = np.outer(dy, w3) * (z2 > 0)
+    d1 = (d2 @ W2) * (z1 > 0)
+    o = mlp_layout(n_in)
+    g = np.zeros(o["count"])
+    g[o["w1"]:o["b1"]] = (d1.T @ X).ravel()
+    g[o["b1"]:o["w2"]] = d1.sum(0)
+    g[o["w2"]:o["b2"]] = (d2.T @ h1).ravel()
+    g[o["b2"]:o["w3"]] = d2.sum(0)
+    g[o["w3"]:o["b3"]] = dy @ h2
+    g[o["b3"]] = dy.sum()
+    return 0.5 * float(np.mean(e * e)), g
+
+
+def adam_step(p, m, v, g, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1); returns new (p, m, v)."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** step)
+    vh = v / (1 - b2 ** step)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -356,3 +356,56 @@ def test_host_mt_seeds_match_glib_grand_fixture(oracle):
         assert [int(x) for x in st.view(np.uint32)] == rec["first32"], seed
         st = oracle.rng_init_host_mt("mwc64x", 16, int(seed))  # 16 states x 2 words
         assert [int(x) for x in st.view(np.uint32)] == rec["first32"], seed
+
+
+# --------------------------------------------------------------------------
+# Cost-model MLP (FP64 oracle): finite differences and closed forms.
+
+def _mlp_case(oracle, n_in=7, rows=33, seed=5):
+    rng = np.random.default_rng(seed)
+    o = oracle.mlp_layout(n_in)
+    p = rng.normal(0, 0.4, o["count"])
+    X = rng.uniform(-1, 1, (rows, n_in))
+    t = rng.normal(0, 1, rows)
+    return p, X, t
+
+
+def test_mlp_gradient_finite_differences(oracle):
+    p, X, t = _mlp_case(oracle)
+    loss, g = oracle.mlp_grad(p, X, t)
+    assert loss == pytest.approx(oracle.mlp_loss(p, X, t), rel=1e-14)
+    h = 1e-6
+    for q in range(p.size):
+        e = np.zeros_like(p)
+        e[q] = h
+        fd = (oracle.mlp_loss(p + e, X, t) - oracle.mlp_loss(p - e, X, t)) / (2 * h)
+        assert g[q] == pytest.approx(fd, rel=1e-5, abs=1e-9), q
+
+
+def test_mlp_closed_forms(oracle):
+    n_in = 3
+    o = oracle.mlp_layout(n_in)
+    assert o["count"] == 32 * 3 + 32 + 32 * 32 + 32 + 32 + 1
+    p = np.zeros(o["count"])
+    p[o["b3"]] = 2.5  # all weights zero: y = b3 everywhere
+    X = np.ones((4, n_in))
+    assert np.allclose(oracle.mlp_forward(p, X), 2.5)
+    loss, g = oracle.mlp_grad(p, X, np.zeros(4))
+    assert loss == pytest.approx(0.5 * 2.5 ** 2)
+    assert g[o["b3"]] == pytest.approx(2.5) and np.count_nonzero(g) == 1
+    # one path: x0 -> unit 0 -> unit 0 -> y, all positive: y = w * x0 + b
+    p[:] = 0
+    p[o["w1"]] = 2.0
+    p[o["w2"]] = 3.0
+    p[o["w3"]] = 0.5
+    p[o["b3"]] = 1.0
+    X = np.array([[1.0, 9, 9], [2.0, 9, 9]])
+    assert np.allclose(oracle.mlp_forward(p, X), [4.0, 7.0])
+
+
+def test_adam_closed_form(oracle):
+    # first step moves every weight by lr * sign(g) (bias-corrected moments)
+    p = np.array([1.0, -2.0, 0.5])
+    g = np.array([0.3, -7.0, 1e-3])
+    p1, m, v = oracle.adam_step(p, np.zeros(3), np.zeros(3), g, 1, 0.01, eps=0.0)
+    assert np.allclose(p1, p - 0.01 * np.sign(g))
