@@ -29,6 +29,7 @@ SEED_DEV_GID, SEED_HOST_MT, SEED_EXT_DEV, SEED_EXT_HOST = 0, 1, 2, 3
 # CloGenKind (include/clo_util.h)
 GEN_U64, GEN_U32_HI, GEN_U64_MASK, GEN_U32_MASK, GEN_IOTA_U32, GEN_ZIPF_U64 = range(6)
 GEN_U32_MOD, GEN_U64_MOD = 6, 7
+GEN_NEARLY_SORTED_U32, GEN_NEARLY_SORTED_U64, GEN_REVERSED_U32, GEN_REVERSED_U64 = 8, 9, 10, 11
 
 _DTYPE_TO_CLO = {
     np.dtype(np.int8): CLO_CHAR, np.dtype(np.uint8): CLO_UCHAR,
@@ -146,11 +147,29 @@ def lib():
         # util
         "clo_util_generate": (vp, [vp, vp, sz, i32, u64, u64, u64, ep]),
         "clo_util_generate_zipf": (vp, [vp, vp, sz, vp, sz, u64, ep]),
+        "clo_util_generate_zipf_u32": (vp, [vp, vp, sz, vp, sz, u64, ep]),
         "clo_util_zipf_cdf": (None, [vp, sz, d]),
         "clo_util_copy": (vp, [vp, vp, vp, sz, ep]),
         "clo_util_multiset_hash": (u64, [vp, vp, sz, sz, ep]),
         "clo_util_check_sorted": (sz, [vp, vp, vp, sz, sz, ep]),
         "clo_util_build_info": (cp, []),
+        "clo_util_count_diff": (sz, [vp, vp, vp, sz, ep]),
+        # dispatcher
+        "clo_dispatch_new": (vp, [vp, ep]),
+        "clo_dispatch_destroy": (None, [vp]),
+        "clo_dispatch_config_name": (cp, [u32]),
+        "clo_dispatch_config_feasible": (u32, [u32, i32, sz, sz]),
+        "clo_dispatch_fixed_config": (u32, [i32, sz]),
+        "clo_dispatch_features": (u32, [vp, vp, i32, vp, sz, sz, vp, ep]),
+        "clo_dispatch_model_row": (None, [vp, u32, vp]),
+        "clo_dispatch_run": (vp, [vp, vp, u32, i32, vp, vp, sz, sz, ep]),
+        "clo_dispatch_tune": (u32, [vp, vp, i32, vp, vp, sz, sz, u32, vp, ep]),
+        "clo_dispatch_table_add": (None, [vp, i32, sz, sz, vp, vp]),
+        "clo_dispatch_table_choose": (i32, [vp, i32, sz, sz, vp]),
+        "clo_dispatch_table_save": (u32, [vp, cp, ep]),
+        "clo_dispatch_table_load": (u32, [vp, cp, ep]),
+        "clo_dispatch_set_model": (u32, [vp, vp, ep]),
+        "clo_dispatch_choose": (u32, [vp, vp, i32, sz, vp, vp, vp, vp, vp, ep]),
         # cost-model MLP
         "clo_mlp_new": (vp, [vp, u32, u64, ep]),
         "clo_mlp_destroy": (None, [vp]),
@@ -329,6 +348,103 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+
+OP_SORT, OP_SCAN = 0, 1
+POLICY_FIXED, POLICY_TABLE, POLICY_MODEL = 0, 1, 2
+N_CONFIGS, N_FEATURES = 9, 8
+MODEL_INPUTS = N_FEATURES + N_CONFIGS
+
+
+def _vp(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Dispatch:
+    """Configuration choice for sort/scan requests (clo_dispatch_*)."""
+
+    def __init__(self, ctx):
+        self.L = lib()
+        self.ctx = ctx
+        self.ptr = _call(self.L.clo_dispatch_new, ctx.ptr)
+        self.model = None
+
+    @staticmethod
+    def config_name(c):
+        return lib().clo_dispatch_config_name(c).decode()
+
+    @staticmethod
+    def feasible(c, op, n, width):
+        return bool(lib().clo_dispatch_config_feasible(c, op, n, width))
+
+    @staticmethod
+    def fixed(op, width):
+        return lib().clo_dispatch_fixed_config(op, width)
+
+    @staticmethod
+    def model_row(features, cfg):
+        f = _f32(features)
+        row = np.empty(MODEL_INPUTS, dtype=np.float32)
+        lib().clo_dispatch_model_row(_vp(f), cfg, _vp(row))
+        return row
+
+    def features(self, q, op, buf, n, width):
+        f = np.empty(N_FEATURES, dtype=np.float32)
+        _call(self.L.clo_dispatch_features, self.ptr, q.ptr, op, buf.ptr, n, width, _vp(f))
+        return f
+
+    def run(self, q, cfg, op, bin_, bout, n, width):
+        return _call(self.L.clo_dispatch_run, self.ptr, q.ptr, cfg, op, bin_.ptr, bout.ptr, n,
+                     width)
+
+    def tune(self, q, op, bin_, bout, n, width, reps=3):
+        t = np.empty(N_CONFIGS, dtype=np.float32)
+        _call(self.L.clo_dispatch_tune, self.ptr, q.ptr, op, bin_.ptr, bout.ptr, n, width, reps,
+              _vp(t))
+        return t
+
+    def table_add(self, op, n, width, features, times):
+        f, t = _f32(features), _f32(times)
+        self.L.clo_dispatch_table_add(self.ptr, op, n, width, _vp(f), _vp(t))
+
+    def table_choose(self, op, n, width, features):
+        f = _f32(features)
+        return self.L.clo_dispatch_table_choose(self.ptr, op, n, width, _vp(f))
+
+    def table_save(self, path):
+        _call(self.L.clo_dispatch_table_save, self.ptr, _b(path))
+
+    def table_load(self, path):
+        _call(self.L.clo_dispatch_table_load, self.ptr, _b(path))
+
+    def set_model(self, mlp):
+        self.model = mlp
+        _call(self.L.clo_dispatch_set_model, self.ptr, mlp.ptr if mlp else None)
+
+    def choose(self, q, policy, ops, ns, widths, features):
+        ops = np.ascontiguousarray(ops, dtype=np.int32)
+        ns = np.ascontiguousarray(ns, dtype=np.uint64)
+        ws = np.ascontiguousarray(widths, dtype=np.uint64)
+        f = _f32(features, N_FEATURES)
+        out = np.empty(ops.size, dtype=np.int32)
+        _call(self.L.clo_dispatch_choose, self.ptr, q.ptr if q else None, policy, ops.size,
+              _vp(ops), _vp(ns), _vp(ws), _vp(f), _vp(out))
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.L.clo_dispatch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def count_diff(q, a, b, words):
+    return _call(lib().clo_util_count_diff, q.ptr, a.ptr, b.ptr, words)
 
 
 class Graph:

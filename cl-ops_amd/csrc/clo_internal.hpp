@@ -162,3 +162,7 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	CCLBuffer* vin, CCLBuffer* kout, CCLBuffer* vout, CCLBuffer* offsets, size_t nseg,
 	size_t numel, uint32_t tile_items, uint32_t grid, void** cache, GError** err);
 void clo_satradix_segsort_cache_free(void* cache);
+// Plain blelloch scan with 8, 16 (the default) or 32 items per thread (8 and
+// 32 for unsigned 32/64-bit elements only).
+CCLEvent* clo_blelloch_scan_items(CloScan* scanner, CCLQueue* cq, CCLBuffer* in,
+	CCLBuffer* out, size_t numel, uint32_t items, GError** err);

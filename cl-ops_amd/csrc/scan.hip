@@ -19,6 +19,7 @@
 // restarts at every start.
 
 #include <cstring>
+#include <type_traits>
 
 #include "clo_internal.hpp"
 #include "dev_common.hpp"
@@ -29,8 +30,8 @@ using namespace clo::dev;
 namespace {
 
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 16;
-constexpr u32 kScanTile = kScanBlock * kScanItems;
+constexpr int kScanItems = 16;  // default items per thread (4096-element tiles)
+constexpr u32 kScanTileMax = kScanBlock * 32;
 constexpr int kTilesBlock = 1024;  // threads of the tile-sum scan
 
 __host__ __device__ constexpr u32 pad(u32 i) { return i + (i >> 5); }
@@ -93,8 +94,10 @@ __device__ __forceinline__ u32 first_at_or_after(const u32* off, u32 nseg, u64 x
 
 // Bitmap (in LDS) of the segment starts inside the tile at `base`.  Contains
 // barriers.
+template <int IT>
 __device__ void mark_heads(u32* s_head, u32* s_j0, const u32* __restrict__ off, u32 nseg,
 	u64 base) {
+	constexpr u32 kScanTile = kScanBlock * IT;
 	const u32 tid = threadIdx.x;
 	for (u32 i = tid; i < kScanTile / 32; i += kScanBlock) s_head[i] = 0;
 	if (tid == 0) *s_j0 = first_at_or_after(off, nseg, base);
@@ -109,11 +112,11 @@ __device__ void mark_heads(u32* s_head, u32* s_j0, const u32* __restrict__ off, 
 }
 
 // Stages the tile (striped, coalesced loads) in LDS converted to the sum type.
-template <typename In, typename Acc>
+template <typename In, typename Acc, int IT>
 __device__ __forceinline__ void stage_tile(Acc* s_t, const In* __restrict__ in, u64 base, u64 n) {
 	const u32 tid = threadIdx.x;
 #pragma unroll
-	for (int j = 0; j < kScanItems; ++j) {
+	for (int j = 0; j < IT; ++j) {
 		const u32 i = j * kScanBlock + tid;
 		const u64 g = base + i;
 		s_t[pad(i)] = g < n ? static_cast<Acc>(in[g]) : Acc(0);
@@ -122,16 +125,17 @@ __device__ __forceinline__ void stage_tile(Acc* s_t, const In* __restrict__ in, 
 }
 
 // Thread's blocked items: (any segment start, sum after the last start).
-template <typename Acc, bool SEG>
+template <typename Acc, bool SEG, int IT>
 __device__ __forceinline__ SegPair<Acc> thread_items(const Acc* s_t, const u32* s_head,
 	u64* heads_out) {
+	static_assert(IT <= 64, "one head bit per item");
 	const u32 tid = threadIdx.x;
 	u64 heads = 0;
 	Acc run = 0;
 	u32 any = 0;
 #pragma unroll
-	for (int j = 0; j < kScanItems; ++j) {
-		const u32 p = tid * kScanItems + j;
+	for (int j = 0; j < IT; ++j) {
+		const u32 p = tid * IT + j;
 		const Acc x = s_t[pad(p)];
 		if (SEG && ((s_head[p >> 5] >> (p & 31)) & 1u)) {
 			heads |= 1ull << j;
@@ -144,10 +148,11 @@ __device__ __forceinline__ SegPair<Acc> thread_items(const Acc* s_t, const u32* 
 	return {any, run};
 }
 
-template <typename In, typename Acc, bool SEG>
+template <typename In, typename Acc, bool SEG, int IT>
 __global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce(const In* __restrict__ in, u64 n,
 	u32 tiles, const u32* __restrict__ off, u32 nseg, Acc* __restrict__ tsum,
 	u32* __restrict__ tflag) {
+	constexpr u32 kScanTile = kScanBlock * IT;
 	__shared__ Acc s_t[SEG ? pad(kScanTile) + 1 : 1];
 	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
 	__shared__ Acc s_wsum[kScanBlock / kWave];
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce(const In* __r
 		// Order does not matter for a plain sum: striped loads, block reduce.
 		Acc s = 0;
 #pragma unroll
-		for (int j = 0; j < kScanItems; ++j) {
+		for (int j = 0; j < IT; ++j) {
 			const u64 g = base + j * kScanBlock + tid;
 			if (g < n) s += static_cast<Acc>(in[g]);
 		}
@@ -170,10 +175,10 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce(const In* __r
 		if (tid == 0) tsum[tile] = tot;
 		return;
 	}
-	mark_heads(s_head, &s_j0, off, nseg, base);
-	stage_tile<In, Acc>(s_t, in, base, n);
+	mark_heads<IT>(s_head, &s_j0, off, nseg, base);
+	stage_tile<In, Acc, IT>(s_t, in, base, n);
 	u64 heads;
-	const SegPair<Acc> mine = thread_items<Acc, true>(s_t, s_head, &heads);
+	const SegPair<Acc> mine = thread_items<Acc, true, IT>(s_t, s_head, &heads);
 	SegPair<Acc> tot;
 	block_exclusive_segscan<kScanBlock, Acc>(mine, s_pw, &tot);
 	if (tid == 0) {
@@ -183,34 +188,67 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce(const In* __r
 }
 
 // One block: tsum[t] <- carry into tile t (exclusive, segmented when SEG).
+// Rounds of kTilesBlock * kTilesItems tile sums are staged in LDS with
+// coalesced loads; each thread then combines kTilesItems consecutive ones.
+constexpr u32 kTilesItems = 8;
+constexpr u32 kTilesRound = kTilesBlock * kTilesItems;
+
 template <typename Acc, bool SEG>
 __global__ __launch_bounds__(kTilesBlock) void blelloch_tiles_scan(Acc* __restrict__ tsum,
-	const u32* __restrict__ tflag, u32 tiles) {
+	u32* __restrict__ tflag, u32 tiles) {
+	__shared__ Acc s_v[pad(kTilesRound)];
+	__shared__ u8 s_f[SEG ? kTilesRound : 1];
 	__shared__ Acc s_wsum[kTilesBlock / kWave];
 	__shared__ SegPair<Acc> s_pw[kTilesBlock / kWave];
-	const u32 per = (tiles + kTilesBlock - 1) / kTilesBlock;
-	const u32 t0 = threadIdx.x * per;
-	const u32 t1 = t0 + per < tiles ? t0 + per : tiles;
-	SegPair<Acc> mine = {0u, Acc(0)};
-	for (u32 t = t0; t < t1; ++t) mine = seg_combine(mine, SegPair<Acc>{SEG ? tflag[t] : 0u, tsum[t]});
-	SegPair<Acc> carry, tot;
-	if (SEG) {
-		carry = block_exclusive_segscan<kTilesBlock, Acc>(mine, s_pw, &tot);
-	} else {
-		Acc total;
-		carry = {0u, block_exclusive_sum<kTilesBlock>(mine.v, s_wsum, &total)};
-	}
-	for (u32 t = t0; t < t1; ++t) {
-		const SegPair<Acc> x = {SEG ? tflag[t] : 0u, tsum[t]};
-		tsum[t] = carry.v;
-		carry = seg_combine(carry, x);
+	const u32 tid = threadIdx.x;
+	SegPair<Acc> carry = {0u, Acc(0)};
+	for (u32 base = 0; base < tiles; base += kTilesRound) {
+#pragma unroll
+		for (u32 j = 0; j < kTilesItems; ++j) {
+			const u32 i = j * kTilesBlock + tid;
+			const bool in = base + i < tiles;
+			s_v[pad(i)] = in ? tsum[base + i] : Acc(0);
+			if (SEG) s_f[i] = in ? static_cast<u8>(tflag[base + i] != 0) : u8(0);
+		}
+		__syncthreads();
+		SegPair<Acc> x[kTilesItems];
+		SegPair<Acc> mine = {0u, Acc(0)};
+#pragma unroll
+		for (u32 k = 0; k < kTilesItems; ++k) {
+			const u32 i = tid * kTilesItems + k;
+			x[k] = {SEG ? static_cast<u32>(s_f[i]) : 0u, s_v[pad(i)]};
+			mine = seg_combine(mine, x[k]);
+		}
+		SegPair<Acc> run, tot;
+		if (SEG) {
+			run = block_exclusive_segscan<kTilesBlock, Acc>(mine, s_pw, &tot);
+		} else {
+			Acc total;
+			run = {0u, block_exclusive_sum<kTilesBlock>(mine.v, s_wsum, &total)};
+			tot = {0u, total};
+		}
+		run = seg_combine(carry, run);
+#pragma unroll
+		for (u32 k = 0; k < kTilesItems; ++k) {
+			s_v[pad(tid * kTilesItems + k)] = run.v;
+			run = seg_combine(run, x[k]);
+		}
+		__syncthreads();
+#pragma unroll
+		for (u32 j = 0; j < kTilesItems; ++j) {
+			const u32 i = j * kTilesBlock + tid;
+			if (base + i < tiles) tsum[base + i] = s_v[pad(i)];
+		}
+		carry = seg_combine(carry, tot);
+		__syncthreads();
 	}
 }
 
-template <typename In, typename Acc, int OUTW, bool SEG>
+template <typename In, typename Acc, int OUTW, bool SEG, int IT>
 __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan(const In* __restrict__ in,
 	void* __restrict__ out, u64 n, u32 tiles, const Acc* __restrict__ carry_in,
 	const u32* __restrict__ off, u32 nseg) {
+	constexpr u32 kScanTile = kScanBlock * IT;
 	__shared__ Acc s_t[pad(kScanTile) + 1];
 	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
 	__shared__ Acc s_wsum[kScanBlock / kWave];
@@ -221,10 +259,10 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan(const In* __res
 	const u64 base = static_cast<u64>(tile) * kScanTile;
 	const u32 tid = threadIdx.x;
 	const Acc carry = carry_in[tile];
-	if (SEG) mark_heads(s_head, &s_j0, off, nseg, base);
-	stage_tile<In, Acc>(s_t, in, base, n);
+	if (SEG) mark_heads<IT>(s_head, &s_j0, off, nseg, base);
+	stage_tile<In, Acc, IT>(s_t, in, base, n);
 	u64 heads;
-	const SegPair<Acc> mine = thread_items<Acc, SEG>(s_t, s_head, &heads);
+	const SegPair<Acc> mine = thread_items<Acc, SEG, IT>(s_t, s_head, &heads);
 	Acc acc;
 	if (SEG) {
 		SegPair<Acc> tot;
@@ -235,16 +273,16 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan(const In* __res
 		acc = carry + block_exclusive_sum<kScanBlock>(mine.v, s_wsum, &tot);
 	}
 #pragma unroll
-	for (int j = 0; j < kScanItems; ++j) {
+	for (int j = 0; j < IT; ++j) {
 		if (SEG && ((heads >> j) & 1ull)) acc = 0;
-		const u32 p = pad(tid * kScanItems + j);
+		const u32 p = pad(tid * IT + j);
 		const Acc x = s_t[p];
 		s_t[p] = acc;
 		acc += x;
 	}
 	__syncthreads();
 #pragma unroll
-	for (int j = 0; j < kScanItems; ++j) {
+	for (int j = 0; j < IT; ++j) {
 		const u32 i = j * kScanBlock + tid;
 		const u64 g = base + i;
 		if (g < n) {
@@ -270,10 +308,11 @@ struct ScanLaunch {
 	u32* tflag;
 	const u32* off;  // segment starts (segmented scan) or nullptr
 	u32 nseg;
+	u32 items;  // items per thread: 16, or 8 / 32 for unsigned plain scans
 	CCLQueue* q;
 };
 
-template <typename In, typename Acc, int OUTW, bool SEG>
+template <typename In, typename Acc, int OUTW, bool SEG, int IT>
 bool launch_scan3(const ScanLaunch& a, GError** err) {
 	hipStream_t st = a.q->stream;
 	const u32 grid = xcd_grid(a.tiles);
@@ -281,7 +320,7 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 	Acc* tsum = static_cast<Acc*>(a.tsum);
 	{
 		Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_REDUCE);
-		blelloch_tile_reduce<In, Acc, SEG><<<grid, kScanBlock, 0, st>>>(in, a.n, a.tiles, a.off,
+		blelloch_tile_reduce<In, Acc, SEG, IT><<<grid, kScanBlock, 0, st>>>(in, a.n, a.tiles, a.off,
 			a.nseg, tsum, a.tflag);
 		if (!L.done(err)) return false;
 	}
@@ -292,7 +331,7 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 	}
 	{
 		Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_SCAN);
-		blelloch_tile_scan<In, Acc, OUTW, SEG><<<grid, kScanBlock, 0, st>>>(in, a.out, a.n,
+		blelloch_tile_scan<In, Acc, OUTW, SEG, IT><<<grid, kScanBlock, 0, st>>>(in, a.out, a.n,
 			a.tiles, tsum, a.off, a.nseg);
 		if (!L.done(err)) return false;
 	}
@@ -301,7 +340,17 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 
 template <typename In, typename Acc, int OUTW>
 bool launch_scan(const ScanLaunch& a, GError** err) {
-	return a.off ? launch_scan3<In, Acc, OUTW, true>(a, err) : launch_scan3<In, Acc, OUTW, false>(a, err);
+	if (a.off) return launch_scan3<In, Acc, OUTW, true, kScanItems>(a, err);
+	if constexpr ((std::is_same<In, u32>::value || std::is_same<In, u64>::value) && OUTW >= 4) {
+		if (a.items == 8) return launch_scan3<In, Acc, OUTW, false, 8>(a, err);
+		if (a.items == 32) return launch_scan3<In, Acc, OUTW, false, 32>(a, err);
+	}
+	if (a.items != kScanItems) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"items per thread %u needs an unsigned 32/64-bit plain scan", a.items);
+		return false;
+	}
+	return launch_scan3<In, Acc, OUTW, false, kScanItems>(a, err);
 }
 
 template <typename In>
@@ -332,10 +381,15 @@ void blelloch_finalize(CloScan* scanner) {
 }
 
 CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
-	CCLBuffer* data_out, size_t numel, const u32* off, u32 nseg, GError** err) {
+	CCLBuffer* data_out, size_t numel, const u32* off, u32 nseg, GError** err,
+	u32 items = kScanItems) {
 	BlellochData* bd = static_cast<BlellochData*>(scanner->data);
 	if (numel == 0) return record_event(cq_exec, "blelloch_done", err);
-	const size_t tiles = div_ceil(numel, kScanTile);
+	if (items != 8 && items != 16 && items != 32) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "items per thread must be 8, 16 or 32");
+		return nullptr;
+	}
+	const size_t tiles = div_ceil(numel, static_cast<size_t>(kScanBlock) * items);
 	if (tiles > 0x7FFFFFFFull) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "numel too large");
 		return nullptr;
@@ -353,6 +407,7 @@ CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
 	a.tflag = reinterpret_cast<u32*>(static_cast<char*>(bd->tiles.ptr) + sum_bytes);
 	a.off = off;
 	a.nseg = nseg;
+	a.items = items;
 	a.q = cq_exec;
 	bool ok = false;
 	switch (scanner->elem_type) {
@@ -404,11 +459,25 @@ size_t blelloch_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, 
 		return 0;
 	}
 	const size_t acc = clo_type_sizeof(scanner->sum_type) > 4 ? 8 : 4;
-	if (i == 1) return (kTilesBlock / kWave) * (acc + 8);
-	return (kScanTile + kScanTile / 32 + 1 + 2 * (kScanBlock / kWave)) * acc + kScanTile / 8 + 4;
+	constexpr size_t kScanTile = static_cast<size_t>(kScanBlock) * kScanItems;
+	if (i == 1) return pad(kTilesRound) * acc + kTilesRound + (kTilesBlock / kWave) * (acc + 8);
+	return (pad(kScanTile) + 1 + 2 * (kScanBlock / kWave)) * acc + kScanTile / 8 + 4;
 }
 
 }  // namespace
+
+CCLEvent* clo_blelloch_scan_items(CloScan* scanner, CCLQueue* cq, CCLBuffer* in, CCLBuffer* out,
+	size_t numel, uint32_t items, GError** err) {
+	if (err && *err) return nullptr;
+	if (!scanner || !cq || !in || !out || strcmp(scanner->impl.name, "blelloch") != 0 ||
+		in->size < numel * clo_type_sizeof(scanner->elem_type) ||
+		out->size < numel * clo_type_sizeof(scanner->sum_type)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "bad blelloch scan arguments");
+		return nullptr;
+	}
+	if (!use_device(cq->ctx, err)) return nullptr;
+	return scan_common(scanner, cq, in, out, numel, nullptr, 0, err, items);
+}
 
 extern "C" {
 

@@ -13,6 +13,7 @@ using namespace clo::dev;
 namespace {
 
 constexpr u64 kGolden = 0x9E3779B97F4A7C15ull;
+constexpr u64 kPerturb = 0x5bd1e995ull;
 
 // splitmix64 finalizer (Steele, Lea & Flood, OOPSLA 2014).
 __host__ __device__ __forceinline__ u64 mix64(u64 z) {
@@ -51,11 +52,25 @@ __global__ __launch_bounds__(256) void util_generate(void* out, u64 n, u64 seed,
 		case CLO_GEN_U64_MOD:
 			static_cast<u64*>(out)[i] = U(seed, c) % param;
 			break;
+		case CLO_GEN_NEARLY_SORTED_U32:
+			static_cast<u32*>(out)[i] = U(seed ^ kPerturb, c) % 100 == 0
+				? static_cast<u32>(U(seed, c) >> 32) : static_cast<u32>(c * param);
+			break;
+		case CLO_GEN_NEARLY_SORTED_U64:
+			static_cast<u64*>(out)[i] = U(seed ^ kPerturb, c) % 100 == 0 ? U(seed, c) : c * param;
+			break;
+		case CLO_GEN_REVERSED_U32:
+			static_cast<u32*>(out)[i] = 0xFFFFFFFFu - static_cast<u32>(c * param);
+			break;
+		case CLO_GEN_REVERSED_U64:
+			static_cast<u64*>(out)[i] = ~0ull - c * param;
+			break;
 		}
 	}
 }
 
-__global__ __launch_bounds__(256) void util_generate_zipf(u64* out, u64 n, const double* cdf,
+template <typename T>
+__global__ __launch_bounds__(256) void util_generate_zipf(T* out, u64 n, const double* cdf,
 	u64 ranks, u64 seed) {
 	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
 	for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -67,7 +82,8 @@ __global__ __launch_bounds__(256) void util_generate_zipf(u64* out, u64 n, const
 			if (cdf[mid] > u) hi = mid;
 			else lo = mid + 1;
 		}
-		out[i] = mix64(lo);
+		const u64 key = mix64(lo);
+		out[i] = sizeof(T) == 8 ? static_cast<T>(key) : static_cast<T>(key >> 32);
 	}
 }
 
@@ -105,6 +121,17 @@ __global__ __launch_bounds__(256) void util_check_sorted(const T* k, const u32* 
 		static_cast<unsigned long long>(c));
 }
 
+__global__ __launch_bounds__(256) void util_count_diff(const u32* a, const u32* b, u64 n,
+	u64* bad) {
+	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+	u64 c = 0;
+	for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+		c += a[i] != b[i] ? 1u : 0u;
+	c = wave_sum(c);
+	if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(bad),
+		static_cast<unsigned long long>(c));
+}
+
 u32 grid_for(u64 n) {
 	u64 g = (n + 255) / 256;
 	if (g > 8192) g = 8192;
@@ -123,8 +150,9 @@ CCLEvent* clo_util_generate(CCLQueue* cq, CCLBuffer* out, size_t n, CloGenKind k
 		return nullptr;
 	}
 	size_t w = (kind == CLO_GEN_U64 || kind == CLO_GEN_U64_MASK || kind == CLO_GEN_ZIPF_U64 ||
-		kind == CLO_GEN_U64_MOD) ? 8 : 4;
-	if (kind == CLO_GEN_ZIPF_U64 || static_cast<int>(kind) < 0 || kind > CLO_GEN_U64_MOD) {
+		kind == CLO_GEN_U64_MOD || kind == CLO_GEN_NEARLY_SORTED_U64 ||
+		kind == CLO_GEN_REVERSED_U64) ? 8 : 4;
+	if (kind == CLO_GEN_ZIPF_U64 || static_cast<int>(kind) < 0 || kind > CLO_GEN_REVERSED_U64) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "generator kind %d not valid here", kind);
 		return nullptr;
 	}
@@ -147,6 +175,10 @@ CCLEvent* clo_util_generate(CCLQueue* cq, CCLBuffer* out, size_t n, CloGenKind k
 	case CLO_GEN_IOTA_U32: util_generate<CLO_GEN_IOTA_U32><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
 	case CLO_GEN_U32_MOD: util_generate<CLO_GEN_U32_MOD><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
 	case CLO_GEN_U64_MOD: util_generate<CLO_GEN_U64_MOD><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_NEARLY_SORTED_U32: util_generate<CLO_GEN_NEARLY_SORTED_U32><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_NEARLY_SORTED_U64: util_generate<CLO_GEN_NEARLY_SORTED_U64><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_REVERSED_U32: util_generate<CLO_GEN_REVERSED_U32><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
+	case CLO_GEN_REVERSED_U64: util_generate<CLO_GEN_REVERSED_U64><<<g, 256, 0, cq->stream>>>(out->ptr, n, seed, param, start); break;
 	default: break;
 	}
 	if (!L.done(err)) return nullptr;
@@ -165,18 +197,39 @@ void clo_util_zipf_cdf(double* cdf, size_t ranks, double s) {
 	if (ranks) cdf[ranks - 1] = 1.0;
 }
 
-CCLEvent* clo_util_generate_zipf(CCLQueue* cq, CCLBuffer* out, size_t n, CCLBuffer* cdf,
-	size_t ranks, cl_ulong seed, GError** err) {
-	if (!cq || !out || !cdf || ranks == 0 || out->size < n * 8 || cdf->size < ranks * 8) {
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+CCLEvent* generate_zipf(CCLQueue* cq, CCLBuffer* out, size_t n, CCLBuffer* cdf, size_t ranks,
+	cl_ulong seed, GError** err) {
+	if (err && *err) return nullptr;
+	if (!cq || !out || !cdf || ranks == 0 || out->size < n * sizeof(T) ||
+		cdf->size < ranks * 8) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "invalid zipf generator arguments");
 		return nullptr;
 	}
 	if (!use_device(cq->ctx, err)) return nullptr;
 	Launch L(cq, "util_generate_zipf");
-	util_generate_zipf<<<grid_for(n), 256, 0, cq->stream>>>(static_cast<u64*>(out->ptr), n,
+	util_generate_zipf<T><<<grid_for(n), 256, 0, cq->stream>>>(static_cast<T*>(out->ptr), n,
 		static_cast<const double*>(cdf->ptr), ranks, seed);
 	if (!L.done(err)) return nullptr;
 	return record_event(cq, "util_generate_zipf", err);
+}
+
+}  // namespace
+
+extern "C" {
+
+CCLEvent* clo_util_generate_zipf(CCLQueue* cq, CCLBuffer* out, size_t n, CCLBuffer* cdf,
+	size_t ranks, cl_ulong seed, GError** err) {
+	return generate_zipf<u64>(cq, out, n, cdf, ranks, seed, err);
+}
+
+CCLEvent* clo_util_generate_zipf_u32(CCLQueue* cq, CCLBuffer* out, size_t n, CCLBuffer* cdf,
+	size_t ranks, cl_ulong seed, GError** err) {
+	return generate_zipf<u32>(cq, out, n, cdf, ranks, seed, err);
 }
 
 CCLEvent* clo_util_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src, size_t bytes,
@@ -269,8 +322,26 @@ size_t clo_util_check_sorted(CCLQueue* cq, CCLBuffer* keys, CCLBuffer* vals, siz
 	return static_cast<size_t>(r);
 }
 
+size_t clo_util_count_diff(CCLQueue* cq, CCLBuffer* a, CCLBuffer* b, size_t words,
+	GError** err) {
+	if (!cq || !a || !b || a->size < words * 4 || b->size < words * 4) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "invalid compare arguments");
+		return static_cast<size_t>(-1);
+	}
+	if (!use_device(cq->ctx, err)) return static_cast<size_t>(-1);
+	u64* bad = nullptr;
+	if (!scratch_u64(cq, &bad, err)) return static_cast<size_t>(-1);
+	u32 g = grid_for(words);
+	if (g > 2048) g = 2048;
+	util_count_diff<<<g, 256, 0, cq->stream>>>(static_cast<const u32*>(a->ptr),
+		static_cast<const u32*>(b->ptr), words, bad);
+	u64 r = 0;
+	if (!read_u64(cq, bad, &r, err)) return static_cast<size_t>(-1);
+	return static_cast<size_t>(r);
+}
+
 const char* clo_util_build_info(void) {
-	return "cl-ops MI355X: gfx950, clang " __clang_version__ ", wave64, satradix onesweep 256x16";
+	return "cl-ops MI355X: gfx950, clang " __clang_version__ ", wave64, satradix reduce-then-scan";
 }
 
 }  // extern "C"

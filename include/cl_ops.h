@@ -16,5 +16,6 @@
 #include "clo_util.h"
 #include "clo_graph.h"
 #include "clo_mlp.h"
+#include "clo_dispatch.h"
 
 #endif

@@ -1,0 +1,111 @@
+/*
+ * clo_dispatch.h - per-request choice of kernel configuration for sorts and
+ * scans: an exhaustive autotuner, its persisted per-bucket table, and a
+ * learned cost model (clo_mlp.h).  No counterpart in CL_Ops (an addition);
+ * every configuration computes the same bit-exact result, so the choice only
+ * changes the time taken.
+ *
+ * Requests: sort (keys only, ascending, unsigned 32- or 64-bit keys, out of
+ * place in -> out) or scan (exclusive, element = sum type = the same
+ * unsigned width, in -> out).
+ *
+ * Configurations (index: name, op, feasibility):
+ *   0 satradix 256x32      sort   any n
+ *   1 satradix 512x16      sort   any n
+ *   2 satradix 256x16      sort   any n
+ *   3 abitonic             sort   any n
+ *   4 segment-local sort   sort   n <= 65536 (one segment, LDS-resident)
+ *   5 gselect              sort   n <= 4096 (rank sort)
+ *   6 blelloch 8 items     scan   any n
+ *   7 blelloch 16 items    scan   any n (the plain clo_scan path)
+ *   8 blelloch 32 items    scan   any n
+ * The fixed configuration (what the plain API runs): satradix's default
+ * tile shape (0 for 32-bit keys, 1 for 64-bit keys), and 7 for scans.
+ *
+ * Features (CLO_DISPATCH_NUM_FEATURES floats, from a GPU kernel over a
+ * sample of up to 4096 evenly spaced elements):
+ *   0 log2(n) / 32                 4 fraction of constant key bytes
+ *   1 key bytes / 8                5 fraction of sampled neighbour pairs
+ *   2 mean key-byte entropy / 8      (x[i], x[i+1]) in ascending order
+ *   3 max key-byte entropy / 8     6 fraction in descending order
+ *                                  7 op (0 sort, 1 scan)
+ * Model input row (CLO_DISPATCH_MODEL_INPUTS floats) = the features followed
+ * by a one-hot of the configuration; the model predicts log2 of the time in
+ * microseconds, and the choice is the feasible configuration with the least
+ * prediction.
+ */
+#ifndef CLO_DISPATCH_H_
+#define CLO_DISPATCH_H_
+
+#include "clo_ccl.h"
+#include "clo_mlp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLO_DISPATCH_NUM_CONFIGS 9
+#define CLO_DISPATCH_NUM_FEATURES 8
+#define CLO_DISPATCH_MODEL_INPUTS (CLO_DISPATCH_NUM_FEATURES + CLO_DISPATCH_NUM_CONFIGS)
+
+typedef enum { CLO_OP_SORT = 0, CLO_OP_SCAN = 1 } CloOp;
+
+typedef enum {
+	CLO_POLICY_FIXED = 0,  /* the plain path's configuration */
+	CLO_POLICY_TABLE = 1,  /* the tuner table's best for the request's bucket, else fixed */
+	CLO_POLICY_MODEL = 2   /* the learned model's choice, else the table, else fixed */
+} CloPolicy;
+
+typedef struct clo_dispatch CloDispatch;
+
+CloDispatch* clo_dispatch_new(CCLContext* ctx, GError** err);
+void clo_dispatch_destroy(CloDispatch* d);
+
+const char* clo_dispatch_config_name(cl_uint cfg);
+cl_bool clo_dispatch_config_feasible(cl_uint cfg, CloOp op, size_t n, size_t width);
+cl_uint clo_dispatch_fixed_config(CloOp op, size_t width);
+
+/* Features of a request (blocking: the sample statistics come back to the
+ * host).  width = 4 or 8 bytes. */
+cl_bool clo_dispatch_features(CloDispatch* d, CCLQueue* cq, CloOp op, CCLBuffer* data,
+	size_t n, size_t width, float* features, GError** err);
+/* Model input row for one configuration. */
+void clo_dispatch_model_row(const float* features, cl_uint cfg, float* row);
+
+/* Runs configuration cfg on the request (asynchronous). */
+CCLEvent* clo_dispatch_run(CloDispatch* d, CCLQueue* cq, cl_uint cfg, CloOp op, CCLBuffer* in,
+	CCLBuffer* out, size_t n, size_t width, GError** err);
+
+/* Exhaustive tuner: every feasible configuration of the op runs once
+ * untimed, then `reps` times timed with events on cq; times_ms[cfg] = the
+ * least time (ms), -1 for configurations that are not feasible or belong to
+ * the other op.  Blocking. */
+cl_bool clo_dispatch_tune(CloDispatch* d, CCLQueue* cq, CloOp op, CCLBuffer* in,
+	CCLBuffer* out, size_t n, size_t width, cl_uint reps, float* times_ms, GError** err);
+
+/* Tuner table: buckets of (op, width, floor(log2 n), order class) keep the
+ * mean log2 time of every configuration timed in them. */
+void clo_dispatch_table_add(CloDispatch* d, CloOp op, size_t n, size_t width,
+	const float* features, const float* times_ms);
+/* Best configuration of the request's bucket, -1 if the bucket is empty. */
+int clo_dispatch_table_choose(CloDispatch* d, CloOp op, size_t n, size_t width,
+	const float* features);
+/* Text file "CLOTUNE01" followed by one bucket entry per line. */
+cl_bool clo_dispatch_table_save(CloDispatch* d, const char* path, GError** err);
+cl_bool clo_dispatch_table_load(CloDispatch* d, const char* path, GError** err);
+
+/* Learned model (borrowed; NULL removes it).  Its input count must be
+ * CLO_DISPATCH_MODEL_INPUTS. */
+cl_bool clo_dispatch_set_model(CloDispatch* d, CloMlp* mlp, GError** err);
+/* Batched choice for `count` requests with one model launch over every
+ * (request, feasible configuration) row.  ops/ns/widths/features
+ * (count x NUM_FEATURES) describe the requests; cfg_out receives the
+ * choices.  Policy as in CloPolicy. */
+cl_bool clo_dispatch_choose(CloDispatch* d, CCLQueue* cq, CloPolicy policy, size_t count,
+	const int* ops, const size_t* ns, const size_t* widths, const float* features, int* cfg_out,
+	GError** err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

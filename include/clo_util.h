@@ -23,7 +23,15 @@ typedef enum {
 	CLO_GEN_IOTA_U32 = 4,  /* (uint32) i */
 	CLO_GEN_ZIPF_U64 = 5,  /* mix64(rank), rank from the CDF table, see below */
 	CLO_GEN_U32_MOD = 6,   /* (U(seed, i) >> 32) % param */
-	CLO_GEN_U64_MOD = 7    /* U(seed, i) % param */
+	CLO_GEN_U64_MOD = 7,   /* U(seed, i) % param */
+	/* Ascending i * param, except that 1% of the positions (those with
+	 * U(seed ^ 0x5bd1e995, i) % 100 == 0) hold a random word (U(seed, i) >> 32
+	 * or U(seed, i)): "presorted with random perturbations". */
+	CLO_GEN_NEARLY_SORTED_U32 = 8,
+	CLO_GEN_NEARLY_SORTED_U64 = 9,
+	/* Descending: all-ones minus i * param. */
+	CLO_GEN_REVERSED_U32 = 10,
+	CLO_GEN_REVERSED_U64 = 11
 } CloGenKind;
 
 /* Writes n words of the kind's width (4 or 8 bytes) to out at element
@@ -37,6 +45,9 @@ CCLEvent* clo_util_generate(CCLQueue* cq, CCLBuffer* out, size_t n,
  * cdf[r] = P(rank <= r) (built on the host by clo_util_zipf_cdf).  Element
  * i = mix64(rank) where rank is the smallest r with cdf[r] > u,
  * u = (U(seed, i) >> 11) * 2^-53. */
+/* As clo_util_generate_zipf with 32-bit elements: (uint32)(mix64(rank) >> 32). */
+CCLEvent* clo_util_generate_zipf_u32(CCLQueue* cq, CCLBuffer* out, size_t n,
+	CCLBuffer* cdf, size_t ranks, cl_ulong seed, GError** err);
 CCLEvent* clo_util_generate_zipf(CCLQueue* cq, CCLBuffer* out, size_t n,
 	CCLBuffer* cdf, size_t ranks, cl_ulong seed, GError** err);
 void clo_util_zipf_cdf(double* cdf, size_t ranks, double s);
@@ -60,6 +71,11 @@ size_t clo_util_check_sorted(CCLQueue* cq, CCLBuffer* keys, CCLBuffer* vals,
 
 /* Library build description (compiler, target, kernel variants). */
 const char* clo_util_build_info(void);
+
+/* Number of 32-bit words that differ between two device buffers of `words`
+ * words (blocking). */
+size_t clo_util_count_diff(CCLQueue* cq, CCLBuffer* a, CCLBuffer* b, size_t words,
+	GError** err);
 
 #ifdef __cplusplus
 }

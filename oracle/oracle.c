@@ -83,6 +83,16 @@ void orc_generate(int kind, uint64_t seed, uint64_t param, uint64_t start, size_
 		case 4: ((uint32_t*)out)[i] = (uint32_t)c; break;
 		case 6: ((uint32_t*)out)[i] = (uint32_t)(orc_U(seed, c) >> 32) % (uint32_t)param; break;
 		case 7: ((uint64_t*)out)[i] = orc_U(seed, c) % param; break;
+		case 8:
+			((uint32_t*)out)[i] = orc_U(seed ^ 0x5bd1e995ull, c) % 100 == 0
+				? (uint32_t)(orc_U(seed, c) >> 32) : (uint32_t)(c * param);
+			break;
+		case 9:
+			((uint64_t*)out)[i] = orc_U(seed ^ 0x5bd1e995ull, c) % 100 == 0 ? orc_U(seed, c)
+				: c * param;
+			break;
+		case 10: ((uint32_t*)out)[i] = 0xFFFFFFFFu - (uint32_t)(c * param); break;
+		case 11: ((uint64_t*)out)[i] = ~0ull - c * param; break;
 		default: break;
 		}
 	}

@@ -14,6 +14,7 @@ _LIB = None
 
 GEN_U64, GEN_U32_HI, GEN_U64_MASK, GEN_U32_MASK, GEN_IOTA_U32 = 0, 1, 2, 3, 4
 GEN_U32_MOD, GEN_U64_MOD = 6, 7
+GEN_NEARLY_SORTED_U32, GEN_NEARLY_SORTED_U64, GEN_REVERSED_U32, GEN_REVERSED_U64 = 8, 9, 10, 11
 RNG_NAMES = ["lcg", "xorshift64", "xorshift128", "mwc64x", "parkmiller"]
 STATE_SIZE = {"lcg": 8, "xorshift64": 8, "xorshift128": 16, "mwc64x": 8, "parkmiller": 4}
 
@@ -66,7 +67,8 @@ def mix64(z):
 
 
 def generate(kind, seed, n, param=0, start=0):
-    dt = np.uint64 if kind in (GEN_U64, GEN_U64_MASK, GEN_U64_MOD) else np.uint32
+    dt = np.uint64 if kind in (GEN_U64, GEN_U64_MASK, GEN_U64_MOD, GEN_NEARLY_SORTED_U64,
+                               GEN_REVERSED_U64) else np.uint32
     out = np.empty(n, dtype=dt)
     lib().orc_generate(kind, seed, param, start, n, _p(out))
     return out

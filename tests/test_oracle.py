@@ -35,6 +35,22 @@ def test_generator_kinds(oracle):
                           (u >> np.uint64(32)).astype(np.uint32) % np.uint32(1000))
     assert np.array_equal(oracle.generate(oracle.GEN_IOTA_U32, 0, 10, start=3),
                           np.arange(3, 13, dtype=np.uint32))
+    # presorted with 1% perturbed positions; reversed
+    n = 20000
+    ns = oracle.generate(oracle.GEN_NEARLY_SORTED_U64, 77, n, param=1000)
+    hit = oracle.generate(oracle.GEN_U64, 77 ^ 0x5bd1e995, n) % np.uint64(100) == 0
+    full = oracle.generate(oracle.GEN_U64, 77, n)
+    idx = np.arange(n, dtype=np.uint64)
+    assert np.array_equal(ns, np.where(hit, full, idx * np.uint64(1000)))
+    assert 0.005 < hit.mean() < 0.015
+    ns32 = oracle.generate(oracle.GEN_NEARLY_SORTED_U32, 77, n, param=1000)
+    assert np.array_equal(ns32, np.where(hit, (full >> np.uint64(32)).astype(np.uint32),
+                                         (idx * np.uint64(1000)).astype(np.uint32)))
+    rv = oracle.generate(oracle.GEN_REVERSED_U32, 0, n, param=7)
+    assert np.array_equal(rv, (0xFFFFFFFF - 7 * idx).astype(np.uint32))
+    assert np.all(np.diff(rv.astype(np.int64)) < 0)
+    rv64 = oracle.generate(oracle.GEN_REVERSED_U64, 0, n, param=3)
+    assert rv64[0] == np.uint64(2 ** 64 - 1) and np.all(rv64[:-1] > rv64[1:])
 
 
 def test_zipf_distribution(oracle):
