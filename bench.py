@@ -10,6 +10,10 @@ timed region.  `cpu_baseline` times the oracle's reference-structured satradix
 (4-bit digits, 256-element blocks, blocks spread over host threads) on a
 bounded sample of the same workload.
 
+`dispatch` reports the learned dispatcher on the mixed sort/scan request
+stream: regret against the exhaustive tuner and speedup over the fixed
+configuration (see dispatch_eval).
+
 Other modes (not the headline line):
   --copy-probe     runs only the calibration copy kernel (for memory counters)
   --suite          also measures the other configurations, one JSON line each
@@ -280,6 +284,148 @@ def suite_segmented(ctx, q, baseline_stride=16):
     return res
 
 
+M64 = (1 << 64) - 1
+DISTS = ("uniform", "16 unique", "presorted+1%", "reversed", "zipf1.1")
+
+
+def _mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def _U(seed, i):
+    return _mix64((seed + (i + 1) * 0x9E3779B97F4A7C15) & M64)
+
+
+def request_stream(seed, count, max_log2=28):
+    """Request j of the mixed stream: size floor(2^(10 + 18 u)) with
+    u = (U(seed, 4j) >> 11) 2^-53 (capped at 2^max_log2), key width 4 or 8,
+    sort or scan, one of five distributions; all from U(seed, 4j..4j+3)."""
+    out = []
+    for j in range(count):
+        u = (_U(seed, 4 * j) >> 11) * 2.0 ** -53
+        n = min(int(2.0 ** (10 + 18 * u)), 1 << max_log2)
+        width = 8 if _U(seed, 4 * j + 1) & 1 else 4
+        op = clops.OP_SCAN if _U(seed, 4 * j + 2) & 1 else clops.OP_SORT
+        dist = _U(seed, 4 * j + 3) % 5
+        out.append((op, n, width, dist, _U(seed, 4 * j + 3) >> 8))
+    return out
+
+
+class RequestData:
+    """Device buffers big enough for any request, filled per request."""
+
+    def __init__(self, ctx, q, max_n):
+        self.q = q
+        self.inb = clops.Buffer(ctx, max_n * 8)
+        self.outb = clops.Buffer(ctx, max_n * 8)
+        self.out2 = clops.Buffer(ctx, max_n * 8)
+        import ctypes
+        cdfh = np.empty(1 << 24, dtype=np.float64)
+        clops.lib().clo_util_zipf_cdf(cdfh.ctypes.data_as(ctypes.c_void_p), cdfh.size, 1.1)
+        self.cdf = clops.Buffer.from_array(ctx, cdfh)
+        self.ranks = cdfh.size
+
+    def fill(self, n, width, dist, seed):
+        q, b = self.q, self.inb
+        w8 = width == 8
+        step = max(((1 << (8 * width)) - 1) // max(n, 1), 1)
+        if dist == 0:
+            clops.generate(q, b, n, clops.GEN_U64 if w8 else clops.GEN_U32_HI, seed)
+        elif dist == 1:
+            clops.generate(q, b, n, clops.GEN_U64_MOD if w8 else clops.GEN_U32_MOD, seed, 16)
+        elif dist == 2:
+            clops.generate(q, b, n, clops.GEN_NEARLY_SORTED_U64 if w8 else
+                           clops.GEN_NEARLY_SORTED_U32, seed, step)
+        elif dist == 3:
+            clops.generate(q, b, n, clops.GEN_REVERSED_U64 if w8 else clops.GEN_REVERSED_U32,
+                           seed, step)
+        else:
+            fn = clops.lib().clo_util_generate_zipf if w8 else \
+                clops.lib().clo_util_generate_zipf_u32
+            clops._call(fn, q.ptr, b.ptr, n, self.cdf.ptr, self.ranks, seed)
+
+
+def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=4000, lr=3e-3):
+    """Learned dispatch on the mixed request stream.
+
+    Training: n_train requests (seed 0xC50, sizes up to 2^train_max_log2),
+    every feasible configuration timed by the exhaustive tuner (min of 2
+    runs); the cost
+    model is trained on the GPU on those timings (log2 microseconds).
+    Evaluation: the first n_eval requests of the stream of seed 0xC5 (sizes
+    2^10..2^28); each is tuned exhaustively (ground truth), and the model's,
+    the tuner table's and the fixed configuration's choices are priced with
+    those same timings.  Every chosen output is compared word for word with
+    the fixed configuration's output."""
+    q = clops.Queue(ctx)
+    d = clops.Dispatch(ctx)
+    data = RequestData(ctx, q, 1 << 28)
+    t0 = time.perf_counter()
+    X, y = [], []
+    for (op, n, width, dist, s) in request_stream(0xC50, n_train, train_max_log2):
+        data.fill(n, width, dist, s)
+        f = d.features(q, op, data.inb, n, width)
+        t = d.tune(q, op, data.inb, data.outb, n, width, reps=2)
+        d.table_add(op, n, width, f, t)
+        for c in range(clops.N_CONFIGS):
+            if t[c] > 0:
+                X.append(clops.Dispatch.model_row(f, c))
+                y.append(np.log2(t[c] * 1000.0))
+    t_collect = time.perf_counter() - t0
+    X, y = np.array(X, dtype=np.float32), np.array(y, dtype=np.float32)
+    m = clops.Mlp(ctx, clops.MODEL_INPUTS, seed=0xC5)
+    t0 = time.perf_counter()
+    loss = m.train(q, X, y, steps, lr)
+    t_train = time.perf_counter() - t0
+    d.set_model(m)
+    reqs = request_stream(0xC5, n_eval, 28)
+    feats, times = [], []
+    for (op, n, width, dist, s) in reqs:
+        data.fill(n, width, dist, s)
+        feats.append(d.features(q, op, data.inb, n, width))
+        times.append(d.tune(q, op, data.inb, data.outb, n, width, reps=3))
+    feats, times = np.array(feats), np.array(times)
+    ops = [r[0] for r in reqs]
+    ns = [r[1] for r in reqs]
+    ws = [r[2] for r in reqs]
+    t0 = time.perf_counter()
+    model = d.choose(q, clops.POLICY_MODEL, ops, ns, ws, feats)
+    t_choose = (time.perf_counter() - t0) / len(reqs)
+    table = d.choose(q, clops.POLICY_TABLE, ops, ns, ws, feats)
+    fixed = d.choose(q, clops.POLICY_FIXED, ops, ns, ws, feats)
+    best = np.array([int(np.argmin(np.where(t > 0, t, np.inf))) for t in times])
+    rows = np.arange(len(reqs))
+    tot = {k: float(times[rows, v].sum()) for k, v in
+           (("model", model), ("table", table), ("fixed", fixed), ("best", best))}
+    # bit-exact: chosen configuration vs the fixed one, word for word
+    diffs = 0
+    for r, (op, n, width, dist, s) in enumerate(reqs):
+        if model[r] == fixed[r]:
+            continue
+        data.fill(n, width, dist, s)
+        d.run(q, int(model[r]), op, data.inb, data.outb, n, width)
+        d.run(q, int(fixed[r]), op, data.inb, data.out2, n, width)
+        diffs += clops.count_diff(q, data.outb, data.out2, n * width // 4)
+    return {
+        "train_requests": n_train, "train_timings": int(y.size),
+        "train_collect_s": round(t_collect, 1), "train_s": round(t_train, 2),
+        "train_loss": round(loss, 4), "eval_requests": n_eval,
+        "eval_keys": int(sum(ns)),
+        "regret_model": round(tot["model"] / tot["best"] - 1, 4),
+        "regret_table": round(tot["table"] / tot["best"] - 1, 4),
+        "regret_fixed": round(tot["fixed"] / tot["best"] - 1, 4),
+        "speedup_model_vs_fixed": round(tot["fixed"] / tot["model"], 4),
+        "model_matches_best": round(float(np.mean(model == best)), 3),
+        "choose_us_per_request": round(t_choose * 1e6, 1),
+        "eval_ms": {k: round(v, 3) for k, v in tot.items()},
+        "differing_words_vs_fixed": int(diffs),
+        "sample": "first %d requests of the seed-0xC5 stream (sizes 2^10..2^28), exhaustive "
+                  "tuner ground truth (min of 3 timed runs per configuration)" % n_eval,
+    }
+
+
 def copy_probe(ctx, nbytes):
     q = clops.Queue(ctx)
     a = clops.Buffer(ctx, nbytes)
@@ -301,6 +447,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--suite", action="store_true")
+    ap.add_argument("--no-dispatch", action="store_true")
+    ap.add_argument("--dispatch-train", type=int, default=2700)
+    ap.add_argument("--dispatch-eval", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -313,6 +462,10 @@ def main():
     h = headline(args, ctx)
     if args.suite:
         suite(ctx)
+    disp = None
+    if not args.no_dispatch:
+        disp = dispatch_eval(ctx, args.dispatch_train, args.dispatch_eval)
+        log("dispatch: " + json.dumps(disp))
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -352,6 +505,7 @@ def main():
             "bytes_per_launch": h["bytes_per_launch"],
         },
         "cpu_baseline": cpu,
+        "dispatch": disp,
     }
     print(json.dumps(line), flush=True)
 
