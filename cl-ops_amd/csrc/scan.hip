@@ -295,6 +295,114 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan(const In* __res
 	}
 }
 
+// ---- vectorised plain scans (unsigned 32/64-bit elements, 16-byte aligned
+// buffers).  The reduce reads 16-byte vectors in striped order; the scan has
+// every thread own IT consecutive elements (loaded and stored as 16-byte
+// vectors) and needs LDS only for the block-level scan of thread sums.
+
+template <typename T, int IT>
+__device__ __forceinline__ void load_vec(const T* __restrict__ p, T (&x)[IT]) {
+	static_assert((IT * sizeof(T)) % 16 == 0, "whole 16-byte vectors");
+	const uint4* __restrict__ q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+	for (int k = 0; k < static_cast<int>(IT * sizeof(T) / 16); ++k) {
+		const uint4 t = q[k];
+		if constexpr (sizeof(T) == 4) {
+			x[4 * k] = t.x;
+			x[4 * k + 1] = t.y;
+			x[4 * k + 2] = t.z;
+			x[4 * k + 3] = t.w;
+		} else {
+			x[2 * k] = static_cast<T>(t.x) | (static_cast<T>(t.y) << 32);
+			x[2 * k + 1] = static_cast<T>(t.z) | (static_cast<T>(t.w) << 32);
+		}
+	}
+}
+
+template <typename T, int IT>
+__device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&x)[IT]) {
+	static_assert((IT * sizeof(T)) % 16 == 0, "whole 16-byte vectors");
+	uint4* __restrict__ q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+	for (int k = 0; k < static_cast<int>(IT * sizeof(T) / 16); ++k) {
+		uint4 t;
+		if constexpr (sizeof(T) == 4) {
+			t = make_uint4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+		} else {
+			t = make_uint4(static_cast<u32>(x[2 * k]), static_cast<u32>(x[2 * k] >> 32),
+				static_cast<u32>(x[2 * k + 1]), static_cast<u32>(x[2 * k + 1] >> 32));
+		}
+		q[k] = t;
+	}
+}
+
+template <typename In, typename Acc, int IT>
+__global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce_vec(const In* __restrict__ in,
+	u64 n, u32 tiles, Acc* __restrict__ tsum) {
+	constexpr u32 kScanTile = kScanBlock * IT;
+	constexpr int PER = 16 / sizeof(In);
+	__shared__ Acc s_wsum[kScanBlock / kWave];
+	const u32 tile = xcd_tile(tiles);
+	if (tile >= tiles) return;
+	const u64 base = static_cast<u64>(tile) * kScanTile;
+	const u32 tid = threadIdx.x;
+	Acc s = 0;
+	if (base + kScanTile <= n) {
+		In x[IT];
+#pragma unroll
+		for (int j = 0; j < IT / PER; ++j) {
+			In v[PER];
+			load_vec<In, PER>(in + base + static_cast<u64>(j * kScanBlock + tid) * PER, v);
+#pragma unroll
+			for (int e = 0; e < PER; ++e) x[j * PER + e] = v[e];
+		}
+#pragma unroll
+		for (int j = 0; j < IT; ++j) s += static_cast<Acc>(x[j]);
+	} else {
+		for (u64 g = base + tid; g < n; g += kScanBlock) s += static_cast<Acc>(in[g]);
+	}
+	Acc tot;
+	block_exclusive_sum<kScanBlock>(s, s_wsum, &tot);
+	if (tid == 0) tsum[tile] = tot;
+}
+
+template <typename In, typename Acc, int IT>
+__global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan_vec(const In* __restrict__ in,
+	Acc* __restrict__ out, u64 n, u32 tiles, const Acc* __restrict__ carry_in) {
+	constexpr u32 kScanTile = kScanBlock * IT;
+	__shared__ Acc s_wsum[kScanBlock / kWave];
+	const u32 tile = xcd_tile(tiles);
+	if (tile >= tiles) return;
+	const u32 tid = threadIdx.x;
+	const u64 first = static_cast<u64>(tile) * kScanTile + static_cast<u64>(tid) * IT;
+	const bool full = first + IT <= n;
+	In x[IT];
+	if (full) {
+		load_vec<In, IT>(in + first, x);
+	} else {
+#pragma unroll
+		for (int j = 0; j < IT; ++j) x[j] = first + j < n ? in[first + j] : In(0);
+	}
+	Acc sum = 0;
+#pragma unroll
+	for (int j = 0; j < IT; ++j) sum += static_cast<Acc>(x[j]);
+	Acc tot;
+	Acc run = carry_in[tile] + block_exclusive_sum<kScanBlock>(sum, s_wsum, &tot);
+	Acc y[IT];
+#pragma unroll
+	for (int j = 0; j < IT; ++j) {
+		y[j] = run;
+		run += static_cast<Acc>(x[j]);
+	}
+	if (full) {
+		store_vec<Acc, IT>(out + first, y);
+	} else {
+#pragma unroll
+		for (int j = 0; j < IT; ++j)
+			if (first + j < n) out[first + j] = y[j];
+	}
+}
+
 struct BlellochData {
 	Scratch tiles;  // per-tile sums (then carries) and segment flags
 };
@@ -318,6 +426,26 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 	const u32 grid = xcd_grid(a.tiles);
 	const In* in = static_cast<const In*>(a.in);
 	Acc* tsum = static_cast<Acc*>(a.tsum);
+	if constexpr (!SEG && (std::is_same<In, u32>::value || std::is_same<In, u64>::value) &&
+		sizeof(Acc) == OUTW) {
+		if (((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0) {
+			{
+				Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_REDUCE);
+				blelloch_tile_reduce_vec<In, Acc, IT><<<grid, kScanBlock, 0, st>>>(in, a.n,
+					a.tiles, tsum);
+				if (!L.done(err)) return false;
+			}
+			{
+				Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_TILES);
+				blelloch_tiles_scan<Acc, false><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles);
+				if (!L.done(err)) return false;
+			}
+			Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_SCAN);
+			blelloch_tile_scan_vec<In, Acc, IT><<<grid, kScanBlock, 0, st>>>(in,
+				static_cast<Acc*>(a.out), a.n, a.tiles, tsum);
+			return L.done(err);
+		}
+	}
 	{
 		Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_REDUCE);
 		blelloch_tile_reduce<In, Acc, SEG, IT><<<grid, kScanBlock, 0, st>>>(in, a.n, a.tiles, a.off,
