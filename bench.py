@@ -180,8 +180,10 @@ def suite(ctx):
         sc = clops.Scanner(ctx, elem, sumt)
         ms = time_steps(q, lambda: sc.scan_device(q, bi, bo, n), 10, 3)
         by = n * (np.dtype(elem).itemsize + np.dtype(sumt).itemsize)
+        kern = {nm: round(q.prof(nm)[1] / 10, 4) for nm in q.prof_names()}
         out.append({"config": "scan %s->%s n=%d" % (np.dtype(elem).name, np.dtype(sumt).name, n),
-                    "ms": ms, "GB/s": by / ms / 1e6, "hbm_frac": by / ms / 1e6 / HBM_PEAK_GBS})
+                    "ms": ms, "GB/s": by / ms / 1e6, "hbm_frac": by / ms / 1e6 / HBM_PEAK_GBS,
+                    "kernel_ms_per_scan": kern})
     # RNG fills (C2).
     G, draws = 1 << 26, 16
     ob = clops.Buffer(ctx, G * draws * 4)
@@ -200,8 +202,10 @@ def suite(ctx):
         s = clops.Sorter(ctx, algo, np.uint32)
         ms = time_steps(q, lambda: s.sort_device(q, kin, kout, n), 5, 2)
         sw = s.last_sweeps()
+        kern = {nm: round(q.prof(nm)[1] / 5, 4) for nm in q.prof_names()}
         out.append({"config": "%s u32 2^24" % algo, "ms": ms, "Gkeys/s": n / ms / 1e6,
-                    "sweeps": sw, "hbm_frac": sw * 2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+                    "sweeps": sw, "hbm_frac": sw * 2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "kernel_ms_per_sort": kern})
     out.extend(suite_segmented(ctx, q))
     for o in out:
         log(json.dumps(o))

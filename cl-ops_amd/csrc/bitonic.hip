@@ -16,6 +16,7 @@
 // signed keys flip the sign bit, descending order complements the key, and the
 // kernels then sort unsigned words ascending.
 
+#include <cstdint>
 #include <cstring>
 
 #include "clo_internal.hpp"
@@ -161,6 +162,189 @@ __global__ __launch_bounds__(kBitBlock) void bitonic_local(T* __restrict__ data,
 	}
 }
 
+// ---- tile kernels with register tails --------------------------------------
+// Each thread owns kRegN = 32 consecutive tile positions in registers.  Steps
+// whose comparator distance is below 32 (t <= 5, and the flip steps of stages
+// s <= 5) run in registers; the other steps go through LDS, addressed with a
+// 1-in-32 padding so that a thread's 32 consecutive keys sit in distinct banks.
+constexpr int kRegN = 32;
+
+__host__ __device__ constexpr u32 bpad(u32 i) { return i + (i >> 5); }
+
+template <typename T>
+__device__ __forceinline__ void cmpswap(T& a, T& b) {
+	const T mn = a < b ? a : b;
+	const T mx = a < b ? b : a;
+	a = mn;
+	b = mx;
+}
+
+// Half-cleaner of distance H on the thread's keys; `valid` keys are real
+// (positions >= valid are virtual +inf: a comparator reaching one does nothing).
+template <typename T, int H>
+__device__ __forceinline__ void reg_half(T (&x)[kRegN], u32 valid) {
+#pragma unroll
+	for (int j = 0; j < kRegN; ++j)
+		if (!(j & H) && static_cast<u32>(j | H) < valid) cmpswap(x[j], x[j | H]);
+}
+
+// Flip step of blocks of B positions: j pairs with j ^ (B - 1).
+template <typename T, int B>
+__device__ __forceinline__ void reg_flip(T (&x)[kRegN], u32 valid) {
+#pragma unroll
+	for (int j = 0; j < kRegN; ++j)
+		if (!(j & (B / 2)) && static_cast<u32>(j ^ (B - 1)) < valid) cmpswap(x[j], x[j ^ (B - 1)]);
+}
+
+// Steps t = t_hi..1 of stage s (the step t == s is the flip).
+template <typename T>
+__device__ __forceinline__ void reg_tail(T (&x)[kRegN], u32 s, u32 t_hi, u32 valid) {
+	if (t_hi >= 5) {
+		if (s == 5) reg_flip<T, 32>(x, valid);
+		else reg_half<T, 16>(x, valid);
+	}
+	if (t_hi >= 4) {
+		if (s == 4) reg_flip<T, 16>(x, valid);
+		else reg_half<T, 8>(x, valid);
+	}
+	if (t_hi >= 3) {
+		if (s == 3) reg_flip<T, 8>(x, valid);
+		else reg_half<T, 4>(x, valid);
+	}
+	if (t_hi >= 2) {
+		if (s == 2) reg_flip<T, 4>(x, valid);
+		else reg_half<T, 2>(x, valid);
+	}
+	if (t_hi >= 1) {
+		if (s == 1) reg_flip<T, 2>(x, valid);
+		else reg_half<T, 1>(x, valid);
+	}
+}
+
+// Q consecutive steps t, t-1, .., t-Q+1 of stage s inside the LDS tile (each
+// thread takes whole comparator groups of 2^Q positions), the same grouping as
+// bitonic_group uses for global steps.
+template <typename T, int Q>
+__device__ __forceinline__ void lds_group(T* s_t, u64 base, u64 n, u32 tile_n, u32 s, u32 t) {
+	constexpr int V = 1 << Q;
+	const u32 pos = t - Q;
+	u32 masks[Q];
+#pragma unroll
+	for (int r = 0; r < Q; ++r) masks[r] = 1u << (t - 1 - r);
+	if (t == s) masks[0] = (1u << s) - 1u;
+	for (u32 g = threadIdx.x; g < (tile_n >> Q); g += kBitBlock) {
+		const u32 x = static_cast<u32>(insert_zero_bits(g, pos, Q));
+		u32 idx[V];
+		T val[V];
+#pragma unroll
+		for (int j = 0; j < V; ++j) {
+			u32 e = x;
+#pragma unroll
+			for (int r = 0; r < Q; ++r)
+				if (j & (1 << r)) e ^= masks[r];
+			idx[j] = e;
+			val[j] = s_t[bpad(e)];
+		}
+#pragma unroll
+		for (int r = 0; r < Q; ++r) {
+#pragma unroll
+			for (int j = 0; j < V; ++j) {
+				if (j & (1 << r)) continue;
+				const int k = j | (1 << r);
+				const bool jlo = idx[j] < idx[k];
+				const u32 hi = jlo ? idx[k] : idx[j];
+				if (base + hi >= n) continue;
+				const T a = val[j], b = val[k];
+				const T mn = a < b ? a : b, mx = a < b ? b : a;
+				val[j] = jlo ? mn : mx;
+				val[k] = jlo ? mx : mn;
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < V; ++j) s_t[bpad(idx[j])] = val[j];
+	}
+}
+
+// Stages s_first..s_last of the network restricted to one tile of 2^k
+// positions (k >= 5): steps t = min(s, k)..1 of each stage.  Serves both the
+// tile sort (stages 1..k) and the local half-cleaners of a stage s > k.
+template <typename T>
+__global__ __launch_bounds__(kBitBlock) void bitonic_tile_reg(T* __restrict__ data, u64 n, u32 k,
+	u32 s_first, u32 s_last, T xmask) {
+	__shared__ T s_t[bpad(1u << TileMax<T>::kLog)];
+	const u32 tile_n = 1u << k;
+	const u64 base = static_cast<u64>(blockIdx.x) << k;
+	if (base >= n) return;
+	const u32 tid = threadIdx.x;
+	const bool owner = tid * kRegN < tile_n;
+	const u32 p0 = tid * kRegN;
+	const u64 left = n - base;  // > 0
+	const u32 valid = !owner ? 0u : (left <= p0 ? 0u : (left - p0 >= kRegN ? kRegN
+		: static_cast<u32>(left - p0)));
+	// Every owner's 32 keys come straight from global memory (16-byte vectors
+	// when whole and aligned), all loads in flight together.
+	T* const mine = data + base + p0;
+	const bool vec = valid == kRegN && (reinterpret_cast<uintptr_t>(mine) & 15) == 0;
+	constexpr int NV = kRegN * sizeof(T) / 16;
+	T x[kRegN];
+	if (vec) {
+		uint4 t4[NV];
+#pragma unroll
+		for (int v = 0; v < NV; ++v) t4[v] = reinterpret_cast<const uint4*>(mine)[v];
+#pragma unroll
+		for (int v = 0; v < NV; ++v) __builtin_memcpy(&x[v * (16 / sizeof(T))], &t4[v], 16);
+#pragma unroll
+		for (int j = 0; j < kRegN; ++j) x[j] = static_cast<T>(x[j] ^ xmask);
+	} else {
+#pragma unroll
+		for (int j = 0; j < kRegN; ++j)
+			x[j] = static_cast<u32>(j) < valid ? static_cast<T>(mine[j] ^ xmask) : T(0);
+	}
+	for (u32 s = s_first; s <= s_last; ++s) {
+		const u32 t_top = s < k ? s : k;
+		if (t_top > 5) {
+			__syncthreads();
+			if (owner) {
+#pragma unroll
+				for (int j = 0; j < kRegN; ++j) s_t[bpad(p0 + j)] = x[j];
+			}
+			__syncthreads();
+			for (u32 t = t_top; t > 5;) {
+				const u32 m = t - 5;
+				if (m >= 3) {
+					lds_group<T, 3>(s_t, base, n, tile_n, s, t);
+					t -= 3;
+				} else if (m == 2) {
+					lds_group<T, 2>(s_t, base, n, tile_n, s, t);
+					t -= 2;
+				} else {
+					lds_group<T, 1>(s_t, base, n, tile_n, s, t);
+					t -= 1;
+				}
+				__syncthreads();
+			}
+			if (owner) {
+#pragma unroll
+				for (int j = 0; j < kRegN; ++j) x[j] = s_t[bpad(p0 + j)];
+			}
+		}
+		if (owner) reg_tail<T>(x, s, t_top < 5 ? t_top : 5, valid);
+	}
+#pragma unroll
+	for (int j = 0; j < kRegN; ++j) x[j] = static_cast<T>(x[j] ^ xmask);
+	if (vec) {
+		uint4 t4[NV];
+#pragma unroll
+		for (int v = 0; v < NV; ++v) __builtin_memcpy(&t4[v], &x[v * (16 / sizeof(T))], 16);
+#pragma unroll
+		for (int v = 0; v < NV; ++v) reinterpret_cast<uint4*>(mine)[v] = t4[v];
+	} else if (owner) {
+#pragma unroll
+		for (int j = 0; j < kRegN; ++j)
+			if (static_cast<u32>(j) < valid) mine[j] = x[j];
+	}
+}
+
 struct BitonicData {
 	bool advanced;
 	u32 minps = 1, maxps = 4, maxsfs = 0xFFFFFFFFu;
@@ -220,7 +404,10 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 	if (k >= 1) {
 		const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
 		Launch L(q, CLO_SORT_ABITONIC_KNAME_TILESORT);
-		bitonic_tilesort<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, xm);
+		if (k >= 5)
+			bitonic_tile_reg<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, 1, k, xm);
+		else
+			bitonic_tilesort<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, xm);
 		if (!L.done(err)) return false;
 		++sweeps;
 	}
@@ -242,7 +429,10 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 		if (k >= 1) {
 			const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
 			Launch L(q, CLO_SORT_ABITONIC_KNAME_LOCAL);
-			bitonic_local<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, s, xm);
+			if (k >= 5)
+				bitonic_tile_reg<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, s, s, xm);
+			else
+				bitonic_local<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, s, xm);
 			if (!L.done(err)) return false;
 			++sweeps;
 		}

@@ -426,27 +426,45 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	}
 	__syncthreads();
 
-	// Rank inside the wave.
-	u32 rank[ITEMS];
+	// Rank inside the wave.  The lanes holding one digit (found with one ballot
+	// per digit bit) form a peer group; its lowest lane adds the group's size
+	// to the wave's running digit count with one LDS atomic that returns the
+	// previous count.  The atomics of all items issue back to back (a wave's
+	// LDS operations complete in order, so item j sees the counts of items
+	// before it); each lane then takes its group's base from the leader.
+	// Items go in groups of kGroup (8 atomics in flight, 8 extra registers).
+	constexpr int kGroup = ITEMS < 8 ? ITEMS : 8;
+	static_assert(ITEMS % kGroup == 0, "whole groups");
+	u32 rank[ITEMS];  // (leader lane << 16) | peers below, then the rank
 #pragma unroll
-	for (int j = 0; j < ITEMS; ++j) {
-		const bool valid = (wbase + j * kWave + lane) < n;
-		const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
-		u64 peers = ballot(valid);
+	for (int j0 = 0; j0 < ITEMS; j0 += kGroup) {
+		u32 got[kGroup];
 #pragma unroll
-		for (u32 bt = 0; bt < (FIXED_BITS ? FIXED_BITS : 8); ++bt) {
-			if (!FIXED_BITS && bt >= bits) break;
-			const bool bit = (dig >> bt) & 1u;
-			const u64 bb = ballot(bit);
-			peers &= bit ? bb : ~bb;
+		for (int jj = 0; jj < kGroup; ++jj) {
+			const int j = j0 + jj;
+			const bool valid = (wbase + j * kWave + lane) < n;
+			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
+			u64 peers = ballot(valid);
+#pragma unroll
+			for (u32 bt = 0; bt < (FIXED_BITS ? FIXED_BITS : 8); ++bt) {
+				if (!FIXED_BITS && bt >= bits) break;
+				const bool bit = (dig >> bt) & 1u;
+				const u64 bb = ballot(bit);
+				peers &= bit ? bb : ~bb;
+			}
+			const u32 below = count_below(peers);
+			const u32 leader = peers ? static_cast<u32>(__builtin_ctzll(peers)) : 0u;
+			got[jj] = 0;
+			if (valid && below == 0)
+				got[jj] = atomicAdd(&s_whist[w][dig], static_cast<u32>(__popcll(peers)));
+			rank[j] = (leader << 16) | below;
 		}
-		const u32 below = count_below(peers);
-		u32 base = 0;
-		if (valid) {
-			base = s_whist[w][dig];
-			if (below == 0) s_whist[w][dig] = base + static_cast<u32>(__popcll(peers));
+#pragma unroll
+		for (int jj = 0; jj < kGroup; ++jj) {
+			const int j = j0 + jj;
+			rank[j] = static_cast<u32>(__shfl(static_cast<int>(got[jj]),
+				static_cast<int>(rank[j] >> 16), kWave)) + (rank[j] & 0xFFFFu);
 		}
-		rank[j] = base + below;
 	}
 	__syncthreads();
 

@@ -202,15 +202,30 @@ __global__ __launch_bounds__(kTilesBlock) void blelloch_tiles_scan(Acc* __restri
 	__shared__ SegPair<Acc> s_pw[kTilesBlock / kWave];
 	const u32 tid = threadIdx.x;
 	SegPair<Acc> carry = {0u, Acc(0)};
+	// Values of the next round are loaded while the current one is scanned.
+	Acc nv[kTilesItems];
+	u8 nf[kTilesItems];
+#pragma unroll
+	for (u32 j = 0; j < kTilesItems; ++j) {
+		const u32 i = j * kTilesBlock + tid;
+		nv[j] = i < tiles ? tsum[i] : Acc(0);
+		nf[j] = (SEG && i < tiles) ? static_cast<u8>(tflag[i] != 0) : u8(0);
+	}
 	for (u32 base = 0; base < tiles; base += kTilesRound) {
 #pragma unroll
 		for (u32 j = 0; j < kTilesItems; ++j) {
 			const u32 i = j * kTilesBlock + tid;
-			const bool in = base + i < tiles;
-			s_v[pad(i)] = in ? tsum[base + i] : Acc(0);
-			if (SEG) s_f[i] = in ? static_cast<u8>(tflag[base + i] != 0) : u8(0);
+			s_v[pad(i)] = nv[j];
+			if (SEG) s_f[i] = nf[j];
 		}
 		__syncthreads();
+		const u32 nb = base + kTilesRound;
+#pragma unroll
+		for (u32 j = 0; j < kTilesItems; ++j) {
+			const u32 i = nb + j * kTilesBlock + tid;
+			nv[j] = i < tiles ? tsum[i] : Acc(0);
+			nf[j] = (SEG && i < tiles) ? static_cast<u8>(tflag[i] != 0) : u8(0);
+		}
 		SegPair<Acc> x[kTilesItems];
 		SegPair<Acc> mine = {0u, Acc(0)};
 #pragma unroll

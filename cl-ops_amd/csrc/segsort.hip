@@ -100,6 +100,60 @@ __device__ __forceinline__ u32 digit_of(K k, u32 shift) {
 	return static_cast<u32>(k >> shift) & 0xFFu;
 }
 
+// Copies with the loads of up to kBatch rounds issued before their stores (a
+// plain load-store loop would wait on each load in turn), kBatch registers
+// per thread.
+constexpr int kBatch = 8;
+
+// Global -> LDS copy of n <= IPT * kRB elements.
+template <typename T, int IPT>
+__device__ __forceinline__ void g2s(T* dst, const T* __restrict__ src, u32 n) {
+	constexpr int B = IPT < kBatch ? IPT : kBatch;
+#pragma unroll
+	for (int r0 = 0; r0 < IPT; r0 += B) {
+		T t[B];
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const u32 i = (r0 + r) * kRB + threadIdx.x;
+			t[r] = i < n ? src[i] : T(0);
+		}
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const u32 i = (r0 + r) * kRB + threadIdx.x;
+			if (i < n) dst[i] = t[r];
+		}
+	}
+}
+
+// LDS -> global copy of n <= IPT * kRB elements.
+template <typename T, int IPT>
+__device__ __forceinline__ void s2g(T* __restrict__ dst, const T* src, u32 n) {
+#pragma unroll
+	for (int r = 0; r < IPT; ++r) {
+		const u32 i = r * kRB + threadIdx.x;
+		if (i < n) dst[i] = src[i];
+	}
+}
+
+// Global -> global copy of any length.
+template <typename T, int IPT>
+__device__ __forceinline__ void g2g(T* __restrict__ dst, const T* __restrict__ src, u32 n) {
+	constexpr u32 R = kBatch * kRB;
+	for (u32 c0 = 0; c0 < n; c0 += R) {
+		T t[kBatch];
+#pragma unroll
+		for (int r = 0; r < kBatch; ++r) {
+			const u32 i = c0 + r * kRB + threadIdx.x;
+			t[r] = i < n ? src[i] : T(0);
+		}
+#pragma unroll
+		for (int r = 0; r < kBatch; ++r) {
+			const u32 i = c0 + r * kRB + threadIdx.x;
+			if (i < n) dst[i] = t[r];
+		}
+	}
+}
+
 // Block-wide AND and OR of the keys in s.k[0, n).
 template <typename K, bool KV, int IPT>
 __device__ void and_or_lds(SegSmem<K, KV, IPT>& s, u32 n, K* a_out, K* o_out) {
@@ -179,16 +233,12 @@ template <typename K, bool KV, int IPT>
 __device__ void sort_medium(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 	const u32 start = a.off[j];
 	const u32 len = seg_end(a.off, j, a.nseg, a.numel) - start;
-	for (u32 i = threadIdx.x; i < len; i += kRB) {
-		s.k[i] = a.kin[start + i];
-		if (KV) s.v[i] = a.vin[start + i];
-	}
+	g2s<K, IPT>(s.k, a.kin + start, len);
+	if (KV) g2s<u32, IPT>(s.v, a.vin + start, len);
 	__syncthreads();
 	sort_key_passes<K, KV, IPT>(s, len, false);
-	for (u32 i = threadIdx.x; i < len; i += kRB) {
-		a.kout[start + i] = s.k[i];
-		if (KV) a.vout[start + i] = s.v[i];
-	}
+	s2g<K, IPT>(a.kout + start, s.k, len);
+	if (KV) s2g<u32, IPT>(a.vout + start, s.v, len);
 	__syncthreads();
 }
 
@@ -219,7 +269,7 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	const u32 j0 = s.misc[0];
 	const u32 mall = s.misc[1];  // segments starting in the window (empty ones included)
 	// Compact the non-empty small segments (distinct starts inside a window of
-	// W elements, so at most W of them): segj[r] = segment index, lpos[r] =
+	// W elements, so at most W of them): segj[r] = segment start, lpos[r] =
 	// its start in the LDS tile.
 	u32 carry = 0, m = 0;
 	for (u32 r0 = 0; r0 < mall; r0 += kRB) {
@@ -234,7 +284,7 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 		const u32 ex = block_exclusive_sum<kRB>(len, s.rs.wsum, &tot);
 		const u32 exc = block_exclusive_sum<kRB>(len ? 1u : 0u, s.rs.wsum, &cnt);
 		if (len) {
-			s.segj[m + exc] = j;
+			s.segj[m + exc] = a.off[j];  // the segment's start in the array
 			s.lpos[m + exc] = carry + ex;
 		}
 		carry += tot;
@@ -245,19 +295,42 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	const u32 n = carry;  // <= CAP
 	if (n == 0) return;
 	// Gather: element i belongs to the included segment r with
-	// lpos[r] <= i < lpos[r+1].
-	for (u32 i = threadIdx.x; i < n; i += kRB) {
-		u32 l = 0, h = m;  // last r with lpos[r] <= i
-		while (h - l > 1) {
-			const u32 mid = (l + h) >> 1;
-			if (s.lpos[mid] <= i) l = mid;
-			else h = mid;
+	// lpos[r] <= i < lpos[r+1] (fixed-length binary search, m <= W < 2^13),
+	// then every thread's global loads are issued together.
+	constexpr int B = IPT < kBatch ? IPT : kBatch;
+#pragma unroll
+	for (int r0 = 0; r0 < IPT; r0 += B) {
+		u32 gidx[B];
+		u32 gsid[B];
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const u32 i = (r0 + r) * kRB + threadIdx.x;
+			u32 l = 0;
+#pragma unroll
+			for (int b = 12; b >= 0; --b) {
+				const u32 c = l + (1u << b);
+				if (c < m && s.lpos[c] <= i) l = c;
+			}
+			gsid[r] = l;
+			gidx[r] = s.segj[l] + (i - s.lpos[l]);
 		}
-		while (l + 1 < m && s.lpos[l + 1] <= i) ++l;
-		const u32 g = a.off[s.segj[l]] + (i - s.lpos[l]);
-		s.k[i] = a.kin[g];
-		if (KV) s.v[i] = a.vin[g];
-		s.sid[i] = static_cast<u16>(l);
+		K tk[B];
+		u32 tv[B];
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const bool in = (r0 + r) * kRB + threadIdx.x < n;
+			tk[r] = in ? a.kin[gidx[r]] : K(0);
+			if (KV) tv[r] = in ? a.vin[gidx[r]] : 0u;
+		}
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const u32 i = (r0 + r) * kRB + threadIdx.x;
+			if (i < n) {
+				s.k[i] = tk[r];
+				if (KV) s.v[i] = tv[r];
+				s.sid[i] = static_cast<u16>(gsid[r]);
+			}
+		}
 	}
 	__syncthreads();
 	const bool multi = m > 1;
@@ -267,11 +340,15 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 		lds_pass<K, KV, IPT>(s, n, 0, true, true);
 		if (m > 256) lds_pass<K, KV, IPT>(s, n, 8, true, true);
 	}
-	for (u32 i = threadIdx.x; i < n; i += kRB) {
-		const u32 r = multi ? s.sid[i] : 0u;
-		const u32 g = a.off[s.segj[r]] + (i - s.lpos[r]);
-		a.kout[g] = s.k[i];
-		if (KV) a.vout[g] = s.v[i];
+#pragma unroll
+	for (int rr = 0; rr < IPT; ++rr) {
+		const u32 i = rr * kRB + threadIdx.x;
+		if (i < n) {
+			const u32 r = multi ? s.sid[i] : 0u;
+			const u32 g = s.segj[r] + (i - s.lpos[r]);
+			a.kout[g] = s.k[i];
+			if (KV) a.vout[g] = s.v[i];
+		}
 	}
 	__syncthreads();
 }
@@ -288,12 +365,21 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 	__syncthreads();
 	// One read: every digit histogram plus AND/OR of the keys.
 	K ka = ~K(0), ko = K(0);
-	for (u32 i = tid; i < len; i += kRB) {
-		const K k = a.kin[start + i];
-		ka &= k;
-		ko |= k;
+	for (u32 c0 = 0; c0 < len; c0 += kBatch * kRB) {
+		K t[kBatch];
 #pragma unroll
-		for (u32 p = 0; p < P; ++p) atomicAdd(&s.hist[p][digit_of(k, 8 * p)], 1u);
+		for (int r = 0; r < kBatch; ++r) {
+			const u32 i = c0 + r * kRB + tid;
+			t[r] = i < len ? a.kin[start + i] : K(0);
+		}
+#pragma unroll
+		for (int r = 0; r < kBatch; ++r) {
+			if (c0 + r * kRB + tid >= len) continue;
+			ka &= t[r];
+			ko |= t[r];
+#pragma unroll
+			for (u32 p = 0; p < P; ++p) atomicAdd(&s.hist[p][digit_of(t[r], 8 * p)], 1u);
+		}
 	}
 #pragma unroll
 	for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -396,10 +482,8 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 	// with nothing to do.
 	const K* fin = src;
 	if (fin != a.kout + start) {
-		for (u32 i = tid; i < len; i += kRB) {
-			a.kout[start + i] = fin[i];
-			if (KV) a.vout[start + i] = vsrc[i];
-		}
+		g2g<K, IPT>(a.kout + start, fin, len);
+		if (KV) g2g<u32, IPT>(a.vout + start, vsrc, len);
 	}
 	__syncthreads();
 }
