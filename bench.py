@@ -146,8 +146,33 @@ def suite(ctx):
     kout = clops.Buffer(ctx, n * 4)
     clops.generate(q, kin, n, clops.GEN_U32_HI, 0xC1)
     s = clops.Sorter(ctx, "satradix", np.uint32)
+    qp = q
+    q = clops.Queue(ctx)  # no per-kernel events: this size is launch-bound
     ms = time_steps(q, lambda: s.sort_device(q, kin, kout, n), 20, 5)
-    out.append({"config": "satradix u32 2^20 keys", "ms": ms, "Gkeys/s": n / ms / 1e6})
+    ms_flush = None
+    flush = clops.Buffer(ctx, 512 << 20)
+    t = 0.0
+    for _ in range(5):  # each sort after a 512 MiB write that evicts the caches
+        clops.generate(q, flush, (512 << 20) // 4, clops.GEN_U32_HI, 1)
+        e0 = q.marker()
+        s.sort_device(q, kin, kout, n)
+        e1 = q.marker()
+        q.finish()
+        t += q.elapsed_ms(e0, e1)
+    ms_flush = t / 5
+    g = clops.Graph(q)
+    g.begin()
+    try:
+        s.sort_device(q, kin, kout, n)
+    finally:
+        g.end()
+    ms_graph = time_steps(q, g.launch, 20, 5)
+    bad = clops.check_sorted(q, kout, n, 4, None)
+    g.close()
+    out.append({"config": "satradix u32 2^20 keys", "ms": ms, "Gkeys/s": n / ms / 1e6,
+                "ms_after_cache_flush": ms_flush, "ms_graph_replay": ms_graph,
+                "unsorted_pairs": bad})
+    q = qp
     # C3 variants: [0, 2^20) keys and Zipf.
     n = 1 << 28
     kin = clops.Buffer(ctx, n * 8)
@@ -351,13 +376,17 @@ class RequestData:
             clops._call(fn, q.ptr, b.ptr, n, self.cdf.ptr, self.ranks, seed)
 
 
-def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=4000, lr=3e-3):
+def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=8000, lr=2e-3,
+                  margin=0.02):
     """Learned dispatch on the mixed request stream.
 
     Training: n_train requests (seed 0xC50, sizes up to 2^train_max_log2),
     every feasible configuration timed by the exhaustive tuner (min of 2
-    runs); the cost
-    model is trained on the GPU on those timings (log2 microseconds).
+    runs); the cost model is trained on the GPU on those timings, as log2 of
+    each configuration's time over the fixed configuration's time for the
+    same request (the choice only needs one request's configurations in
+    order).  The model policy leaves the fixed configuration only for a
+    predicted gain above `margin` (log2).
     Evaluation: the first n_eval requests of the stream of seed 0xC5 (sizes
     2^10..2^28); each is tuned exhaustively (ground truth), and the model's,
     the tuner table's and the fixed configuration's choices are priced with
@@ -373,10 +402,11 @@ def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=4000, 
         f = d.features(q, op, data.inb, n, width)
         t = d.tune(q, op, data.inb, data.outb, n, width, reps=2)
         d.table_add(op, n, width, f, t)
+        t_fixed = t[clops.Dispatch.fixed(op, n, width)]
         for c in range(clops.N_CONFIGS):
             if t[c] > 0:
                 X.append(clops.Dispatch.model_row(f, c))
-                y.append(np.log2(t[c] * 1000.0))
+                y.append(np.log2(t[c] / t_fixed))
     t_collect = time.perf_counter() - t0
     X, y = np.array(X, dtype=np.float32), np.array(y, dtype=np.float32)
     m = clops.Mlp(ctx, clops.MODEL_INPUTS, seed=0xC5)
@@ -384,6 +414,7 @@ def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=4000, 
     loss = m.train(q, X, y, steps, lr)
     t_train = time.perf_counter() - t0
     d.set_model(m)
+    d.set_margin(margin)
     reqs = request_stream(0xC5, n_eval, 28)
     feats, times = [], []
     for (op, n, width, dist, s) in reqs:
@@ -415,7 +446,8 @@ def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=4000, 
     return {
         "train_requests": n_train, "train_timings": int(y.size),
         "train_collect_s": round(t_collect, 1), "train_s": round(t_train, 2),
-        "train_loss": round(loss, 4), "eval_requests": n_eval,
+        "train_loss": round(loss, 4), "train_target": "log2(time / fixed-config time)",
+        "margin_log2": margin, "eval_requests": n_eval,
         "eval_keys": int(sum(ns)),
         "regret_model": round(tot["model"] / tot["best"] - 1, 4),
         "regret_table": round(tot["table"] / tot["best"] - 1, 4),

@@ -159,7 +159,7 @@ def lib():
         "clo_dispatch_destroy": (None, [vp]),
         "clo_dispatch_config_name": (cp, [u32]),
         "clo_dispatch_config_feasible": (u32, [u32, i32, sz, sz]),
-        "clo_dispatch_fixed_config": (u32, [i32, sz]),
+        "clo_dispatch_fixed_config": (u32, [i32, sz, sz]),
         "clo_dispatch_features": (u32, [vp, vp, i32, vp, sz, sz, vp, ep]),
         "clo_dispatch_model_row": (None, [vp, u32, vp]),
         "clo_dispatch_run": (vp, [vp, vp, u32, i32, vp, vp, sz, sz, ep]),
@@ -169,6 +169,7 @@ def lib():
         "clo_dispatch_table_save": (u32, [vp, cp, ep]),
         "clo_dispatch_table_load": (u32, [vp, cp, ep]),
         "clo_dispatch_set_model": (u32, [vp, vp, ep]),
+        "clo_dispatch_set_margin": (None, [vp, ctypes.c_float]),
         "clo_dispatch_choose": (u32, [vp, vp, i32, sz, vp, vp, vp, vp, vp, ep]),
         # cost-model MLP
         "clo_mlp_new": (vp, [vp, u32, u64, ep]),
@@ -378,8 +379,8 @@ class Dispatch:
         return bool(lib().clo_dispatch_config_feasible(c, op, n, width))
 
     @staticmethod
-    def fixed(op, width):
-        return lib().clo_dispatch_fixed_config(op, width)
+    def fixed(op, n, width):
+        return lib().clo_dispatch_fixed_config(op, n, width)
 
     @staticmethod
     def model_row(features, cfg):
@@ -420,6 +421,9 @@ class Dispatch:
     def set_model(self, mlp):
         self.model = mlp
         _call(self.L.clo_dispatch_set_model, self.ptr, mlp.ptr if mlp else None)
+
+    def set_margin(self, log2_margin):
+        self.L.clo_dispatch_set_margin(self.ptr, log2_margin)
 
     def choose(self, q, policy, ops, ns, widths, features):
         ops = np.ascontiguousarray(ops, dtype=np.int32)

@@ -47,18 +47,31 @@ const CfgInfo kCfg[kCfgs] = {
 template <typename K>
 __global__ __launch_bounds__(256) void dispatch_features(const K* __restrict__ x, u64 n, u32 op,
 	float* __restrict__ out) {
+	constexpr u32 kPer = kSample / 256;
+	__shared__ K s_x[kSample];
 	__shared__ u32 s_h[256];
 	__shared__ float s_f[4];
 	__shared__ u32 s_u[3][4];
 	const u32 tid = threadIdx.x;
 	const u32 S = n < kSample ? static_cast<u32>(n) : kSample;
+	// The sample and its right neighbours, every load in flight at once.
+	K a[kPer], b[kPer];
+#pragma unroll
+	for (u32 r = 0; r < kPer; ++r) {
+		const u32 i = r * 256 + tid;
+		const u64 p = static_cast<u64>(i) * n / (S ? S : 1);
+		a[r] = i < S ? x[p] : K(0);
+		b[r] = (i < S && p + 1 < n) ? x[p + 1] : K(0);
+	}
 	u32 asc = 0, dsc = 0, pairs = 0;
-	for (u32 i = tid; i < S; i += 256) {
-		const u64 p = static_cast<u64>(i) * n / S;
-		if (p + 1 < n) {
-			const K a = x[p], b = x[p + 1];
-			asc += a < b ? 1u : 0u;
-			dsc += a > b ? 1u : 0u;
+#pragma unroll
+	for (u32 r = 0; r < kPer; ++r) {
+		const u32 i = r * 256 + tid;
+		if (i < S) s_x[i] = a[r];
+		const u64 p = static_cast<u64>(i) * n / (S ? S : 1);
+		if (i < S && p + 1 < n) {
+			asc += a[r] < b[r] ? 1u : 0u;
+			dsc += a[r] > b[r] ? 1u : 0u;
 			++pairs;
 		}
 	}
@@ -72,13 +85,11 @@ __global__ __launch_bounds__(256) void dispatch_features(const K* __restrict__ x
 	}
 	float esum = 0.f, emax = 0.f;
 	u32 constant = 0;
-	for (u32 b = 0; b < sizeof(K); ++b) {
+	for (u32 bt = 0; bt < sizeof(K); ++bt) {
 		s_h[tid] = 0;
 		__syncthreads();
-		for (u32 i = tid; i < S; i += 256) {
-			const u64 p = static_cast<u64>(i) * n / S;
-			atomicAdd(&s_h[static_cast<u32>(x[p] >> (8 * b)) & 255u], 1u);
-		}
+		for (u32 i = tid; i < S; i += 256)
+			atomicAdd(&s_h[static_cast<u32>(s_x[i] >> (8 * bt)) & 255u], 1u);
 		__syncthreads();
 		const u32 c = s_h[tid];
 		float e = 0.f;
@@ -143,6 +154,7 @@ struct clo_dispatch {
 	CCLBuffer* off1 = nullptr;  // one segment starting at 0
 	Scratch feat;
 	CloMlp* mlp = nullptr;
+	float margin = 0.f;  // model policy: least predicted gain (log2) to leave the fixed config
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	std::map<BucketKey, Bucket> table;
 };
@@ -216,9 +228,11 @@ cl_bool clo_dispatch_config_feasible(cl_uint cfg, CloOp op, size_t n, size_t wid
 	return (kCfg[cfg].max_n == 0 || n <= kCfg[cfg].max_n) ? CL_TRUE : CL_FALSE;
 }
 
-cl_uint clo_dispatch_fixed_config(CloOp op, size_t width) {
+cl_uint clo_dispatch_fixed_config(CloOp op, size_t n, size_t width) {
 	if (op == CLO_OP_SCAN) return 7;
-	return width == 8 ? 1 : 0;  // satradix's default tile shape for keys only
+	// satradix's default tile shape for keys only
+	if (n <= (static_cast<size_t>(1) << 20)) return 2;
+	return width == 8 ? 1 : 0;
 }
 
 void clo_dispatch_model_row(const float* features, cl_uint cfg, float* row) {
@@ -397,6 +411,10 @@ cl_bool clo_dispatch_set_model(CloDispatch* d, CloMlp* mlp, GError** err) {
 	return CL_TRUE;
 }
 
+void clo_dispatch_set_margin(CloDispatch* d, float log2_margin) {
+	if (d) d->margin = log2_margin > 0.f ? log2_margin : 0.f;
+}
+
 cl_bool clo_dispatch_choose(CloDispatch* d, CCLQueue* cq, CloPolicy policy, size_t count,
 	const int* ops, const size_t* ns, const size_t* widths, const float* features, int* cfg_out,
 	GError** err) {
@@ -410,7 +428,7 @@ cl_bool clo_dispatch_choose(CloDispatch* d, CCLQueue* cq, CloPolicy policy, size
 		int c = -1;
 		if (policy != CLO_POLICY_FIXED)
 			c = clo_dispatch_table_choose(d, op, ns[r], widths[r], features + r * kFeat);
-		cfg_out[r] = c >= 0 ? c : static_cast<int>(clo_dispatch_fixed_config(op, widths[r]));
+		cfg_out[r] = c >= 0 ? c : static_cast<int>(clo_dispatch_fixed_config(op, ns[r], widths[r]));
 	}
 	if (policy != CLO_POLICY_MODEL || !d->mlp || count == 0) return CL_TRUE;
 	if (!cq) {
@@ -432,13 +450,23 @@ cl_bool clo_dispatch_choose(CloDispatch* d, CCLQueue* cq, CloPolicy policy, size
 	if (who.empty()) return CL_TRUE;
 	std::vector<float> pred(who.size());
 	if (!clo_mlp_predict(d->mlp, cq, rows.data(), who.size(), pred.data(), err)) return CL_FALSE;
-	std::vector<float> best(count, INFINITY);
+	std::vector<float> best(count, INFINITY), fixed_pred(count, INFINITY);
+	std::vector<int> arg(count, -1);
 	for (size_t i = 0; i < who.size(); ++i) {
 		const size_t r = who[i].first;
+		if (who[i].second == clo_dispatch_fixed_config(static_cast<CloOp>(ops[r]), ns[r], widths[r]))
+			fixed_pred[r] = pred[i];
 		if (pred[i] < best[r]) {
 			best[r] = pred[i];
-			cfg_out[r] = static_cast<int>(who[i].second);
+			arg[r] = static_cast<int>(who[i].second);
 		}
+	}
+	for (size_t r = 0; r < count; ++r) {
+		if (arg[r] < 0) continue;
+		// Leave the fixed configuration only for a predicted gain above the margin.
+		cfg_out[r] = best[r] < fixed_pred[r] - d->margin
+			? arg[r] : static_cast<int>(clo_dispatch_fixed_config(static_cast<CloOp>(ops[r]),
+				ns[r], widths[r]));
 	}
 	return CL_TRUE;
 }

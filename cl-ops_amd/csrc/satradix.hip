@@ -344,6 +344,39 @@ __global__ __launch_bounds__(kPartThreads) void satradix_scan_partials(
 	}
 }
 
+// The whole counter scan in one block when all tiles form one group
+// (tiles <= kScanTiles): counts[t][d] <- (#keys with a smaller digit) +
+// (#keys with digit d in tiles < t).
+__global__ __launch_bounds__(kGroupThreads) void satradix_scan_one(const SortPlan* __restrict__ plan,
+	u32 pass, u32* __restrict__ counts, u32 tiles) {
+	__shared__ u32 s_q[4][kRadixMax];
+	__shared__ u32 s_wsum[kGroupThreads / kWave];
+	if (!plan->pass[pass].active) return;
+	const u32 d = threadIdx.x % kRadixMax, q = threadIdx.x / kRadixMax;
+	const u32 t0 = q * kQuarter;
+	u32 c[kQuarter];
+	load_quarter(counts, t0, tiles, d, c);
+	u32 s = 0;
+#pragma unroll
+	for (u32 i = 0; i < kQuarter; ++i) s += c[i];
+	s_q[q][d] = s;
+	__syncthreads();
+	const u32 total = q == 0 ? s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d] : 0u;
+	// Threads are ordered (q, d), so the q == 0 threads are the first 256 and
+	// their exclusive sum is the digit base.
+	u32 run = block_exclusive_sum<kGroupThreads>(total, s_wsum, static_cast<u32*>(nullptr));
+	__shared__ u32 s_base[kRadixMax];
+	if (q == 0) s_base[d] = run;
+	__syncthreads();
+	run = s_base[d];
+	for (u32 qq = 0; qq < q; ++qq) run += s_q[qq][d];
+#pragma unroll
+	for (u32 i = 0; i < kQuarter; ++i) {
+		if (t0 + i < tiles) counts[static_cast<size_t>(t0 + i) * kRadixMax + d] = run;
+		run += c[i];
+	}
+}
+
 // Turns the tile counts into global output offsets, in place:
 // counts[t][d] <- (#keys with a smaller digit) + (#keys with digit d in tiles < t).
 __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* __restrict__ plan,
@@ -642,18 +675,22 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 				counts, andor, 0, digits);
 			if (!L.done(err)) return false;
 		}
-		{
-			Launch L(q, CLO_SORT_SATRADIX_KNAME_REDUCE);
-			satradix_reduce<<<nblk, kGroupThreads, 0, st>>>(plan, p, counts, tiles, partial);
-			if (!L.done(err)) return false;
-		}
-		{
+		if (nblk == 1) {
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
-			satradix_scan_partials<<<1, kPartThreads, 0, st>>>(plan, p, partial,
-				nblk * kRadixMax);
+			satradix_scan_one<<<1, kGroupThreads, 0, st>>>(plan, p, counts, tiles);
 			if (!L.done(err)) return false;
-		}
-		{
+		} else {
+			{
+				Launch L(q, CLO_SORT_SATRADIX_KNAME_REDUCE);
+				satradix_reduce<<<nblk, kGroupThreads, 0, st>>>(plan, p, counts, tiles, partial);
+				if (!L.done(err)) return false;
+			}
+			{
+				Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
+				satradix_scan_partials<<<1, kPartThreads, 0, st>>>(plan, p, partial,
+					nblk * kRadixMax);
+				if (!L.done(err)) return false;
+			}
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
 			satradix_scan<<<nblk, kGroupThreads, 0, st>>>(plan, p, counts, tiles, partial);
 			if (!L.done(err)) return false;
@@ -684,11 +721,13 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	return true;
 }
 
-// Default tile shape per key width (measured on MI355X at 2^28 keys: 512x16
-// is 7% faster for u64 keys alone and 12% for u32 keys with values; 256x32
-// wins or ties elsewhere; 16384-key tiles run one block per CU and lose
-// 20-40%).
-inline int auto_shape(size_t ksize, bool kv) {
+// Default tile shape, measured on MI355X.  At 2^28 keys 512x16 is 7% faster
+// for u64 keys alone and 12% for u32 keys with values, 256x32 wins or ties
+// elsewhere, and 16384-key tiles run one block per CU and lose 20-40%.  Up to
+// 2^20 keys the 4096-key tile (256x16) is 10-25% faster: twice the blocks,
+// and still one group of tiles for the one-block counter scan.
+inline int auto_shape(size_t ksize, bool kv, size_t n) {
+	if (n <= (static_cast<size_t>(1) << 20)) return 2;
 	if ((ksize == 8 && !kv) || (ksize == 4 && kv)) return 1;
 	return 0;
 }
@@ -696,7 +735,7 @@ inline int auto_shape(size_t ksize, bool kv) {
 template <typename K, bool KV>
 bool launch_shape(SatradixData* sd, CCLQueue* q, void* kin, u32* vin, void* kout, u32* vout,
 	size_t n, GError** err) {
-	int sh = sd->shape >= 0 ? sd->shape : auto_shape(sizeof(K), KV);
+	int sh = sd->shape >= 0 ? sd->shape : auto_shape(sizeof(K), KV, n);
 	if (sd->bits != 8) sh = 0;  // other digit widths: the generic-width kernels
 	K* ki = static_cast<K*>(kin);
 	K* ko = static_cast<K*>(kout);
@@ -871,7 +910,7 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	(void)numel;
 	const size_t ks = clo_type_sizeof(sorter->elem_type);
 	const SatradixData* sd = sdata(sorter);
-	const int sh = sd && sd->shape >= 0 ? sd->shape : auto_shape(ks, true);
+	const int sh = sd && sd->shape >= 0 ? sd->shape : auto_shape(ks, true, numel);
 	const size_t nt = kShapes[sh].nt, ipt = kShapes[sh].ipt;
 	switch (i) {
 	case CLO_SORT_SATRADIX_KIDX_UPSWEEP:

@@ -19,8 +19,9 @@
  *   6 blelloch 8 items     scan   any n
  *   7 blelloch 16 items    scan   any n (the plain clo_scan path)
  *   8 blelloch 32 items    scan   any n
- * The fixed configuration (what the plain API runs): satradix's default
- * tile shape (0 for 32-bit keys, 1 for 64-bit keys), and 7 for scans.
+ * The fixed configuration is what the plain API runs: satradix's default
+ * tile shape (2 up to 2^20 keys, else 0 for 32-bit and 1 for 64-bit keys),
+ * and 7 for scans.
  *
  * Features (CLO_DISPATCH_NUM_FEATURES floats, from a GPU kernel over a
  * sample of up to 4096 evenly spaced elements):
@@ -30,9 +31,10 @@
  *   3 max key-byte entropy / 8     6 fraction in descending order
  *                                  7 op (0 sort, 1 scan)
  * Model input row (CLO_DISPATCH_MODEL_INPUTS floats) = the features followed
- * by a one-hot of the configuration; the model predicts log2 of the time in
- * microseconds, and the choice is the feasible configuration with the least
- * prediction.
+ * by a one-hot of the configuration; the model predicts a log2 time (absolute,
+ * or relative to the fixed configuration of the same request: only the
+ * order of one request's predictions matters), and the choice is the feasible
+ * configuration with the least prediction.
  */
 #ifndef CLO_DISPATCH_H_
 #define CLO_DISPATCH_H_
@@ -63,7 +65,7 @@ void clo_dispatch_destroy(CloDispatch* d);
 
 const char* clo_dispatch_config_name(cl_uint cfg);
 cl_bool clo_dispatch_config_feasible(cl_uint cfg, CloOp op, size_t n, size_t width);
-cl_uint clo_dispatch_fixed_config(CloOp op, size_t width);
+cl_uint clo_dispatch_fixed_config(CloOp op, size_t n, size_t width);
 
 /* Features of a request (blocking: the sample statistics come back to the
  * host).  width = 4 or 8 bytes. */
@@ -97,6 +99,9 @@ cl_bool clo_dispatch_table_load(CloDispatch* d, const char* path, GError** err);
 /* Learned model (borrowed; NULL removes it).  Its input count must be
  * CLO_DISPATCH_MODEL_INPUTS. */
 cl_bool clo_dispatch_set_model(CloDispatch* d, CloMlp* mlp, GError** err);
+/* Model policy: keep the fixed configuration unless the predicted log2 time
+ * of the best one is lower by more than log2_margin (default 0). */
+void clo_dispatch_set_margin(CloDispatch* d, float log2_margin);
 /* Batched choice for `count` requests with one model launch over every
  * (request, feasible configuration) row.  ops/ns/widths/features
  * (count x NUM_FEATURES) describe the requests; cfg_out receives the

@@ -127,7 +127,7 @@ def test_tuner_and_table(clops, oracle, ctx, tmp_path):
     assert d2.table_choose(clops.OP_SORT, n, 4, f) == best
     # fixed / table policies
     ch = d2.choose(None, clops.POLICY_TABLE, [0, 0], [n, 1 << 24], [4, 8], np.stack([f, f]))
-    assert ch[0] == best and ch[1] == clops.Dispatch.fixed(clops.OP_SORT, 8)
+    assert ch[0] == best and ch[1] == clops.Dispatch.fixed(clops.OP_SORT, 1 << 24, 8)
 
 
 def _synthetic_cost(ops, ns, widths, feats, c):
