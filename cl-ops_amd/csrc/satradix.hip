@@ -444,6 +444,10 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const u32 wbase = tile_base + w * kWave * ITEMS;
 	K k[ITEMS];
 	u32 v[ITEMS];
+	// A wave issuing its loads (and later its scatter stores) runs at raised
+	// priority so its memory requests go out ahead of other waves' ranking
+	// work (measured: 2% on the 2^28 u64 KV downsweep).
+	__builtin_amdgcn_s_setprio(3);
 #pragma unroll
 	for (int j = 0; j < ITEMS; ++j) {
 		const u32 idx = wbase + j * kWave + lane;
@@ -457,6 +461,7 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 			v[j] = idx < n ? vsrc[idx] : 0u;
 		}
 	}
+	__builtin_amdgcn_s_setprio(0);
 	__syncthreads();
 
 	// Rank inside the wave.  The lanes holding one digit (found with one ballot
@@ -533,6 +538,7 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 
 	const u32 count = (n - tile_base) < kTile ? (n - tile_base) : kTile;
 	u32 oidx[ITEMS];
+	__builtin_amdgcn_s_setprio(2);
 	if (d.nact) {
 		const u32 nshift = d.nshift;
 #pragma unroll
@@ -576,6 +582,7 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 			if (i < count) vdst[oidx[r]] = s_vals[i];
 		}
 	}
+	__builtin_amdgcn_s_setprio(0);
 }
 
 // ---------------------------------------------------------------------------

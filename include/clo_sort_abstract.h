@@ -94,7 +94,7 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
  * device buffer of nseg uint32 segment starts, non-decreasing, offsets[0] = 0.
  * vals_in may be NULL (keys only).  keys_out == NULL sorts in place.
  * Sorter options tune it: segtile=8|16|32 (LDS tile = 256*segtile elements;
- * 8|16 for 64-bit keys), seggrid=<blocks> (0 = two per compute unit). */
+ * 8|16 for 64-bit keys), seggrid=<blocks> (0 = three per compute unit). */
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
 	CCLBuffer* vals_out, CCLBuffer* offsets, size_t nseg, size_t numel, GError** err);
