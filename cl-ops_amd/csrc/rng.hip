@@ -16,10 +16,10 @@ namespace {
 
 // Generated at build time from include/clo_rng_device.h (the @@section
 // blocks): kSrcLcg, kSrcXorshift64, kSrcXorshift128, kSrcMwc64x,
-// kSrcParkmiller, kSrcApi.
+// kSrcParkmiller, kSrcTauslcg, kSrcApi.
 #include "rng_sources.inc"
 
-enum GenId { kLcg = 0, kXs64 = 1, kXs128 = 2, kMwc = 3, kPm = 4 };
+enum GenId { kLcg = 0, kXs64 = 1, kXs128 = 2, kMwc = 3, kPm = 4, kTaus = 5 };
 
 // Seed hash of the DEV_GID initialisation (clo_rng_init.cl:27-38).
 enum HashId { kHashNone = 0, kHashKnuth = 1, kHashXs1 = 2 };
@@ -81,6 +81,7 @@ const Info kInfos[] = {
 	{"xorshift128", 16, kSrcXorshift128},
 	{"mwc64x", 8, kSrcMwc64x},
 	{"parkmiller", 4, kSrcParkmiller},
+	{"tauslcg", 16, kSrcTauslcg},
 };
 
 // MT19937 (Matsumoto & Nishimura 1998) with init_genrand seeding: the
@@ -130,6 +131,7 @@ const struct clo_rng_info clo_rng_infos[] = {
 	{"xorshift128", kSrcXorshift128, 16},
 	{"mwc64x", kSrcMwc64x, 8},
 	{"parkmiller", kSrcParkmiller, 4},
+	{"tauslcg", kSrcTauslcg, 16},
 	{nullptr, nullptr, 0},
 };
 #endif
@@ -166,6 +168,11 @@ static bool launch_init(int gen, void* seeds, u64 count, u64 main_seed, u32 hash
 			<<<static_cast<u32>(g), 256, 0, st>>>(
 			static_cast<clo_rng_parkmiller::clo_statetype*>(seeds), count, main_seed, hash);
 		break;
+	case kTaus:
+		rng_init_kernel<clo_rng_tauslcg::clo_statetype, clo_rng_tauslcg::clo_ulong2statetype>
+			<<<static_cast<u32>(g), 256, 0, st>>>(
+			static_cast<clo_rng_tauslcg::clo_statetype*>(seeds), count, main_seed, hash);
+		break;
 	}
 	return L.done(err);
 }
@@ -185,7 +192,7 @@ CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type, void* seeds, siz
 		return nullptr;
 	}
 	int gen = -1;
-	for (int i = 0; i < 5; ++i)
+	for (int i = 0; i < static_cast<int>(sizeof(kInfos) / sizeof(kInfos[0])); ++i)
 		if (type && strcmp(type, kInfos[i].name) == 0) gen = i;
 	if (gen < 0) {
 		set_error(err, kQuarkClo, CLO_ERROR_IMPL_NOT_FOUND,
@@ -346,6 +353,11 @@ CCLEvent* clo_rng_fill(CloRng* rng, CCLQueue* cq, CCLBuffer* out, cl_uint draws,
 	case kPm:
 		rng_fill_kernel<clo_rng_parkmiller::clo_statetype, clo_rng_parkmiller::clo_rng_step>
 			<<<g, 256, 0, s>>>(static_cast<clo_rng_parkmiller::clo_statetype*>(st), o, G, draws,
+			shift, maxint);
+		break;
+	case kTaus:
+		rng_fill_kernel<clo_rng_tauslcg::clo_statetype, clo_rng_tauslcg::clo_rng_step>
+			<<<g, 256, 0, s>>>(static_cast<clo_rng_tauslcg::clo_statetype*>(st), o, G, draws,
 			shift, maxint);
 		break;
 	}

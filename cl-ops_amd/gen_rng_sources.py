@@ -5,7 +5,8 @@ import re
 import sys
 
 NAMES = {"lcg": "kSrcLcg", "xorshift64": "kSrcXorshift64", "xorshift128": "kSrcXorshift128",
-         "mwc64x": "kSrcMwc64x", "parkmiller": "kSrcParkmiller", "api": "kSrcApi"}
+         "mwc64x": "kSrcMwc64x", "parkmiller": "kSrcParkmiller", "tauslcg": "kSrcTauslcg",
+         "api": "kSrcApi"}
 
 
 def main(header, out):

@@ -8,9 +8,8 @@
  * callers that compile their own kernels with hipRTC; the same functions are
  * shipped as a header in clo_rng_device.h.
  *
- * Generators: lcg, xorshift64, xorshift128, mwc64x, parkmiller.
- * ("tauslcg" is listed by the reference and is not provided here;
- * clo_rng_new fails for it with CLO_ERROR_IMPL_NOT_FOUND.)
+ * Generators: lcg, xorshift64, xorshift128, mwc64x, parkmiller, tauslcg (the
+ * reference's clo_rng_infos order, clo_rng.c:60-68).
  */
 #ifndef CLO_RNG_H_
 #define CLO_RNG_H_
@@ -22,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CLO_RNG_IMPLS "lcg, xorshift64, xorshift128, mwc64x, parkmiller"
+#define CLO_RNG_IMPLS "lcg, xorshift64, xorshift128, mwc64x, parkmiller, tauslcg"
 
 /* clo_rng.in.h:50-65 */
 struct clo_rng_info {

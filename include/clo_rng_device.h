@@ -155,6 +155,45 @@ __device__ inline unsigned int clo_rng_next(clo_statetype* states, unsigned int 
 }  // namespace clo_rng_parkmiller
 /* @@end */
 
+/* @@section tauslcg */
+/* Hybrid Tausworthe/LCG generator as CL_Ops runs it
+ * (src/cl_ops/rng/clo_rng_tauslcg.cl:31-105): state (x, y, z, w) = the 64-bit
+ * seed's (low, high, low, high) 32-bit halves.  One step moves every word one
+ * slot down while advancing it: x <- T1(y), y <- T2(z), z <- T3(w),
+ * w <- 1664525 * x + 1013904223 (the old x), and returns the new x, where
+ * T(v) = ((v & m) << s3) ^ (((v << s1) ^ v) >> s2) with (s1, s2, s3, m) =
+ * (13, 19, 12, ~1), (2, 25, 4, ~7), (3, 11, 17, ~1). */
+namespace clo_rng_tauslcg {
+typedef uint4 clo_statetype;
+__host__ __device__ inline clo_statetype clo_ulong2statetype(unsigned long long seed) {
+	clo_statetype s;
+	s.x = (unsigned int)(seed & 0xFFFFFFFFULL);
+	s.y = (unsigned int)(seed >> 32);
+	s.z = s.x;
+	s.w = s.y;
+	return s;
+}
+__host__ __device__ inline unsigned int clo_taus(unsigned int v, int s1, int s2, int s3,
+	unsigned int m) {
+	return ((v & m) << s3) ^ (((v << s1) ^ v) >> s2);
+}
+__host__ __device__ inline unsigned int clo_rng_step(clo_statetype& s) {
+	const unsigned int x = s.x;
+	s.x = clo_taus(s.y, 13, 19, 12, 4294967294U);
+	s.y = clo_taus(s.z, 2, 25, 4, 4294967288U);
+	s.z = clo_taus(s.w, 3, 11, 17, 4294967294U);
+	s.w = 1664525U * x + 1013904223U;
+	return s.x;
+}
+__device__ inline unsigned int clo_rng_next(clo_statetype* states, unsigned int index) {
+	clo_statetype s = states[index];
+	unsigned int r = clo_rng_step(s);
+	states[index] = s;
+	return r;
+}
+}  // namespace clo_rng_tauslcg
+/* @@end */
+
 /* @@section api */
 /* Workitem API (src/cl_ops/rng/clo_rng_api.cl:33-105 and
  * clo_rng_workitem.cl:24-32) over whichever generator namespace is in use:
@@ -191,5 +230,6 @@ CLO_RNG_DEFINE_API(clo_rng_xorshift64)
 CLO_RNG_DEFINE_API(clo_rng_xorshift128)
 CLO_RNG_DEFINE_API(clo_rng_mwc64x)
 CLO_RNG_DEFINE_API(clo_rng_parkmiller)
+CLO_RNG_DEFINE_API(clo_rng_tauslcg)
 
 #endif /* CLO_RNG_DEVICE_H_ */

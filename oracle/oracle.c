@@ -473,8 +473,8 @@ void orc_scan_blelloch(const uint32_t* in, void* out, int sum_width, size_t n, s
  * RNG (src/cl_ops/rng/clo_rng_*.cl).
  */
 size_t orc_rng_state_size(int gen) {
-	static const size_t sz[5] = {8, 8, 16, 8, 4}; /* clo_rng.c:60-68 */
-	return (gen >= 0 && gen < 5) ? sz[gen] : 0;
+	static const size_t sz[6] = {8, 8, 16, 8, 4, 16}; /* clo_rng.c:60-68 */
+	return (gen >= 0 && gen < 6) ? sz[gen] : 0;
 }
 
 void orc_rng_from_seed(int gen, uint64_t seed, void* state) {
@@ -500,8 +500,20 @@ void orc_rng_from_seed(int gen, uint64_t seed, void* state) {
 	case 4: /* clo_rng_parkmiller.cl:31 */
 		*(int32_t*)state = (int32_t)(uint32_t)(seed & 0xFFFFFFFFull);
 		break;
+	case 5: { /* clo_rng_tauslcg.cl:35, as_uint4((ulong2)(seed, seed)) */
+		uint32_t* s = (uint32_t*)state;
+		s[0] = s[2] = (uint32_t)seed;
+		s[1] = s[3] = (uint32_t)(seed >> 32);
+		break;
+	}
 	default: break;
 	}
+}
+
+/* One Tausworthe step (clo_rng_tauslcg.cl:52-55). */
+uint32_t orc_taus(uint32_t z, int s1, int s2, int s3, uint32_t m) {
+	uint32_t b = ((z << s1) ^ z) >> s2;
+	return ((z & m) << s3) ^ b;
 }
 
 uint32_t orc_rng_next(int gen, void* state) {
@@ -546,6 +558,15 @@ uint32_t orc_rng_next(int gen, void* state) {
 		s = (int32_t)(((int64_t)s * 16807) % 2147483647);
 		*(int32_t*)state = s;
 		return ((uint32_t)s) << 1;
+	}
+	case 5: { /* clo_rng_tauslcg.cl:52-105 */
+		uint32_t* s = (uint32_t*)state;
+		uint32_t x = s[0];
+		s[0] = orc_taus(s[1], 13, 19, 12, 4294967294u);
+		s[1] = orc_taus(s[2], 2, 25, 4, 4294967288u);
+		s[2] = orc_taus(s[3], 3, 11, 17, 4294967294u);
+		s[3] = 1664525u * x + 1013904223u;
+		return s[0];
 	}
 	default: return 0;
 	}

@@ -69,13 +69,16 @@ void orc_scan_blelloch(const uint32_t* in, void* out, int sum_width, size_t n, s
 	int threads);
 
 /* ---- RNG ------------------------------------------------------------------ */
-/* gen: 0 lcg, 1 xorshift64, 2 xorshift128, 3 mwc64x, 4 parkmiller
+/* gen: 0 lcg, 1 xorshift64, 2 xorshift128, 3 mwc64x, 4 parkmiller, 5 tauslcg
  * (src/cl_ops/rng/clo_rng.c:60-68).  States are stored as in device memory:
  * lcg/xorshift64 uint64, xorshift128 4 x uint32 (x,y,z,w), mwc64x 2 x uint32
- * (x,c), parkmiller int32. */
+ * (x,c), parkmiller int32, tauslcg 4 x uint32 (x,y,z,w). */
 size_t orc_rng_state_size(int gen);
 /* clo_ulong2statetype of each generator. */
 void orc_rng_from_seed(int gen, uint64_t seed, void* state);
+/* The Tausworthe step of tauslcg (src/cl_ops/rng/clo_rng_tauslcg.cl:52-55):
+ * ((z & m) << s3) ^ (((z << s1) ^ z) >> s2). */
+uint32_t orc_taus(uint32_t z, int s1, int s2, int s3, uint32_t m);
 /* One clo_rng_next (src/cl_ops/rng/clo_rng_<gen>.cl). */
 uint32_t orc_rng_next(int gen, void* state);
 /* DEV_GID seeds (src/cl_ops/rng/clo_rng_init.cl:47-60): state[gid] =

@@ -15,8 +15,9 @@ _LIB = None
 GEN_U64, GEN_U32_HI, GEN_U64_MASK, GEN_U32_MASK, GEN_IOTA_U32 = 0, 1, 2, 3, 4
 GEN_U32_MOD, GEN_U64_MOD = 6, 7
 GEN_NEARLY_SORTED_U32, GEN_NEARLY_SORTED_U64, GEN_REVERSED_U32, GEN_REVERSED_U64 = 8, 9, 10, 11
-RNG_NAMES = ["lcg", "xorshift64", "xorshift128", "mwc64x", "parkmiller"]
-STATE_SIZE = {"lcg": 8, "xorshift64": 8, "xorshift128": 16, "mwc64x": 8, "parkmiller": 4}
+RNG_NAMES = ["lcg", "xorshift64", "xorshift128", "mwc64x", "parkmiller", "tauslcg"]
+STATE_SIZE = {"lcg": 8, "xorshift64": 8, "xorshift128": 16, "mwc64x": 8, "parkmiller": 4,
+              "tauslcg": 16}
 
 _U = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
 _S = {1: np.int8, 2: np.int16, 4: np.int32, 8: np.int64}
@@ -48,6 +49,7 @@ def lib():
             "orc_rng_init_dev_gid": (None, [i32, sz, u64, i32, vp]),
             "orc_rng_init_host_mt": (None, [i32, sz, u64, vp]),
             "orc_mt19937": (None, [u32, sz, vp]),
+            "orc_taus": (u32, [u32, i32, i32, i32, u32]),
             "orc_rng_fill": (None, [i32, vp, sz, u32, u32, u32, vp, i32]),
         }
         for name, (res, args) in sig.items():

@@ -60,11 +60,13 @@ def test_rng_infos_table(clops):
         _fields_ = [("name", ctypes.c_char_p), ("src", ctypes.c_char_p),
                     ("seed_size", ctypes.c_size_t)]
 
-    tab = (Info * 6).in_dll(clops.lib(), "clo_rng_infos")
-    got = [(tab[i].name, tab[i].seed_size) for i in range(6)]
+    # the reference's table, in its order (src/cl_ops/rng/clo_rng.c:60-68)
+    tab = (Info * 7).in_dll(clops.lib(), "clo_rng_infos")
+    got = [(tab[i].name, tab[i].seed_size) for i in range(7)]
     assert got == [(b"lcg", 8), (b"xorshift64", 8), (b"xorshift128", 16), (b"mwc64x", 8),
-                   (b"parkmiller", 4), (None, 0)]
+                   (b"parkmiller", 4), (b"tauslcg", 16), (None, 0)]
     assert b"namespace clo_rng_xorshift128" in tab[2].src
+    assert b"namespace clo_rng_tauslcg" in tab[5].src
 
 
 def test_type_table(clops):

@@ -4,7 +4,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-GENS = ["lcg", "xorshift64", "xorshift128", "mwc64x", "parkmiller"]
+GENS = ["lcg", "xorshift64", "xorshift128", "mwc64x", "parkmiller", "tauslcg"]
 
 
 @pytest.mark.parametrize("gen", GENS)
@@ -81,7 +81,7 @@ def test_config2_fills_prefix(clops, oracle, ctx, queue):
 
 def test_errors(clops, ctx):
     with pytest.raises(clops.ClopsError) as e:
-        clops.Rng(ctx, "tauslcg", clops.SEED_DEV_GID, 10)
+        clops.Rng(ctx, "mersenne", clops.SEED_DEV_GID, 10)
     assert e.value.code == clops.CLO_ERROR_IMPL_NOT_FOUND
     with pytest.raises(clops.ClopsError) as e:
         clops.Rng(ctx, "lcg", clops.SEED_DEV_GID, 10, hash="(x * 3 + 1)")

@@ -425,3 +425,66 @@ def test_adam_closed_form(oracle):
     g = np.array([0.3, -7.0, 1e-3])
     p1, m, v = oracle.adam_step(p, np.zeros(3), np.zeros(3), g, 1, 0.01, eps=0.0)
     assert np.allclose(p1, p - 0.01 * np.sign(g))
+
+
+# --------------------------------------------------------------------------
+# tauslcg (src/cl_ops/rng/clo_rng_tauslcg.cl): Tausworthe components and the
+# LCG, each pinned by what the mathematics fixes.
+
+def _taus_fn(oracle, s1, s2, s3, m):
+    return lambda z: int(oracle.lib().orc_taus(z & 0xFFFFFFFF, s1, s2, s3, m))
+
+
+def test_tauslcg_component1_period(oracle):
+    # (13, 19, 12, ~1) is L'Ecuyer's taus88 first component: a 31-bit LFSR of
+    # period 2^31 - 1 (a prime), so T^(2^31-1) fixes every reachable state and
+    # T itself fixes none.
+    T = _taus_fn(oracle, 13, 19, 12, 0xFFFFFFFE)
+    cols = _gf2_matrix(T, 32)
+    P = _gf2_pow(cols, (1 << 31) - 1, 32)
+    for z0 in (0x12345678, 0xDEADBEEF, 0x80000001):
+        z1 = T(z0)
+        assert _gf2_apply(P, z1) == z1
+        assert T(z1) != z1
+
+
+def test_tauslcg_component2_period(oracle):
+    # (2, 25, 4, ~7) is taus88's second component: period 2^29 - 1
+    # = 233 * 1103 * 2089.
+    T = _taus_fn(oracle, 2, 25, 4, 0xFFFFFFF8)
+    cols = _gf2_matrix(T, 32)
+    e = (1 << 29) - 1
+    assert 233 * 1103 * 2089 == e
+    P = _gf2_pow(cols, e, 32)
+    for z0 in (0x12345678, 0xCAFEBABE):
+        z = T(T(T(z0)))
+        assert _gf2_apply(P, z) == z
+        for p in (233, 1103, 2089):
+            assert _gf2_apply(_gf2_pow(cols, e // p, 32), z) != z
+
+
+def test_tauslcg_lcg_word(oracle):
+    # The fourth word is Numerical Recipes' quick generator (ranqd1)
+    # 1664525 * x + 1013904223; its published sequence from 0:
+    seq = [0, 1013904223, 1196435762, 3519870697, 2868466484, 1649599747, 2670642822,
+           1476291629, 2748932008, 2180890343, 2498801434]
+    for a, b in zip(seq, seq[1:]):
+        st = np.array([a, 0, 0, 0], dtype=np.uint32).view(np.uint8)
+        oracle.rng_next("tauslcg", st)
+        assert int(st.view(np.uint32)[3]) == b
+
+
+def test_tauslcg_seed_and_rotation(oracle):
+    s = 0x0123456789ABCDEF
+    st = oracle.rng_from_seed("tauslcg", s).view(np.uint32)
+    assert [int(w) for w in st] == [0x89ABCDEF, 0x01234567, 0x89ABCDEF, 0x01234567]
+    # one step moves each word one slot towards x through its own map
+    T1 = _taus_fn(oracle, 13, 19, 12, 0xFFFFFFFE)
+    T2 = _taus_fn(oracle, 2, 25, 4, 0xFFFFFFF8)
+    T3 = _taus_fn(oracle, 3, 11, 17, 0xFFFFFFFE)
+    x, y, z, w = (int(v) for v in st)
+    b = st.view(np.uint8).copy()
+    out = oracle.rng_next("tauslcg", b)
+    nx, ny, nz, nw = (int(v) for v in b.view(np.uint32))
+    assert (nx, ny, nz) == (T1(y), T2(z), T3(w)) and out == nx
+    assert nw == (1664525 * x + 1013904223) & 0xFFFFFFFF
