@@ -136,8 +136,9 @@ def headline(args, ctx):
     }
 
 
-def suite(ctx):
-    """Other configurations, one stderr JSON line each."""
+def suite(ctx, cpu_threads=0):
+    """Other configurations, one stderr JSON line each (with the CPU
+    restatement timed beside them when cpu_threads > 0)."""
     q = clops.Queue(ctx, profiling=True)
     out = []
     # C1: 2^20 uint32 keys only.
@@ -232,8 +233,73 @@ def suite(ctx):
                     "sweeps": sw, "hbm_frac": sw * 2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                     "kernel_ms_per_sort": kern})
     out.extend(suite_segmented(ctx, q))
+    if cpu_threads:
+        cpu = cpu_suite(cpu_threads)
+        for o in out:
+            for key, rec in cpu.items():
+                if o["config"].startswith(key):
+                    o["cpu_baseline"] = rec
     for o in out:
         log(json.dumps(o))
+
+
+def _best_time(fn, reps=2):
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    return best
+
+
+def cpu_suite(threads):
+    """The oracle's reference-structured CPU restatements, timed on bounded
+    samples of each suite configuration (rates in the GPU line's units)."""
+    sys.path[:0] = [os.path.join(ROOT, "oracle")]
+    import oracle
+    res = {}
+    n = 1 << 20
+    k = oracle.generate(oracle.GEN_U32_HI, 0xC1, n)
+    t = _best_time(lambda: oracle.satradix(k, bits=4, lws=256, threads=threads))
+    res["satradix u32 2^20"] = {"value": n / t / 1e9, "unit": "Gkeys/s", "cores": threads,
+                                "kind": "port", "sample": "the full 2^20 keys, 4-bit digits, "
+                                "256-element blocks, %.3f s" % t}
+    m = 1 << 24
+    a = oracle.generate(oracle.GEN_U32_MASK, 0x2C, m, 0xFF)
+    for sumt, key in ((np.uint32, "scan uint32->uint32"), (np.uint64, "scan uint32->uint64")):
+        t = _best_time(lambda: oracle.scan_blelloch(a, sumt, lws=256, threads=threads))
+        by = m * (4 + np.dtype(sumt).itemsize)
+        res[key] = {"value": by / t / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+                    "sample": "2^24 elements, 3-phase Blelloch with 2*256-element blocks, %.3f s" % t}
+    G, draws = 1 << 20, 16
+    for gen, ss in (("xorshift128", 16), ("mwc64x", 8)):
+        st = oracle.rng_init_dev_gid(gen, G, 0, 0)
+        t = _best_time(lambda: oracle.rng_fill(gen, st.copy(), draws, threads=threads))
+        res["rng fill " + gen] = {"value": G * (2 * ss + 4 * draws) / t / 1e9, "unit": "GB/s",
+                                  "cores": threads, "kind": "port",
+                                  "sample": "2^20 work-items x 16 draws, %.3f s" % t}
+    b = oracle.generate(oracle.GEN_U32_HI, 0xC3D, 1 << 20)
+    t = _best_time(lambda: oracle.sbitonic(b), 1)
+    res["sbitonic u32"] = res["abitonic u32"] = {
+        "value": (1 << 20) / t / 1e9, "unit": "Gkeys/s", "cores": 1, "kind": "port",
+        "sample": "2^20 keys through the full network, one thread, %.2f s" % t}
+    lens = np.clip(np.floor(np.exp(np.random.default_rng(0xC4).uniform(0, np.log(65536), 512))),
+                   1, 65536).astype(np.int64)
+    keys = oracle.generate(oracle.GEN_U32_HI, 0xC4B, int(lens.sum()))
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
+
+    def segs():
+        for o, ln in zip(offs, lens):
+            oracle.stable_sort(keys[o:o + ln])
+
+    t = _best_time(segs, 1)
+    res["segmented sort u32"] = {"value": int(lens.sum()) / t / 1e9, "unit": "Gkeys/s",
+                                 "cores": 1, "kind": "port",
+                                 "sample": "512 segments of the same length distribution "
+                                           "(%d keys), stable LSD per segment, %.2f s"
+                                           % (int(lens.sum()), t)}
+    return res
 
 
 def c4_offsets(ctx, q, nseg=32768, max_len=65536, seed=0xC4A):
@@ -497,7 +563,7 @@ def main():
         return
     h = headline(args, ctx)
     if args.suite:
-        suite(ctx)
+        suite(ctx, 0 if args.no_cpu_baseline else (args.cpu_threads or min(os.cpu_count() or 1, 16)))
     disp = None
     if not args.no_dispatch:
         disp = dispatch_eval(ctx, args.dispatch_train, args.dispatch_eval)
