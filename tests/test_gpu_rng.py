@@ -93,3 +93,73 @@ def test_source_text(clops, ctx):
     src = r.source()
     assert "namespace clo_rng_mwc64x" in src and "4294883355U" in src
     assert "clo_rng_next_int" in src
+
+
+def _hiprtc_kernel(src, name):
+    """Compiles `src` with hipRTC for the current GPU and returns
+    (module, function) handles (ctypes voidp)."""
+    import ctypes
+    rtc = ctypes.CDLL("libhiprtc.so")
+    hip = ctypes.CDLL("libamdhip64.so")
+    prog = ctypes.c_void_p()
+    assert rtc.hiprtcCreateProgram(ctypes.byref(prog), src.encode(), b"user.hip", 0, None,
+                                   None) == 0
+    opts = (ctypes.c_char_p * 1)(b"-O3")  # target: the current device
+    rc = rtc.hiprtcCompileProgram(prog, 1, opts)
+    if rc != 0:
+        sz = ctypes.c_size_t()
+        rtc.hiprtcGetProgramLogSize(prog, ctypes.byref(sz))
+        log = ctypes.create_string_buffer(sz.value + 1)
+        rtc.hiprtcGetProgramLog(prog, log)
+        raise AssertionError("hipRTC failed: " + log.value.decode(errors="replace")[-2000:])
+    sz = ctypes.c_size_t()
+    assert rtc.hiprtcGetCodeSize(prog, ctypes.byref(sz)) == 0
+    code = ctypes.create_string_buffer(sz.value)
+    assert rtc.hiprtcGetCode(prog, code) == 0
+    mod, fn = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipModuleLoadData(ctypes.byref(mod), code) == 0
+    assert hip.hipModuleGetFunction(ctypes.byref(fn), mod, name.encode()) == 0
+    return hip, mod, fn, code
+
+
+@pytest.mark.parametrize("gen", ["xorshift128", "tauslcg", "mwc64x"])
+def test_source_serves_user_kernels(clops, oracle, ctx, queue, gen):
+    """clo_rng_get_source() is what a user kernel compiles against (the
+    reference hands it to OpenCL programs): a hipRTC kernel drawing with
+    clo_rng_next_int4 gives the oracle's words."""
+    import ctypes
+    G, nmod = 1024, 1000003
+    r = clops.Rng(ctx, gen, clops.SEED_DEV_GID, 4 * G, main_seed=77)
+    st = r.read_states(queue)
+    src = r.source() + """
+extern "C" __global__ void user_draw(clo_statetype* st, unsigned int* out, unsigned int n) {
+	const unsigned int g = CLO_RNG_GID1();
+	const uint4 v = clo_rng_next_int4(st, n);
+	out[4 * g] = v.x;
+	out[4 * g + 1] = v.y;
+	out[4 * g + 2] = v.z;
+	out[4 * g + 3] = v.w;
+}
+"""
+    hip, mod, fn, _code = _hiprtc_kernel(src, "user_draw")
+    out = clops.Buffer(ctx, 4 * G * 4)
+    L = clops.lib()
+    p_st = ctypes.c_void_p(L.ccl_buffer_get_device_ptr(L.clo_rng_get_device_seeds(r.ptr)))
+    p_out = ctypes.c_void_p(L.ccl_buffer_get_device_ptr(out.ptr))
+    n = ctypes.c_uint(nmod)
+    args = (ctypes.c_void_p * 3)(ctypes.cast(ctypes.byref(p_st), ctypes.c_void_p),
+                                 ctypes.cast(ctypes.byref(p_out), ctypes.c_void_p),
+                                 ctypes.cast(ctypes.byref(n), ctypes.c_void_p))
+    queue.finish()
+    assert hip.hipModuleLaunchKernel(fn, G // 256, 1, 1, 256, 1, 1, 0, None, args, None) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    got = out.read(queue, np.uint32, 4 * G).reshape(G, 4)
+    ss = oracle.STATE_SIZE[gen]
+    per = st.reshape(4 * G, ss)
+    want = np.zeros((G, 4), dtype=np.uint32)
+    for g in range(G):
+        for k in range(4):
+            s = per[k * G + g].copy()
+            want[g, k] = oracle.rng_next(gen, s) % nmod
+    assert np.array_equal(got, want)
+    hip.hipModuleUnload(mod)
