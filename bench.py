@@ -233,6 +233,7 @@ def suite(ctx, cpu_threads=0):
                     "sweeps": sw, "hbm_frac": sw * 2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                     "kernel_ms_per_sort": kern})
     out.extend(suite_segmented(ctx, q))
+    out.extend(suite_snippets(ctx, q))
     if cpu_threads:
         cpu = cpu_suite(cpu_threads)
         for o in out:
@@ -241,6 +242,48 @@ def suite(ctx, cpu_threads=0):
                     o["cpu_baseline"] = rec
     for o in out:
         log(json.dumps(o))
+
+
+FLOAT_KEY = "(as_uint(x) ^ ((uint)(as_int(x) >> 31) | 0x80000000u))"
+
+
+def suite_snippets(ctx, q):
+    """Sorters compiled at run time from compare / key snippets, each timed
+    beside the precompiled sorter of the same size and element width."""
+    out = []
+    t0 = time.perf_counter()
+    clops.Sorter(ctx, "sbitonic", np.uint64, key_dtype=np.uint32,
+                 get_key="((uint)((x) >> SHIFT))", compiler_opts="-DSHIFT=%d" % (int(t0) % 7 + 30))
+    compile_s = time.perf_counter() - t0
+    cases = [
+        # (label, n, elem width, snippet sorter, precompiled sorter)
+        ("satradix float32 2^28 via key snippet", 1 << 28, 4,
+         clops.Sorter(ctx, "satradix", np.float32, key_dtype=np.uint32, get_key=FLOAT_KEY),
+         clops.Sorter(ctx, "satradix", np.uint32)),
+        ("abitonic u64 2^24 keyed by high 32 bits", 1 << 24, 8,
+         clops.Sorter(ctx, "abitonic", np.uint64, key_dtype=np.uint32,
+                      get_key="((uint)((x) >> 32))"),
+         clops.Sorter(ctx, "abitonic", np.uint64)),
+        ("gselect float32 2^14", 1 << 14, 4, clops.Sorter(ctx, "gselect", np.float32),
+         clops.Sorter(ctx, "gselect", np.uint32)),
+    ]
+    for label, n, w, snip, pre in cases:
+        kin = clops.Buffer(ctx, n * w)
+        kout = clops.Buffer(ctx, n * w)
+        # Finite floats of both signs for the float sorters: uniform bits with
+        # the lowest exponent bit cleared, so the exponent is never 0xFF.
+        clops.generate(q, kin, n, clops.GEN_U32_MASK if w == 4 else clops.GEN_U64, 0x5A,
+                       0xFF7FFFFF)
+        reps = 3 if n > (1 << 20) else 10
+        ms = time_steps(q, lambda: snip.sort_device(q, kin, kout, n), reps, 1)
+        kern = {nm: round(q.prof(nm)[1] / reps, 4) for nm in q.prof_names()}
+        ms_pre = time_steps(q, lambda: pre.sort_device(q, kin, kout, n), reps, 1)
+        out.append({"config": "snippet " + label, "ms": ms, "Gkeys/s": n / ms / 1e6,
+                    "precompiled_ms": ms_pre, "slowdown": ms / ms_pre,
+                    "kernel_ms_per_sort": kern, "first_compile_s": compile_s})
+        kin.close()
+        kout.close()
+    return out
 
 
 def _best_time(fn, reps=2):

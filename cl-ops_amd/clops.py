@@ -36,6 +36,8 @@ _DTYPE_TO_CLO = {
     np.dtype(np.int16): CLO_SHORT, np.dtype(np.uint16): CLO_USHORT,
     np.dtype(np.int32): CLO_INT, np.dtype(np.uint32): CLO_UINT,
     np.dtype(np.int64): CLO_LONG, np.dtype(np.uint64): CLO_ULONG,
+    np.dtype(np.float16): CLO_HALF, np.dtype(np.float32): CLO_FLOAT,
+    np.dtype(np.float64): CLO_DOUBLE,
 }
 _CLO_TO_DTYPE = {v: k for k, v in _DTYPE_TO_CLO.items()}
 
@@ -118,6 +120,7 @@ def lib():
         "clo_sort_get_element_type": (i32, [vp]),
         "clo_sort_get_element_size": (sz, [vp]),
         "clo_sort_get_key_type": (i32, [vp]),
+        "clo_sort_get_source": (cp, [vp]),
         "clo_sort_get_key_size": (sz, [vp]),
         "clo_sort_get_num_kernels": (u32, [vp, ep]),
         "clo_sort_get_kernel_name": (cp, [vp, u32, ep]),
@@ -544,14 +547,19 @@ def _b(s):
 
 class Sorter:
     def __init__(self, ctx, algo, dtype, options=None, compare=None, get_key=None,
-                 key_dtype=None):
+                 key_dtype=None, compiler_opts=None):
         self.L = lib()
         self.ctx = ctx
         self.dtype = np.dtype(dtype)
         et = ctypes.c_int(clo_type(dtype))
         kt = None if key_dtype is None else ctypes.byref(ctypes.c_int(clo_type(key_dtype)))
         self.ptr = _call(self.L.clo_sort_new, _b(algo), _b(options), ctx.ptr, ctypes.byref(et),
-                         kt, _b(compare), _b(get_key), None)
+                         kt, _b(compare), _b(get_key), _b(compiler_opts))
+
+    def source(self):
+        """HIP source of the run-time compiled kernels (None: precompiled)."""
+        s = self.L.clo_sort_get_source(self.ptr)
+        return None if s is None else s.decode()
 
     def sort_device(self, q, bin_, bout, n, q_comm=None):
         return _call(self.L.clo_sort_with_device_data, self.ptr, q.ptr,

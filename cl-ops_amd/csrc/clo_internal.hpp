@@ -140,6 +140,9 @@ struct clo_sort {
 	uint32_t seg_tile;
 	uint32_t seg_grid;
 	void* seg_cache;
+	// Run-time compiled kernels for user compare / key snippets (NULL when
+	// the ahead-of-time kernels serve the sorter).
+	void* snip;
 };
 
 struct clo_scan {
@@ -162,6 +165,37 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	CCLBuffer* vin, CCLBuffer* kout, CCLBuffer* vout, CCLBuffer* offsets, size_t nseg,
 	size_t numel, uint32_t tile_items, uint32_t grid, void** cache, GError** err);
 void clo_satradix_segsort_cache_free(void* cache);
+// Stable satradix sort of (key word, u32 payload) pairs with the sorter's
+// radix settings; ksize is 1, 2, 4 or 8.  keys_out/vals_out NULL: in place.
+bool clo_satradix_sort_pairs(CloSort* sorter, CCLQueue* q, size_t ksize, void* keys,
+	uint32_t* vals, void* keys_out, uint32_t* vals_out, size_t n, GError** err);
+
+namespace clo {
+// hipRTC (rtc.cpp): the functions `names` of `src` compiled for the device of
+// ctx, one compilation per distinct source and options in the process.  A
+// source that does not compile fails with CLO_ERROR_ARGS naming `what` and
+// quoting the compiler log.
+bool rtc_functions(CCLContext* ctx, const std::string& src, const char* compiler_opts,
+	const char* what, const char* const* names, int nnames, hipFunction_t* out, GError** err);
+// Launches a run-time compiled kernel over `items` work-items in blocks of
+// 256 on q, profiled under `name`.
+bool rtc_launch(CCLQueue* q, const char* name, hipFunction_t f, size_t items, void** args,
+	GError** err);
+
+// Sorters with user snippets (snippet.cpp).  needed(): the snippets, types
+// or key getter are beyond the ahead-of-time kernels.  init(): compiles the
+// run-time kernels for the sorter's device.  sort(): the algorithm of
+// s->impl.name on them; vin/vout are an optional u32 payload (stable
+// algorithms only).
+bool snippet_sorter_needed(const CloSort* s, const char* compare, const char* get_key);
+bool snippet_sorter_init(CloSort* s, const char* compare, const char* get_key,
+	const char* compiler_opts, GError** err);
+void snippet_sorter_free(CloSort* s);
+const char* snippet_sorter_source(const CloSort* s);
+CCLEvent* snippet_sort(CloSort* s, CCLQueue* q, CCLBuffer* kin, CCLBuffer* vin, CCLBuffer* kout,
+	CCLBuffer* vout, size_t numel, GError** err);
+}  // namespace clo
+
 // Plain blelloch scan with 8, 16 (the default) or 32 items per thread (8 and
 // 32 for unsigned 32/64-bit elements only).
 CCLEvent* clo_blelloch_scan_items(CloScan* scanner, CCLQueue* cq, CCLBuffer* in,

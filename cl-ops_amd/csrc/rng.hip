@@ -137,6 +137,13 @@ const struct clo_rng_info clo_rng_infos[] = {
 #endif
 
 
+// The device source of generator `gen` that clo_rng_get_source() returns:
+// its section, the draw API, and the API bound to it.
+static std::string gen_source(int gen) {
+	return std::string(kInfos[gen].src) + kSrcApi + "CLO_RNG_DEFINE_API(clo_rng_" +
+		kInfos[gen].name + ")\nusing namespace clo_rng_" + kInfos[gen].name + ";\n";
+}
+
 static bool launch_init(int gen, void* seeds, u64 count, u64 main_seed, u32 hash, CCLQueue* cq,
 	GError** err) {
 	u64 g = (count + 255) / 256;
@@ -177,6 +184,23 @@ static bool launch_init(int gen, void* seeds, u64 count, u64 main_seed, u32 hash
 	return L.done(err);
 }
 
+// DEV_GID initialisation for a seed hash other than the predefined ones:
+// clo_rng_init.cl's kernel with `#define CLO_RNG_HASH(x) <hash>` prepended
+// (clo_rng.c:101-109), compiled at run time after the generator's device
+// source.  KNUTH and XS1 stay available to the snippet.
+static const char kInitRtc[] =
+	"#define KNUTH(x) x = ((x * 2654435761) % 0x100000000)\n"
+	"#define XS1(x) x = ((x >> 16) ^ x) * 0x45d9f3b; x = ((x >> 16) ^ x) * 0x45d9f3b; "
+	"x = ((x >> 16) ^ x);\n"
+	"extern \"C\" __global__ __launch_bounds__(256) void clo_rng_init(clo_statetype* seeds,\n"
+	"	ulong count, ulong main_seed) {\n"
+	"	const ulong gid = (ulong)blockIdx.x * 256 + threadIdx.x;\n"
+	"	if (gid >= count) return;\n"
+	"	ulong seed = gid + main_seed;\n"
+	"	CLO_RNG_HASH(seed);\n"
+	"	seeds[gid] = clo_ulong2statetype(seed);\n"
+	"}\n";
+
 static std::string strip(const char* s) {
 	std::string r;
 	for (; s && *s; ++s)
@@ -207,14 +231,19 @@ CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type, void* seeds, siz
 	case CLO_RNG_SEED_DEV_GID: {
 		// clo_rng.c:85-153.
 		std::string h = strip(hash);
-		u32 hid;
+		// The predefined hashes run the precompiled kernel; any other
+		// snippet is compiled with the generator's source.
+		int hid = -1;
 		if (h.empty() || h == "x" || h == "x=x") hid = kHashNone;
 		else if (h == "KNUTH(x)") hid = kHashKnuth;
 		else if (h == "XS1(x)") hid = kHashXs1;
-		else {
-			set_error(err, kQuarkClo, CLO_ERROR_ARGS,
-				"Unsupported seed hash '%s' (use KNUTH(x), XS1(x) or none).", hash);
-			return nullptr;
+		hipFunction_t init_fn = nullptr;
+		if (hid < 0) {
+			const std::string src = std::string("#define CLO_RNG_HASH(x) ") + hash + "\n" +
+				gen_source(gen) + kInitRtc;
+			const char* name = "clo_rng_init";
+			if (!rtc_functions(ctx, src, nullptr, "seed hash snippet", &name, 1, &init_fn, err))
+				return nullptr;
 		}
 		if (seeds_count == 0) {
 			set_error(err, kQuarkClo, CLO_ERROR_ARGS, "seeds_count must be positive");
@@ -224,8 +253,19 @@ CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type, void* seeds, siz
 			own = ccl_queue_new(ctx, nullptr, 0, err);
 			if (!own) return nullptr;
 		}
+		CCLQueue* iq = cq ? cq : own;
 		buf = ccl_buffer_new(ctx, CL_MEM_READ_WRITE, seeds_count * ss, nullptr, err);
-		if (!buf || !launch_init(gen, buf->ptr, seeds_count, main_seed, hid, cq ? cq : own, err)) {
+		bool launched = false;
+		if (buf && init_fn) {
+			void* p = buf->ptr;
+			u64 cnt = seeds_count, ms = main_seed;
+			void* args[] = {&p, &cnt, &ms};
+			launched = rtc_launch(iq, "clo_rng_init", init_fn, seeds_count, args, err);
+		} else if (buf) {
+			launched = launch_init(gen, buf->ptr, seeds_count, main_seed,
+				static_cast<u32>(hid), iq, err);
+		}
+		if (!launched) {
 			if (buf) ccl_buffer_destroy(buf);
 			buf = nullptr;
 		}
@@ -289,8 +329,7 @@ CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type, void* seeds, siz
 	r->gen = gen;
 	r->seeds = buf;
 	r->count = seeds_count;
-	r->src = std::string(kInfos[gen].src) + kSrcApi + "CLO_RNG_DEFINE_API(clo_rng_" +
-		kInfos[gen].name + ")\nusing namespace clo_rng_" + kInfos[gen].name + ";\n";
+	r->src = gen_source(gen);
 	return r;
 }
 

@@ -947,6 +947,20 @@ CCLEvent* clo_satradix_sort_kv(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_
 		err);
 }
 
+bool clo_satradix_sort_pairs(CloSort* sorter, CCLQueue* q, size_t ksize, void* keys,
+	uint32_t* vals, void* keys_out, uint32_t* vals_out, size_t n, GError** err) {
+	if (n <= 1) {
+		if (n == 1 && keys_out &&
+			(hip_failed(hipMemcpyAsync(keys_out, keys, ksize, hipMemcpyDeviceToDevice,
+				q->stream), err, "hipMemcpyAsync") ||
+			 hip_failed(hipMemcpyAsync(vals_out, vals, 4, hipMemcpyDeviceToDevice, q->stream),
+				err, "hipMemcpyAsync")))
+			return false;
+		return true;
+	}
+	return dispatch<true>(sdata(sorter), q, ksize, keys, vals, keys_out, vals_out, n, err);
+}
+
 cl_uint clo_satradix_last_passes(CloSort* sorter, GError** err) {
 	SatradixData* sd = sdata(sorter);
 	if (!sd || !sd->meta.ptr || !sd->last_queue) return 0;

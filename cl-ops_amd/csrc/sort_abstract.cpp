@@ -12,7 +12,6 @@ extern "C" {
 CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 	CloType* elem_type, CloType* key_type, const char* compare, const char* get_key,
 	const char* compiler_opts, GError** err) {
-	(void)compiler_opts;  // kernels are compiled ahead of time
 	if (err && *err) return nullptr;
 	if (!ctx || !elem_type) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "clo_sort_new: NULL context or element type");
@@ -31,22 +30,8 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 	}
 	CloType et = *elem_type;
 	CloType kt = key_type ? *key_type : et;
-	if (!type_is_integer(et)) {
-		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
-			"Element type '%s' is not supported: sorts take integer types (char..ulong).",
-			clo_type_get_name(et) ? clo_type_get_name(et) : "?");
-		return nullptr;
-	}
-	if (kt != et || !get_key_is_identity(get_key)) {
-		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
-			"Custom keys (key type or get_key snippet) are not supported: "
-			"the key is the element.");
-		return nullptr;
-	}
-	int ck = compare_kind(compare);
-	if (ck < 0) {
-		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
-			"Unsupported compare snippet '%s' (use ((a) > (b)) or ((a) < (b))).", compare);
+	if (!clo_type_get_name(et) || !clo_type_get_name(kt)) {
+		set_error(err, kQuarkClo, CLO_ERROR_UNKNOWN_TYPE, "Unknown element or key type.");
 		return nullptr;
 	}
 	CloSort* s = new clo_sort();
@@ -55,12 +40,13 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 	ccl_context_ref(ctx);
 	s->elem_type = et;
 	s->key_type = kt;
-	s->descending = ck;
+	s->descending = 0;
 	s->data = nullptr;
 	s->last_queue = nullptr;
 	s->seg_tile = 0;
 	s->seg_grid = 0;
 	s->seg_cache = nullptr;
+	s->snip = nullptr;
 	GError* e = nullptr;
 	const char* src = s->impl.init(s, options, &e);
 	if (!src || e) {
@@ -73,6 +59,18 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 		clo_sort_destroy(s);
 		return nullptr;
 	}
+	// clo_sort_abstract.c:140-186: the sorter's kernels with the element
+	// type, key type, compare and key-getter macros.  Integer elements sorted
+	// by themselves, ascending or descending, run the ahead-of-time kernels;
+	// everything else is compiled now.
+	if (snippet_sorter_needed(s, compare, get_key)) {
+		if (!snippet_sorter_init(s, compare, get_key, compiler_opts, err)) {
+			clo_sort_destroy(s);
+			return nullptr;
+		}
+	} else if (strcmp(s->impl.name, "satradix") != 0) {
+		s->descending = compare_kind(compare);
+	}
 	return s;
 }
 
@@ -80,6 +78,7 @@ void clo_sort_destroy(CloSort* sorter) {
 	if (!sorter) return;
 	if (sorter->impl.finalize) sorter->impl.finalize(sorter);
 	if (sorter->seg_cache) clo_satradix_segsort_cache_free(sorter->seg_cache);
+	if (sorter->snip) snippet_sorter_free(sorter);
 	ccl_context_destroy(sorter->ctx);
 	delete sorter;
 }
@@ -109,6 +108,8 @@ CCLEvent* clo_sort_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue
 	if (!check_device_args(sorter, cq_exec, data_in, data_out, numel,
 			sorter ? clo_type_sizeof(sorter->elem_type) : 0, err))
 		return nullptr;
+	if (sorter->snip)
+		return snippet_sort(sorter, cq_exec, data_in, nullptr, data_out, nullptr, numel, err);
 	return sorter->impl.sort_with_device_data(sorter, cq_exec, cq_comm, data_in, data_out,
 		numel, lws_max, err);
 }
@@ -175,6 +176,9 @@ CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQu
 			"must both be NULL (in place) or both set");
 		return nullptr;
 	}
+	if (sorter->snip && (strcmp(sorter->impl.name, "satradix") == 0 ||
+			strcmp(sorter->impl.name, "gselect") == 0))
+		return snippet_sort(sorter, cq_exec, keys_in, vals_in, keys_out, vals_out, numel, err);
 	if (strcmp(sorter->impl.name, "satradix") == 0)
 		return clo_satradix_sort_kv(sorter, cq_exec, cq_comm, keys_in, vals_in, keys_out,
 			vals_out, numel, err);
@@ -239,6 +243,11 @@ CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec
 			"segmented sorting is provided by satradix sorters, not '%s'", sorter->impl.name);
 		return nullptr;
 	}
+	if (sorter->snip) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"segmented sorting runs the built-in integer keys only, not key snippets");
+		return nullptr;
+	}
 	if (!offsets || offsets->size < nseg * 4 || nseg > 0xFFFFFFFFull ||
 		(vals_in && (vals_in->size < numel * 4 || (vals_out && vals_out->size < numel * 4))) ||
 		(vals_in && ((keys_out == nullptr) != (vals_out == nullptr))) ||
@@ -268,6 +277,9 @@ CCLProgram* clo_sort_get_program(CloSort* sorter) {
 CloType clo_sort_get_element_type(CloSort* sorter) { return sorter->elem_type; }
 size_t clo_sort_get_element_size(CloSort* sorter) { return clo_type_sizeof(sorter->elem_type); }
 CloType clo_sort_get_key_type(CloSort* sorter) { return sorter->key_type; }
+const char* clo_sort_get_source(CloSort* sorter) {
+	return sorter ? snippet_sorter_source(sorter) : nullptr;
+}
 size_t clo_sort_get_key_size(CloSort* sorter) { return clo_type_sizeof(sorter->key_type); }
 void* clo_sort_get_data(CloSort* sorter) { return sorter ? sorter->data : nullptr; }
 void clo_sort_set_data(CloSort* sorter, void* data) {

@@ -40,9 +40,14 @@ typedef enum clo_rng_seed_type {
 } CloRngSeedType;
 
 /* clo_rng.c:262-405.
- *  DEV_GID : state[gid] = ulong2statetype(HASH(gid + main_seed)); `hash` is
- *            NULL/""/"x" (none), "KNUTH(x)" or "XS1(x)" (clo_rng_init.cl);
- *            other snippets fail with CLO_ERROR_ARGS.
+ *  DEV_GID : seed = gid + main_seed; CLO_RNG_HASH(seed);
+ *            state[gid] = clo_ulong2statetype(seed) (clo_rng_init.cl:47-60).
+ *            `hash` is the body of the CLO_RNG_HASH(x) macro, a statement
+ *            on the ulong x: NULL/""/"x" (none), "KNUTH(x)" and "XS1(x)"
+ *            run precompiled kernels; any other snippet (which may use KNUTH
+ *            and XS1) is compiled with hipRTC, after the generator's device
+ *            source, when the generator is created.  A snippet that does not
+ *            compile fails with CLO_ERROR_ARGS and the compiler log.
  *  HOST_MT : word i of the state buffer = i-th output of MT19937 seeded with
  *            (uint32) main_seed (GLib g_rand_new_with_seed/g_rand_int).
  *  EXT_DEV : `seeds` is a CCLBuffer* of seeds_count states (referenced).

@@ -33,11 +33,26 @@ typedef struct clo_sort_impl_def {
 		size_t numel, GError** err);
 } CloSortImplDef;
 
-/* clo_sort_abstract.c:91-207.  `compare` and `get_key` accept NULL (the
- * defaults "((a) > (b))" ascending and "(x)"); "((a) < (b))" selects
- * descending order for the comparison sorts.  Other snippets, and a key type
- * different from the element type, fail with CLO_ERROR_ARGS: kernels are
- * compiled ahead of time.  `compiler_opts` is accepted and ignored. */
+/* clo_sort_abstract.c:91-207.  Arguments as in CL_Ops: element type, key
+ * type (NULL: the element type), the CLO_SORT_COMPARE(a, b) snippet (NULL:
+ * "((a) > (b))", ascending), the CLO_SORT_KEY_GET(x) snippet (NULL: "(x)")
+ * and compiler options.  Integer elements sorted by themselves, ascending or
+ * descending ("((a) < (b))"), run precompiled kernels.  Any other snippet,
+ * key type or element type (half, float, double) is compiled with hipRTC
+ * when the sorter is created, from the HIP source that clo_sort_get_source()
+ * returns, with the four macros defined as CL_Ops defines them:
+ *   - sbitonic / abitonic run CL_Ops' bitonic network exactly (ties come out
+ *     in its order); sizes that are not powers of two sort as if padded with
+ *     elements that order after all others;
+ *   - gselect ranks with CL_Ops' formula (stable for a strict order);
+ *   - satradix orders stably by the bits of KEY_GET(x), which must be an
+ *     integer type; it ignores `compare`, as CL_Ops' radix kernels do.
+ * Snippets may use OpenCL C type names (uint, ulong, half, ...) and the
+ * as_uint / as_int / as_ulong / as_long / as_ushort / as_short / as_float /
+ * as_double reinterpretations.  compiler_opts pass to hipRTC ("-D..."),
+ * except OpenCL-only "-cl-*" options, which are dropped.  A snippet that does
+ * not compile fails with CLO_ERROR_ARGS and the compiler log.  Segmented
+ * sorting takes precompiled kernels only. */
 CloSort* clo_sort_new(const char* type, const char* options,
 	CCLContext* ctx, CloType* elem_type, CloType* key_type,
 	const char* compare, const char* get_key, const char* compiler_opts,
@@ -98,6 +113,11 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
 	CCLBuffer* vals_out, CCLBuffer* offsets, size_t nseg, size_t numel, GError** err);
+
+/* HIP source of the sorter's run-time compiled kernels (the macros of its
+ * snippets followed by the kernels), NULL when the sorter runs the
+ * precompiled ones.  Owned by the sorter. */
+const char* clo_sort_get_source(CloSort* sorter);
 
 /* Number of digit passes the last satradix call executed (passes whose digit
  * was the same for every key are skipped).  Blocks on the queue of that

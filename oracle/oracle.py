@@ -121,6 +121,81 @@ def gselect(keys, vals=None, desc=False):
     return ko if v is None else (ko, vo)
 
 
+# ---- sorts with user key / compare functions ------------------------------
+# The Python callables stand for the CLO_SORT_KEY_GET(x) and
+# CLO_SORT_COMPARE(a, b) snippets; they take and return numpy arrays.
+
+def _swap_step(d, pad, i1, i2, desc, key, compare):
+    a, b = d[i1], d[i2]
+    swap = np.asarray(compare(key(a), key(b)), dtype=bool) ^ desc
+    if pad is not None:
+        pa, pb = pad[i1], pad[i2]
+        either = pa | pb
+        swap = np.where(either, np.where(desc, pb & ~pa, pa & ~pb), swap)
+        pad[i1[swap]], pad[i2[swap]] = pb[swap], pa[swap]
+    d[i1[swap]], d[i2[swap]] = b[swap], a[swap]
+
+
+def bitonic_network(data, key=lambda x: x, compare=np.greater, padded=False):
+    """CL_Ops' bitonic network with a key getter and a compare: the kernel of
+    sbitonic.cl:38-69 (pair i1 = gid + (gid / 2^(step-1)) * 2^(step-1),
+    i2 = i1 + 2^(step-1); desc = bit (stage-1) of gid; swap iff
+    COMPARE(KEY_GET(d1), KEY_GET(d2)) ^ desc) driven by the stage/step loops
+    of sbitonic.c:102-118.  abitonic.cl's ABIT_CMPXCH applies the same
+    comparators in the same dependency order, so this is its result too.
+
+    The reference needs a power-of-two size.  padded=True instead runs the
+    network on the next power of two, with the extra slots flagged as ordering
+    after every element, and returns the first len(data) slots (the library's
+    rule for other sizes)."""
+    d = np.array(data, copy=True)
+    n = d.size
+    if n < 2:
+        return d
+    npow2 = 1 << (n - 1).bit_length()
+    if npow2 != n and not padded:
+        raise ValueError("the reference network needs a power-of-two size")
+    pad = None
+    if npow2 != n:
+        d = np.concatenate([d, np.zeros(npow2 - n, dtype=d.dtype)])
+        pad = np.arange(npow2) >= n
+    gid = np.arange(npow2 // 2, dtype=np.int64)
+    for stage in range(1, npow2.bit_length()):
+        desc = ((gid >> (stage - 1)) & 1).astype(bool)
+        for step in range(stage, 0, -1):
+            ps = 1 << (step - 1)
+            i1 = gid + (gid // ps) * ps
+            _swap_step(d, pad, i1, i1 + ps, desc, key, compare)
+    return d[:n]
+
+
+def gselect_rank(data, vals=None, key=lambda x: x, compare=np.greater):
+    """gselect.cl:38-57 with a key getter and a compare:
+    pos(g) = #{i : COMPARE(key_g, key_i) || (key_i == key_g && i < g)},
+    out[pos(g)] = data[g].  Ranks that collide or fall outside 0..n-1 (a
+    compare that is not a strict order) raise instead of racing."""
+    d = np.asarray(data)
+    n = d.size
+    k = key(d)
+    pos = np.empty(n, dtype=np.int64)
+    idx = np.arange(n)
+    for g0 in range(0, n, 512):
+        g1 = min(n, g0 + 512)
+        kg = k[g0:g1, None]
+        c = np.asarray(compare(kg, k[None, :]), dtype=bool) | (
+            (k[None, :] == kg) & (idx[None, :] < idx[g0:g1, None]))
+        pos[g0:g1] = c.sum(axis=1)
+    if n and (pos.max() >= n or np.unique(pos).size != n):
+        raise ValueError("ranks are not a permutation")
+    out = np.empty_like(d)
+    out[pos] = d
+    if vals is None:
+        return out
+    vo = np.empty_like(np.asarray(vals))
+    vo[pos] = vals
+    return out, vo
+
+
 def scan(values, sum_dtype):
     a = np.ascontiguousarray(values)
     out = np.empty(a.size, dtype=sum_dtype)

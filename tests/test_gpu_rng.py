@@ -79,13 +79,59 @@ def test_config2_fills_prefix(clops, oracle, ctx, queue):
         assert np.array_equal(got, oracle.rng_fill(gen, st, draws, threads=8))
 
 
+M64 = (1 << 64) - 1
+
+
+def _xs1(s):
+    """clo_rng_init.cl's XS1 on a 64-bit seed."""
+    s = (((s >> 16) ^ s) * 0x45d9f3b) & M64
+    s = (((s >> 16) ^ s) * 0x45d9f3b) & M64
+    return (s >> 16) ^ s
+
+
+# Seed hash snippets beyond the predefined ones, compiled at run time, with
+# their meaning on a Python int.  The reference applies the snippet as the
+# statement CLO_RNG_HASH(seed); (clo_rng_init.cl:54-55), so an expression
+# without an assignment leaves the seed unchanged.
+CUSTOM_HASHES = [
+    ("x ^= x >> 33; x *= 0xff51afd7ed558ccdUL",
+     lambda s: ((s ^ (s >> 33)) * 0xff51afd7ed558ccd) & M64),
+    ("KNUTH(x); x += 7", lambda s: ((s * 2654435761) % (1 << 32)) + 7),
+    ("XS1(x) x = ~x", lambda s: ~_xs1(s) & M64),
+    ("(x * 3 + 1)", lambda s: s),
+]
+
+
+@pytest.mark.parametrize("gen", GENS)
+@pytest.mark.parametrize("which", range(len(CUSTOM_HASHES)))
+def test_dev_gid_custom_hash(clops, oracle, ctx, queue, gen, which):
+    """DEV_GID seeds with a hash snippet other than KNUTH/XS1/none: the
+    snippet is compiled with the generator's source and applied to
+    gid + main_seed, then converted as the predefined ones are."""
+    text, fn = CUSTOM_HASHES[which]
+    count, main_seed = 2000, (1 << 40) + 99
+    r = clops.Rng(ctx, gen, clops.SEED_DEV_GID, count, main_seed=main_seed, hash=text, q=queue)
+    queue.finish()
+    want = np.concatenate([oracle.rng_from_seed(gen, fn(g + main_seed)) for g in range(count)])
+    assert np.array_equal(r.read_states(queue), want)
+
+
+def test_custom_hash_equal_to_predefined(clops, oracle, ctx, queue):
+    """A snippet spelling out KNUTH gives the predefined KNUTH seeds."""
+    r = clops.Rng(ctx, "xorshift128", clops.SEED_DEV_GID, 5000, main_seed=3,
+                  hash="x = ((x * 2654435761UL) % 0x100000000UL)", q=queue)
+    queue.finish()
+    assert np.array_equal(r.read_states(queue),
+                          oracle.rng_init_dev_gid("xorshift128", 5000, 3, 1))
+
+
 def test_errors(clops, ctx):
     with pytest.raises(clops.ClopsError) as e:
         clops.Rng(ctx, "mersenne", clops.SEED_DEV_GID, 10)
     assert e.value.code == clops.CLO_ERROR_IMPL_NOT_FOUND
     with pytest.raises(clops.ClopsError) as e:
-        clops.Rng(ctx, "lcg", clops.SEED_DEV_GID, 10, hash="(x * 3 + 1)")
-    assert e.value.code == clops.CLO_ERROR_ARGS
+        clops.Rng(ctx, "lcg", clops.SEED_DEV_GID, 10, hash="x = y")
+    assert e.value.code == clops.CLO_ERROR_ARGS and "did not compile" in str(e.value)
 
 
 def test_source_text(clops, ctx):
