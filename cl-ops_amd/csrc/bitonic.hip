@@ -29,10 +29,19 @@ namespace {
 
 constexpr int kBitBlock = 256;
 
+// Tiles of the register-tail kernel: 2^11 .. 2^13 keys, LDS sized to the
+// tile so that small tiles keep many blocks resident per CU.
+constexpr u32 kTileLogMin = 11, kTileLogMax = 13;
+// Default tile per key width (measured on MI355X at n = 2^16..2^24 with 16
+// keys per thread: 2^13 best up to 4-byte keys, 2^12 for 8-byte keys).
 template <typename T>
 struct TileMax {
-	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : 13;  // 32 or 64 KiB of LDS
+	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : 13;
 };
+// The plain LDS kernels (one synchronised step at a time) take tiles of up
+// to 2^11 keys.
+constexpr u32 kPlainTileLog = 11;
+constexpr u32 kPlainTileMax = 1u << kPlainTileLog;
 
 __device__ __forceinline__ u64 insert_zero_bits(u64 v, u32 pos, u32 q) {
 	const u64 low = v & ((1ull << pos) - 1ull);
@@ -42,28 +51,30 @@ __device__ __forceinline__ u64 insert_zero_bits(u64 v, u32 pos, u32 q) {
 // Q consecutive steps of stage s starting at step t (t == s: the first is the
 // flip step), each thread holding the 2^Q positions one comparator group
 // touches.  Q = 1 is one network step (the sbitonic kernel).
-template <typename T, int Q>
-__global__ __launch_bounds__(kBitBlock) void bitonic_group(T* __restrict__ data, u64 n, u64 npo2,
+// Indices are 32-bit: sizes are capped at 2^31 keys.  FULL (n a power of
+// two): no position is virtual, so no bounds checks.
+template <typename T, int Q, bool FULL>
+__global__ __launch_bounds__(kBitBlock) void bitonic_group(T* __restrict__ data, u32 n, u32 npo2,
 	u32 s, u32 t, T xmask) {
 	constexpr int V = 1 << Q;
-	const u64 tid = static_cast<u64>(blockIdx.x) * kBitBlock + threadIdx.x;
+	const u32 tid = blockIdx.x * kBitBlock + threadIdx.x;
 	if (tid >= (npo2 >> Q)) return;
 	const u32 pos = t - Q;  // lowest free bit
-	const u64 x = insert_zero_bits(tid, pos, Q);
-	u64 masks[Q];
+	const u32 x = ((tid >> pos) << t) | (tid & ((1u << pos) - 1u));
+	u32 masks[Q];
 #pragma unroll
-	for (int r = 0; r < Q; ++r) masks[r] = 1ull << (t - 1 - r);
-	if (t == s) masks[0] = (1ull << s) - 1ull;
-	u64 idx[V];
+	for (int r = 0; r < Q; ++r) masks[r] = 1u << (t - 1 - r);
+	if (t == s) masks[0] = (1u << s) - 1u;
+	u32 idx[V];
 	T val[V];
 #pragma unroll
 	for (int j = 0; j < V; ++j) {
-		u64 e = x;
+		u32 e = x;
 #pragma unroll
 		for (int r = 0; r < Q; ++r)
 			if (j & (1 << r)) e ^= masks[r];
 		idx[j] = e;
-		val[j] = e < n ? static_cast<T>(data[e] ^ xmask) : T(0);
+		val[j] = (FULL || e < n) ? static_cast<T>(data[e] ^ xmask) : T(0);
 	}
 #pragma unroll
 	for (int r = 0; r < Q; ++r) {
@@ -72,17 +83,16 @@ __global__ __launch_bounds__(kBitBlock) void bitonic_group(T* __restrict__ data,
 			if (j & (1 << r)) continue;
 			const int k = j | (1 << r);
 			const bool jlo = idx[j] < idx[k];
-			const u64 hi = jlo ? idx[k] : idx[j];
-			if (hi >= n) continue;
-			T a = val[j], b = val[k];
-			T mn = a < b ? a : b, mx = a < b ? b : a;
+			if (!FULL && (jlo ? idx[k] : idx[j]) >= n) continue;
+			const T a = val[j], b = val[k];
+			const T mn = a < b ? a : b, mx = a < b ? b : a;
 			val[j] = jlo ? mn : mx;
 			val[k] = jlo ? mx : mn;
 		}
 	}
 #pragma unroll
 	for (int j = 0; j < V; ++j)
-		if (idx[j] < n) data[idx[j]] = static_cast<T>(val[j] ^ xmask);
+		if (FULL || idx[j] < n) data[idx[j]] = static_cast<T>(val[j] ^ xmask);
 }
 
 // One comparator of the step (s, t) inside an LDS tile; tile position p of the
@@ -101,17 +111,20 @@ __device__ __forceinline__ void pair_pos(u32 i, u32 s, u32 t, u32* lo, u32* hi) 
 	}
 }
 
+// Steps t = min(s, t_top_limit)..1 of stages s_first..s_last on an LDS
+// tile whose first `lim` positions are real (comparators reaching past them
+// do nothing: those positions are virtual +inf).
 template <typename T>
-__device__ __forceinline__ void tile_steps(T* s_t, u64 tile_base, u64 n, u32 tile_n, u32 s_first,
-	u32 s_last, u32 t_top_limit) {
+__device__ __forceinline__ void tile_steps(T* s_t, u32 lim, u32 tile_n, u32 s_first, u32 s_last,
+	u32 t_top_limit) {
 	for (u32 s = s_first; s <= s_last; ++s) {
 		const u32 t0 = s < t_top_limit ? s : t_top_limit;
 		for (u32 t = t0; t >= 1; --t) {
 			for (u32 i = threadIdx.x; i < tile_n / 2; i += kBitBlock) {
 				u32 lo, hi;
 				pair_pos(i, s, t, &lo, &hi);
-				if (tile_base + hi < n) {
-					T a = s_t[lo], b = s_t[hi];
+				if (hi < lim) {
+					const T a = s_t[lo], b = s_t[hi];
 					if (b < a) {
 						s_t[lo] = b;
 						s_t[hi] = a;
@@ -123,51 +136,30 @@ __device__ __forceinline__ void tile_steps(T* s_t, u64 tile_base, u64 n, u32 til
 	}
 }
 
-// Stages 1..k of every tile of 2^k positions, in LDS.
+// Stages s_first..s_last inside each tile of 2^k positions (k <= 11), one
+// synchronised LDS step at a time: the tile sort (stages 1..k) or the local
+// half-cleaners of one stage s > k.
 template <typename T>
-__global__ __launch_bounds__(kBitBlock) void bitonic_tilesort(T* __restrict__ data, u64 n, u32 k,
-	T xmask) {
-	__shared__ T s_t[1u << TileMax<T>::kLog];
-	const u32 tile_n = 1u << k;
-	const u64 base = static_cast<u64>(blockIdx.x) << k;
-	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
-		const u64 g = base + i;
-		s_t[i] = g < n ? static_cast<T>(data[g] ^ xmask) : T(0);
-	}
-	__syncthreads();
-	tile_steps(s_t, base, n, tile_n, 1, k, k);
-	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
-		const u64 g = base + i;
-		if (g < n) data[g] = static_cast<T>(s_t[i] ^ xmask);
-	}
-}
-
-// Steps k..1 of stage s > k (half-cleaners inside each tile), in LDS.
-template <typename T>
-__global__ __launch_bounds__(kBitBlock) void bitonic_local(T* __restrict__ data, u64 n, u32 k,
-	u32 s, T xmask) {
-	__shared__ T s_t[1u << TileMax<T>::kLog];
+__global__ __launch_bounds__(kBitBlock) void bitonic_plain(T* __restrict__ data, u64 n, u32 k,
+	u32 s_first, u32 s_last, T xmask) {
+	__shared__ T s_t[kPlainTileMax];
 	const u32 tile_n = 1u << k;
 	const u64 base = static_cast<u64>(blockIdx.x) << k;
 	if (base >= n) return;
-	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
-		const u64 g = base + i;
-		s_t[i] = g < n ? static_cast<T>(data[g] ^ xmask) : T(0);
-	}
+	const u32 lim = n - base < tile_n ? static_cast<u32>(n - base) : tile_n;
+	T* const tile = data + base;
+	for (u32 i = threadIdx.x; i < lim; i += kBitBlock) s_t[i] = static_cast<T>(tile[i] ^ xmask);
 	__syncthreads();
-	tile_steps(s_t, base, n, tile_n, s, s, k);
-	for (u32 i = threadIdx.x; i < tile_n; i += kBitBlock) {
-		const u64 g = base + i;
-		if (g < n) data[g] = static_cast<T>(s_t[i] ^ xmask);
-	}
+	tile_steps(s_t, lim, tile_n, s_first, s_last, k);
+	for (u32 i = threadIdx.x; i < lim; i += kBitBlock) tile[i] = static_cast<T>(s_t[i] ^ xmask);
 }
 
 // ---- tile kernels with register tails --------------------------------------
-// Each thread owns kRegN = 32 consecutive tile positions in registers.  Steps
-// whose comparator distance is below 32 (t <= 5, and the flip steps of stages
-// s <= 5) run in registers; the other steps go through LDS, addressed with a
-// 1-in-32 padding so that a thread's 32 consecutive keys sit in distinct banks.
-constexpr int kRegN = 32;
+// Each thread owns R (16 or 32) consecutive tile positions in registers.
+// Steps whose comparator distance is below R (t <= log2 R, and the flip steps
+// of stages s <= log2 R) run in registers; the other steps go through LDS,
+// addressed with a 1-in-32 padding so that a thread's consecutive keys sit in
+// distinct banks.
 
 __host__ __device__ constexpr u32 bpad(u32 i) { return i + (i >> 5); }
 
@@ -181,50 +173,37 @@ __device__ __forceinline__ void cmpswap(T& a, T& b) {
 
 // Half-cleaner of distance H on the thread's keys; `valid` keys are real
 // (positions >= valid are virtual +inf: a comparator reaching one does nothing).
-template <typename T, int H>
-__device__ __forceinline__ void reg_half(T (&x)[kRegN], u32 valid) {
+template <typename T, int R, int H>
+__device__ __forceinline__ void reg_half(T (&x)[R], u32 valid) {
 #pragma unroll
-	for (int j = 0; j < kRegN; ++j)
+	for (int j = 0; j < R; ++j)
 		if (!(j & H) && static_cast<u32>(j | H) < valid) cmpswap(x[j], x[j | H]);
 }
 
 // Flip step of blocks of B positions: j pairs with j ^ (B - 1).
-template <typename T, int B>
-__device__ __forceinline__ void reg_flip(T (&x)[kRegN], u32 valid) {
+template <typename T, int R, int B>
+__device__ __forceinline__ void reg_flip(T (&x)[R], u32 valid) {
 #pragma unroll
-	for (int j = 0; j < kRegN; ++j)
+	for (int j = 0; j < R; ++j)
 		if (!(j & (B / 2)) && static_cast<u32>(j ^ (B - 1)) < valid) cmpswap(x[j], x[j ^ (B - 1)]);
 }
 
-// Steps t = t_hi..1 of stage s (the step t == s is the flip).
-template <typename T>
-__device__ __forceinline__ void reg_tail(T (&x)[kRegN], u32 s, u32 t_hi, u32 valid) {
-	if (t_hi >= 5) {
-		if (s == 5) reg_flip<T, 32>(x, valid);
-		else reg_half<T, 16>(x, valid);
-	}
-	if (t_hi >= 4) {
-		if (s == 4) reg_flip<T, 16>(x, valid);
-		else reg_half<T, 8>(x, valid);
-	}
-	if (t_hi >= 3) {
-		if (s == 3) reg_flip<T, 8>(x, valid);
-		else reg_half<T, 4>(x, valid);
-	}
-	if (t_hi >= 2) {
-		if (s == 2) reg_flip<T, 4>(x, valid);
-		else reg_half<T, 2>(x, valid);
-	}
-	if (t_hi >= 1) {
-		if (s == 1) reg_flip<T, 2>(x, valid);
-		else reg_half<T, 1>(x, valid);
+// Steps t = min(t_hi, TT)..1 of stage s (the step t == s is the flip).
+template <typename T, int R, int TT>
+__device__ __forceinline__ void reg_tail(T (&x)[R], u32 s, u32 t_hi, u32 valid) {
+	if constexpr (TT >= 1) {
+		if (t_hi >= TT) {
+			if (s == TT) reg_flip<T, R, (1 << TT)>(x, valid);
+			else reg_half<T, R, (1 << (TT - 1))>(x, valid);
+		}
+		reg_tail<T, R, TT - 1>(x, s, t_hi, valid);
 	}
 }
 
 // Q consecutive steps t, t-1, .., t-Q+1 of stage s inside the LDS tile (each
 // thread takes whole comparator groups of 2^Q positions), the same grouping as
 // bitonic_group uses for global steps.
-template <typename T, int Q>
+template <typename T, int Q, int NT>
 __device__ __forceinline__ void lds_group(T* s_t, u64 base, u64 n, u32 tile_n, u32 s, u32 t) {
 	constexpr int V = 1 << Q;
 	const u32 pos = t - Q;
@@ -232,7 +211,7 @@ __device__ __forceinline__ void lds_group(T* s_t, u64 base, u64 n, u32 tile_n, u
 #pragma unroll
 	for (int r = 0; r < Q; ++r) masks[r] = 1u << (t - 1 - r);
 	if (t == s) masks[0] = (1u << s) - 1u;
-	for (u32 g = threadIdx.x; g < (tile_n >> Q); g += kBitBlock) {
+	for (u32 g = threadIdx.x; g < (tile_n >> Q); g += NT) {
 		const u32 x = static_cast<u32>(insert_zero_bits(g, pos, Q));
 		u32 idx[V];
 		T val[V];
@@ -268,88 +247,119 @@ __device__ __forceinline__ void lds_group(T* s_t, u64 base, u64 n, u32 tile_n, u
 // Stages s_first..s_last of the network restricted to one tile of 2^k
 // positions (k >= 5): steps t = min(s, k)..1 of each stage.  Serves both the
 // tile sort (stages 1..k) and the local half-cleaners of a stage s > k.
-template <typename T>
-__global__ __launch_bounds__(kBitBlock) void bitonic_tile_reg(T* __restrict__ data, u64 n, u32 k,
-	u32 s_first, u32 s_last, T xmask) {
-	__shared__ T s_t[bpad(1u << TileMax<T>::kLog)];
+// R keys per thread; LDS for 2^LOGT keys (k <= LOGT); 2^LOGT / R threads.
+template <typename T, int R, int LOGT>
+__global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_reg(
+	T* __restrict__ data, u64 n, u32 k, u32 s_first, u32 s_last, T xmask) {
+	constexpr u32 NT = (1u << LOGT) / R;
+	constexpr u32 LR = R == 32 ? 5 : 4;  // log2 R
+	static_assert(R == 16 || R == 32, "16 or 32 keys per thread");
+	__shared__ T s_t[bpad(1u << LOGT)];
 	const u32 tile_n = 1u << k;
 	const u64 base = static_cast<u64>(blockIdx.x) << k;
 	if (base >= n) return;
 	const u32 tid = threadIdx.x;
-	const bool owner = tid * kRegN < tile_n;
-	const u32 p0 = tid * kRegN;
+	const bool owner = tid * R < tile_n;
+	const u32 p0 = tid * R;
 	const u64 left = n - base;  // > 0
-	const u32 valid = !owner ? 0u : (left <= p0 ? 0u : (left - p0 >= kRegN ? kRegN
+	const u32 valid = !owner ? 0u : (left <= p0 ? 0u : (left - p0 >= R ? R
 		: static_cast<u32>(left - p0)));
-	// Every owner's 32 keys come straight from global memory (16-byte vectors
-	// when whole and aligned), all loads in flight together.
-	T* const mine = data + base + p0;
-	const bool vec = valid == kRegN && (reinterpret_cast<uintptr_t>(mine) & 15) == 0;
-	constexpr int NV = kRegN * sizeof(T) / 16;
-	T x[kRegN];
+	// The tile moves between HBM and LDS in lane-contiguous 16-byte words
+	// (whole, aligned tiles) or lane-contiguous keys, so every wave access is
+	// coalesced; each thread then takes its R consecutive keys from LDS.
+	T* const tile = data + base;
+	const u32 cnt = left < tile_n ? static_cast<u32>(left) : tile_n;
+	constexpr u32 E = 16 / sizeof(T);  // keys per 16-byte word
+	const bool vec = cnt == tile_n && (reinterpret_cast<uintptr_t>(tile) & 15) == 0;
 	if (vec) {
-		uint4 t4[NV];
+		for (u32 w = tid; w < tile_n / E; w += NT) {
+			const uint4 v = reinterpret_cast<const uint4*>(tile)[w];
+			T e[E];
+			__builtin_memcpy(e, &v, 16);
 #pragma unroll
-		for (int v = 0; v < NV; ++v) t4[v] = reinterpret_cast<const uint4*>(mine)[v];
-#pragma unroll
-		for (int v = 0; v < NV; ++v) __builtin_memcpy(&x[v * (16 / sizeof(T))], &t4[v], 16);
-#pragma unroll
-		for (int j = 0; j < kRegN; ++j) x[j] = static_cast<T>(x[j] ^ xmask);
+			for (u32 i = 0; i < E; ++i) s_t[bpad(w * E + i)] = static_cast<T>(e[i] ^ xmask);
+		}
 	} else {
+		for (u32 i = tid; i < cnt; i += NT) s_t[bpad(i)] = static_cast<T>(tile[i] ^ xmask);
+	}
+	__syncthreads();
+	T x[R];
+	if (owner) {
 #pragma unroll
-		for (int j = 0; j < kRegN; ++j)
-			x[j] = static_cast<u32>(j) < valid ? static_cast<T>(mine[j] ^ xmask) : T(0);
+		for (int j = 0; j < R; ++j)
+			x[j] = static_cast<u32>(j) < valid ? s_t[bpad(p0 + j)] : T(0);
 	}
 	for (u32 s = s_first; s <= s_last; ++s) {
 		const u32 t_top = s < k ? s : k;
-		if (t_top > 5) {
+		if (t_top > LR) {
 			__syncthreads();
 			if (owner) {
 #pragma unroll
-				for (int j = 0; j < kRegN; ++j) s_t[bpad(p0 + j)] = x[j];
+				for (int j = 0; j < R; ++j) s_t[bpad(p0 + j)] = x[j];
 			}
 			__syncthreads();
-			for (u32 t = t_top; t > 5;) {
-				const u32 m = t - 5;
+			for (u32 t = t_top; t > LR;) {
+				const u32 m = t - LR;
 				if (m >= 3) {
-					lds_group<T, 3>(s_t, base, n, tile_n, s, t);
+					lds_group<T, 3, NT>(s_t, base, n, tile_n, s, t);
 					t -= 3;
 				} else if (m == 2) {
-					lds_group<T, 2>(s_t, base, n, tile_n, s, t);
+					lds_group<T, 2, NT>(s_t, base, n, tile_n, s, t);
 					t -= 2;
 				} else {
-					lds_group<T, 1>(s_t, base, n, tile_n, s, t);
+					lds_group<T, 1, NT>(s_t, base, n, tile_n, s, t);
 					t -= 1;
 				}
 				__syncthreads();
 			}
 			if (owner) {
 #pragma unroll
-				for (int j = 0; j < kRegN; ++j) x[j] = s_t[bpad(p0 + j)];
+				for (int j = 0; j < R; ++j) x[j] = s_t[bpad(p0 + j)];
 			}
 		}
-		if (owner) reg_tail<T>(x, s, t_top < 5 ? t_top : 5, valid);
+		if (owner) reg_tail<T, R, LR>(x, s, t_top < LR ? t_top : LR, valid);
 	}
+	__syncthreads();  // every thread has read its keys of the last LDS step
+	if (owner) {
 #pragma unroll
-	for (int j = 0; j < kRegN; ++j) x[j] = static_cast<T>(x[j] ^ xmask);
+		for (int j = 0; j < R; ++j) s_t[bpad(p0 + j)] = x[j];
+	}
+	__syncthreads();
 	if (vec) {
-		uint4 t4[NV];
+		for (u32 w = tid; w < tile_n / E; w += NT) {
+			T e[E];
 #pragma unroll
-		for (int v = 0; v < NV; ++v) __builtin_memcpy(&t4[v], &x[v * (16 / sizeof(T))], 16);
-#pragma unroll
-		for (int v = 0; v < NV; ++v) reinterpret_cast<uint4*>(mine)[v] = t4[v];
-	} else if (owner) {
-#pragma unroll
-		for (int j = 0; j < kRegN; ++j)
-			if (static_cast<u32>(j) < valid) mine[j] = x[j];
+			for (u32 i = 0; i < E; ++i) e[i] = static_cast<T>(s_t[bpad(w * E + i)] ^ xmask);
+			uint4 v;
+			__builtin_memcpy(&v, e, 16);
+			reinterpret_cast<uint4*>(tile)[w] = v;
+		}
+	} else {
+		for (u32 i = tid; i < cnt; i += NT) tile[i] = static_cast<T>(s_t[bpad(i)] ^ xmask);
 	}
 }
+
+// Keys per thread held in registers by the tile kernels: 0 runs the plain
+// LDS kernel (tiles of 2^11), 16 or 32 the register-tail kernel.
+constexpr int kDefaultRegKeys = 16;
 
 struct BitonicData {
 	bool advanced;
 	u32 minps = 1, maxps = 4, maxsfs = 0xFFFFFFFFu;
+	int regkeys = -1;  // keys per thread of the tile kernels; -1: the default
 	u32 last_sweeps = 0;
 };
+
+// log2 of the LDS tile: the maxsfs option (at most 2^13 keys) or the
+// default of the key width.
+inline int regkeys_of(const BitonicData* bd) {
+	return bd->regkeys >= 0 ? bd->regkeys : kDefaultRegKeys;
+}
+inline u32 tile_log(const BitonicData* bd, u32 dflt) {
+	const u32 cap = regkeys_of(bd) == 0 ? kPlainTileLog : kTileLogMax;
+	if (bd->maxsfs == 0xFFFFFFFFu) return dflt < cap ? dflt : cap;
+	return bd->maxsfs < cap ? bd->maxsfs : cap;
+}
 
 template <typename T>
 T make_xmask(const CloSort* s) {
@@ -359,18 +369,55 @@ T make_xmask(const CloSort* s) {
 	return m;
 }
 
+template <typename T, bool FULL>
+void group_kernel(u32 grid, hipStream_t st, T* d, u32 n, u32 npo2, u32 s, u32 t, u32 Qn, T xm) {
+	switch (Qn) {
+	case 1: bitonic_group<T, 1, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
+	case 2: bitonic_group<T, 2, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
+	case 3: bitonic_group<T, 3, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
+	case 4: bitonic_group<T, 4, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
+	}
+}
+
 template <typename T>
 bool launch_group(CCLQueue* q, T* d, u64 n, u64 npo2, u32 s, u32 t, u32 Qn, T xm, const char* nm,
 	GError** err) {
 	const u64 threads = npo2 >> Qn;
 	const u32 grid = static_cast<u32>((threads + kBitBlock - 1) / kBitBlock);
 	Launch L(q, nm);
-	switch (Qn) {
-	case 1: bitonic_group<T, 1><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
-	case 2: bitonic_group<T, 2><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
-	case 3: bitonic_group<T, 3><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
-	case 4: bitonic_group<T, 4><<<grid, kBitBlock, 0, q->stream>>>(d, n, npo2, s, t, xm); break;
+	if (n == npo2)
+		group_kernel<T, true>(grid, q->stream, d, static_cast<u32>(n), static_cast<u32>(npo2), s,
+			t, Qn, xm);
+	else
+		group_kernel<T, false>(grid, q->stream, d, static_cast<u32>(n), static_cast<u32>(npo2),
+			s, t, Qn, xm);
+	return L.done(err);
+}
+
+// Stages s_first..s_last inside tiles of 2^k keys: the register-tail kernel
+// for k >= 5 (16 keys per thread in 512-thread blocks or 32 in 256), the
+// plain LDS kernels below that.
+template <typename T>
+bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_first,
+	u32 s_last, T xm, const char* nm, GError** err) {
+	const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
+	const int r = regkeys_of(bd);
+	const u32 lt = k < kTileLogMin ? kTileLogMin : k;  // LDS tile of the instance
+	Launch L(q, nm);
+#define CLO_TILE_REG(R_, LT_)                                                                    \
+	bitonic_tile_reg<T, R_, LT_><<<grid, (1u << LT_) / R_, 0, q->stream>>>(d, n, k, s_first,   \
+		s_last, xm)
+	if (k >= 5 && r != 0) {
+		if (r == 16 && lt == 11) CLO_TILE_REG(16, 11);
+		else if (r == 16 && lt == 12) CLO_TILE_REG(16, 12);
+		else if (r == 16) CLO_TILE_REG(16, 13);
+		else if (lt == 11) CLO_TILE_REG(32, 11);
+		else if (lt == 12) CLO_TILE_REG(32, 12);
+		else CLO_TILE_REG(32, 13);
 	}
+#undef CLO_TILE_REG
+	else
+		bitonic_plain<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, s_first, s_last, xm);
 	return L.done(err);
 }
 
@@ -398,17 +445,11 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 	// abitonic strategy: stages 1..k in one LDS tile kernel, then per stage the
 	// global steps in register groups of up to maxps steps (at least minps when
 	// that many remain) and the last k steps in one LDS kernel.
-	u32 k = TileMax<T>::kLog;
-	if (bd->maxsfs < k) k = bd->maxsfs;
+	u32 k = tile_log(bd, TileMax<T>::kLog);
 	if (k > S) k = S;
 	if (k >= 1) {
-		const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
-		Launch L(q, CLO_SORT_ABITONIC_KNAME_TILESORT);
-		if (k >= 5)
-			bitonic_tile_reg<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, 1, k, xm);
-		else
-			bitonic_tilesort<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, xm);
-		if (!L.done(err)) return false;
+		if (!launch_tile<T>(q, bd, d, n, k, 1, k, xm, CLO_SORT_ABITONIC_KNAME_TILESORT, err))
+			return false;
 		++sweeps;
 	}
 	static const char* gname[5] = {"", CLO_SORT_ABITONIC_KNAME_ANY,
@@ -427,13 +468,8 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 			t -= Qn;
 		}
 		if (k >= 1) {
-			const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
-			Launch L(q, CLO_SORT_ABITONIC_KNAME_LOCAL);
-			if (k >= 5)
-				bitonic_tile_reg<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, s, s, xm);
-			else
-				bitonic_local<T><<<grid, kBitBlock, 0, q->stream>>>(d, n, k, s, xm);
-			if (!L.done(err)) return false;
+			if (!launch_tile<T>(q, bd, d, n, k, s, s, xm, CLO_SORT_ABITONIC_KNAME_LOCAL, err))
+				return false;
 			++sweeps;
 		}
 	}
@@ -502,6 +538,12 @@ const char* abitonic_init(CloSort* sorter, const char* options, GError** err) {
 			bd->maxps = static_cast<u32>(v);
 		} else if (o.key == "maxsfs") {
 			bd->maxsfs = static_cast<u32>(v);
+		} else if (o.key == "regkeys") {
+			if (v != 0 && v != 16 && v != 32) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "Option 'regkeys' must be 0, 16 or 32.");
+				return nullptr;
+			}
+			bd->regkeys = static_cast<int>(v);
 		} else {
 			set_error(err, kQuarkClo, CLO_ERROR_ARGS, "Invalid option key '%s' for abitonic sort.",
 				o.key.c_str());
@@ -547,7 +589,10 @@ size_t abitonic_localmem(CloSort* sorter, cl_uint i, size_t, size_t, GError** er
 	}
 	const size_t es = clo_type_sizeof(sorter->elem_type);
 	if (i < 4) return 0;  // register-only kernels
-	return (es >= 8 ? (1u << 12) : (1u << 13)) * es;
+	const BitonicData* bd = static_cast<const BitonicData*>(sorter->data);
+	const u32 k = tile_log(bd, es >= 8 ? TileMax<u64>::kLog : TileMax<u32>::kLog);
+	if (k < 5 || regkeys_of(bd) == 0) return kPlainTileMax * es;
+	return bpad(1u << (k < kTileLogMin ? kTileLogMin : k)) * es;
 }
 
 }  // namespace

@@ -61,8 +61,14 @@ bool copy(CCLQueue* q, void* dst, const void* src, size_t bytes, GError** err) {
 // up to log2(tile) run in one LDS launch; each later stage runs its steps
 // with pair distance >= tile up to four per launch (in registers) and the
 // rest in one LDS launch.
-bool bitonic_network(const SnipSorter* ss, CCLQueue* q, const char* name, void* d,
+// Profiling names: sbitonic launches all carry the sbitonic kernel name;
+// abitonic launches carry the names of the precompiled kernels of the same
+// role (tile sort, local steps, 1-4 fused global steps).
+bool bitonic_network(const SnipSorter* ss, CCLQueue* q, bool advanced, void* d,
 	unsigned char* pad, uint32_t npow2, GError** err) {
+	static const char* const gname[5] = {"", CLO_SORT_ABITONIC_KNAME_ANY,
+		CLO_SORT_ABITONIC_KNAME_PRIV_2S4V, CLO_SORT_ABITONIC_KNAME_PRIV_3S8V,
+		CLO_SORT_ABITONIC_KNAME_PRIV_4S16V};
 	const int P = pad ? 1 : 0;
 	uint32_t tile = npow2 < kTile ? npow2 : kTile;
 	const uint32_t stages = static_cast<uint32_t>(__builtin_ctz(npow2));
@@ -71,6 +77,8 @@ bool bitonic_network(const SnipSorter* ss, CCLQueue* q, const char* name, void* 
 	auto local = [&](uint32_t lo, uint32_t hi, uint32_t first) {
 		void* args[] = {&d, &pad, &lo, &hi, &first, &tile};
 		const size_t items = static_cast<size_t>(nblocks) * 256;  // one block per tile
+		const char* name = !advanced ? CLO_SORT_SBITONIC_KNAME
+			: lo == 1 ? CLO_SORT_ABITONIC_KNAME_TILESORT : CLO_SORT_ABITONIC_KNAME_LOCAL;
 		return launch(q, name, ss->fn[kLocal0 + P], items, args, err);
 	};
 	if (!local(1, tstages, 1)) return false;
@@ -79,6 +87,7 @@ bool bitonic_network(const SnipSorter* ss, CCLQueue* q, const char* name, void* 
 			const uint32_t fused = step - tstages < 4 ? step - tstages : 4;
 			uint32_t st = stage, hi = step, items = npow2 >> fused;
 			void* args[] = {&d, &pad, &st, &hi, &items};
+			const char* name = advanced ? gname[fused] : CLO_SORT_SBITONIC_KNAME;
 			if (!launch(q, name, ss->fn[kGroup + 4 * P + fused - 1], items, args, err))
 				return false;
 			step -= fused;
@@ -162,8 +171,7 @@ CCLEvent* snippet_sort(CloSort* s, CCLQueue* q, CCLBuffer* kin, CCLBuffer* vin, 
 		return record_event(q, done_name.c_str(), err);
 	}
 	if (strcmp(algo, "sbitonic") == 0 || strcmp(algo, "abitonic") == 0) {
-		const char* kname = strcmp(algo, "sbitonic") == 0 ? CLO_SORT_SBITONIC_KNAME
-			: CLO_SORT_ABITONIC_KNAME_ANY;
+		const bool advanced = strcmp(algo, "abitonic") == 0;
 		void* d = kin->ptr;
 		if (kout) {
 			// sbitonic.c:84-96: copy, then sort the copy.
@@ -172,7 +180,7 @@ CCLEvent* snippet_sort(CloSort* s, CCLQueue* q, CCLBuffer* kin, CCLBuffer* vin, 
 		}
 		const uint32_t npow2 = static_cast<uint32_t>(clo_nlpo2(numel));
 		if (npow2 == n) {
-			if (!bitonic_network(ss, q, kname, d, nullptr, n, err)) return nullptr;
+			if (!bitonic_network(ss, q, advanced, d, nullptr, n, err)) return nullptr;
 		} else {
 			if (!ss->a.ensure(static_cast<size_t>(npow2) * es, err) ||
 				!ss->b.ensure(npow2, err) || !copy(q, ss->a.ptr, d, numel * es, err))
@@ -181,7 +189,7 @@ CCLEvent* snippet_sort(CloSort* s, CCLQueue* q, CCLBuffer* kin, CCLBuffer* vin, 
 			uint32_t nn = n, np = npow2;
 			void* args[] = {&pad, &nn, &np};
 			if (!launch(q, "bitonic_pad", ss->fn[kPad], npow2, args, err) ||
-				!bitonic_network(ss, q, kname, ss->a.ptr, pad, npow2, err) ||
+				!bitonic_network(ss, q, advanced, ss->a.ptr, pad, npow2, err) ||
 				!copy(q, d, ss->a.ptr, numel * es, err))
 				return nullptr;
 		}

@@ -49,8 +49,20 @@ def test_abitonic_options(clops, oracle, ctx, opts):
     assert np.array_equal(s.sort_host(k), np.sort(k))
 
 
+@pytest.mark.parametrize("regkeys", [0, 16, 32])
+@pytest.mark.parametrize("maxsfs", [5, 9, 11, 12, 13])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint32, np.int64])
+def test_abitonic_tile_kernels(clops, oracle, ctx, regkeys, maxsfs, dtype):
+    """Every tile kernel (plain LDS steps, 16 or 32 register keys per
+    thread) at every tile size, on power-of-two and ragged sizes."""
+    s = clops.Sorter(ctx, "abitonic", dtype, options="regkeys=%d,maxsfs=%d" % (regkeys, maxsfs))
+    for n in (1 << 15, 40003):
+        k = _keys(oracle, dtype, n, 0xBB + n)
+        assert np.array_equal(s.sort_host(k), np.sort(k))
+
+
 def test_abitonic_bad_options(clops, ctx):
-    for bad in ("minps=0", "maxps=5", "minps=3,maxps=2", "foo=1", "minps"):
+    for bad in ("minps=0", "maxps=5", "minps=3,maxps=2", "foo=1", "minps", "regkeys=8"):
         with pytest.raises(clops.ClopsError) as e:
             clops.Sorter(ctx, "abitonic", np.uint32, options=bad)
         assert e.value.code == clops.CLO_ERROR_ARGS
