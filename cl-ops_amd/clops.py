@@ -569,17 +569,19 @@ class Sorter:
         return _call(self.L.clo_sort_kv_with_device_data, self.ptr, q.ptr, None, kin.ptr, vin.ptr,
                      kout.ptr if kout else None, vout.ptr if vout else None, n, 0)
 
-    def sort_host(self, a, q=None):
+    def sort_host(self, a, q=None, q_comm=None):
         a = np.ascontiguousarray(a, dtype=self.dtype)
         out = np.empty_like(a)
-        _call(self.L.clo_sort_with_host_data, self.ptr, q.ptr if q else None, None,
+        _call(self.L.clo_sort_with_host_data, self.ptr, q.ptr if q else None,
+              q_comm.ptr if q_comm else None,
               a.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), a.size, 0)
         return out
 
-    def sort_kv_host(self, keys, vals, q=None):
+    def sort_kv_host(self, keys, vals, q=None, q_comm=None):
         k = np.array(keys, dtype=self.dtype, copy=True)
         v = np.array(vals, dtype=np.uint32, copy=True)
-        _call(self.L.clo_sort_kv_with_host_data, self.ptr, q.ptr if q else None, None,
+        _call(self.L.clo_sort_kv_with_host_data, self.ptr, q.ptr if q else None,
+              q_comm.ptr if q_comm else None,
               k.ctypes.data_as(ctypes.c_void_p), v.ctypes.data_as(ctypes.c_void_p), k.size, 0)
         return k, v
 
@@ -627,10 +629,11 @@ class Scanner:
         return _call(self.L.clo_scan_segmented_with_device_data, self.ptr, q.ptr, None, bin_.ptr,
                      bout.ptr, offsets.ptr, nseg, n)
 
-    def scan_host(self, a, q=None):
+    def scan_host(self, a, q=None, q_comm=None):
         a = np.ascontiguousarray(a, dtype=self.elem)
         out = np.empty(a.size, dtype=self.sum)
-        _call(self.L.clo_scan_with_host_data, self.ptr, q.ptr if q else None, None,
+        _call(self.L.clo_scan_with_host_data, self.ptr, q.ptr if q else None,
+              q_comm.ptr if q_comm else None,
               a.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), a.size, 0)
         return out
 

@@ -143,6 +143,8 @@ struct clo_sort {
 	// Run-time compiled kernels for user compare / key snippets (NULL when
 	// the ahead-of-time kernels serve the sorter).
 	void* snip;
+	// Device buffers of the host-data path, kept between calls.
+	void* host_cache;
 };
 
 struct clo_scan {
@@ -151,6 +153,7 @@ struct clo_scan {
 	CloType elem_type;
 	CloType sum_type;
 	void* data;
+	void* host_cache = nullptr;  // device buffers of the host-data path
 };
 
 // KV entry points of the algorithms that support them.
@@ -182,6 +185,15 @@ bool rtc_functions(CCLContext* ctx, const std::string& src, const char* compiler
 bool rtc_launch(CCLQueue* q, const char* name, hipFunction_t f, size_t items, void** args,
 	GError** err);
 
+// Host-data paths through pinned staging (host_path.cpp).  Blocking.
+// sort: keys (and optional u32 vals) in place on the host, except that the
+// sorted keys go to keys_out.  scan: `in` -> `out`, chunk by chunk.
+bool sort_host_path(CloSort* s, CCLQueue* cq_exec, CCLQueue* cq_comm, void* keys, void* vals,
+	void* keys_out, size_t numel, GError** err);
+bool scan_host_path(CloScan* sc, CCLQueue* cq_exec, CCLQueue* cq_comm, const void* in, void* out,
+	size_t numel, GError** err);
+void host_cache_free(void* cache);
+
 // Sorters with user snippets (snippet.cpp).  needed(): the snippets, types
 // or key getter are beyond the ahead-of-time kernels.  init(): compiles the
 // run-time kernels for the sorter's device.  sort(): the algorithm of
@@ -195,6 +207,12 @@ const char* snippet_sorter_source(const CloSort* s);
 CCLEvent* snippet_sort(CloSort* s, CCLQueue* q, CCLBuffer* kin, CCLBuffer* vin, CCLBuffer* kout,
 	CCLBuffer* vout, size_t numel, GError** err);
 }  // namespace clo
+
+// One chunk of a plain blelloch scan of a longer array: the running total
+// before the chunk is read from the device word `chain` (sum type) and the
+// total after it is left there.  Chunks must run in order on one queue.
+bool clo_blelloch_scan_chunk(CloScan* scanner, CCLQueue* cq, const void* in, void* out,
+	size_t numel, void* chain, GError** err);
 
 // Plain blelloch scan with 8, 16 (the default) or 32 items per thread (8 and
 // 32 for unsigned 32/64-bit elements only).

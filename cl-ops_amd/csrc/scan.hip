@@ -190,18 +190,21 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce(const In* __r
 // One block: tsum[t] <- carry into tile t (exclusive, segmented when SEG).
 // Rounds of kTilesBlock * kTilesItems tile sums are staged in LDS with
 // coalesced loads; each thread then combines kTilesItems consecutive ones.
+// `chain` (plain scans, may be NULL) carries a running total between calls
+// on consecutive chunks of one array: the scan starts from *chain and leaves
+// *chain + the chunk's total there.
 constexpr u32 kTilesItems = 8;
 constexpr u32 kTilesRound = kTilesBlock * kTilesItems;
 
 template <typename Acc, bool SEG>
 __global__ __launch_bounds__(kTilesBlock) void blelloch_tiles_scan(Acc* __restrict__ tsum,
-	u32* __restrict__ tflag, u32 tiles) {
+	u32* __restrict__ tflag, u32 tiles, Acc* __restrict__ chain) {
 	__shared__ Acc s_v[pad(kTilesRound)];
 	__shared__ u8 s_f[SEG ? kTilesRound : 1];
 	__shared__ Acc s_wsum[kTilesBlock / kWave];
 	__shared__ SegPair<Acc> s_pw[kTilesBlock / kWave];
 	const u32 tid = threadIdx.x;
-	SegPair<Acc> carry = {0u, Acc(0)};
+	SegPair<Acc> carry = {0u, (!SEG && chain) ? *chain : Acc(0)};
 	// Values of the next round are loaded while the current one is scanned.
 	Acc nv[kTilesItems];
 	u8 nf[kTilesItems];
@@ -257,6 +260,8 @@ __global__ __launch_bounds__(kTilesBlock) void blelloch_tiles_scan(Acc* __restri
 		carry = seg_combine(carry, tot);
 		__syncthreads();
 	}
+	// Every thread read *chain before the first barrier above.
+	if (!SEG && chain && tid == 0) *chain = carry.v;
 }
 
 template <typename In, typename Acc, int OUTW, bool SEG, int IT>
@@ -432,6 +437,7 @@ struct ScanLaunch {
 	const u32* off;  // segment starts (segmented scan) or nullptr
 	u32 nseg;
 	u32 items;  // items per thread: 16, or 8 / 32 for unsigned plain scans
+	void* chain;  // running total across chunks (plain scans) or nullptr
 	CCLQueue* q;
 };
 
@@ -452,7 +458,8 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 			}
 			{
 				Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_TILES);
-				blelloch_tiles_scan<Acc, false><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles);
+				blelloch_tiles_scan<Acc, false><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles,
+					static_cast<Acc*>(a.chain));
 				if (!L.done(err)) return false;
 			}
 			Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_SCAN);
@@ -469,7 +476,8 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 	}
 	{
 		Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_TILES);
-		blelloch_tiles_scan<Acc, SEG><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles);
+		blelloch_tiles_scan<Acc, SEG><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles,
+			static_cast<Acc*>(a.chain));
 		if (!L.done(err)) return false;
 	}
 	{
@@ -523,27 +531,26 @@ void blelloch_finalize(CloScan* scanner) {
 	scanner->data = nullptr;
 }
 
-CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
-	CCLBuffer* data_out, size_t numel, const u32* off, u32 nseg, GError** err,
-	u32 items = kScanItems) {
+// Launches the scan of numel elements at `in` into `out` on cq_exec.
+bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, size_t numel,
+	const u32* off, u32 nseg, GError** err, u32 items, void* chain) {
 	BlellochData* bd = static_cast<BlellochData*>(scanner->data);
-	if (numel == 0) return record_event(cq_exec, "blelloch_done", err);
 	if (items != 8 && items != 16 && items != 32) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "items per thread must be 8, 16 or 32");
-		return nullptr;
+		return false;
 	}
 	const size_t tiles = div_ceil(numel, static_cast<size_t>(kScanBlock) * items);
 	if (tiles > 0x7FFFFFFFull) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "numel too large");
-		return nullptr;
+		return false;
 	}
 	const size_t ss = clo_type_sizeof(scanner->sum_type);
 	const size_t acc = ss == 8 ? 8 : 4;
 	const size_t sum_bytes = (tiles * acc + 255) / 256 * 256;
-	if (!bd->tiles.ensure(sum_bytes + tiles * 4, err)) return nullptr;
+	if (!bd->tiles.ensure(sum_bytes + tiles * 4, err)) return false;
 	ScanLaunch a;
-	a.in = data_in->ptr;
-	a.out = data_out->ptr;
+	a.in = in;
+	a.out = out;
 	a.n = numel;
 	a.tiles = static_cast<u32>(tiles);
 	a.tsum = bd->tiles.ptr;
@@ -551,6 +558,7 @@ CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
 	a.off = off;
 	a.nseg = nseg;
 	a.items = items;
+	a.chain = chain;
 	a.q = cq_exec;
 	bool ok = false;
 	switch (scanner->elem_type) {
@@ -566,7 +574,16 @@ CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "unsupported scan types");
 		break;
 	}
-	if (!ok) return nullptr;
+	return ok;
+}
+
+CCLEvent* scan_common(CloScan* scanner, CCLQueue* cq_exec, CCLBuffer* data_in,
+	CCLBuffer* data_out, size_t numel, const u32* off, u32 nseg, GError** err,
+	u32 items = kScanItems) {
+	if (numel > 0 &&
+		!scan_ptrs(scanner, cq_exec, data_in->ptr, data_out->ptr, numel, off, nseg, err, items,
+			nullptr))
+		return nullptr;
 	return record_event(cq_exec, "blelloch_done", err);
 }
 
@@ -608,6 +625,12 @@ size_t blelloch_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, 
 }
 
 }  // namespace
+
+bool clo_blelloch_scan_chunk(CloScan* scanner, CCLQueue* cq, const void* in, void* out,
+	size_t numel, void* chain, GError** err) {
+	if (numel == 0) return true;
+	return scan_ptrs(scanner, cq, in, out, numel, nullptr, 0, err, kScanItems, chain);
+}
 
 CCLEvent* clo_blelloch_scan_items(CloScan* scanner, CCLQueue* cq, CCLBuffer* in, CCLBuffer* out,
 	size_t numel, uint32_t items, GError** err) {
@@ -676,6 +699,7 @@ CloScan* clo_scan_new(const char* type, const char* options, CCLContext* ctx, Cl
 void clo_scan_destroy(CloScan* scan) {
 	if (!scan) return;
 	if (scan->impl.finalize && scan->data) scan->impl.finalize(scan);
+	if (scan->host_cache) host_cache_free(scan->host_cache);
 	ccl_context_destroy(scan->ctx);
 	delete scan;
 }
@@ -724,39 +748,22 @@ CCLEvent* clo_scan_segmented_with_device_data(CloScan* scanner, CCLQueue* cq_exe
 
 cl_bool clo_scan_with_host_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	void* data_in, void* data_out, size_t numel, size_t lws_max, GError** err) {
+	(void)lws_max;
 	if (err && *err) return CL_FALSE;
 	if (!scanner || !data_in || !data_out) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL scanner or host pointer");
 		return CL_FALSE;
 	}
-	const size_t ib = numel * clo_type_sizeof(scanner->elem_type);
-	const size_t ob = numel * clo_type_sizeof(scanner->sum_type);
 	CCLQueue* own = nullptr;
-	CCLBuffer *din = nullptr, *dout = nullptr;
-	CCLEvent* evt = nullptr;
-	CCLEventWaitList ewl = nullptr;
-	cl_bool ok = CL_FALSE;
 	if (!cq_exec) {
 		own = ccl_queue_new(scanner->ctx, nullptr, 0, err);
 		if (!own) return CL_FALSE;
 		cq_exec = own;
 	}
 	if (!cq_comm) cq_comm = cq_exec;
-	din = ccl_buffer_new(scanner->ctx, CL_MEM_READ_ONLY, ib, nullptr, err);
-	dout = din ? ccl_buffer_new(scanner->ctx, CL_MEM_WRITE_ONLY, ob, nullptr, err) : nullptr;
-	if (!din || !dout) goto done;
-	if (!ccl_buffer_enqueue_write(din, cq_comm, CL_TRUE, 0, ib, data_in, nullptr, err)) goto done;
-	evt = clo_scan_with_device_data(scanner, cq_exec, cq_comm, din, dout, numel, lws_max, err);
-	if (!evt) goto done;
-	ccl_event_wait_list_add(&ewl, evt, nullptr);
-	if (!ccl_buffer_enqueue_read(dout, cq_comm, CL_TRUE, 0, ob, data_out, &ewl, err)) goto done;
-	ok = CL_TRUE;
-done:
-	ccl_event_wait_list_clear(&ewl);
-	if (din) ccl_buffer_destroy(din);
-	if (dout) ccl_buffer_destroy(dout);
+	const bool ok = scan_host_path(scanner, cq_exec, cq_comm, data_in, data_out, numel, err);
 	if (own) ccl_queue_destroy(own);
-	return ok;
+	return ok ? CL_TRUE : CL_FALSE;
 }
 
 CCLContext* clo_scan_get_context(CloScan* scanner) { return scanner ? scanner->ctx : nullptr; }

@@ -47,6 +47,7 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 	s->seg_grid = 0;
 	s->seg_cache = nullptr;
 	s->snip = nullptr;
+	s->host_cache = nullptr;
 	GError* e = nullptr;
 	const char* src = s->impl.init(s, options, &e);
 	if (!src || e) {
@@ -79,6 +80,7 @@ void clo_sort_destroy(CloSort* sorter) {
 	if (sorter->impl.finalize) sorter->impl.finalize(sorter);
 	if (sorter->seg_cache) clo_satradix_segsort_cache_free(sorter->seg_cache);
 	if (sorter->snip) snippet_sorter_free(sorter);
+	if (sorter->host_cache) host_cache_free(sorter->host_cache);
 	ccl_context_destroy(sorter->ctx);
 	delete sorter;
 }
@@ -116,49 +118,24 @@ CCLEvent* clo_sort_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue
 
 cl_bool clo_sort_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	void* data_in, void* data_out, size_t numel, size_t lws_max, GError** err) {
+	(void)lws_max;
 	if (err && *err) return CL_FALSE;
 	if (!sorter || !data_in || !data_out) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL sorter or host pointer");
 		return CL_FALSE;
 	}
-	size_t bytes = numel * clo_type_sizeof(sorter->elem_type);
-	CCLQueue* own = nullptr;
-	CCLBuffer* din = nullptr;
-	CCLBuffer* dout = nullptr;
-	cl_bool ok = CL_FALSE;
-	CCLEvent* evt = nullptr;
-	CCLEventWaitList ewl = nullptr;
 	// clo_sort_abstract.c:333-341: internal queue when cq_exec is NULL.
+	CCLQueue* own = nullptr;
 	if (!cq_exec) {
 		own = ccl_queue_new(sorter->ctx, nullptr, 0, err);
 		if (!own) return CL_FALSE;
 		cq_exec = own;
 	}
 	if (!cq_comm) cq_comm = cq_exec;
-	din = ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, bytes, nullptr, err);
-	if (!din) goto done;
-	if (!sorter->impl.in_place) {
-		dout = ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, bytes, nullptr, err);
-		if (!dout) goto done;
-	}
-	evt = ccl_buffer_enqueue_write(din, cq_comm, CL_FALSE, 0, bytes, data_in, nullptr, err);
-	if (!evt) goto done;
-	// The copy on cq_comm must finish before the sort on cq_exec starts.
-	ccl_event_wait_list_add(&ewl, evt, nullptr);
-	if (!ccl_event_wait(&ewl, err)) goto done;
-	evt = clo_sort_with_device_data(sorter, cq_exec, cq_comm, din, dout, numel, lws_max, err);
-	if (!evt) goto done;
-	ccl_event_wait_list_add(&ewl, evt, nullptr);
-	evt = ccl_buffer_enqueue_read(dout ? dout : din, cq_comm, CL_TRUE, 0, bytes, data_out,
-		&ewl, err);
-	if (!evt) goto done;
-	ok = CL_TRUE;
-done:
-	ccl_event_wait_list_clear(&ewl);
-	if (din) ccl_buffer_destroy(din);
-	if (dout) ccl_buffer_destroy(dout);
+	const bool ok = sort_host_path(sorter, cq_exec, cq_comm, data_in, nullptr, data_out, numel,
+		err);
 	if (own) ccl_queue_destroy(own);
-	return ok;
+	return ok ? CL_TRUE : CL_FALSE;
 }
 
 CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
@@ -193,41 +170,22 @@ CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQu
 
 cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	void* keys, void* vals, size_t numel, size_t lws_max, GError** err) {
+	(void)lws_max;
 	if (err && *err) return CL_FALSE;
 	if (!sorter || !keys || !vals) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL sorter or host pointer");
 		return CL_FALSE;
 	}
-	size_t kb = numel * clo_type_sizeof(sorter->elem_type), vb = numel * 4;
 	CCLQueue* own = nullptr;
-	CCLBuffer *dk = nullptr, *dv = nullptr;
-	CCLEvent* evt = nullptr;
-	CCLEventWaitList ewl = nullptr;
-	cl_bool ok = CL_FALSE;
 	if (!cq_exec) {
 		own = ccl_queue_new(sorter->ctx, nullptr, 0, err);
 		if (!own) return CL_FALSE;
 		cq_exec = own;
 	}
 	if (!cq_comm) cq_comm = cq_exec;
-	dk = ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, kb, nullptr, err);
-	dv = dk ? ccl_buffer_new(sorter->ctx, CL_MEM_READ_WRITE, vb, nullptr, err) : nullptr;
-	if (!dk || !dv) goto done;
-	if (!ccl_buffer_enqueue_write(dk, cq_comm, CL_TRUE, 0, kb, keys, nullptr, err)) goto done;
-	if (!ccl_buffer_enqueue_write(dv, cq_comm, CL_TRUE, 0, vb, vals, nullptr, err)) goto done;
-	evt = clo_sort_kv_with_device_data(sorter, cq_exec, cq_comm, dk, dv, nullptr, nullptr,
-		numel, lws_max, err);
-	if (!evt) goto done;
-	ccl_event_wait_list_add(&ewl, evt, nullptr);
-	if (!ccl_buffer_enqueue_read(dk, cq_comm, CL_TRUE, 0, kb, keys, &ewl, err)) goto done;
-	if (!ccl_buffer_enqueue_read(dv, cq_comm, CL_TRUE, 0, vb, vals, nullptr, err)) goto done;
-	ok = CL_TRUE;
-done:
-	ccl_event_wait_list_clear(&ewl);
-	if (dk) ccl_buffer_destroy(dk);
-	if (dv) ccl_buffer_destroy(dv);
+	const bool ok = sort_host_path(sorter, cq_exec, cq_comm, keys, vals, keys, numel, err);
 	if (own) ccl_queue_destroy(own);
-	return ok;
+	return ok ? CL_TRUE : CL_FALSE;
 }
 
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
