@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "cl_ops.h"
+#include "cli_common.hpp"
 
 namespace {
 
@@ -56,83 +57,43 @@ void usage(FILE* f) {
 		"  -m, --max                 Maximum integer to produce, overrides --bits\n");
 }
 
-bool parse_uint(const char* s, unsigned long long* v) {
-	char* end = nullptr;
-	errno = 0;
-	*v = strtoull(s, &end, 0);
-	return errno == 0 && end && *end == '\0' && end != s;
-}
-
-// GOption-style parsing: -x VALUE, -xVALUE, --long VALUE, --long=VALUE.
+// GOption's forms (cli_common.hpp); -d takes -1 for the default device.
 int parse(int argc, char** argv, Args* a) {
-	struct Opt {
-		char s;
-		const char* l;
-	} const opts[] = {{'r', "rng"}, {'o', "output"}, {'g', "globalsize"}, {'l', "localsize"},
-		{'n', "runs"}, {'d', "device"}, {'s', "rng-seed"}, {'h', "gid-hash"}, {'b', "bits"},
-		{'m', "max"}};
-	for (int i = 1; i < argc; ++i) {
-		const char* arg = argv[i];
-		if (!strcmp(arg, "--help") || !strcmp(arg, "-?")) {
-			usage(stdout);
-			exit(0);
-		}
-		char key = 0;
-		const char* val = nullptr;
-		if (arg[0] == '-' && arg[1] == '-') {
-			const char* name = arg + 2;
-			const char* eq = strchr(name, '=');
-			const size_t len = eq ? static_cast<size_t>(eq - name) : strlen(name);
-			for (const Opt& o : opts)
-				if (strlen(o.l) == len && !strncmp(o.l, name, len)) key = o.s;
-			if (eq) val = eq + 1;
-		} else if (arg[0] == '-' && arg[1]) {
-			for (const Opt& o : opts)
-				if (o.s == arg[1]) key = o.s;
-			if (arg[2]) val = arg + 2;
-		}
-		if (!key) {
-			fprintf(stderr, "Error: Unknown option %s\n", arg);
-			return CLO_ERROR_ARGS;
-		}
-		if (!val) {
-			if (i + 1 >= argc) {
-				fprintf(stderr, "Error: Missing argument for %s\n", arg);
-				return CLO_ERROR_ARGS;
-			}
-			val = argv[++i];
-		}
-		unsigned long long v = 0;
-		const bool numeric = strchr("glndsbm", key) != nullptr;
-		if (numeric && !parse_uint(val, &v) && !(key == 'd' && !strcmp(val, "-1"))) {
-			fprintf(stderr, "Error: Cannot parse integer value '%s' for %s\n", val, arg);
-			return CLO_ERROR_ARGS;
-		}
-		switch (key) {
-		case 'r': a->rng = val; break;
-		case 'o': a->output = val; break;
-		case 'g': a->gws = static_cast<size_t>(v); break;
-		case 'l': a->lws = static_cast<size_t>(v); break;
-		case 'n': a->runs = static_cast<unsigned>(v); break;
-		case 'd': a->dev = strcmp(val, "-1") ? static_cast<int>(v) : -1; break;
-		case 's': a->seed = static_cast<uint32_t>(v); break;
-		case 'h':
-			a->hash = val;
-			a->have_hash = true;
-			break;
-		case 'b': a->bits = static_cast<unsigned>(v); break;
-		case 'm': a->maxint = static_cast<unsigned>(v); break;
-		}
+	unsigned long long gws = a->gws, lws = a->lws, runs = a->runs, seed = a->seed, bits = a->bits,
+		maxint = a->maxint;
+	long long dev = a->dev;
+	std::string usage_text;
+	{
+		char* buf = nullptr;
+		size_t len = 0;
+		FILE* f = open_memstream(&buf, &len);
+		usage(f);
+		fclose(f);
+		usage_text.assign(buf, len);
+		free(buf);
 	}
-	return 0;
+	const std::vector<cli::Option> opts = {{'r', "rng", cli::kString, &a->rng, nullptr},
+		{'o', "output", cli::kString, &a->output, nullptr},
+		{'g', "globalsize", cli::kUInt, &gws, nullptr},
+		{'l', "localsize", cli::kUInt, &lws, nullptr},
+		{'n', "runs", cli::kUInt, &runs, nullptr},
+		{'d', "device", cli::kInt, &dev, nullptr},
+		{'s', "rng-seed", cli::kUInt, &seed, nullptr},
+		{'h', "gid-hash", cli::kString, &a->hash, &a->have_hash},
+		{'b', "bits", cli::kUInt, &bits, nullptr},
+		{'m', "max", cli::kUInt, &maxint, nullptr}};
+	const int st = cli::parse(argc, argv, opts, usage_text.c_str());
+	a->gws = static_cast<size_t>(gws);
+	a->lws = static_cast<size_t>(lws);
+	a->runs = static_cast<unsigned>(runs);
+	a->dev = static_cast<int>(dev);
+	a->seed = static_cast<uint32_t>(seed);
+	a->bits = static_cast<unsigned>(bits);
+	a->maxint = static_cast<unsigned>(maxint);
+	return st;
 }
 
-int fail(GError* err) {
-	const int status = err ? err->code : CLO_ERROR_ARGS;
-	fprintf(stderr, "Error: %s\n", err && err->message ? err->message : "unknown");
-	clo_error_free(err);
-	return status;
-}
+int fail(GError* err) { return cli::fail(err); }
 
 // Appends the decimal digits of v.
 inline char* put_uint(char* p, uint32_t v) {

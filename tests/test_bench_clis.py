@@ -1,6 +1,7 @@
-"""clo_rng_bench, the command-line counterpart of the reference's RNG bench:
-option checks (CPU) and every output format against the oracle's streams
-(GPU)."""
+"""The command-line counterparts of the reference's benchmarks: clo_rng_bench
+(option checks on the CPU; every output format against the oracle's streams
+on the GPU), clo_sort_bench and clo_scan_bench (report lines, file output,
+the sortedness / serial-scan checks)."""
 import os
 import subprocess
 
@@ -99,3 +100,70 @@ def test_cli_continuous_stdout(oracle):
     assert rc == 0
     want = _stream(oracle, "xorshift128", gws, 37, 3)
     assert np.array_equal(np.frombuffer(data, dtype=np.uint32), want.ravel())
+
+
+# ---- clo_sort_bench / clo_scan_bench -----------------------------------------
+
+SORT_CLI = os.path.join(ROOT, "cl-ops_amd", "bin", "clo_sort_bench")
+SCAN_CLI = os.path.join(ROOT, "cl-ops_amd", "bin", "clo_scan_bench")
+
+
+def _run_cli(cli, args, cwd=None, timeout=120):
+    return subprocess.run([cli] + args, cwd=cwd, capture_output=True, timeout=timeout)
+
+
+def test_bench_clis_built_and_check_arguments():
+    for cli in (SORT_CLI, SCAN_CLI):
+        r = _run_cli(cli, ["--help"])
+        assert r.returncode == 0 and r.stdout.startswith(b"Usage: ")
+    r = _run_cli(SORT_CLI, ["-t", "foo"])
+    assert r.returncode == 6 and b"Error: Unknown type 'foo'" in r.stderr  # CLO_ERROR_UNKNOWN_TYPE
+    r = _run_cli(SCAN_CLI, ["--bogus"])
+    assert r.returncode == 2
+
+
+def _rates(stdout, pattern):
+    import re
+    return re.findall(pattern, stdout.decode())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algo,typ,ok", [
+    ("satradix", "ulong", True), ("abitonic", "uint", True), ("gselect", "short", True),
+    ("sbitonic", "float", True), ("abitonic", "double", True),
+    ("satradix", "int", False),  # unsigned key order, as the reference: a signed check fails
+])
+def test_sort_bench_cli(tmp_path, algo, typ, ok):
+    maxpo2 = 12 if algo != "gselect" else 10
+    r = _run_cli(SORT_CLI, ["-a", algo, "-t", typ, "-n", str(maxpo2), "-r", "2", "-o", "b.tsv"],
+                 cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert b"Type of elements to sort: %s" % typ.encode() in r.stdout
+    lines = _rates(r.stdout, r"- 2\^(\d+): ([0-9.]+) Mkeys/s (.*)")
+    assert [int(n) for n, _, _ in lines] == list(range(4, maxpo2 + 1))
+    assert all(float(v) > 0 for _, v, _ in lines)
+    if ok:
+        assert all(msg.strip() == "" for _, _, msg in lines)
+    else:  # 2^4 random ints include negatives: they sort after the positives
+        assert any("(sort did not work)" in msg for _, _, msg in lines)
+    rows = (tmp_path / "b.tsv").read_text().splitlines()
+    assert len(rows) == maxpo2 and all(len(row.split("\t")) == 3 for row in rows)
+    assert int(rows[maxpo2 - 1].split("\t")[1]) > 0 and rows[0] == "0\t0\t0"
+
+
+@pytest.mark.gpu
+def test_scan_bench_cli(tmp_path):
+    r = _run_cli(SCAN_CLI, ["-n", "12", "-r", "2", "-o", "s.tsv"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = _rates(r.stdout, r"- +(\d+) : ([0-9.]+) MValues/s (.*)")
+    assert [int(n) for n, _, _ in lines] == [4 << k for k in range(12)]
+    assert all(msg.strip() == "" for _, _, msg in lines)
+    rows = (tmp_path / "s.tsv").read_text().splitlines()
+    assert len(rows) == 12 and all(len(row.split("\t")) == 3 for row in rows)
+    # a uchar sum overflows past a few elements; --no-check skips the check
+    r = _run_cli(SCAN_CLI, ["-t", "uchar", "-y", "uchar", "-i", "64", "-n", "3"], cwd=tmp_path)
+    assert r.returncode == 0 and all("[Overflow]" in m for _, _, m in
+                                     _rates(r.stdout, r"- +(\d+) : ([0-9.]+) MValues/s (.*)"))
+    r = _run_cli(SCAN_CLI, ["-u", "-n", "3", "-t", "int", "-y", "long"], cwd=tmp_path)
+    assert r.returncode == 0 and all("[Unverified]" in m for _, _, m in
+                                     _rates(r.stdout, r"- +(\d+) : ([0-9.]+) MValues/s (.*)"))
