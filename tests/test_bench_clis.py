@@ -167,3 +167,19 @@ def test_scan_bench_cli(tmp_path):
     r = _run_cli(SCAN_CLI, ["-u", "-n", "3", "-t", "int", "-y", "long"], cwd=tmp_path)
     assert r.returncode == 0 and all("[Unverified]" in m for _, _, m in
                                      _rates(r.stdout, r"- +(\d+) : ([0-9.]+) MValues/s (.*)"))
+
+
+# ---- a C99 caller of the public headers ----------------------------------------
+
+C_CALLER = os.path.join(ROOT, "cl-ops_amd", "bin", "c_caller")
+
+
+def test_c_caller_built():
+    """Built by gcc -std=c99 -pedantic -Werror against include/ alone."""
+    assert os.access(C_CALLER, os.X_OK)
+
+
+@pytest.mark.gpu
+def test_c_caller_runs():
+    r = subprocess.run([C_CALLER], capture_output=True, timeout=120)
+    assert r.returncode == 0 and r.stdout == b"c_caller: ok\n", r.stderr
