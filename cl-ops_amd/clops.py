@@ -463,10 +463,27 @@ class Graph:
         self.ptr = None
 
     def begin(self):
-        _call(self.L.clo_graph_begin_capture, self.q.ptr)
+        # A finaliser that frees device memory (hipFree) on this thread while
+        # the capture is open would invalidate it: collect garbage now and
+        # keep the cyclic collector off until end().
+        import gc
+        gc.collect()
+        self._gc_was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            _call(self.L.clo_graph_begin_capture, self.q.ptr)
+        except Exception:
+            if self._gc_was_enabled:
+                gc.enable()
+            raise
 
     def end(self):
-        self.ptr = _call(self.L.clo_graph_end_capture, self.q.ptr)
+        import gc
+        try:
+            self.ptr = _call(self.L.clo_graph_end_capture, self.q.ptr)
+        finally:
+            if getattr(self, "_gc_was_enabled", False):
+                gc.enable()
         return self
 
     def launch(self, q=None):

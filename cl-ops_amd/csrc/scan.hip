@@ -423,8 +423,231 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan_vec(const In* _
 	}
 }
 
+// ---- single pass: decoupled look-back --------------------------------------
+// Plain unsigned 32/64-bit scans on 16-byte aligned buffers of at most
+// kOnePassMaxTiles tiles.  On MI355X a device-coherent status read takes about
+// a microsecond, and a tile's inclusive prefix only appears after its own
+// look-back, so the frontier of known prefixes advances about one poll width
+// per microsecond: for large scans that is slower than reading the input
+// twice (2^26 u32: 0.32 ms single pass vs 0.17 ms reduce-then-scan), while up
+// to 2^21 elements the single launch beats three (2^20: 10.6 vs 14.2 us).
+// Tile t = block
+// t (linear order: each XCD dispatches its blocks in index order, so the
+// lowest unfinished tile is always resident and every wait below ends).  A
+// tile loads its elements, publishes its aggregate (A) in its status words,
+// then wave 0 walks back over the statuses 64 tiles at a time, adding
+// aggregates until it meets an inclusive prefix (P); the tile publishes its
+// own P and writes its scanned elements.  The input is read once.
+//
+// Status words (one per 32 bits of the sum) pack value, epoch and flag into
+// one 64-bit relaxed device-scope atomic, so no fences are needed.  The
+// epoch comes from a device word (the first word of the status buffer; the
+// abort word and the tiles' words follow) that the last tile advances once its
+// look-back is done: by then every tile has published and so has read the
+// epoch; words of earlier launches therefore read as "not ready" and the
+// status array never needs clearing (also across graph replays).
+//
+// A guard bounds every wait: a wave that polls kSpinLimit times stores this
+// launch's abort marker, and every waiting wave that sees the marker stops,
+// so the kernel always ends (with a wrong result; correct launches never get
+// near the limit).
+constexpr u32 kSpinLimit = 1u << 20;
+constexpr u32 kOnePassMaxTiles = 512;
+
+template <typename Acc>
+__device__ __forceinline__ void publish(u64* st, u32 tile, u32 epoch, u32 flag, Acc v) {
+	if constexpr (sizeof(Acc) == 8) {
+		status_store(st + 2 * static_cast<u64>(tile), epoch, flag, static_cast<u32>(v));
+		status_store(st + 2 * static_cast<u64>(tile) + 1, epoch, flag, static_cast<u32>(v >> 32));
+	} else {
+		status_store(st + tile, epoch, flag, static_cast<u32>(v));
+	}
+}
+
+// Flag (0 = not ready) and value of tile `i`'s status.
+template <typename Acc>
+__device__ __forceinline__ u32 peek(const u64* st, u64 i, u32 epoch, Acc* v) {
+	if constexpr (sizeof(Acc) == 8) {
+		const u64 lo = status_load(st + 2 * i), hi = status_load(st + 2 * i + 1);
+		const u32 f = status_flag(lo, epoch);
+		if (f != status_flag(hi, epoch)) return 0u;  // halves of different updates
+		*v = static_cast<Acc>(static_cast<u32>(lo)) | (static_cast<Acc>(static_cast<u32>(hi)) << 32);
+		return f;
+	} else {
+		const u64 w = status_load(st + i);
+		*v = static_cast<Acc>(static_cast<u32>(w));
+		return status_flag(w, epoch);
+	}
+}
+
+// Exclusive prefix of tile t > 0, by the whole block: each round every
+// thread reads kLookPer statuses, so one round covers kLookSpan predecessors
+// (the distance back to the nearest inclusive prefix is about the number of
+// tiles in flight, so wide rounds keep the walk to one or two device-coherent
+// reads).  Offset d of a round is tile j - d; thread tid takes offsets
+// tid + kScanBlock * k (coalesced).  `abort` holds epoch + 1 once a wait of
+// this launch has given up.
+constexpr u32 kLookPer = 4;
+constexpr u32 kLookSpan = kScanBlock * kLookPer;
+
+template <typename Acc>
+struct LookShared {
+	u32 wmin[kScanBlock / kWave];
+	u32 wx[kScanBlock / kWave];
+	Acc wsum[kScanBlock / kWave];
+	bool stop;
+};
+
+template <typename Acc>
+__device__ Acc look_back(const u64* st, u32 t, u32 epoch, u64* abort, LookShared<Acc>& sh) {
+	constexpr int W = kScanBlock / kWave;
+	const u32 tid = threadIdx.x, lane = lane_id(), w = wave_id();
+	Acc excl = 0;
+	int64_t j = static_cast<int64_t>(t) - 1;
+	u32 spins = 0;
+	while (j >= 0) {
+		u32 f[kLookPer];
+		Acc v[kLookPer];
+		u32 my_min = kLookSpan;  // smallest offset holding an inclusive prefix
+#pragma unroll
+		for (u32 k = 0; k < kLookPer; ++k) {
+			const u32 d = tid + kScanBlock * k;
+			const int64_t idx = j - static_cast<int64_t>(d);
+			v[k] = 0;
+			f[k] = kFlagPrefix;  // before tile 0: never reached (tile 0 publishes P)
+			if (idx >= 0) f[k] = peek<Acc>(st, static_cast<u64>(idx), epoch, &v[k]);
+			if (f[k] == kFlagPrefix && d < my_min) my_min = d;
+		}
+		// block minimum of the prefix offsets
+		u32 m = my_min;
+#pragma unroll
+		for (int off = kWave / 2; off > 0; off >>= 1) {
+			const u32 o = static_cast<u32>(__shfl_xor(static_cast<int>(m), off, kWave));
+			m = o < m ? o : m;
+		}
+		if (lane == 0) sh.wmin[w] = m;
+		__syncthreads();
+		u32 first_p = sh.wmin[0];
+#pragma unroll
+		for (int i = 1; i < W; ++i) first_p = sh.wmin[i] < first_p ? sh.wmin[i] : first_p;
+		// any tile up to the prefix not published yet, and the sum up to it
+		bool missing = false;
+		Acc part = 0;
+#pragma unroll
+		for (u32 k = 0; k < kLookPer; ++k) {
+			const u32 d = tid + kScanBlock * k;
+			if (d <= first_p) {
+				missing |= f[k] == 0u;
+				part += v[k];
+			}
+		}
+		const u64 mb = ballot(missing);
+		const Acc wsum = wave_sum(part);
+		if (lane == 0) {
+			sh.wx[w] = mb != 0;
+			sh.wsum[w] = wsum;
+		}
+		__syncthreads();
+		bool any_missing = false;
+		Acc sum = 0;
+#pragma unroll
+		for (int i = 0; i < W; ++i) {
+			any_missing |= sh.wx[i] != 0;
+			sum += sh.wsum[i];
+		}
+		__syncthreads();  // the shared slots are reused by the next round
+		if (any_missing) {
+			// Thread 0 decides whether to stop (limit reached, or another wait
+			// of this launch gave up) and the block follows it.
+			++spins;
+			if (tid == 0) {
+				bool stop = status_load(abort) == static_cast<u64>(epoch) + 1u;
+				if (spins > kSpinLimit) {
+					__hip_atomic_store(abort, static_cast<u64>(epoch) + 1u, __ATOMIC_RELAXED,
+						__HIP_MEMORY_SCOPE_AGENT);
+					stop = true;
+				}
+				sh.stop = stop;
+			}
+			__syncthreads();
+			const bool stop = sh.stop;
+			__syncthreads();
+			if (stop) break;
+			__builtin_amdgcn_s_sleep(1);
+			continue;
+		}
+		excl += sum;
+		if (first_p < kLookSpan) break;
+		j -= kLookSpan;
+	}
+	return excl;
+}
+
+template <typename In, typename Acc, int IT>
+__global__ __launch_bounds__(kScanBlock) void blelloch_onepass(const In* __restrict__ in,
+	Acc* __restrict__ out, u64 n, u32 tiles, u64* __restrict__ status, u32* __restrict__ epoch_p,
+	Acc* __restrict__ chain) {
+	constexpr u32 kScanTile = kScanBlock * IT;
+	__shared__ Acc s_wsum[kScanBlock / kWave];
+	__shared__ Acc s_prefix;
+	__shared__ LookShared<Acc> s_look;
+	const u32 t = blockIdx.x;
+	const u32 tid = threadIdx.x;
+	const u32 epoch = *epoch_p & 0x3FFFFFFFu;
+	const u64 first = static_cast<u64>(t) * kScanTile + static_cast<u64>(tid) * IT;
+	const bool full = first + IT <= n;
+	In x[IT];
+	if (full) {
+		load_vec<In, IT>(in + first, x);
+	} else {
+#pragma unroll
+		for (int j = 0; j < IT; ++j) x[j] = first + j < n ? in[first + j] : In(0);
+	}
+	Acc sum = 0;
+#pragma unroll
+	for (int j = 0; j < IT; ++j) sum += static_cast<Acc>(x[j]);
+	Acc agg;
+	Acc run = block_exclusive_sum<kScanBlock>(sum, s_wsum, &agg);
+	if (t == 0) {
+		if (tid == 0) {
+			const Acc base = chain ? *chain : Acc(0);
+			publish<Acc>(status, 0, epoch, kFlagPrefix, base + agg);
+			s_prefix = base;
+		}
+	} else {
+		if (tid == 0) publish<Acc>(status, t, epoch, kFlagAggregate, agg);
+		const Acc excl = look_back<Acc>(status, t, epoch, status - 1, s_look);
+		if (tid == 0) {
+			publish<Acc>(status, t, epoch, kFlagPrefix, excl + agg);
+			s_prefix = excl;
+		}
+	}
+	__syncthreads();
+	const Acc prefix = s_prefix;
+	if (tid == 0 && t == tiles - 1) {
+		if (chain) *chain = prefix + agg;
+		*epoch_p = (epoch + 1u) & 0x3FFFFFFFu;  // every tile has read the epoch by now
+	}
+	run += prefix;
+	Acc y[IT];
+#pragma unroll
+	for (int j = 0; j < IT; ++j) {
+		y[j] = run;
+		run += static_cast<Acc>(x[j]);
+	}
+	if (full) {
+		store_vec<Acc, IT>(out + first, y);
+	} else {
+#pragma unroll
+		for (int j = 0; j < IT; ++j)
+			if (first + j < n) out[first + j] = y[j];
+	}
+}
+
 struct BlellochData {
-	Scratch tiles;  // per-tile sums (then carries) and segment flags
+	Scratch tiles;   // per-tile sums (then carries) and segment flags
+	Scratch status;  // single pass: status words, then the epoch word (zeroed once)
+	BlellochData() { status.zero_on_alloc = true; }
 };
 
 struct ScanLaunch {
@@ -438,6 +661,7 @@ struct ScanLaunch {
 	u32 nseg;
 	u32 items;  // items per thread: 16, or 8 / 32 for unsigned plain scans
 	void* chain;  // running total across chunks (plain scans) or nullptr
+	u64* status;  // single pass: epoch word, abort word, then the tiles' status words
 	CCLQueue* q;
 };
 
@@ -449,6 +673,15 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 	Acc* tsum = static_cast<Acc*>(a.tsum);
 	if constexpr (!SEG && (std::is_same<In, u32>::value || std::is_same<In, u64>::value) &&
 		sizeof(Acc) == OUTW) {
+		if (a.status && a.tiles <= kOnePassMaxTiles &&
+			((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0) {
+			Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_ONEPASS);
+			// status[0]: epoch; status[1]: abort marker; tiles' words from status[2]
+			blelloch_onepass<In, Acc, IT><<<a.tiles, kScanBlock, 0, st>>>(in,
+				static_cast<Acc*>(a.out), a.n, a.tiles, a.status + 2,
+				reinterpret_cast<u32*>(a.status), static_cast<Acc*>(a.chain));
+			return L.done(err);
+		}
 		if (((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0) {
 			{
 				Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_REDUCE);
@@ -548,6 +781,14 @@ bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, s
 	const size_t acc = ss == 8 ? 8 : 4;
 	const size_t sum_bytes = (tiles * acc + 255) / 256 * 256;
 	if (!bd->tiles.ensure(sum_bytes + tiles * 4, err)) return false;
+	// Single pass (plain scans): the epoch word, the abort word and (sum
+	// bytes / 4) status words per tile.  Growing reallocates zeroed memory:
+	// epoch 0, no status.
+	u64* status = nullptr;
+	if (!off && tiles <= kOnePassMaxTiles) {
+		if (!bd->status.ensure((2 + tiles * (acc / 4)) * sizeof(u64), err)) return false;
+		status = static_cast<u64*>(bd->status.ptr);
+	}
 	ScanLaunch a;
 	a.in = in;
 	a.out = out;
@@ -559,6 +800,7 @@ bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, s
 	a.nseg = nseg;
 	a.items = items;
 	a.chain = chain;
+	a.status = status;
 	a.q = cq_exec;
 	bool ok = false;
 	switch (scanner->elem_type) {
@@ -621,6 +863,7 @@ size_t blelloch_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, 
 	const size_t acc = clo_type_sizeof(scanner->sum_type) > 4 ? 8 : 4;
 	constexpr size_t kScanTile = static_cast<size_t>(kScanBlock) * kScanItems;
 	if (i == 1) return pad(kTilesRound) * acc + kTilesRound + (kTilesBlock / kWave) * (acc + 8);
+	if (i == 3) return (kScanBlock / kWave + 1) * acc;
 	return (pad(kScanTile) + 1 + 2 * (kScanBlock / kWave)) * acc + kScanTile / 8 + 4;
 }
 

@@ -32,8 +32,9 @@ def test_introspection(clops, ctx):
     assert clops._call(L.clo_sort_get_localmem_usage, s.ptr, 4, 256, 1000) > 0
     sc = clops.Scanner(ctx, np.uint32, np.uint64)
     assert L.clo_scan_get_sum_size(sc.ptr) == 8
-    assert [clops._call(L.clo_scan_get_kernel_name, sc.ptr, i) for i in range(3)] == [
-        b"blelloch_tile_reduce", b"blelloch_tiles_scan", b"blelloch_tile_scan"]
+    assert [clops._call(L.clo_scan_get_kernel_name, sc.ptr, i) for i in range(4)] == [
+        b"blelloch_tile_reduce", b"blelloch_tiles_scan", b"blelloch_tile_scan",
+        b"blelloch_onepass"]
 
 
 def test_profiling_records_kernels(clops, oracle, ctx):
