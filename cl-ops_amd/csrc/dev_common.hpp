@@ -85,6 +85,28 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T* s_wsum, T* total) {
 	return pre + inc - v;
 }
 
+// Lanes of the wave holding the same `bits`-bit digit as this lane, among
+// the lanes where `valid` is set: AND over the digit bits of
+// (ballot of bit b) matched against this lane's bit b.  The mismatches are
+// gathered as OR of (ballot ^ sext(bit)) on 32-bit halves, one
+// v_bitop3_b32 (0xF6: a | (b ^ c)) per half and bit on gfx950.
+// MAXBITS bounds the unrolled loop; bits <= MAXBITS.
+template <u32 MAXBITS>
+__device__ __forceinline__ u64 match_peers(u32 dig, u32 bits, bool valid) {
+	const u64 pv = ballot(valid);
+	u32 mlo = 0, mhi = 0;
+#pragma unroll
+	for (u32 bt = 0; bt < MAXBITS; ++bt) {
+		if (bt >= bits) break;
+		const u32 sb = static_cast<u32>(__builtin_amdgcn_sbfe(static_cast<int>(dig),
+			static_cast<int>(bt), 1));
+		const u64 bb = ballot(sb != 0u);
+		mlo = __builtin_amdgcn_bitop3_b32(mlo, static_cast<u32>(bb), sb, 0xF6);
+		mhi = __builtin_amdgcn_bitop3_b32(mhi, static_cast<u32>(bb >> 32), sb, 0xF6);
+	}
+	return pv & ~((static_cast<u64>(mhi) << 32) | mlo);
+}
+
 // ---- look-back status words -----------------------------------------------
 // 64-bit word: high half = (epoch << 2) | flag, low half = 32-bit value.
 // The epoch (30 bits) is unique per kernel launch, so words left over from

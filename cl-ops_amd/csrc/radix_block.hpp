@@ -34,12 +34,7 @@ __device__ __forceinline__ void block_rank(const u32 (&dig)[IPT], const bool (&v
 #pragma unroll
 	for (int j = 0; j < IPT; ++j) {
 		const u32 d = dig[j];
-		u64 peers = ballot(valid[j]);
-		for (u32 b = 0; b < bits; ++b) {
-			const bool bit = (d >> b) & 1u;
-			const u64 bb = ballot(bit);
-			peers &= bit ? bb : ~bb;
-		}
+		const u64 peers = match_peers<8>(d, bits, valid[j]);
 		const u32 below = count_below(peers);
 		u32 base = 0;
 		if (valid[j]) {

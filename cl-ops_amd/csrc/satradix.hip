@@ -482,14 +482,8 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 			const int j = j0 + jj;
 			const bool valid = (wbase + j * kWave + lane) < n;
 			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
-			u64 peers = ballot(valid);
-#pragma unroll
-			for (u32 bt = 0; bt < (FIXED_BITS ? FIXED_BITS : 8); ++bt) {
-				if (!FIXED_BITS && bt >= bits) break;
-				const bool bit = (dig >> bt) & 1u;
-				const u64 bb = ballot(bit);
-				peers &= bit ? bb : ~bb;
-			}
+			const u64 peers = match_peers<FIXED_BITS ? FIXED_BITS : 8>(dig,
+				FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits, valid);
 			const u32 below = count_below(peers);
 			const u32 leader = peers ? static_cast<u32>(__builtin_ctzll(peers)) : 0u;
 			got[jj] = 0;
