@@ -66,6 +66,12 @@ def cpu_baseline_kv(n_sample, threads):
         times.append(time.perf_counter() - t0)
     t = min(times)
     ok = bool(np.all(ko[:-1] <= ko[1:]))
+    del ko, vo
+    # Context: the same decomposition on one thread (a smaller sample), and a
+    # plain serial LSD radix sort (8-bit digits) of the same pairs.
+    n1 = min(n_sample, 1 << 22)
+    t1 = _best_time(lambda: oracle.satradix(keys[:n1], vals[:n1], bits=4, lws=256, threads=1), 1)
+    t_std = _best_time(lambda: oracle.stable_sort(keys[:n1], vals[:n1]), 1)
     return {
         "value": n_sample / t / 1e9,
         "unit": "Gkeys/s",
@@ -73,9 +79,25 @@ def cpu_baseline_kv(n_sample, threads):
         "kind": "port",
         "sample": "%d KV pairs (u64 key U(0xC3A,i), u32 index), oracle satradix restating "
                   "the reference decomposition (4-bit digits, 16 passes, 256-element blocks: "
-                  "split-sort, histogram, counter scan, scatter), best of 2, %.2f s, sorted=%s"
-                  % (n_sample, t, ok),
+                  "split-sort, histogram, counter scan, scatter) on %d threads of %s, best of "
+                  "2, %.2f s, sorted=%s" % (n_sample, threads, cpu_model(), t, ok),
+        "value_1_thread": n1 / t1 / 1e9,
+        "serial_lsd_1_thread": n1 / t_std / 1e9,
+        "sample_1_thread": "%d KV pairs, the same decomposition on 1 thread (%.2f s); a "
+                           "plain serial LSD radix sort, 8-bit digits (%.2f s)"
+                           % (n1, t1, t_std),
     }
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def traffic_from_profiles():
@@ -587,7 +609,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=28)
-    ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    ap.add_argument("--cpu-sample-log2", type=int, default=27)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
