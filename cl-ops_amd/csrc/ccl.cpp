@@ -69,7 +69,7 @@ CCLEvent* record_event(CCLQueue* q, const char* name, GError** err) {
 		return nullptr;
 	}
 	if (hip_failed(hipEventRecord(e->ev, q->stream), err, "hipEventRecord")) {
-		hipEventDestroy(e->ev);
+		(void)hipEventDestroy(e->ev);
 		delete e;
 		return nullptr;
 	}
@@ -96,14 +96,23 @@ Launch::Launch(CCLQueue* q_, const char* name_) : q(q_), name(name_), rec(nullpt
 		(k == 0 ? r.start : r.end) = ev;
 	}
 	if (!ok) return;
-	hipEventRecord(r.start, q->stream);
+	if (hipEventRecord(r.start, q->stream) != hipSuccess) {
+		// No record for this launch: the events go back to the pool and the
+		// launch's error check still runs in done().
+		q->prof_pool.push_back(r.start);
+		q->prof_pool.push_back(r.end);
+		return;
+	}
 	q->prof_recs.push_back(r);
 	rec = &q->prof_recs.back();
 }
 
 bool Launch::done(GError** err) {
 	hipError_t e = hipGetLastError();
-	if (q->prof && rec) hipEventRecord(q->prof_recs.back().end, q->stream);
+	if (q->prof && rec) {
+		const hipError_t re = hipEventRecord(q->prof_recs.back().end, q->stream);
+		if (e == hipSuccess) e = re;
+	}
 	if (e != hipSuccess) {
 		set_error(err, kQuarkCcl, static_cast<int>(e), "launch of %s: %s", name,
 			hipGetErrorString(e));
@@ -126,7 +135,7 @@ bool Scratch::ensure(size_t bytes, GError** err) {
 	if (hip_failed(dev_malloc(&p, want), err, "device scratch allocation"))
 		return false;
 	if (zero_on_alloc && hip_failed(hipMemset(p, 0, want), err, "hipMemset")) {
-		hipFree(p);
+		(void)hipFree(p);
 		return false;
 	}
 	ptr = p;
@@ -134,8 +143,10 @@ bool Scratch::ensure(size_t bytes, GError** err) {
 	return true;
 }
 
+// Teardown paths (release, destroy) have no error channel, as in the C API's
+// void destroy functions: their HIP return codes are dropped on purpose.
 void Scratch::release() {
-	if (ptr) hipFree(ptr);
+	if (ptr) (void)hipFree(ptr);
 	ptr = nullptr;
 	size = 0;
 }
@@ -304,19 +315,19 @@ cl_bool ccl_queue_finish(CCLQueue* cq, GError** err) {
 
 void ccl_queue_destroy(CCLQueue* cq) {
 	if (!cq) return;
-	hipSetDevice(cq->ctx->dev);
-	hipStreamSynchronize(cq->stream);
+	(void)hipSetDevice(cq->ctx->dev);
+	(void)hipStreamSynchronize(cq->stream);
 	for (CCLEvent* e : cq->events) {
-		hipEventDestroy(e->ev);
+		(void)hipEventDestroy(e->ev);
 		delete e;
 	}
 	for (auto& r : cq->prof_recs) {
-		hipEventDestroy(r.start);
-		hipEventDestroy(r.end);
+		(void)hipEventDestroy(r.start);
+		(void)hipEventDestroy(r.end);
 	}
-	for (hipEvent_t e : cq->prof_pool) hipEventDestroy(e);
+	for (hipEvent_t e : cq->prof_pool) (void)hipEventDestroy(e);
 	delete cq->capture_event;
-	if (cq->owns_stream) hipStreamDestroy(cq->stream);
+	if (cq->owns_stream) (void)hipStreamDestroy(cq->stream);
 	ccl_context_destroy(cq->ctx);
 	delete cq;
 }
@@ -345,7 +356,7 @@ CCLBuffer* ccl_buffer_new(CCLContext* ctx, cl_mem_flags flags, size_t size,
 	if (host_ptr && size &&
 		hip_failed(hipMemcpy(p, host_ptr, size, hipMemcpyHostToDevice), err,
 			"hipMemcpy H2D")) {
-		hipFree(p);
+		(void)hipFree(p);
 		return nullptr;
 	}
 	CCLBuffer* b = new ccl_buffer();
@@ -377,8 +388,8 @@ void ccl_buffer_ref(CCLBuffer* buf) {
 void ccl_buffer_destroy(CCLBuffer* buf) {
 	if (!buf || --buf->refc > 0) return;
 	if (buf->owns && buf->ptr) {
-		hipSetDevice(buf->ctx->dev);
-		hipFree(buf->ptr);
+		(void)hipSetDevice(buf->ctx->dev);
+		(void)hipFree(buf->ptr);
 	}
 	ccl_context_destroy(buf->ctx);
 	delete buf;
@@ -498,7 +509,9 @@ double ccl_event_elapsed_ms(CCLEvent* start, CCLEvent* end, GError** err) {
 
 void ccl_prof_reset(CCLQueue* cq) {
 	if (!cq) return;
-	hipStreamSynchronize(cq->stream);
+	// A void call: a failed synchronisation resurfaces in the queue's next
+	// checked call.
+	(void)hipStreamSynchronize(cq->stream);
 	for (auto& r : cq->prof_recs) {
 		cq->prof_pool.push_back(r.start);
 		cq->prof_pool.push_back(r.end);

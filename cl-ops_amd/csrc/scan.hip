@@ -31,7 +31,6 @@ namespace {
 
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;  // default items per thread (4096-element tiles)
-constexpr u32 kScanTileMax = kScanBlock * 32;
 constexpr int kTilesBlock = 1024;  // threads of the tile-sum scan
 
 __host__ __device__ constexpr u32 pad(u32 i) { return i + (i >> 5); }

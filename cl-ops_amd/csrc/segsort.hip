@@ -488,8 +488,18 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 	__syncthreads();
 }
 
+// Blocks per CU the register allocation aims for: three for the default
+// 32-bit keys-only shape (measured best), fewer where the shape's registers
+// and LDS cannot reach three.
 template <typename K, bool KV, int IPT>
-__global__ __launch_bounds__(kRB, 3) void seg_sort_persistent(SegArgs<K> a) {
+constexpr int seg_min_blocks() {
+	return (sizeof(K) == 4 && !KV && IPT <= 16) ? 3
+		: (IPT >= 32 || (KV && sizeof(K) == 8)) ? 1 : 2;
+}
+
+template <typename K, bool KV, int IPT>
+__global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_persistent(
+	SegArgs<K> a) {
 	__shared__ SegSmem<K, KV, IPT> s;
 	const u32 nl = a.meta->n_large;
 	const u32 nm = a.meta->n_medium;

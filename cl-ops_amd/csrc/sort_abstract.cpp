@@ -215,7 +215,6 @@ CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec
 			"buffers of numel elements (keys_out/vals_out both NULL or both set)");
 		return nullptr;
 	}
-	const size_t ks = clo_type_sizeof(sorter->elem_type);
 	uint32_t tile = sorter->seg_tile ? sorter->seg_tile : 16u;  // 16: measured best on MI355X
 	return clo_satradix_sort_segmented(sorter, cq_exec, keys_in, vals_in, keys_out, vals_out,
 		offsets, nseg, numel, tile, sorter->seg_grid, &sorter->seg_cache, err);
