@@ -425,10 +425,21 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 			s.base[tid] = ex;
 		}
 		__syncthreads();
+		// Chunks are software-pipelined: the loads of chunk c+1 are issued
+		// before chunk c is ranked and scattered, so their latency overlaps
+		// that work instead of stalling the block at the top of every chunk.
+		const u32 lane = lane_id();
+		const u32 w = wave_id();
+		K kn[IPT];
+		u32 vn[IPT];
+#pragma unroll
+		for (int jj = 0; jj < IPT; ++jj) {
+			const u32 e = w * kWave * IPT + jj * kWave + lane;
+			kn[jj] = e < len ? src[e] : K(0);
+			if (KV) vn[jj] = e < len ? vsrc[e] : 0u;
+		}
 		for (u32 c0 = 0; c0 < len; c0 += CAP) {
 			const u32 cnt = (len - c0) < CAP ? (len - c0) : CAP;
-			const u32 lane = lane_id();
-			const u32 w = wave_id();
 			K k[IPT];
 			u32 v[IPT];
 			u32 dig[IPT];
@@ -438,9 +449,19 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 			for (int jj = 0; jj < IPT; ++jj) {
 				const u32 e = w * kWave * IPT + jj * kWave + lane;
 				valid[jj] = e < cnt;
-				k[jj] = valid[jj] ? src[c0 + e] : K(0);
-				if (KV) v[jj] = valid[jj] ? vsrc[c0 + e] : 0u;
+				k[jj] = kn[jj];
+				if (KV) v[jj] = vn[jj];
 				dig[jj] = digit_of(k[jj], shift);
+			}
+			const u32 c1 = c0 + CAP;
+			if (c1 < len) {
+				const u32 cnt1 = (len - c1) < CAP ? (len - c1) : CAP;
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj) {
+					const u32 e = w * kWave * IPT + jj * kWave + lane;
+					kn[jj] = e < cnt1 ? src[c1 + e] : K(0);
+					if (KV) vn[jj] = e < cnt1 ? vsrc[c1 + e] : 0u;
+				}
 			}
 			block_rank<IPT>(dig, valid, pos, 8, s.rs);
 #pragma unroll
