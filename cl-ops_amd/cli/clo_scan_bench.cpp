@@ -7,8 +7,9 @@
 // sum no longer fits the sum type, "[Scan did not work]" on a mismatch,
 // "[Unverified]" with --no-check).
 //
-// Values come from the library's seeded generator (clo_util_generate, top
-// 7 bits of each 64-bit draw) instead of GLib's GRand.
+// Values are the reference's: one GRand seeded with the bench seed, value =
+// (ulong)(g_rand_double * 128) per element (grand.hpp), so a given seed
+// scans the same arrays as the reference benchmark.
 
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,7 @@
 
 #include "cl_ops.h"
 #include "cli_common.hpp"
+#include "grand.hpp"
 
 namespace {
 
@@ -94,23 +96,16 @@ int main(int argc, char** argv) {
 
 	const size_t max_elems = static_cast<size_t>(init_elems) << num_doub;
 	std::vector<unsigned char> host(bytes * max_elems), scanned(bytes_sum * max_elems);
-	std::vector<uint64_t> bits(max_elems);
-	CCLBuffer* gen = ccl_buffer_new(ctx, CL_MEM_READ_WRITE, max_elems * 8, nullptr, &err);
-	if (!gen) return cli::fail(err);
+	cli::GRand rng(static_cast<uint32_t>(seed));
 	std::vector<std::vector<double>> bench(num_doub, std::vector<double>(runs, 0.0));
 	const uint64_t max_sum = bytes_sum >= 8 ? ~0ull : ((1ull << (8 * bytes_sum)) - 1);
-	unsigned long long drawn = 0;
 	size_t num = init_elems;
 	for (unsigned N = 1; N <= num_doub; ++N, num *= 2) {
 		const char* scan_ok = "";
 		for (unsigned long long r = 0; r < runs; ++r) {
-			if (!clo_util_generate(cq_comm, gen, num, CLO_GEN_U64, seed, 0, drawn, &err) ||
-				!ccl_buffer_enqueue_read(gen, cq_comm, CL_TRUE, 0, num * 8, bits.data(), nullptr,
-					&err))
-				return cli::fail(err);
-			drawn += num;
+			// clo_scan_bench.c:219-224: 0..127, so any element type holds it.
 			for (size_t i = 0; i < num; ++i) {
-				const uint64_t v = bits[i] >> 57;  // 0..127, any element type holds it
+				const uint64_t v = static_cast<uint64_t>(rng.next_double() * 128);
 				memcpy(host.data() + bytes * i, &v, bytes);
 			}
 			ccl_prof_reset(cq_exec);
@@ -166,7 +161,6 @@ int main(int argc, char** argv) {
 		}
 		fclose(f);
 	}
-	ccl_buffer_destroy(gen);
 	clo_scan_destroy(scanner);
 	ccl_queue_destroy(cq_exec);
 	ccl_queue_destroy(cq_comm);

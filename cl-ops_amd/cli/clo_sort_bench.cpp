@@ -4,9 +4,9 @@
 // time of a run is the kernel time recorded on the profiling exec queue;
 // each size prints its Mkeys/s and whether the result came out sorted.
 //
-// Inputs are uniform random bits from the library's seeded generator
-// (clo_util_generate) instead of GLib's GRand ranges; floating-point types
-// take uniform values in [0, 1).
+// Inputs are the reference's: one GRand seeded with the bench seed, each
+// element drawn with clo_bench_rand (grand.hpp), so a given seed sorts the
+// same arrays as the reference benchmark.
 
 #include <cstdio>
 #include <cstring>
@@ -15,6 +15,7 @@
 
 #include "cl_ops.h"
 #include "cli_common.hpp"
+#include "grand.hpp"
 
 namespace {
 
@@ -58,19 +59,6 @@ bool in_order(CloType t, const unsigned char* a, const unsigned char* b) {
 	}
 }
 
-// One element of type t from 64 random bits.
-void from_bits(CloType t, uint64_t u, unsigned char* dst) {
-	if (t == CLO_FLOAT) {
-		const float f = static_cast<float>(u >> 40) * (1.0f / 16777216.0f);
-		memcpy(dst, &f, 4);
-	} else if (t == CLO_DOUBLE) {
-		const double d = static_cast<double>(u >> 11) * (1.0 / 9007199254740992.0);
-		memcpy(dst, &d, 8);
-	} else {
-		memcpy(dst, &u, clo_type_sizeof(t));  // little endian: the low bytes
-	}
-}
-
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -92,7 +80,9 @@ int main(int argc, char** argv) {
 	CloType et = clo_type_by_name(type.c_str(), &err);
 	if (err) return cli::fail(err);
 	if (et == CLO_HALF) {
-		fprintf(stderr, "Error: Type 'half' has no host representation in this benchmark.\n");
+		// The reference checks half results as unsigned shorts on the host
+		// (clo_bench.c:52) while the device compares half floats.
+		fprintf(stderr, "Error: Type 'half' is not supported by this benchmark.\n");
 		return CLO_ERROR_ARGS;
 	}
 	if (maxpo2 < 4 || maxpo2 > 31 || runs == 0) {
@@ -122,22 +112,14 @@ int main(int argc, char** argv) {
 
 	const size_t max_elems = static_cast<size_t>(1) << maxpo2;
 	std::vector<unsigned char> host(bytes * max_elems);
-	std::vector<uint64_t> bits(max_elems);
-	CCLBuffer* gen = ccl_buffer_new(ctx, CL_MEM_READ_WRITE, max_elems * 8, nullptr, &err);
-	if (!gen) return cli::fail(err);
+	cli::GRand rng(static_cast<uint32_t>(seed));
 	std::vector<std::vector<unsigned long>> bench(maxpo2, std::vector<unsigned long>(runs, 0));
-	unsigned long long drawn = 0;  // generator position: fresh values every run
 
 	for (unsigned N = 4; N <= maxpo2; ++N) {
 		const size_t num = static_cast<size_t>(1) << N;
 		bool sorted_ok = true;
 		for (unsigned long long r = 0; r < runs; ++r) {
-			if (!clo_util_generate(cq_comm, gen, num, CLO_GEN_U64, seed, 0, drawn, &err) ||
-				!ccl_buffer_enqueue_read(gen, cq_comm, CL_TRUE, 0, num * 8, bits.data(), nullptr,
-					&err))
-				return cli::fail(err);
-			drawn += num;
-			for (size_t i = 0; i < num; ++i) from_bits(et, bits[i], host.data() + bytes * i);
+			for (size_t i = 0; i < num; ++i) cli::bench_rand(rng, et, host.data() + bytes * i);
 			ccl_prof_reset(cq_exec);
 			if (!clo_sort_with_host_data(sorter, cq_exec, cq_comm, host.data(), host.data(), num,
 					static_cast<size_t>(lws), &err))
@@ -174,7 +156,6 @@ int main(int argc, char** argv) {
 		}
 		fclose(f);
 	}
-	ccl_buffer_destroy(gen);
 	clo_sort_destroy(sorter);
 	ccl_queue_destroy(cq_exec);
 	ccl_queue_destroy(cq_comm);
