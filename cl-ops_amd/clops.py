@@ -107,6 +107,7 @@ def lib():
         "ccl_prof_reset": (None, [vp]),
         "ccl_prof_get": (sz, [vp, cp, ctypes.POINTER(d), ep]),
         "ccl_prof_get_name": (cp, [vp, sz]),
+        "ccl_prof_set_filter": (None, [vp, cp]),
         # sort
         "clo_sort_new": (vp, [cp, cp, vp, ctypes.POINTER(i32), ctypes.POINTER(i32), cp, cp, cp,
                               ep]),
@@ -264,6 +265,11 @@ class Queue:
         tot = ctypes.c_double()
         n = _call(self.L.ccl_prof_get, self.ptr, prefix.encode(), ctypes.byref(tot))
         return n, tot.value
+
+    def prof_filter(self, prefix=""):
+        """Record events only around kernels whose name starts with prefix
+        ("" = every kernel)."""
+        self.L.ccl_prof_set_filter(self.ptr, prefix.encode())
 
     def prof_names(self):
         out, i = [], 0

@@ -81,6 +81,9 @@ CCLEvent* record_event(CCLQueue* q, const char* name, GError** err) {
 Launch::Launch(CCLQueue* q_, const char* name_) : q(q_), name(name_), rec(nullptr) {
 	(void)hipGetLastError();  // clear stale launch errors
 	if (!q->prof || q->capturing) return;
+	if (!q->prof_filter.empty() &&
+		strncmp(name, q->prof_filter.c_str(), q->prof_filter.size()) != 0)
+		return;
 	ccl_prof_rec r;
 	r.name = name;
 	bool ok = true;
@@ -506,6 +509,10 @@ double ccl_event_elapsed_ms(CCLEvent* start, CCLEvent* end, GError** err) {
 }
 
 /* ---------------------------- profiling --------------------------------- */
+
+void ccl_prof_set_filter(CCLQueue* cq, const char* prefix) {
+	if (cq) cq->prof_filter = prefix ? prefix : "";
+}
 
 void ccl_prof_reset(CCLQueue* cq) {
 	if (!cq) return;

@@ -39,6 +39,7 @@ struct ccl_queue {
 	std::vector<ccl_prof_rec> prof_recs;  // used records since last reset
 	std::vector<hipEvent_t> prof_pool;    // spare events for prof records
 	std::vector<std::string> prof_names;  // scratch for ccl_prof_get_name
+	std::string prof_filter;              // record only kernels with this name prefix
 };
 
 struct ccl_buffer {

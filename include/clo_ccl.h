@@ -142,6 +142,11 @@ size_t ccl_prof_get(CCLQueue* cq, const char* prefix, double* total_ms,
 	GError** err);
 /* i-th distinct kernel name recorded since the last reset, or NULL. */
 const char* ccl_prof_get_name(CCLQueue* cq, size_t i);
+/* Records events only for kernels whose name starts with `prefix` (NULL or
+ * "" = every kernel, the default), so that timing one kernel adds no event
+ * pairs around the others.  No counterpart in cf4ocl, whose profiling records
+ * every event of the queue. */
+void ccl_prof_set_filter(CCLQueue* cq, const char* prefix);
 
 /* Test hook: the next `count` device allocations made by the library fail
  * with hipErrorOutOfMemory (0 disables). */

@@ -48,6 +48,18 @@ def test_profiling_records_kernels(clops, oracle, ctx):
     cnt, ms = q.prof("satradix_downsweep")
     assert cnt == 8 and ms > 0
     assert "satradix_upsweep" in q.prof_names()
+    # With a name filter only the matching kernels are recorded.
+    q.prof_filter("satradix_downsweep")
+    q.prof_reset()
+    s.sort_device(q, b, None, n)
+    assert q.prof_names() == ["satradix_downsweep"]
+    cnt2, ms2 = q.prof("")
+    assert cnt2 == 8 and ms2 > 0
+    q.prof_filter("")
+    q.prof_reset()
+    s.sort_device(q, b, None, n)
+    assert "satradix_upsweep" in q.prof_names()
+    assert np.array_equal(b.read(q, np.uint64, n), np.sort(k))
 
 
 def test_alloc_failure_reported(clops, oracle, ctx, queue):
