@@ -244,11 +244,21 @@ def suite(ctx, cpu_threads=0):
     G, draws = 1 << 26, 16
     ob = clops.Buffer(ctx, G * draws * 4)
     for gen, ss in (("xorshift128", 16), ("mwc64x", 8)):
-        r = clops.Rng(ctx, gen, clops.SEED_DEV_GID, G, 0, q=q)
+        # Seeding (clo_rng_new) in both of the reference bench's modes, main
+        # seed 0: HOST_MT draws the seed words on the host and copies them in;
+        # DEV_GID derives them on the device (one launch).  Wall time to a
+        # finished queue.
+        init_ms = {}
+        for mode, key in ((clops.SEED_HOST_MT, "host_mt"), (clops.SEED_DEV_GID, "dev_gid")):
+            q.finish()
+            t0 = time.perf_counter()
+            r = clops.Rng(ctx, gen, mode, G, 0, q=q)
+            q.finish()
+            init_ms[key] = round((time.perf_counter() - t0) * 1e3, 3)
         ms = time_steps(q, lambda: r.fill(q, ob, draws), 10, 3)
         by = G * (2 * ss + 4 * draws)
         out.append({"config": "rng fill %s 2^26x16" % gen, "ms": ms, "GB/s": by / ms / 1e6,
-                    "hbm_frac": by / ms / 1e6 / HBM_PEAK_GBS})
+                    "hbm_frac": by / ms / 1e6 / HBM_PEAK_GBS, "seed_init_ms": init_ms})
     # Bitonic 2^24 u32.
     n = 1 << 24
     kin = clops.Buffer(ctx, n * 4)
