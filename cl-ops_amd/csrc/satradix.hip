@@ -15,7 +15,9 @@
 //                       passes are skipped) and the buffers of each pass
 //   satradix_reduce     per 256 tiles and quarter of the digits: sums of the
 //                       tiles' digit counts
-//   satradix_scan       (two kernels) tile counts -> global output offsets
+//   satradix_scan       (two kernels) tile counts -> global output offsets;
+//                       sorts of at most 256 tiles skip reduce and scan: each
+//                       downsweep block sums the tile counts itself
 //   satradix_downsweep  per tile: wave64 ballot ranking, LDS-staged scatter
 //                       to the offsets (contiguous runs per digit), and the
 //                       next executed pass's digit byte of every key
@@ -267,7 +269,10 @@ __global__ __launch_bounds__(kPlanThreads) void satradix_plan(SortPlan* __restri
 // Counter scan.  Tiles form groups of kScanTiles, handled by blocks of
 // kGroupThreads = 4 x 256 threads.
 constexpr int kGroupThreads = 4 * kRadixMax;
-constexpr u32 kQuarter = kScanTiles / 4;
+// Sorts of at most kScanTiles tiles (one group) launch no counter scan: each
+// downsweep block sums the tile counts itself, up to 256 KB of reads per
+// block, which measured cheaper than a one-block scan launch (2^14..2^20
+// u32 keys: 0.078-0.099 -> 0.059-0.094 ms per sort).
 
 // Blocks of the grouped counter kernels split the digits DSPLIT ways
 // (grid.y): a block handles kDigits digits of one group of kScanTiles tiles,
@@ -366,39 +371,6 @@ __global__ __launch_bounds__(kPartThreads) void satradix_scan_partials(
 	}
 }
 
-// The whole counter scan in one block when all tiles form one group
-// (tiles <= kScanTiles): counts[t][d] <- (#keys with a smaller digit) +
-// (#keys with digit d in tiles < t).
-__global__ __launch_bounds__(kGroupThreads) void satradix_scan_one(const SortPlan* __restrict__ plan,
-	u32 pass, u32* __restrict__ counts, u32 tiles) {
-	__shared__ u32 s_q[4][kRadixMax];
-	__shared__ u32 s_wsum[kGroupThreads / kWave];
-	if (!plan->pass[pass].active) return;
-	const u32 d = threadIdx.x % kRadixMax, q = threadIdx.x / kRadixMax;
-	const u32 t0 = q * kQuarter;
-	u32 c[kQuarter];
-	load_part(counts, t0, tiles, d, c);
-	u32 s = 0;
-#pragma unroll
-	for (u32 i = 0; i < kQuarter; ++i) s += c[i];
-	s_q[q][d] = s;
-	__syncthreads();
-	const u32 total = q == 0 ? s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d] : 0u;
-	// Threads are ordered (q, d), so the q == 0 threads are the first 256 and
-	// their exclusive sum is the digit base.
-	u32 run = block_exclusive_sum<kGroupThreads>(total, s_wsum, static_cast<u32*>(nullptr));
-	__shared__ u32 s_base[kRadixMax];
-	if (q == 0) s_base[d] = run;
-	__syncthreads();
-	run = s_base[d];
-	for (u32 qq = 0; qq < q; ++qq) run += s_q[qq][d];
-#pragma unroll
-	for (u32 i = 0; i < kQuarter; ++i) {
-		if (t0 + i < tiles) counts[static_cast<size_t>(t0 + i) * kRadixMax + d] = run;
-		run += c[i];
-	}
-}
-
 // Turns the tile counts into global output offsets, in place:
 // counts[t][d] <- (#keys with a smaller digit) + (#keys with digit d in tiles < t).
 template <u32 DSPLIT>
@@ -435,10 +407,13 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 // each element the number of equal-digit elements before it (stability).
 // FIXED_BITS = 8 compiles the digit-match loop for the default 8-bit digits;
 // 0 takes the digit width from `bits` (radix option).
+// fused_scan (sorts of at most kScanTiles tiles): `offsets` holds the
+// upsweep's tile counts and every block derives its own tile's offsets from
+// them, in place of a separate counter-scan launch.
 template <typename K, bool KV, int NT, int ITEMS, int FIXED_BITS>
 __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits_arg,
-	const u32* __restrict__ offsets, u8* __restrict__ ndig) {
+	const u32* __restrict__ offsets, u8* __restrict__ ndig, u32 fused_scan) {
 	const u32 bits = FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits_arg;
 	constexpr u32 kTile = NT * ITEMS;
 	constexpr int NW = NT / kWave;
@@ -463,7 +438,36 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const u32 shift = d.shift;
 	const K* __restrict__ src = b.k[d.src];
 	K* __restrict__ dst = b.k[d.dst];
-	const u32 goff = tid < kRadixMax ? offsets[static_cast<size_t>(tile) * kRadixMax + tid] : 0u;
+	u32 goff = 0;
+	if (fused_scan) {
+		// Offset of digit d = (#keys with a smaller digit) + (#keys with digit
+		// d in earlier tiles), from the counts of all tiles (<= kScanTiles
+		// rows of 1 KB, read row by row, 16 rows in flight).
+		u32 pre = 0, tot = 0;
+		if (tid < kRadixMax) {
+			u32 t2 = 0;
+			for (; t2 + 16 <= tiles; t2 += 16) {
+				u32 c[16];
+#pragma unroll
+				for (u32 j = 0; j < 16; ++j) c[j] = offsets[static_cast<size_t>(t2 + j) * kRadixMax + tid];
+#pragma unroll
+				for (u32 j = 0; j < 16; ++j) {
+					tot += c[j];
+					pre += (t2 + j < tile) ? c[j] : 0u;
+				}
+			}
+			for (; t2 < tiles; ++t2) {
+				const u32 c = offsets[static_cast<size_t>(t2) * kRadixMax + tid];
+				tot += c;
+				pre += (t2 < tile) ? c : 0u;
+			}
+		}
+		const u32 base = block_exclusive_sum<NT>(tid < kRadixMax ? tot : 0u, s_wsum,
+			static_cast<u32*>(nullptr));
+		goff = tid < kRadixMax ? base + pre : 0u;
+	} else if (tid < kRadixMax) {
+		goff = offsets[static_cast<size_t>(tile) * kRadixMax + tid];
+	}
 
 	// Load (coalesced: 64 consecutive elements per wave instruction).
 	const u32 wbase = tile_base + w * kWave * ITEMS;
@@ -701,11 +705,8 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 				counts, andor, 0, digits);
 			if (!L.done(err)) return false;
 		}
-		if (nblk == 1) {
-			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
-			satradix_scan_one<<<1, kGroupThreads, 0, st>>>(plan, p, counts, tiles);
-			if (!L.done(err)) return false;
-		} else {
+		const bool fused = nblk == 1;
+		if (!fused) {
 			{
 				Launch L(q, CLO_SORT_SATRADIX_KNAME_REDUCE);
 				satradix_reduce<kDigitSplit><<<dim3(nblk, kDigitSplit), kGroupThreads, 0, st>>>(plan, p,
@@ -728,13 +729,13 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 			if constexpr (NT == 256 && ITEMS == 32) {
 				if (bits != 8) {
 					satradix_downsweep<K, KV, NT, ITEMS, 0><<<grid, NT, 0, st>>>(b, plan, p, nn,
-						tiles, bits, counts, digits);
+						tiles, bits, counts, digits, fused ? 1u : 0u);
 					if (!L.done(err)) return false;
 					continue;
 				}
 			}
 			satradix_downsweep<K, KV, NT, ITEMS, 8><<<grid, NT, 0, st>>>(b, plan, p, nn, tiles,
-				bits, counts, digits);
+				bits, counts, digits, fused ? 1u : 0u);
 			if (!L.done(err)) return false;
 		}
 	}
