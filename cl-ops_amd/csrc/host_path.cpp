@@ -43,7 +43,9 @@ public:
 		}
 		std::lock_guard<std::mutex> call(call_mu_);
 		const size_t parts = static_cast<size_t>(nworkers_) + 1;
-		const size_t per = (bytes / parts + 63) & ~static_cast<size_t>(63);
+		// Shares of ceil(bytes / parts) rounded up to 64 bytes, so that the
+		// parts cover every byte.
+		const size_t per = ((bytes + parts - 1) / parts + 63) & ~static_cast<size_t>(63);
 		char* d = static_cast<char*>(dst);
 		const char* s = static_cast<const char*>(src);
 		{

@@ -15,6 +15,10 @@ pytestmark = pytest.mark.gpu
     (np.int64, np.int64, 3000017),
     (np.uint32, np.uint32, 1),
     (np.uint32, np.uint32, 0),
+    # One chunk of 2 MiB + 1 B in, 4 MiB + 2 B out: byte counts whose share
+    # per copy thread, floor(bytes / threads), is a multiple of 64 for 2, 4
+    # and 8 threads, so a split that rounds the share down drops the tail.
+    (np.uint8, np.uint16, 2097153),
 ])
 @pytest.mark.parametrize("two_queues", [False, True])
 def test_scan_host_chunks(clops, oracle, ctx, elem, sumt, n, two_queues):
@@ -36,6 +40,7 @@ def test_scan_host_chunks(clops, oracle, ctx, elem, sumt, n, two_queues):
     ("abitonic", np.uint64, 2500001),
     ("gselect", np.uint32, 3001),       # not in place: a second device buffer
     ("sbitonic", np.int8, 10000000),
+    ("satradix", np.uint8, 4194305),    # one chunk of 4 MiB + 1 B (see the scan case)
 ])
 @pytest.mark.parametrize("two_queues", [False, True])
 def test_sort_host_chunks(clops, oracle, ctx, algo, dtype, n, two_queues):
@@ -52,11 +57,11 @@ def test_sort_host_chunks(clops, oracle, ctx, algo, dtype, n, two_queues):
     assert np.array_equal(s.sort_host(x[: n // 3], q, qc), want(x[: n // 3]))
 
 
+@pytest.mark.parametrize("n", [3000017, 1048577])  # 1048577: values of 4 MiB + 4 B
 @pytest.mark.parametrize("two_queues", [False, True])
-def test_sort_kv_host_chunks(clops, oracle, ctx, two_queues):
+def test_sort_kv_host_chunks(clops, oracle, ctx, two_queues, n):
     q = clops.Queue(ctx)
     qc = clops.Queue(ctx) if two_queues else None
-    n = 3000017
     k = oracle.generate(oracle.GEN_U64_MASK, 7, n, param=(1 << 12) - 1)
     v = np.arange(n, dtype=np.uint32)
     s = clops.Sorter(ctx, "satradix", np.uint64)

@@ -147,3 +147,23 @@ def test_zipf_kv(clops, oracle, ctx, queue):
     ko, vo = s.sort_kv_host(k, v)
     rk, rv = oracle.stable_sort(k, v)
     assert np.array_equal(ko, rk) and np.array_equal(vo, rv)
+
+
+@pytest.mark.parametrize("opt", ["", "wg=512,ipt=16", "wg=256,ipt=16"])
+@pytest.mark.parametrize("delta", [-1, 0, 1])
+def test_counter_scan_boundary(clops, oracle, ctx, opt, delta):
+    """Sorts of at most 256 tiles have each downsweep block sum the tile
+    counts itself; one more tile switches to the grouped reduce + scan.  Both
+    sides of the switch, for the tile sizes of each shape (8192 or 4096 keys),
+    keys-only and KV."""
+    tile = 4096 if opt == "wg=256,ipt=16" else 8192
+    n = 256 * tile + delta
+    s = clops.Sorter(ctx, "satradix", np.uint64, options=opt or None)
+    k = _keys(oracle, np.uint64, n, 0xB0 + delta)
+    v = np.arange(n, dtype=np.uint32)
+    ko, vo = s.sort_kv_host(k, v)
+    rk, rv = oracle.stable_sort(k, v)
+    assert np.array_equal(ko, rk) and np.array_equal(vo, rv)
+    s32 = clops.Sorter(ctx, "satradix", np.uint32, options=opt or None)
+    k32 = k.astype(np.uint32)
+    assert np.array_equal(s32.sort_host(k32), np.sort(k32))
