@@ -78,6 +78,40 @@ def test_segmented_sort_tiles_out_of_place(clops, oracle, ctx, queue, tile):
     assert np.array_equal(kb.read(queue, np.uint32, n), k)  # input untouched
 
 
+@pytest.mark.parametrize("tile", [8, 16, 32])
+@pytest.mark.parametrize("dtype,kv", [(np.uint32, False), (np.uint32, True), (np.uint64, True)])
+def test_segmented_sort_class_boundaries(clops, oracle, ctx, queue, tile, dtype, kv):
+    """Segment lengths on both sides of each work-class boundary: chunked
+    small segments (<= W), one-tile medium segments (<= CAP) and multi-chunk
+    large ones, with CAP = 256 * tile items and W = CAP / 2; many equal keys,
+    so stability is checked through the payloads."""
+    if dtype == np.uint64 and tile == 32:
+        pytest.skip("64-bit keys take tiles of 8 or 16 items")
+    cap = 256 * tile
+    w = cap // 2
+    edges = [0, 1, 2, w - 1, w, w + 1, cap - 1, cap, cap + 1, 2 * cap - 1, 2 * cap,
+             2 * cap + 1, 3 * cap + 7]
+    rs = np.random.default_rng(tile)
+    lens = []
+    for e in edges:  # each boundary length between short random segments
+        lens += [int(rs.integers(0, 40)), e, int(rs.integers(0, 40))]
+    lens = np.array(lens, dtype=np.uint32)
+    off = _offsets(lens)
+    nseg, n = lens.size, int(lens.sum(dtype=np.uint64))
+    k = (oracle.generate(oracle.GEN_U64, 0x5E + tile, n) % np.uint64(1000)).astype(dtype)
+    v = np.arange(n, dtype=np.uint32) if kv else None
+    s = clops.Sorter(ctx, "satradix", dtype, options="segtile=%d" % tile)
+    kb = clops.Buffer.from_array(ctx, k)
+    vb = clops.Buffer.from_array(ctx, v) if kv else None
+    ob = clops.Buffer.from_array(ctx, off)
+    s.sort_segmented_device(queue, kb, ob, nseg, n, vin=vb)
+    queue.finish()
+    rk, rv = _ref_sort(oracle, k, v, off, n)
+    assert np.array_equal(kb.read(queue, dtype, n), rk)
+    if kv:
+        assert np.array_equal(vb.read(queue, np.uint32, n), rv)
+
+
 def _ref_segscan(oracle, a, off, n, sumt):
     out = np.empty(n, dtype=sumt)
     starts = np.append(off, n)
