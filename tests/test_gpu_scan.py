@@ -12,13 +12,23 @@ def _vals(oracle, dtype, n, seed):
     return u.astype(np.dtype("u%d" % np.dtype(dtype).itemsize)).view(dtype)
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 255, 4095, 4096, 4097, 8193, 100000, 1 << 20])
+# 2^21 +- 1: both sides of the switch from the single-pass scan (<= 512 tiles
+# of 4096) to reduce-then-scan.
+@pytest.mark.parametrize("n", [0, 1, 2, 255, 4095, 4096, 4097, 8193, 100000, 1 << 20,
+                               (1 << 21) - 1, 1 << 21, (1 << 21) + 1])
 @pytest.mark.parametrize("elem,sumt", [(np.uint32, np.uint32), (np.uint32, np.uint64),
                                        (np.uint64, np.uint64)])
-def test_scan_sizes(clops, oracle, ctx, n, elem, sumt):
+def test_scan_sizes(clops, oracle, ctx, queue, n, elem, sumt):
     sc = clops.Scanner(ctx, elem, sumt)
     a = _vals(oracle, elem, n, 0x5C + n)
-    assert np.array_equal(sc.scan_host(a), oracle.scan(a, sumt))
+    want = oracle.scan(a, sumt)
+    assert np.array_equal(sc.scan_host(a), want)
+    if n:  # device data: one call over the whole array (no host chunking)
+        bi = clops.Buffer.from_array(ctx, a)
+        bo = clops.Buffer(ctx, n * np.dtype(sumt).itemsize)
+        sc.scan_device(queue, bi, bo, n)
+        queue.finish()
+        assert np.array_equal(bo.read(queue, sumt, n), want)
 
 
 @pytest.mark.parametrize("elem", INT_TYPES)
