@@ -21,6 +21,16 @@ def test_bitonic_sizes(clops, oracle, ctx, algo, n):
 
 
 @pytest.mark.parametrize("algo", ["sbitonic", "abitonic"])
+@pytest.mark.parametrize("n", [2047, 2048, 4095, 4096, 4097, 1 << 15, (1 << 17) + 3])
+def test_bitonic_sizes_u64(clops, oracle, ctx, algo, n):
+    """8-byte keys run 2^12-key tiles: sizes around the tile and the
+    register/LDS step split, power-of-two (check-free) and padded networks."""
+    s = clops.Sorter(ctx, algo, np.uint64)
+    k = _keys(oracle, np.uint64, n, 0xB3 + n)
+    assert np.array_equal(s.sort_host(k), np.sort(k))
+
+
+@pytest.mark.parametrize("algo", ["sbitonic", "abitonic"])
 @pytest.mark.parametrize("dtype", INT_TYPES)
 @pytest.mark.parametrize("desc", [False, True])
 def test_bitonic_types_order(clops, oracle, ctx, algo, dtype, desc):
