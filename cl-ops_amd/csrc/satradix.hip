@@ -15,9 +15,11 @@
 //                       passes are skipped) and the buffers of each pass
 //   satradix_reduce     per 256 tiles and quarter of the digits: sums of the
 //                       tiles' digit counts
-//   satradix_scan       (two kernels) tile counts -> global output offsets;
-//                       sorts of at most 256 tiles skip reduce and scan: each
-//                       downsweep block sums the tile counts itself
+//   satradix_scan       per 256 tiles and quarter of the digits: tile counts
+//                       -> global output offsets (each block derives its
+//                       bases from all group sums); sorts of at most 256
+//                       tiles skip reduce and scan: each downsweep block sums
+//                       the tile counts itself
 //   satradix_downsweep  per tile: wave64 ballot ranking, LDS-staged scatter
 //                       to the offsets (contiguous runs per digit), and the
 //                       next executed pass's digit byte of every key
@@ -296,8 +298,7 @@ __device__ __forceinline__ void load_part(const u32* __restrict__ counts, u32 t0
 		c[i] = (t0 + i < tiles) ? counts[static_cast<size_t>(t0 + i) * kRadixMax + d] : 0u;
 }
 
-// partial[d][g] = sum of counts[t][d] over the tiles of group g (digit-major,
-// so the next kernel scans it as one flat array).
+// partial[g][d] = sum of counts[t][d] over the tiles of group g.
 template <u32 DSPLIT>
 __global__ __launch_bounds__(kGroupThreads) void satradix_reduce(const SortPlan* __restrict__ plan,
 	u32 pass, const u32* __restrict__ counts, u32 tiles, u32* __restrict__ partial) {
@@ -317,79 +318,76 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_reduce(const SortPlan*
 		u32 tot = 0;
 #pragma unroll
 		for (u32 qq = 0; qq < S::kParts; ++qq) tot += s_q[qq][dl];
-		partial[d * gridDim.x + blockIdx.x] = tot;
-	}
-}
-
-// One block: flat exclusive scan of the digit-major partials, so that
-// partial[d][g] <- (#keys with a smaller digit) + (#keys with digit d in
-// groups < g), the offset of group g's first key of digit d.  Rounds of
-// kPartThreads * kPartItems values are staged in LDS with coalesced loads,
-// then each thread scans kPartItems consecutive values.
-constexpr int kPartThreads = 1024;
-constexpr u32 kPartItems = 16;
-constexpr u32 kPartRound = kPartThreads * kPartItems;
-
-__host__ __device__ constexpr u32 ppad(u32 i) { return i + (i >> 5); }
-
-__global__ __launch_bounds__(kPartThreads) void satradix_scan_partials(
-	const SortPlan* __restrict__ plan, u32 pass, u32* __restrict__ partial, u32 total) {
-	__shared__ u32 s_v[ppad(kPartRound)];
-	__shared__ u32 s_wsum[kPartThreads / kWave];
-	if (!plan->pass[pass].active) return;
-	const u32 tid = threadIdx.x;
-	u32 carry = 0;
-	for (u32 base = 0; base < total; base += kPartRound) {
-#pragma unroll
-		for (u32 j = 0; j < kPartItems; ++j) {
-			const u32 i = j * kPartThreads + tid;
-			s_v[ppad(i)] = base + i < total ? partial[base + i] : 0u;
-		}
-		__syncthreads();
-		u32 v[kPartItems];
-		u32 sum = 0;
-#pragma unroll
-		for (u32 k = 0; k < kPartItems; ++k) {
-			v[k] = s_v[ppad(tid * kPartItems + k)];
-			sum += v[k];
-		}
-		u32 round_total;
-		u32 run = carry + block_exclusive_sum<kPartThreads>(sum, s_wsum, &round_total);
-#pragma unroll
-		for (u32 k = 0; k < kPartItems; ++k) {
-			s_v[ppad(tid * kPartItems + k)] = run;
-			run += v[k];
-		}
-		__syncthreads();
-#pragma unroll
-		for (u32 j = 0; j < kPartItems; ++j) {
-			const u32 i = j * kPartThreads + tid;
-			if (base + i < total) partial[base + i] = s_v[ppad(i)];
-		}
-		carry += round_total;
-		__syncthreads();
+		partial[blockIdx.x * kRadixMax + d] = tot;
 	}
 }
 
 // Turns the tile counts into global output offsets, in place:
 // counts[t][d] <- (#keys with a smaller digit) + (#keys with digit d in tiles < t).
+// Each block first derives, from the group sums of all groups, its digits'
+// bases: (#keys with a smaller digit) + (#keys with digit d in earlier
+// groups); no separate launch scans the group sums.
 template <u32 DSPLIT>
 __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* __restrict__ plan,
 	u32 pass, u32* __restrict__ counts, u32 tiles, const u32* __restrict__ partial) {
 	using S = GroupShape<DSPLIT>;
+	constexpr u32 kGQ = kGroupThreads / kRadixMax;  // threads per digit over the group sums
 	__shared__ u32 s_q[S::kParts][S::kDigits];
+	__shared__ u32 s_tot[kGQ][kRadixMax];
+	__shared__ u32 s_pre[kGQ][kRadixMax];
+	__shared__ u32 s_base[kRadixMax];
+	__shared__ u32 s_wsum[kGroupThreads / kWave];
 	if (!plan->pass[pass].active) return;
-	const u32 dl = threadIdx.x % S::kDigits, q = threadIdx.x / S::kDigits;
+	const u32 tid = threadIdx.x;
+	const u32 dl = tid % S::kDigits, q = tid / S::kDigits;
 	const u32 d = blockIdx.y * S::kDigits + dl;
 	const u32 t0 = blockIdx.x * kScanTiles + q * S::kPer;
 	u32 c[S::kPer];
 	load_part(counts, t0, tiles, d, c);
+	{
+		// Digit totals over all groups, and the sums over groups before this
+		// one; thread (gq, dd) takes every kGQ-th group, 8 loads in flight.
+		const u32 g = blockIdx.x, groups = gridDim.x;
+		const u32 dd = tid % kRadixMax, gq = tid / kRadixMax;
+		u32 tot = 0, pre = 0, g2 = gq;
+		for (; g2 + 7 * kGQ < groups; g2 += 8 * kGQ) {
+			u32 v[8];
+#pragma unroll
+			for (u32 j = 0; j < 8; ++j) v[j] = partial[(g2 + j * kGQ) * kRadixMax + dd];
+#pragma unroll
+			for (u32 j = 0; j < 8; ++j) {
+				tot += v[j];
+				pre += (g2 + j * kGQ < g) ? v[j] : 0u;
+			}
+		}
+		for (; g2 < groups; g2 += kGQ) {
+			const u32 v = partial[g2 * kRadixMax + dd];
+			tot += v;
+			pre += (g2 < g) ? v : 0u;
+		}
+		s_tot[gq][dd] = tot;
+		s_pre[gq][dd] = pre;
+	}
+	__syncthreads();
+	{
+		u32 tsum = 0, psum = 0;
+		if (tid < kRadixMax) {
+#pragma unroll
+			for (u32 k = 0; k < kGQ; ++k) {
+				tsum += s_tot[k][tid];
+				psum += s_pre[k][tid];
+			}
+		}
+		const u32 ex = block_exclusive_sum<kGroupThreads>(tsum, s_wsum,
+			static_cast<u32*>(nullptr));
+		if (tid < kRadixMax) s_base[tid] = ex + psum;
+	}
 	u32 s = 0;
 #pragma unroll
 	for (u32 i = 0; i < S::kPer; ++i) s += c[i];
 	s_q[q][dl] = s;
 	__syncthreads();
-	u32 run = partial[d * gridDim.x + blockIdx.x];
+	u32 run = s_base[d];
 	for (u32 qq = 0; qq < q; ++qq) run += s_q[qq][dl];
 #pragma unroll
 	for (u32 i = 0; i < S::kPer; ++i) {
@@ -713,12 +711,6 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 					counts, tiles, partial);
 				if (!L.done(err)) return false;
 			}
-			{
-				Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
-				satradix_scan_partials<<<1, kPartThreads, 0, st>>>(plan, p, partial,
-					nblk * kRadixMax);
-				if (!L.done(err)) return false;
-			}
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
 			satradix_scan<kDigitSplit><<<dim3(nblk, kDigitSplit), kGroupThreads, 0, st>>>(plan, p, counts,
 				tiles, partial);
@@ -949,7 +941,7 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	case CLO_SORT_SATRADIX_KIDX_REDUCE:
 		return 4 * kRadixMax * 4;
 	case CLO_SORT_SATRADIX_KIDX_SCAN:
-		return (ppad(kPartRound) + kPartThreads / kWave) * 4;
+		return (3 * kGroupThreads + kRadixMax + kGroupThreads / kWave) * 4;
 	case CLO_SORT_SATRADIX_KIDX_DOWNSWEEP:
 		return nt * ipt * (ks > 4 ? ks : 4) + (nt / kWave) * kRadixMax * 4 +
 			2 * kRadixMax * 4 + (nt / kWave) * 4;
