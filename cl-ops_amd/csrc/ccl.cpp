@@ -137,7 +137,12 @@ bool Scratch::ensure(size_t bytes, GError** err) {
 	void* p = nullptr;
 	if (hip_failed(dev_malloc(&p, want), err, "device scratch allocation"))
 		return false;
-	if (zero_on_alloc && hip_failed(hipMemset(p, 0, want), err, "hipMemset")) {
+	// hipMemset runs on the null stream, which the library's non-blocking
+	// queues do not wait for, and may return before the fill is done: wait
+	// for it here, or a kernel on a queue could read the memory unzeroed.
+	if (zero_on_alloc &&
+		(hip_failed(hipMemset(p, 0, want), err, "hipMemset") ||
+			hip_failed(hipStreamSynchronize(nullptr), err, "hipStreamSynchronize"))) {
 		(void)hipFree(p);
 		return false;
 	}
@@ -356,9 +361,12 @@ CCLBuffer* ccl_buffer_new(CCLContext* ctx, cl_mem_flags flags, size_t size,
 	if (!use_device(ctx, err)) return nullptr;
 	void* p = nullptr;
 	if (hip_failed(dev_malloc(&p, size), err, "hipMalloc")) return nullptr;
+	// From pageable memory hipMemcpy may return before its DMA lands, and the
+	// queues (non-blocking streams) do not wait for the null stream: wait.
 	if (host_ptr && size &&
-		hip_failed(hipMemcpy(p, host_ptr, size, hipMemcpyHostToDevice), err,
-			"hipMemcpy H2D")) {
+		(hip_failed(hipMemcpy(p, host_ptr, size, hipMemcpyHostToDevice), err,
+			"hipMemcpy H2D") ||
+			hip_failed(hipStreamSynchronize(nullptr), err, "hipStreamSynchronize"))) {
 		(void)hipFree(p);
 		return nullptr;
 	}

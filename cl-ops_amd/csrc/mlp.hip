@@ -332,7 +332,9 @@ cl_bool clo_mlp_set_params(CloMlp* m, const float* in, GError** err) {
 	const size_t bytes = m->L.count * 4;
 	if (hip_failed(hipMemcpy(m->w, in, bytes, hipMemcpyHostToDevice), err, "hipMemcpy H2D") ||
 		hip_failed(hipMemset(m->m, 0, bytes), err, "hipMemset") ||
-		hip_failed(hipMemset(m->v, 0, bytes), err, "hipMemset"))
+		hip_failed(hipMemset(m->v, 0, bytes), err, "hipMemset") ||
+		// the fills run on the null stream, which queue streams do not wait for
+		hip_failed(hipStreamSynchronize(nullptr), err, "hipStreamSynchronize"))
 		return CL_FALSE;
 	m->step = 0;
 	return CL_TRUE;

@@ -90,7 +90,10 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 	constexpr int NW = NT / kWave;
 	static_assert(ITEMS % 4 == 0, "digit bytes are read as words");
 	static_assert(NT >= kRadixMax, "one thread per digit bin");
-	__shared__ u32 s_h[NW][kRadixMax];
+	// Two copies of each wave's histogram, one per lane parity: lanes that
+	// hold the same digit collide on one address half as often (the later
+	// passes' counting is bound by these LDS atomics; 0.129 -> 0.123 ms).
+	__shared__ u32 s_h[NW][2 * kRadixMax];
 	__shared__ K s_red[2 * NW];
 	u32 shift = 0;
 	u32 dsrc = 0;
@@ -107,8 +110,9 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 	const u32 tid = threadIdx.x;
 	const u32 lane = lane_id();
 	const u32 w = wave_id();
-	for (u32 i = lane; i < kRadixMax; i += kWave) s_h[w][i] = 0;
+	for (u32 i = lane; i < 2 * kRadixMax; i += kWave) s_h[w][i] = 0;
 	__syncthreads();
+	const u32 par = lane & 1u;
 	if (dsrc) {
 		const u32 base = tile * kTile;
 		if (base + kTile <= n) {
@@ -120,16 +124,16 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 #pragma unroll
 			for (int j = 0; j < ITEMS / 4; ++j) {
 #pragma unroll
-				for (int q = 0; q < 4; ++q) atomicAdd(&s_h[w][(x[j] >> (8 * q)) & 0xFFu], 1u);
+				for (int q = 0; q < 4; ++q) atomicAdd(&s_h[w][((x[j] >> (8 * q)) & 0xFFu) * 2 + par], 1u);
 			}
 		} else {
-			for (u32 i = base + tid; i < n; i += NT) atomicAdd(&s_h[w][digits[i]], 1u);
+			for (u32 i = base + tid; i < n; i += NT) atomicAdd(&s_h[w][digits[i] * 2u + par], 1u);
 		}
 		__syncthreads();
 		if (tid < kRadixMax) {
 			u32 c = 0;
 #pragma unroll
-			for (int ww = 0; ww < NW; ++ww) c += s_h[ww][tid];
+			for (int ww = 0; ww < NW; ++ww) c += s_h[ww][2 * tid] + s_h[ww][2 * tid + 1];
 			counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
 		}
 		return;
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 #pragma unroll
 	for (int j = 0; j < ITEMS; ++j) {
 		if (wbase + j * kWave + lane < n) {
-			atomicAdd(&s_h[w][static_cast<u32>(k[j] >> shift) & mask], 1u);
+			atomicAdd(&s_h[w][(static_cast<u32>(k[j] >> shift) & mask) * 2 + par], 1u);
 			ka &= k[j];
 			ko |= k[j];
 		}
@@ -166,7 +170,7 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 	if (tid < kRadixMax) {
 		u32 c = 0;
 #pragma unroll
-		for (int ww = 0; ww < NW; ++ww) c += s_h[ww][tid];
+		for (int ww = 0; ww < NW; ++ww) c += s_h[ww][2 * tid] + s_h[ww][2 * tid + 1];
 		counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
 	}
 	if (first && tid == 0) {
@@ -935,7 +939,7 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	const size_t nt = kShapes[sh].nt, ipt = kShapes[sh].ipt;
 	switch (i) {
 	case CLO_SORT_SATRADIX_KIDX_UPSWEEP:
-		return (nt / kWave) * kRadixMax * 4 + 2 * (nt / kWave) * ks;
+		return (nt / kWave) * 2 * kRadixMax * 4 + 2 * (nt / kWave) * ks;
 	case CLO_SORT_SATRADIX_KIDX_PLAN:
 		return 2 * (kPlanThreads / kWave) * ks;
 	case CLO_SORT_SATRADIX_KIDX_REDUCE:
