@@ -342,10 +342,16 @@ __global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_reg(
 // Keys per thread held in registers by the tile kernels: 0 runs the plain
 // LDS kernel (tiles of 2^11), 16 or 32 the register-tail kernel.
 constexpr int kDefaultRegKeys = 16;
+// Global steps fused per launch when the maxps option is not given: 5 (2^5
+// keys per thread) for keys of up to 4 bytes, 4 for 8-byte keys (measured on
+// 2^24 keys: u32 1.34 -> 1.29 ms with 5; 6 steps, and 5 for u64, were slower).
+// The option keeps the reference's range, 1..4.
+constexpr u32 kAutoMaxSteps = 5;
 
 struct BitonicData {
 	bool advanced;
 	u32 minps = 1, maxps = 4, maxsfs = 0xFFFFFFFFu;
+	bool maxps_set = false;  // without the option, see kAutoMaxSteps
 	int regkeys = -1;  // keys per thread of the tile kernels; -1: the default
 	u32 last_sweeps = 0;
 };
@@ -376,6 +382,7 @@ void group_kernel(u32 grid, hipStream_t st, T* d, u32 n, u32 npo2, u32 s, u32 t,
 	case 2: bitonic_group<T, 2, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
 	case 3: bitonic_group<T, 3, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
 	case 4: bitonic_group<T, 4, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
+	case 5: bitonic_group<T, 5, FULL><<<grid, kBitBlock, 0, st>>>(d, n, npo2, s, t, xm); break;
 	}
 }
 
@@ -452,15 +459,16 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 			return false;
 		++sweeps;
 	}
-	static const char* gname[5] = {"", CLO_SORT_ABITONIC_KNAME_ANY,
+	static const char* gname[kAutoMaxSteps + 1] = {"", CLO_SORT_ABITONIC_KNAME_ANY,
 		CLO_SORT_ABITONIC_KNAME_PRIV_2S4V, CLO_SORT_ABITONIC_KNAME_PRIV_3S8V,
-		CLO_SORT_ABITONIC_KNAME_PRIV_4S16V};
+		CLO_SORT_ABITONIC_KNAME_PRIV_4S16V, CLO_SORT_ABITONIC_KNAME_PRIV_5S32V};
+	const u32 qmax = bd->maxps_set ? bd->maxps : (sizeof(T) <= 4 ? kAutoMaxSteps : 4u);
 	for (u32 s = k + 1; s <= S; ++s) {
 		u32 t = s;
 		const u32 t_stop = k;  // steps t > t_stop are global
 		while (t > t_stop) {
 			u32 rem = t - t_stop;
-			u32 Qn = rem < bd->maxps ? rem : bd->maxps;
+			u32 Qn = rem < qmax ? rem : qmax;
 			if (Qn < bd->minps && rem >= bd->minps) Qn = bd->minps;
 			if (Qn > t) Qn = t;
 			if (!launch_group<T>(q, d, n, npo2, s, t, Qn, xm, gname[Qn], err)) return false;
@@ -536,6 +544,7 @@ const char* abitonic_init(CloSort* sorter, const char* options, GError** err) {
 				return nullptr;
 			}
 			bd->maxps = static_cast<u32>(v);
+			bd->maxps_set = true;
 		} else if (o.key == "maxsfs") {
 			bd->maxsfs = static_cast<u32>(v);
 		} else if (o.key == "regkeys") {
@@ -588,7 +597,7 @@ size_t abitonic_localmem(CloSort* sorter, cl_uint i, size_t, size_t, GError** er
 		return 0;
 	}
 	const size_t es = clo_type_sizeof(sorter->elem_type);
-	if (i < 4) return 0;  // register-only kernels
+	if (i < 4 || i > 5) return 0;  // register-only kernels
 	const BitonicData* bd = static_cast<const BitonicData*>(sorter->data);
 	const u32 k = tile_log(bd, es >= 8 ? TileMax<u64>::kLog : TileMax<u32>::kLog);
 	if (k < 5 || regkeys_of(bd) == 0) return kPlainTileMax * es;

@@ -16,17 +16,18 @@
 extern "C" {
 #endif
 
-#define CLO_SORT_ABITONIC_NUM_KERNELS 6
+#define CLO_SORT_ABITONIC_NUM_KERNELS 7
 #define CLO_SORT_ABITONIC_KNAME_ANY "abit_any"
 #define CLO_SORT_ABITONIC_KNAME_PRIV_2S4V "abit_priv_2s4v"
 #define CLO_SORT_ABITONIC_KNAME_PRIV_3S8V "abit_priv_3s8v"
 #define CLO_SORT_ABITONIC_KNAME_PRIV_4S16V "abit_priv_4s16v"
 #define CLO_SORT_ABITONIC_KNAME_LOCAL "abit_local"
 #define CLO_SORT_ABITONIC_KNAME_TILESORT "abit_tilesort"
+#define CLO_SORT_ABITONIC_KNAME_PRIV_5S32V "abit_priv_5s32v"
 #define CLO_SORT_ABITONIC_KERNELNAMES { CLO_SORT_ABITONIC_KNAME_ANY, \
 	CLO_SORT_ABITONIC_KNAME_PRIV_2S4V, CLO_SORT_ABITONIC_KNAME_PRIV_3S8V, \
 	CLO_SORT_ABITONIC_KNAME_PRIV_4S16V, CLO_SORT_ABITONIC_KNAME_LOCAL, \
-	CLO_SORT_ABITONIC_KNAME_TILESORT }
+	CLO_SORT_ABITONIC_KNAME_TILESORT, CLO_SORT_ABITONIC_KNAME_PRIV_5S32V }
 
 extern const CloSortImplDef clo_sort_abitonic_def;
 

@@ -116,3 +116,23 @@ def test_bitonic_rejects_kv(clops, ctx, queue):
     with pytest.raises(clops.ClopsError) as e:
         s.sort_kv_device(queue, b, b, None, None, 4)
     assert e.value.code == clops.CLO_ERROR_ARGS
+
+
+def test_abitonic_default_step_groups(clops, oracle, ctx):
+    """Without maxps, 4-byte keys fuse up to 5 global steps per launch
+    (abit_priv_5s32v); maxps=4 keeps the reference's limit.  Same result."""
+    n = (1 << 20) + 12345
+    k = _keys(oracle, np.uint32, n, 0xAB)
+    q = clops.Queue(ctx, profiling=True)
+    outs = {}
+    for opt in (None, "maxps=4"):
+        s = clops.Sorter(ctx, "abitonic", np.uint32, options=opt)
+        kb = clops.Buffer.from_array(ctx, k)
+        q.prof_reset()
+        s.sort_device(q, kb, None, n)
+        q.finish()
+        outs[opt] = (kb.read(q, np.uint32, n), q.prof_names())
+    assert np.array_equal(outs[None][0], np.sort(k))
+    assert np.array_equal(outs["maxps=4"][0], np.sort(k))
+    assert "abit_priv_5s32v" in outs[None][1]
+    assert "abit_priv_5s32v" not in outs["maxps=4"][1]
