@@ -761,7 +761,7 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 // for u64 keys alone and 12% for u32 keys with values, 256x32 wins or ties
 // elsewhere, and 16384-key tiles run one block per CU and lose 20-40%.  Up to
 // 2^20 keys the 4096-key tile (256x16) is 10-25% faster: twice the blocks,
-// and still one group of tiles for the one-block counter scan.
+// and still one group of tiles, whose offsets the downsweep sums itself.
 inline int auto_shape(size_t ksize, bool kv, size_t n) {
 	if (n <= (static_cast<size_t>(1) << 20)) return 2;
 	if ((ksize == 8 && !kv) || (ksize == 4 && kv)) return 1;
