@@ -44,6 +44,14 @@ __device__ __forceinline__ u32 xcd_tile(u32 tiles) {
 
 inline u32 xcd_grid(u32 tiles) { return kXcds * ((tiles + kXcds - 1) / kXcds); }
 
+// xcd_tile with each XCD's range walked from its end: a kernel that follows
+// one over the same tiles starts on the data the first read last, which the
+// memory-side cache may still hold.
+__device__ __forceinline__ u32 xcd_tile_rev(u32 tiles) {
+	const u32 per = (tiles + kXcds - 1) / kXcds;
+	return (blockIdx.x % kXcds) * per + (per - 1 - blockIdx.x / kXcds);
+}
+
 // Inclusive wave-wide prefix sum.
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_sum(T x) {

@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan(const In* __res
 	__shared__ Acc s_wsum[kScanBlock / kWave];
 	__shared__ SegPair<Acc> s_pw[kScanBlock / kWave];
 	__shared__ u32 s_j0;
-	const u32 tile = xcd_tile(tiles);
+	const u32 tile = xcd_tile_rev(tiles);
 	if (tile >= tiles) return;
 	const u64 base = static_cast<u64>(tile) * kScanTile;
 	const u32 tid = threadIdx.x;
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan_vec(const In* _
 	Acc* __restrict__ out, u64 n, u32 tiles, const Acc* __restrict__ carry_in) {
 	constexpr u32 kScanTile = kScanBlock * IT;
 	__shared__ Acc s_wsum[kScanBlock / kWave];
-	const u32 tile = xcd_tile(tiles);
+	const u32 tile = xcd_tile_rev(tiles);
 	if (tile >= tiles) return;
 	const u32 tid = threadIdx.x;
 	const u64 first = static_cast<u64>(tile) * kScanTile + static_cast<u64>(tid) * IT;
