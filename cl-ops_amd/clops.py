@@ -86,6 +86,7 @@ def lib():
         "clo_tzc": (ctypes.c_uint, [ctypes.c_int]),
         "clo_sum": (ctypes.c_uint, [ctypes.c_uint]),
         "ccl_debug_fail_next_allocs": (None, [i32]),
+        "clo_scan_debug_lookback_polls": (None, [i32]),
         # thin HIP layer
         "ccl_context_new_from_device_index": (vp, [i32, ep]),
         "ccl_context_destroy": (None, [vp]),

@@ -18,6 +18,7 @@
 // Segmented scans take segment starts from a sorted offsets array: the sum
 // restarts at every start.
 
+#include <atomic>
 #include <cstring>
 #include <type_traits>
 
@@ -441,17 +442,27 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan_vec(const In* _
 // Status words (one per 32 bits of the sum) pack value, epoch and flag into
 // one 64-bit relaxed device-scope atomic, so no fences are needed.  The
 // epoch comes from a device word (the first word of the status buffer; the
-// abort word and the tiles' words follow) that the last tile advances once its
-// look-back is done: by then every tile has published and so has read the
-// epoch; words of earlier launches therefore read as "not ready" and the
-// status array never needs clearing (also across graph replays).
+// tiles' words start at the third).  Every tile reads the epoch and the chunk
+// base before it publishes its aggregate; the last tile advances the epoch
+// once every tile has published, so words of earlier launches read as "not
+// ready" and the status array never needs clearing (also across graph
+// replays).
 //
-// A guard bounds every wait: a wave that polls kSpinLimit times stores this
-// launch's abort marker, and every waiting wave that sees the marker stops,
-// so the kernel always ends (with a wrong result; correct launches never get
-// near the limit).
-constexpr u32 kSpinLimit = 1u << 20;
+// Fallback: a tile whose look-back has found a predecessor unpublished for
+// `polls` rounds sums that predecessor's input itself (a tile's aggregate is
+// the sum of its elements, whoever computes it), so every wait is bounded
+// and the result is exact even if a predecessor is stalled, for instance on
+// a preempted GPU.  Correct launches stay far below kLookbackPolls; a test
+// hook lowers it to force the fallback.  A prefix found through a fallback,
+// or through a prefix so found, is published "late" (kFlagPrefixLate): an
+// earlier tile may not have started.  An ordinary prefix means every earlier
+// tile has published, so the last tile advances the epoch at once unless its
+// walk was late; then it first waits until every tile has published (each
+// one will: no wait is unbounded).
+constexpr u32 kLookbackPolls = 1u << 12;
+constexpr u32 kFlagPrefixLate = 3;
 constexpr u32 kOnePassMaxTiles = 512;
+std::atomic<u32> g_lookback_polls{kLookbackPolls};
 
 template <typename Acc>
 __device__ __forceinline__ void publish(u64* st, u32 tile, u32 epoch, u32 flag, Acc v) {
@@ -484,8 +495,11 @@ __device__ __forceinline__ u32 peek(const u64* st, u64 i, u32 epoch, Acc* v) {
 // (the distance back to the nearest inclusive prefix is about the number of
 // tiles in flight, so wide rounds keep the walk to one or two device-coherent
 // reads).  Offset d of a round is tile j - d; thread tid takes offsets
-// tid + kScanBlock * k (coalesced).  `abort` holds epoch + 1 once a wait of
-// this launch has given up.
+// tid + kScanBlock * k (coalesced).  After `polls` rounds that found a
+// predecessor unpublished, the block sums the input of each unpublished tile
+// up to the nearest prefix itself (`base`, the chunk base, belongs to tile 0).
+// `late` is set (uniformly) if the walk used a fallback or ended at a late
+// prefix.
 constexpr u32 kLookPer = 4;
 constexpr u32 kLookSpan = kScanBlock * kLookPer;
 
@@ -494,16 +508,43 @@ struct LookShared {
 	u32 wmin[kScanBlock / kWave];
 	u32 wx[kScanBlock / kWave];
 	Acc wsum[kScanBlock / kWave];
-	bool stop;
+	u32 miss[kLookSpan / 32];  // offsets of this round found unpublished
 };
 
-template <typename Acc>
-__device__ Acc look_back(const u64* st, u32 t, u32 epoch, u64* abort, LookShared<Acc>& sh) {
+// Sum of tile `idx`'s elements, by the whole block (a barrier inside).
+template <typename In, typename Acc, int IT>
+__device__ Acc tile_sum(const In* in, u64 n, u64 idx, LookShared<Acc>& sh) {
+	constexpr int W = kScanBlock / kWave;
+	const u64 first = idx * (kScanBlock * IT) + static_cast<u64>(threadIdx.x) * IT;
+	In x[IT];
+	if (first + IT <= n) {
+		load_vec<In, IT>(in + first, x);
+	} else {
+#pragma unroll
+		for (int j = 0; j < IT; ++j) x[j] = first + j < n ? in[first + j] : In(0);
+	}
+	Acc s = 0;
+#pragma unroll
+	for (int j = 0; j < IT; ++j) s += static_cast<Acc>(x[j]);
+	s = wave_sum(s);
+	if (lane_id() == 0) sh.wsum[wave_id()] = s;
+	__syncthreads();
+	Acc tot = 0;
+#pragma unroll
+	for (int i = 0; i < W; ++i) tot += sh.wsum[i];
+	__syncthreads();
+	return tot;
+}
+
+template <typename In, typename Acc, int IT>
+__device__ Acc look_back(const u64* st, u32 t, u32 epoch, const In* in, u64 n, u32 polls,
+	Acc base, LookShared<Acc>& sh, bool* late) {
 	constexpr int W = kScanBlock / kWave;
 	const u32 tid = threadIdx.x, lane = lane_id(), w = wave_id();
 	Acc excl = 0;
 	int64_t j = static_cast<int64_t>(t) - 1;
 	u32 spins = 0;
+	bool was_late = false;
 	while (j >= 0) {
 		u32 f[kLookPer];
 		Acc v[kLookPer];
@@ -515,7 +556,7 @@ __device__ Acc look_back(const u64* st, u32 t, u32 epoch, u64* abort, LookShared
 			v[k] = 0;
 			f[k] = kFlagPrefix;  // before tile 0: never reached (tile 0 publishes P)
 			if (idx >= 0) f[k] = peek<Acc>(st, static_cast<u64>(idx), epoch, &v[k]);
-			if (f[k] == kFlagPrefix && d < my_min) my_min = d;
+			if (f[k] >= kFlagPrefix && d < my_min) my_min = d;
 		}
 		// block minimum of the prefix offsets
 		u32 m = my_min;
@@ -525,74 +566,103 @@ __device__ Acc look_back(const u64* st, u32 t, u32 epoch, u64* abort, LookShared
 			m = o < m ? o : m;
 		}
 		if (lane == 0) sh.wmin[w] = m;
+		if (tid < kLookSpan / 32) sh.miss[tid] = 0u;
 		__syncthreads();
 		u32 first_p = sh.wmin[0];
 #pragma unroll
 		for (int i = 1; i < W; ++i) first_p = sh.wmin[i] < first_p ? sh.wmin[i] : first_p;
-		// any tile up to the prefix not published yet, and the sum up to it
-		bool missing = false;
+		// any tile up to the prefix not published yet, and the sum of those
+		// that are
+		bool missing = false, lp = false;
 		Acc part = 0;
 #pragma unroll
 		for (u32 k = 0; k < kLookPer; ++k) {
 			const u32 d = tid + kScanBlock * k;
 			if (d <= first_p) {
-				missing |= f[k] == 0u;
-				part += v[k];
+				if (f[k] == 0u) {
+					missing = true;
+					if (spins >= polls) atomicOr(&sh.miss[d >> 5], 1u << (d & 31u));
+				} else {
+					part += v[k];
+					lp |= f[k] == kFlagPrefixLate;
+				}
 			}
 		}
-		const u64 mb = ballot(missing);
+		const u64 mb = ballot(missing), lb = ballot(lp);
 		const Acc wsum = wave_sum(part);
 		if (lane == 0) {
-			sh.wx[w] = mb != 0;
+			sh.wx[w] = (mb != 0 ? 1u : 0u) | (lb != 0 ? 2u : 0u);
 			sh.wsum[w] = wsum;
 		}
 		__syncthreads();
-		bool any_missing = false;
+		bool any_missing = false, any_late = false;
 		Acc sum = 0;
 #pragma unroll
 		for (int i = 0; i < W; ++i) {
-			any_missing |= sh.wx[i] != 0;
+			any_missing |= (sh.wx[i] & 1u) != 0;
+			any_late |= (sh.wx[i] & 2u) != 0;
 			sum += sh.wsum[i];
 		}
-		__syncthreads();  // the shared slots are reused by the next round
+		__syncthreads();  // the shared slots are reused below and by the next round
 		if (any_missing) {
-			// Thread 0 decides whether to stop (limit reached, or another wait
-			// of this launch gave up) and the block follows it.
-			++spins;
-			if (tid == 0) {
-				bool stop = status_load(abort) == static_cast<u64>(epoch) + 1u;
-				if (spins > kSpinLimit) {
-					__hip_atomic_store(abort, static_cast<u64>(epoch) + 1u, __ATOMIC_RELAXED,
-						__HIP_MEMORY_SCOPE_AGENT);
-					stop = true;
-				}
-				sh.stop = stop;
+			if (spins < polls) {
+				++spins;
+				__builtin_amdgcn_s_sleep(1);
+				continue;
 			}
-			__syncthreads();
-			const bool stop = sh.stop;
-			__syncthreads();
-			if (stop) break;
-			__builtin_amdgcn_s_sleep(1);
-			continue;
+			// Fallback: sum the unpublished tiles' inputs (the offsets
+			// flagged in sh.miss; uniform across the block).
+			was_late = true;
+			for (u32 wi = 0; wi < kLookSpan / 32; ++wi) {
+				u32 bits = sh.miss[wi];
+				while (bits) {
+					const u32 d = wi * 32 + static_cast<u32>(__builtin_ctz(bits));
+					bits &= bits - 1u;
+					const u64 idx = static_cast<u64>(j - static_cast<int64_t>(d));
+					sum += tile_sum<In, Acc, IT>(in, n, idx, sh);
+					if (idx == 0) sum += base;
+				}
+			}
 		}
 		excl += sum;
+		was_late |= any_late;
 		if (first_p < kLookSpan) break;
 		j -= kLookSpan;
 	}
+	*late = was_late;
 	return excl;
+}
+
+// Waits, by the whole block, until every tile of this launch has published
+// a status word (and so has read the epoch and the chunk base).
+template <typename Acc>
+__device__ void wait_all_published(const u64* st, u32 tiles, u32 epoch) {
+	constexpr u32 kWords = sizeof(Acc) / 4;
+	for (;;) {
+		int unpublished = 0;
+		for (u32 i = threadIdx.x; i < tiles; i += kScanBlock)
+			unpublished |= status_flag(status_load(st + static_cast<u64>(i) * kWords), epoch) == 0u;
+		if (!__syncthreads_or(unpublished)) return;
+		__builtin_amdgcn_s_sleep(1);
+	}
 }
 
 template <typename In, typename Acc, int IT>
 __global__ __launch_bounds__(kScanBlock) void blelloch_onepass(const In* __restrict__ in,
 	Acc* __restrict__ out, u64 n, u32 tiles, u64* __restrict__ status, u32* __restrict__ epoch_p,
-	Acc* __restrict__ chain) {
+	Acc* __restrict__ chain, u32 polls) {
 	constexpr u32 kScanTile = kScanBlock * IT;
 	__shared__ Acc s_wsum[kScanBlock / kWave];
 	__shared__ Acc s_prefix;
+	__shared__ Acc s_base;
+	__shared__ u32 s_epoch;
 	__shared__ LookShared<Acc> s_look;
 	const u32 t = blockIdx.x;
 	const u32 tid = threadIdx.x;
-	const u32 epoch = *epoch_p & 0x3FFFFFFFu;
+	if (tid == 0) {  // read before this tile publishes (barriers in between)
+		s_epoch = *epoch_p & 0x3FFFFFFFu;
+		s_base = chain ? *chain : Acc(0);
+	}
 	const u64 first = static_cast<u64>(t) * kScanTile + static_cast<u64>(tid) * IT;
 	const bool full = first + IT <= n;
 	In x[IT];
@@ -606,26 +676,30 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_onepass(const In* __restr
 #pragma unroll
 	for (int j = 0; j < IT; ++j) sum += static_cast<Acc>(x[j]);
 	Acc agg;
-	Acc run = block_exclusive_sum<kScanBlock>(sum, s_wsum, &agg);
+	Acc run = block_exclusive_sum<kScanBlock>(sum, s_wsum, &agg);  // barriers: s_epoch, s_base
+	const u32 epoch = s_epoch;
+	const Acc base = s_base;
+	bool late = false;
 	if (t == 0) {
 		if (tid == 0) {
-			const Acc base = chain ? *chain : Acc(0);
 			publish<Acc>(status, 0, epoch, kFlagPrefix, base + agg);
 			s_prefix = base;
 		}
 	} else {
 		if (tid == 0) publish<Acc>(status, t, epoch, kFlagAggregate, agg);
-		const Acc excl = look_back<Acc>(status, t, epoch, status - 1, s_look);
+		const Acc excl = look_back<In, Acc, IT>(status, t, epoch, in, n, polls, base, s_look,
+			&late);
 		if (tid == 0) {
-			publish<Acc>(status, t, epoch, kFlagPrefix, excl + agg);
+			publish<Acc>(status, t, epoch, late ? kFlagPrefixLate : kFlagPrefix, excl + agg);
 			s_prefix = excl;
 		}
 	}
+	if (t == tiles - 1 && late) wait_all_published<Acc>(status, tiles, epoch);
 	__syncthreads();
 	const Acc prefix = s_prefix;
 	if (tid == 0 && t == tiles - 1) {
 		if (chain) *chain = prefix + agg;
-		*epoch_p = (epoch + 1u) & 0x3FFFFFFFu;  // every tile has read the epoch by now
+		*epoch_p = (epoch + 1u) & 0x3FFFFFFFu;
 	}
 	run += prefix;
 	Acc y[IT];
@@ -660,7 +734,7 @@ struct ScanLaunch {
 	u32 nseg;
 	u32 items;  // items per thread: 16, or 8 / 32 for unsigned plain scans
 	void* chain;  // running total across chunks (plain scans) or nullptr
-	u64* status;  // single pass: epoch word, abort word, then the tiles' status words
+	u64* status;  // single pass: epoch word, an unused word, then the tiles' status words
 	CCLQueue* q;
 };
 
@@ -675,10 +749,11 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 		if (a.status && a.tiles <= kOnePassMaxTiles &&
 			((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0) {
 			Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_ONEPASS);
-			// status[0]: epoch; status[1]: abort marker; tiles' words from status[2]
+			// status[0]: epoch; status[1]: unused; tiles' words from status[2]
 			blelloch_onepass<In, Acc, IT><<<a.tiles, kScanBlock, 0, st>>>(in,
 				static_cast<Acc*>(a.out), a.n, a.tiles, a.status + 2,
-				reinterpret_cast<u32*>(a.status), static_cast<Acc*>(a.chain));
+				reinterpret_cast<u32*>(a.status), static_cast<Acc*>(a.chain),
+				g_lookback_polls.load(std::memory_order_relaxed));
 			return L.done(err);
 		}
 		if (((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0) {
@@ -780,7 +855,7 @@ bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, s
 	const size_t acc = ss == 8 ? 8 : 4;
 	const size_t sum_bytes = (tiles * acc + 255) / 256 * 256;
 	if (!bd->tiles.ensure(sum_bytes + tiles * 4, err)) return false;
-	// Single pass (plain scans): the epoch word, the abort word and (sum
+	// Single pass (plain scans): the epoch word, an unused word and (sum
 	// bytes / 4) status words per tile.  Growing reallocates zeroed memory:
 	// epoch 0, no status.
 	u64* status = nullptr;
@@ -1030,6 +1105,11 @@ const char* clo_scan_get_kernel_name(CloScan* scanner, cl_uint i, GError** err) 
 size_t clo_scan_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, size_t numel,
 	GError** err) {
 	return scanner ? scanner->impl.get_localmem_usage(scanner, i, lws_max, numel, err) : 0;
+}
+
+void clo_scan_debug_lookback_polls(int polls) {
+	g_lookback_polls.store(polls < 0 ? kLookbackPolls : static_cast<u32>(polls),
+		std::memory_order_relaxed);
 }
 
 }  // extern "C"

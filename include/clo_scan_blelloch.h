@@ -29,6 +29,13 @@ extern "C" {
 
 extern const CloScanImplDef clo_scan_blelloch_def;
 
+/* Test hook (no counterpart in the reference): the single-pass kernel's
+ * look-back waits `polls` rounds for an unpublished predecessor tile before
+ * summing that tile's input itself; 0 sums every unpublished predecessor at
+ * once, a negative value restores the default (4096).  The result is the same
+ * for any value.  Applies to launches made after the call. */
+void clo_scan_debug_lookback_polls(int polls);
+
 #ifdef __cplusplus
 }
 #endif

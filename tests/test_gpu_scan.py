@@ -61,6 +61,34 @@ def test_scan_repeat_epochs(clops, oracle, ctx, queue):
         assert np.array_equal(sc.scan_host(a), oracle.scan(a, np.uint64))
 
 
+@pytest.mark.parametrize("polls", [0, 1])
+@pytest.mark.parametrize("elem,sumt", [(np.uint32, np.uint32), (np.uint32, np.uint64),
+                                       (np.uint64, np.uint64)])
+def test_scan_lookback_fallback(clops, oracle, ctx, queue, polls, elem, sumt):
+    """The single-pass scan with its look-back wait cut to `polls` rounds:
+    a tile that finds a predecessor unpublished sums that tile's input
+    itself.  The results stay exact, also over repeated launches (epochs and
+    the started count reused) and through the host path's chunk chain."""
+    L = clops.lib()
+    sc = clops.Scanner(ctx, elem, sumt)
+    L.clo_scan_debug_lookback_polls(polls)
+    try:
+        for rep in range(3):
+            for n in [8193, 100000, 1 << 20, (1 << 21) - 1, 1 << 21]:
+                a = _vals(oracle, elem, n, 0xFB + n + rep)
+                want = oracle.scan(a, sumt)
+                bi = clops.Buffer.from_array(ctx, a)
+                bo = clops.Buffer(ctx, n * np.dtype(sumt).itemsize)
+                sc.scan_device(queue, bi, bo, n)
+                queue.finish()
+                assert np.array_equal(bo.read(queue, sumt, n), want), (rep, n)
+                assert np.array_equal(sc.scan_host(a), want), (rep, n)
+    finally:
+        L.clo_scan_debug_lookback_polls(-1)
+    a = _vals(oracle, elem, 1 << 20, 0x3)
+    assert np.array_equal(sc.scan_host(a), oracle.scan(a, sumt))
+
+
 def test_scan_options_rejected(clops, ctx):
     with pytest.raises(clops.ClopsError) as e:
         clops.Scanner(ctx, np.uint32, np.uint32, options="x=1")
