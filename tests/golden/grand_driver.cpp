@@ -26,7 +26,7 @@ int main(int argc, char** argv) {
 		const double b = strtod(argv[3], nullptr), e = strtod(argv[4], nullptr);
 		for (long i = strtol(argv[5], nullptr, 0); i > 0; --i) printf("%a\n", g.double_range(b, e));
 	} else if (!strcmp(cmd, "bench") && argc >= 5) {
-		// Element sizes by CloType (char .. double), so the driver needs only
+		// Element sizes by CloType (char .. double), so this program needs only
 		// the headers, not the library.
 		static const int kSize[] = {1, 1, 2, 2, 4, 4, 8, 8, 2, 4, 8};
 		const long ti = strtol(argv[3], nullptr, 0);
