@@ -112,6 +112,15 @@ def traffic_from_profiles():
     return int(m.group(1)) if m else None
 
 
+def verify_kv(q, kin, kout, vout, n, width):
+    """Device checks of a KV sort whose input values were 0..n-1: keys
+    non-decreasing with values increasing inside runs of equal keys, values a
+    permutation of 0..n-1, and each output key the input key its value names.
+    Zero violations of both means the output is exactly the stable sort."""
+    return (clops.check_sorted(q, kout, n, width, vout),
+            clops.check_pairs(q, kin, kout, vout, n, width))
+
+
 def headline(args, ctx):
     n = 1 << args.log2n
     q = clops.Queue(ctx, profiling=True)
@@ -151,14 +160,15 @@ def headline(args, ctx):
     bytes_per_launch = n * 2 * (8 + 4)
     achieved = bytes_per_launch / (scatter_ms * 1e-3) / 1e9
     # verification of the last step's output on the device
-    bad = clops.check_sorted(q, kout, n, 8, vout)
+    bad, pairs = verify_kv(q, kin, kout, vout, n, 8)
     same = clops.multiset_hash(q, kin, n, 8) == clops.multiset_hash(q, kout, n, 8)
     total_bytes = n * 8 + passes * bytes_per_launch
     log("headline: %.3f ms/step, downsweep %.3f ms x %d, upsweep %.3f ms, passes %d, "
-        "sorted-violations %d, multiset-equal %s, whole-sort HBM fraction %.3f"
-        % (ms, scatter_ms, cnt // max(args.steps, 1), htot / max(hcnt, 1), passes, bad, same,
-           total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
-    if bad != 0 or not same:
+        "sorted/stability violations %d, pairing/permutation violations %d, multiset-equal %s, "
+        "whole-sort HBM fraction %.3f"
+        % (ms, scatter_ms, cnt // max(args.steps, 1), htot / max(hcnt, 1), passes, bad, pairs,
+           same, total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+    if bad != 0 or pairs != 0 or not same:
         raise SystemExit("benchmark output failed verification")
     return {
         "ms": ms, "n": n, "passes": passes, "scatter_ms": scatter_ms, "achieved": achieved,
@@ -215,8 +225,12 @@ def suite(ctx, cpu_threads=0):
     clops.generate(q, kin, n, clops.GEN_U64_MASK, 0xC3B, (1 << 20) - 1)
     ms = time_steps(q, lambda: s.sort_kv_device(q, kin, vin, kout, vout, n), 5, 2)
     p = s.last_passes()
+    bad, pairs = verify_kv(q, kin, kout, vout, n, 8)
     out.append({"config": "satradix KV u64 2^28 keys<2^20", "ms": ms, "Gkeys/s": n / ms / 1e6,
-                "passes": p, "hbm_frac": n * (8 + 24 * p) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+                "passes": p, "hbm_frac": n * (8 + 24 * p) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "order_violations": bad, "pair_violations": pairs})
+    if bad or pairs:
+        raise SystemExit("keys<2^20 sort failed verification")
     cdfh = np.empty(1 << 24, dtype=np.float64)
     import ctypes
     clops.lib().clo_util_zipf_cdf(cdfh.ctypes.data_as(ctypes.c_void_p), cdfh.size, 1.1)
@@ -224,8 +238,12 @@ def suite(ctx, cpu_threads=0):
     clops._call(clops.lib().clo_util_generate_zipf, q.ptr, kin.ptr, n, cdf.ptr, cdfh.size, 0xC3C)
     ms = time_steps(q, lambda: s.sort_kv_device(q, kin, vin, kout, vout, n), 5, 2)
     p = s.last_passes()
+    bad, pairs = verify_kv(q, kin, kout, vout, n, 8)
     out.append({"config": "satradix KV u64 2^28 zipf1.1", "ms": ms, "Gkeys/s": n / ms / 1e6,
-                "passes": p, "hbm_frac": n * (8 + 24 * p) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+                "passes": p, "hbm_frac": n * (8 + 24 * p) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "order_violations": bad, "pair_violations": pairs})
+    if bad or pairs:
+        raise SystemExit("Zipf sort failed verification")
     # Scans (C2 shapes).
     for elem, sumt, n in ((np.uint32, np.uint32, 1 << 28), (np.uint32, np.uint64, 1 << 28),
                           (np.uint64, np.uint64, 1 << 27)):

@@ -159,6 +159,7 @@ def lib():
         "clo_util_check_sorted": (sz, [vp, vp, vp, sz, sz, ep]),
         "clo_util_build_info": (cp, []),
         "clo_util_count_diff": (sz, [vp, vp, vp, sz, ep]),
+        "clo_util_check_pairs": (sz, [vp, vp, vp, vp, sz, sz, ep]),
         # dispatcher
         "clo_dispatch_new": (vp, [vp, ep]),
         "clo_dispatch_destroy": (None, [vp]),
@@ -726,6 +727,13 @@ def generate(q, buf, n, kind, seed, param=0, start=0):
 
 def multiset_hash(q, buf, n, width):
     return _call(lib().clo_util_multiset_hash, q.ptr, buf.ptr, n, width)
+
+
+def check_pairs(q, keys_in, keys_out, vals_out, n, width):
+    """Violations of: vals_out is a permutation of 0..n-1 and
+    keys_out[i] == keys_in[vals_out[i]] (computed on the device)."""
+    return _call(lib().clo_util_check_pairs, q.ptr, keys_in.ptr, keys_out.ptr, vals_out.ptr, n,
+                 width)
 
 
 def check_sorted(q, keys, n, width, vals=None):

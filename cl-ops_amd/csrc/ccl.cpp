@@ -124,12 +124,36 @@ bool Launch::done(GError** err) {
 	return true;
 }
 
+thread_local std::vector<DevBlock*> g_capture_blocks;
+
+void devblock_unref(DevBlock* b) {
+	if (b && b->refs.fetch_sub(1) == 1) {
+		(void)hipFree(b->ptr);
+		delete b;
+	}
+}
+
 bool Scratch::ensure(size_t bytes, GError** err) {
-	if (bytes <= size && ptr) return true;
+	if (bytes <= size && ptr) {
+		// Used inside a capture: logged (the log holds a reference until the
+		// last open capture ends), and the graph will hold the allocation.
+		if (g_capture_depth > 0 && (g_capture_blocks.empty() || g_capture_blocks.back() != blk)) {
+			blk->refs.fetch_add(1);
+			g_capture_blocks.push_back(blk);
+		}
+		return true;
+	}
 	if (g_capture_depth > 0) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
 			"device scratch of %zu bytes needed during graph capture: run the same call once "
 			"before capturing it", bytes);
+		return false;
+	}
+	if (blk && blk->refs.load() > 1) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"device scratch would grow to %zu bytes while a captured graph uses it: destroy "
+			"the graph first, or size the scratch with the largest call before capturing",
+			bytes);
 		return false;
 	}
 	release();
@@ -146,15 +170,17 @@ bool Scratch::ensure(size_t bytes, GError** err) {
 		(void)hipFree(p);
 		return false;
 	}
+	blk = new DevBlock();
+	blk->ptr = p;
+	blk->refs.store(1);
 	ptr = p;
 	size = want;
 	return true;
 }
 
-// Teardown paths (release, destroy) have no error channel, as in the C API's
-// void destroy functions: their HIP return codes are dropped on purpose.
 void Scratch::release() {
-	if (ptr) (void)hipFree(ptr);
+	devblock_unref(blk);
+	blk = nullptr;
 	ptr = nullptr;
 	size = 0;
 }

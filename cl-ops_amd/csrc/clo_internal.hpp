@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdint>
 #include <string>
@@ -34,6 +35,7 @@ struct ccl_queue {
 	bool owns_stream;
 	bool prof;
 	bool capturing = false;                // between clo_graph_begin/end_capture
+	size_t capture_mark = 0;               // g_capture_blocks size at begin_capture
 	ccl_event* capture_event = nullptr;   // returned for work enqueued while capturing
 	std::vector<ccl_event*> events;       // every event this queue recorded
 	std::vector<ccl_prof_rec> prof_recs;  // used records since last reset
@@ -93,13 +95,29 @@ struct Launch {
 	bool done(GError** err);  // after the launch; false on error
 };
 
+// A device allocation shared by the Scratch that made it and by every graph
+// captured over calls that used it; freed when the last holder lets go.
+struct DevBlock {
+	void* ptr;
+	std::atomic<int> refs;
+};
+void devblock_unref(DevBlock* b);
+
+// Blocks used by library calls while a graph capture is open on the calling
+// thread; clo_graph_end_capture hands them to the graph (graph.cpp).
+extern thread_local std::vector<DevBlock*> g_capture_blocks;
+
 // A growable device scratch buffer owned by a sorter/scanner.  Growing frees
 // the previous allocation (hipFree synchronises with the device, so kernels
-// still reading it have finished).
+// still reading it have finished).  A scratch used inside a graph capture is
+// held by that graph: it then cannot grow (ensure fails with CLO_ERROR_ARGS
+// until the graph is destroyed), and releasing it leaves the memory to the
+// graph, which frees it when destroyed.
 struct Scratch {
 	void* ptr = nullptr;
 	size_t size = 0;
 	bool zero_on_alloc = false;
+	DevBlock* blk = nullptr;
 	bool ensure(size_t bytes, GError** err);
 	void release();
 	~Scratch() { release(); }

@@ -10,7 +10,30 @@ struct clo_graph {
 	hipGraphExec_t exec;
 	CCLContext* ctx;
 	size_t nodes;
+	std::vector<DevBlock*> held;  // library scratch the captured kernels use
 };
+
+// The distinct blocks logged since the capture on cq began, each with a new
+// reference for the graph.  When no capture remains open on this thread the
+// log drops its own references and is cleared.  With overlapping captures a
+// graph may hold a block that only another capture used: never too few.
+static std::vector<DevBlock*> take_capture_blocks(CCLQueue* cq) {
+	std::vector<DevBlock*> out;
+	for (size_t i = cq->capture_mark; i < g_capture_blocks.size(); ++i) {
+		DevBlock* b = g_capture_blocks[i];
+		bool seen = false;
+		for (DevBlock* o : out) seen = seen || o == b;
+		if (!seen) {
+			b->refs.fetch_add(1);
+			out.push_back(b);
+		}
+	}
+	if (g_capture_depth == 0) {
+		for (DevBlock* b : g_capture_blocks) devblock_unref(b);
+		g_capture_blocks.clear();
+	}
+	return out;
+}
 
 extern "C" {
 
@@ -31,6 +54,7 @@ cl_bool clo_graph_begin_capture(CCLQueue* cq, GError** err) {
 			"hipStreamBeginCapture"))
 		return CL_FALSE;
 	cq->capturing = true;
+	cq->capture_mark = g_capture_blocks.size();
 	++g_capture_depth;
 	return CL_TRUE;
 }
@@ -45,23 +69,28 @@ CloGraph* clo_graph_end_capture(CCLQueue* cq, GError** err) {
 	hipError_t e = hipStreamEndCapture(cq->stream, &g);
 	cq->capturing = false;
 	--g_capture_depth;
-	if (err && *err) {  // a call inside the capture failed: drop the graph
+	std::vector<DevBlock*> held = take_capture_blocks(cq);
+	auto drop = [&]() {
 		if (g) (void)hipGraphDestroy(g);
+		for (DevBlock* b : held) devblock_unref(b);
+	};
+	if (err && *err) {  // a call inside the capture failed: drop the graph
+		drop();
 		return nullptr;
 	}
 	if (hip_failed(e, err, "hipStreamEndCapture")) {
-		if (g) (void)hipGraphDestroy(g);
+		drop();
 		return nullptr;
 	}
 	size_t nodes = 0;
 	if (hip_failed(hipGraphGetNodes(g, nullptr, &nodes), err, "hipGraphGetNodes")) {
-		(void)hipGraphDestroy(g);
+		drop();
 		return nullptr;
 	}
 	hipGraphExec_t x = nullptr;  // stays null for an empty capture (launch = no-op)
 	if (nodes > 0 &&
 		hip_failed(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), err, "hipGraphInstantiate")) {
-		(void)hipGraphDestroy(g);
+		drop();
 		return nullptr;
 	}
 	CloGraph* cg = new clo_graph();
@@ -70,6 +99,7 @@ CloGraph* clo_graph_end_capture(CCLQueue* cq, GError** err) {
 	cg->ctx = cq->ctx;
 	ccl_context_ref(cq->ctx);
 	cg->nodes = nodes;
+	cg->held = std::move(held);
 	return cg;
 }
 
@@ -97,6 +127,7 @@ void clo_graph_destroy(CloGraph* graph) {
 	(void)hipSetDevice(graph->ctx->dev);
 	if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
 	(void)hipGraphDestroy(graph->graph);
+	for (DevBlock* b : graph->held) devblock_unref(b);
 	ccl_context_destroy(graph->ctx);
 	delete graph;
 }

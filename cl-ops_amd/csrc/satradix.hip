@@ -423,20 +423,245 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 // fused_scan (sorts of at most kScanTiles tiles): `offsets` holds the
 // upsweep's tile counts and every block derives its own tile's offsets from
 // them, in place of a separate counter-scan launch.
+//
+// Staging: 8-byte keys pass through LDS as two 32-bit halves (high halves,
+// then low halves) in a buffer of 4 bytes per key, so the block needs 36 KB
+// of LDS instead of 68 KB.  Full tiles (every tile but a ragged last one) run
+// a copy of the body without per-element bounds checks.  Measured on the 2^28
+// u64-key/u32-value sort: 1.326 -> 1.298 ms per downsweep.
+template <typename K, bool KV, int NT, int ITEMS>
+struct DownSmem {
+	static constexpr u32 kTile = NT * ITEMS;
+	static constexpr bool kSplit = sizeof(K) == 8;
+	static constexpr u32 kStageBytes =
+		kTile * (kSplit ? 4u : (sizeof(K) > 4 ? sizeof(K) : (KV ? 4u : sizeof(K))));
+	unsigned char stage[kStageBytes] __attribute__((aligned(16)));
+	u32 whist[NT / kWave][kRadixMax];
+	u32 tstart[kRadixMax];
+	u32 gofs[kRadixMax];
+	u32 wsum[NT / kWave];
+};
+
+// Lanes holding the same digit when every lane of the wave holds an element.
+template <u32 MAXBITS>
+__device__ __forceinline__ u64 match_peers_full(u32 dig, u32 bits) {
+	u32 mlo = 0, mhi = 0;
+#pragma unroll
+	for (u32 bt = 0; bt < MAXBITS; ++bt) {
+		if (bt >= bits) break;
+		const u32 sb = static_cast<u32>(__builtin_amdgcn_sbfe(static_cast<int>(dig),
+			static_cast<int>(bt), 1));
+		const u64 bb = ballot(sb != 0u);
+		mlo = __builtin_amdgcn_bitop3_b32(mlo, static_cast<u32>(bb), sb, 0xF6);
+		mhi = __builtin_amdgcn_bitop3_b32(mhi, static_cast<u32>(bb >> 32), sb, 0xF6);
+	}
+	return ~((static_cast<u64>(mhi) << 32) | mlo);
+}
+
+template <typename K, bool KV, int NT, int ITEMS, int FIXED_BITS, bool FULL>
+__device__ __forceinline__ void downsweep_tile(DownSmem<K, KV, NT, ITEMS>& sm, const Bufs<K>& b,
+	const PassDesc& d, u32 tile, u32 n, u32 bits, u32 goff, u8* __restrict__ ndig) {
+	using S = DownSmem<K, KV, NT, ITEMS>;
+	constexpr u32 kTile = S::kTile;
+	constexpr int NW = NT / kWave;
+	const u32 tid = threadIdx.x;
+	const u32 lane = lane_id();
+	const u32 w = wave_id();
+	const u32 tile_base = tile * kTile;
+	const u32 mask = (1u << bits) - 1u;
+	const u32 shift = d.shift;
+	const K* __restrict__ src = b.k[d.src];
+	K* __restrict__ dst = b.k[d.dst];
+
+	// Load (coalesced: 64 consecutive elements per wave instruction).  A wave
+	// issuing its loads (and later its scatter stores) runs at raised priority
+	// so its memory requests go out ahead of other waves' ranking work
+	// (measured: 2% on the 2^28 u64 KV downsweep).
+	const u32 wbase = tile_base + w * kWave * ITEMS;
+	K k[ITEMS];
+	u32 v[ITEMS];
+	__builtin_amdgcn_s_setprio(3);
+#pragma unroll
+	for (int j = 0; j < ITEMS; ++j) {
+		const u32 idx = wbase + j * kWave + lane;
+		k[j] = (FULL || idx < n) ? src[idx] : K(0);
+	}
+	if (KV) {
+		const u32* __restrict__ vsrc = b.v[d.src];
+#pragma unroll
+		for (int j = 0; j < ITEMS; ++j) {
+			const u32 idx = wbase + j * kWave + lane;
+			v[j] = (FULL || idx < n) ? vsrc[idx] : 0u;
+		}
+	}
+	__builtin_amdgcn_s_setprio(0);
+	__syncthreads();
+
+	// Rank inside the wave.  The lanes holding one digit (found with one ballot
+	// per digit bit) form a peer group; its lowest lane adds the group's size
+	// to the wave's running digit count with one LDS atomic that returns the
+	// previous count.  The atomics of all items issue back to back (a wave's
+	// LDS operations complete in order, so item j sees the counts of items
+	// before it); each lane then takes its group's base from the leader.
+	// Items go in groups of kGroup (8 atomics in flight, 8 extra registers).
+	constexpr int kGroup = ITEMS < 8 ? ITEMS : 8;
+	static_assert(ITEMS % kGroup == 0, "whole groups");
+	constexpr u32 kMatch = FIXED_BITS ? FIXED_BITS : 8;
+	u32 rank[ITEMS];  // (leader lane << 16) | peers below, then the rank
+#pragma unroll
+	for (int j0 = 0; j0 < ITEMS; j0 += kGroup) {
+		u32 got[kGroup];
+#pragma unroll
+		for (int jj = 0; jj < kGroup; ++jj) {
+			const int j = j0 + jj;
+			const bool valid = FULL || (wbase + j * kWave + lane) < n;
+			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
+			const u64 peers = FULL ? match_peers_full<kMatch>(dig, bits)
+				: match_peers<kMatch>(dig, bits, valid);
+			const u32 below = count_below(peers);
+			// An invalid lane has no peers; its leader (lane 63) is never used.
+			const u32 leader = static_cast<u32>(__builtin_ctzll(peers | (1ull << 63)));
+			got[jj] = 0;
+			if (valid && below == 0)
+				got[jj] = atomicAdd(&sm.whist[w][dig], static_cast<u32>(__popcll(peers)));
+			rank[j] = (leader << 16) | below;
+		}
+#pragma unroll
+		for (int jj = 0; jj < kGroup; ++jj) {
+			const int j = j0 + jj;
+			rank[j] = static_cast<u32>(__shfl(static_cast<int>(got[jj]),
+				static_cast<int>(rank[j] >> 16), kWave)) + (rank[j] & 0xFFFFu);
+		}
+	}
+	__syncthreads();
+
+	// Per-digit tile totals and per-wave exclusive offsets.
+	u32 total = 0;
+	if (tid < kRadixMax) {
+#pragma unroll
+		for (int ww = 0; ww < NW; ++ww) {
+			const u32 c = sm.whist[ww][tid];
+			sm.whist[ww][tid] = total;
+			total += c;
+		}
+	}
+	const u32 tstart = block_exclusive_sum<NT>(total, sm.wsum, static_cast<u32*>(nullptr));
+	if (tid < kRadixMax) {
+		sm.tstart[tid] = tstart;
+		sm.gofs[tid] = goff - tstart;
+	}
+	__syncthreads();
+
+	// Stage the keys in digit order in LDS; thread tid then writes elements
+	// tid, tid + NT, ... of the staged tile to their run in the output.
+	const u32 count = FULL ? kTile : ((n - tile_base) < kTile ? (n - tile_base) : kTile);
+	u32 oidx[ITEMS];
+	const u32 nshift = d.nshift;
+	if constexpr (S::kSplit) {
+		u32* s32 = reinterpret_cast<u32*>(sm.stage);
+#pragma unroll
+		for (int j = 0; j < ITEMS; ++j) {
+			const bool valid = FULL || (wbase + j * kWave + lane) < n;
+			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
+			if (valid) {
+				rank[j] += sm.tstart[dig] + sm.whist[w][dig];
+				s32[rank[j]] = static_cast<u32>(static_cast<u64>(k[j]) >> 32);
+			}
+		}
+		__syncthreads();
+		u32 hi[ITEMS];
+#pragma unroll
+		for (int r = 0; r < ITEMS; ++r) {
+			const u32 i = r * NT + tid;
+			hi[r] = (FULL || i < count) ? s32[i] : 0u;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < ITEMS; ++j) {
+			const bool valid = FULL || (wbase + j * kWave + lane) < n;
+			if (valid) s32[rank[j]] = static_cast<u32>(k[j]);
+		}
+		__syncthreads();
+		__builtin_amdgcn_s_setprio(2);
+#pragma unroll
+		for (int r = 0; r < ITEMS; ++r) {
+			const u32 i = r * NT + tid;
+			if (FULL || i < count) {
+				const K key = static_cast<K>((static_cast<u64>(hi[r]) << 32) | s32[i]);
+				const u32 dig = static_cast<u32>(key >> shift) & mask;
+				const u32 o = sm.gofs[dig] + i;
+				dst[o] = key;
+				if (d.nact) ndig[o] = static_cast<u8>(static_cast<u32>(key >> nshift) & mask);
+				oidx[r] = o;
+			}
+		}
+	} else {
+		K* s_keys = reinterpret_cast<K*>(sm.stage);
+#pragma unroll
+		for (int j = 0; j < ITEMS; ++j) {
+			const bool valid = FULL || (wbase + j * kWave + lane) < n;
+			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
+			if (valid) {
+				rank[j] += sm.tstart[dig] + sm.whist[w][dig];
+				s_keys[rank[j]] = k[j];
+			}
+		}
+		__syncthreads();
+		__builtin_amdgcn_s_setprio(2);
+		if (d.nact) {
+#pragma unroll
+			for (int r = 0; r < ITEMS; ++r) {
+				const u32 i = r * NT + tid;
+				if (FULL || i < count) {
+					const K key = s_keys[i];
+					const u32 dig = static_cast<u32>(key >> shift) & mask;
+					const u32 o = sm.gofs[dig] + i;
+					dst[o] = key;
+					ndig[o] = static_cast<u8>(static_cast<u32>(key >> nshift) & mask);
+					oidx[r] = o;
+				}
+			}
+		} else {
+#pragma unroll
+			for (int r = 0; r < ITEMS; ++r) {
+				const u32 i = r * NT + tid;
+				if (FULL || i < count) {
+					const K key = s_keys[i];
+					const u32 dig = static_cast<u32>(key >> shift) & mask;
+					const u32 o = sm.gofs[dig] + i;
+					dst[o] = key;
+					oidx[r] = o;
+				}
+			}
+		}
+	}
+	if (KV) {
+		u32* s_vals = reinterpret_cast<u32*>(sm.stage);
+		u32* __restrict__ vdst = b.v[d.dst];
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < ITEMS; ++j) {
+			const bool valid = FULL || (wbase + j * kWave + lane) < n;
+			if (valid) s_vals[rank[j]] = v[j];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int r = 0; r < ITEMS; ++r) {
+			const u32 i = r * NT + tid;
+			if (FULL || i < count) vdst[oidx[r]] = s_vals[i];
+		}
+	}
+	__builtin_amdgcn_s_setprio(0);
+}
+
 template <typename K, bool KV, int NT, int ITEMS, int FIXED_BITS>
 __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits_arg,
 	const u32* __restrict__ offsets, u8* __restrict__ ndig, u32 fused_scan) {
 	const u32 bits = FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits_arg;
 	constexpr u32 kTile = NT * ITEMS;
-	constexpr int NW = NT / kWave;
 	static_assert(NT >= kRadixMax, "one thread per digit bin");
-	constexpr u32 kStageBytes = kTile * (sizeof(K) > 4 ? sizeof(K) : (KV ? 4 : sizeof(K)));
-	__shared__ __attribute__((aligned(16))) unsigned char s_stage[kStageBytes];
-	__shared__ u32 s_whist[NW][kRadixMax];
-	__shared__ u32 s_tstart[kRadixMax];
-	__shared__ u32 s_gofs[kRadixMax];
-	__shared__ u32 s_wsum[NW];
+	__shared__ DownSmem<K, KV, NT, ITEMS> sm;
 
 	const PassDesc d = plan->pass[pass];
 	if (!d.active) return;  // uniform: the digit is equal in every key
@@ -445,12 +670,7 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const u32 tid = threadIdx.x;
 	const u32 lane = lane_id();
 	const u32 w = wave_id();
-	for (u32 i = lane; i < kRadixMax; i += kWave) s_whist[w][i] = 0;
-	const u32 tile_base = tile * kTile;
-	const u32 mask = (1u << bits) - 1u;
-	const u32 shift = d.shift;
-	const K* __restrict__ src = b.k[d.src];
-	K* __restrict__ dst = b.k[d.dst];
+	for (u32 i = lane; i < kRadixMax; i += kWave) sm.whist[w][i] = 0;
 	u32 goff = 0;
 	if (fused_scan) {
 		// Offset of digit d = (#keys with a smaller digit) + (#keys with digit
@@ -475,150 +695,16 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 				pre += (t2 < tile) ? c : 0u;
 			}
 		}
-		const u32 base = block_exclusive_sum<NT>(tid < kRadixMax ? tot : 0u, s_wsum,
+		const u32 base = block_exclusive_sum<NT>(tid < kRadixMax ? tot : 0u, sm.wsum,
 			static_cast<u32*>(nullptr));
 		goff = tid < kRadixMax ? base + pre : 0u;
 	} else if (tid < kRadixMax) {
 		goff = offsets[static_cast<size_t>(tile) * kRadixMax + tid];
 	}
-
-	// Load (coalesced: 64 consecutive elements per wave instruction).
-	const u32 wbase = tile_base + w * kWave * ITEMS;
-	K k[ITEMS];
-	u32 v[ITEMS];
-	// A wave issuing its loads (and later its scatter stores) runs at raised
-	// priority so its memory requests go out ahead of other waves' ranking
-	// work (measured: 2% on the 2^28 u64 KV downsweep).
-	__builtin_amdgcn_s_setprio(3);
-#pragma unroll
-	for (int j = 0; j < ITEMS; ++j) {
-		const u32 idx = wbase + j * kWave + lane;
-		k[j] = idx < n ? src[idx] : K(0);
-	}
-	if (KV) {
-		const u32* __restrict__ vsrc = b.v[d.src];
-#pragma unroll
-		for (int j = 0; j < ITEMS; ++j) {
-			const u32 idx = wbase + j * kWave + lane;
-			v[j] = idx < n ? vsrc[idx] : 0u;
-		}
-	}
-	__builtin_amdgcn_s_setprio(0);
-	__syncthreads();
-
-	// Rank inside the wave.  The lanes holding one digit (found with one ballot
-	// per digit bit) form a peer group; its lowest lane adds the group's size
-	// to the wave's running digit count with one LDS atomic that returns the
-	// previous count.  The atomics of all items issue back to back (a wave's
-	// LDS operations complete in order, so item j sees the counts of items
-	// before it); each lane then takes its group's base from the leader.
-	// Items go in groups of kGroup (8 atomics in flight, 8 extra registers).
-	constexpr int kGroup = ITEMS < 8 ? ITEMS : 8;
-	static_assert(ITEMS % kGroup == 0, "whole groups");
-	u32 rank[ITEMS];  // (leader lane << 16) | peers below, then the rank
-#pragma unroll
-	for (int j0 = 0; j0 < ITEMS; j0 += kGroup) {
-		u32 got[kGroup];
-#pragma unroll
-		for (int jj = 0; jj < kGroup; ++jj) {
-			const int j = j0 + jj;
-			const bool valid = (wbase + j * kWave + lane) < n;
-			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
-			const u64 peers = match_peers<FIXED_BITS ? FIXED_BITS : 8>(dig,
-				FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits, valid);
-			const u32 below = count_below(peers);
-			const u32 leader = peers ? static_cast<u32>(__builtin_ctzll(peers)) : 0u;
-			got[jj] = 0;
-			if (valid && below == 0)
-				got[jj] = atomicAdd(&s_whist[w][dig], static_cast<u32>(__popcll(peers)));
-			rank[j] = (leader << 16) | below;
-		}
-#pragma unroll
-		for (int jj = 0; jj < kGroup; ++jj) {
-			const int j = j0 + jj;
-			rank[j] = static_cast<u32>(__shfl(static_cast<int>(got[jj]),
-				static_cast<int>(rank[j] >> 16), kWave)) + (rank[j] & 0xFFFFu);
-		}
-	}
-	__syncthreads();
-
-	// Per-digit tile totals and per-wave exclusive offsets.
-	u32 total = 0;
-	if (tid < kRadixMax) {
-#pragma unroll
-		for (int ww = 0; ww < NW; ++ww) {
-			const u32 c = s_whist[ww][tid];
-			s_whist[ww][tid] = total;
-			total += c;
-		}
-	}
-	const u32 tstart = block_exclusive_sum<NT>(total, s_wsum, static_cast<u32*>(nullptr));
-	if (tid < kRadixMax) {
-		s_tstart[tid] = tstart;
-		s_gofs[tid] = goff - tstart;
-	}
-	__syncthreads();
-
-	// Stage keys in digit order in LDS.
-	K* s_keys = reinterpret_cast<K*>(s_stage);
-#pragma unroll
-	for (int j = 0; j < ITEMS; ++j) {
-		const bool valid = (wbase + j * kWave + lane) < n;
-		const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
-		if (valid) {
-			rank[j] += s_tstart[dig] + s_whist[w][dig];
-			s_keys[rank[j]] = k[j];
-		}
-	}
-	__syncthreads();
-
-	const u32 count = (n - tile_base) < kTile ? (n - tile_base) : kTile;
-	u32 oidx[ITEMS];
-	__builtin_amdgcn_s_setprio(2);
-	if (d.nact) {
-		const u32 nshift = d.nshift;
-#pragma unroll
-		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * NT + tid;
-			if (i < count) {
-				const K key = s_keys[i];
-				const u32 dig = static_cast<u32>(key >> shift) & mask;
-				const u32 o = s_gofs[dig] + i;
-				dst[o] = key;
-				ndig[o] = static_cast<u8>(static_cast<u32>(key >> nshift) & mask);
-				oidx[r] = o;
-			}
-		}
-	} else {
-#pragma unroll
-		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * NT + tid;
-			if (i < count) {
-				const K key = s_keys[i];
-				const u32 dig = static_cast<u32>(key >> shift) & mask;
-				const u32 o = s_gofs[dig] + i;
-				dst[o] = key;
-				oidx[r] = o;
-			}
-		}
-	}
-	if (KV) {
-		u32* s_vals = reinterpret_cast<u32*>(s_stage);
-		u32* __restrict__ vdst = b.v[d.dst];
-		__syncthreads();
-#pragma unroll
-		for (int j = 0; j < ITEMS; ++j) {
-			const bool valid = (wbase + j * kWave + lane) < n;
-			if (valid) s_vals[rank[j]] = v[j];
-		}
-		__syncthreads();
-#pragma unroll
-		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * NT + tid;
-			if (i < count) vdst[oidx[r]] = s_vals[i];
-		}
-	}
-	__builtin_amdgcn_s_setprio(0);
+	if (tile * kTile + kTile <= n)
+		downsweep_tile<K, KV, NT, ITEMS, FIXED_BITS, true>(sm, b, d, tile, n, bits, goff, ndig);
+	else
+		downsweep_tile<K, KV, NT, ITEMS, FIXED_BITS, false>(sm, b, d, tile, n, bits, goff, ndig);
 }
 
 // ---------------------------------------------------------------------------
@@ -958,7 +1044,7 @@ size_t satradix_get_localmem_usage(CloSort* sorter, cl_uint i, size_t lws_max, s
 	case CLO_SORT_SATRADIX_KIDX_SCAN:
 		return (3 * kGroupThreads + kRadixMax + kGroupThreads / kWave) * 4;
 	case CLO_SORT_SATRADIX_KIDX_DOWNSWEEP:
-		return nt * ipt * (ks > 4 ? ks : 4) + (nt / kWave) * kRadixMax * 4 +
+		return nt * ipt * (ks == 8 ? 4 : (ks > 4 ? ks : 4)) + (nt / kWave) * kRadixMax * 4 +
 			2 * kRadixMax * 4 + (nt / kWave) * 4;
 	case CLO_SORT_SATRADIX_KIDX_COPY:
 		return 0;

@@ -105,6 +105,7 @@ __device__ void mark_heads(u32* s_head, u32* s_j0, const u32* __restrict__ off, 
 	for (u32 j = *s_j0 + tid; j < nseg; j += kScanBlock) {
 		const u64 o = off[j];
 		if (o >= base + kScanTile) break;
+		if (o < base) continue;  // offsets that are not monotone
 		const u32 p = static_cast<u32>(o - base);
 		atomicOr(&s_head[p >> 5], 1u << (p & 31));
 	}
@@ -859,7 +860,10 @@ bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, s
 	// bytes / 4) status words per tile.  Growing reallocates zeroed memory:
 	// epoch 0, no status.
 	u64* status = nullptr;
-	if (!off && tiles <= kOnePassMaxTiles) {
+	// In-place scans (in == out, equal widths) take reduce-then-scan: the
+	// single pass may re-read a slow predecessor tile's input after that tile
+	// has overwritten it with its prefixes.
+	if (!off && tiles <= kOnePassMaxTiles && in != out) {
 		if (!bd->status.ensure((2 + tiles * (acc / 4)) * sizeof(u64), err)) return false;
 		status = static_cast<u64*>(bd->status.ptr);
 	}
@@ -1021,6 +1025,24 @@ void clo_scan_destroy(CloScan* scan) {
 	delete scan;
 }
 
+// Input and output either disjoint or the same address with equal element
+// and sum widths (an in-place scan); anything else overlaps destructively.
+static bool scan_placement_ok(CloScan* sc, CCLBuffer* in, CCLBuffer* out, size_t numel,
+	GError** err) {
+	const size_t ib = numel * clo_type_sizeof(sc->elem_type);
+	const size_t ob = numel * clo_type_sizeof(sc->sum_type);
+	const uintptr_t x = reinterpret_cast<uintptr_t>(in->ptr);
+	const uintptr_t y = reinterpret_cast<uintptr_t>(out->ptr);
+	const bool overlap = ib && ob && x < y + ob && y < x + ib;
+	if (overlap && !(x == y && ib == ob)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"scan output overlaps the input (only an in-place scan with equal element and sum "
+			"widths may share it)");
+		return false;
+	}
+	return true;
+}
+
 CCLEvent* clo_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, size_t lws_max, GError** err) {
 	if (err && *err) return nullptr;
@@ -1033,6 +1055,7 @@ CCLEvent* clo_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueu
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "buffers smaller than numel elements");
 		return nullptr;
 	}
+	if (!scan_placement_ok(scanner, data_in, data_out, numel, err)) return nullptr;
 	if (!use_device(cq_exec->ctx, err)) return nullptr;
 	return scanner->impl.scan_with_device_data(scanner, cq_exec, cq_comm, data_in, data_out,
 		numel, lws_max, err);
@@ -1054,6 +1077,7 @@ CCLEvent* clo_scan_segmented_with_device_data(CloScan* scanner, CCLQueue* cq_exe
 			"buffers smaller than numel elements / nseg offsets, or sizes beyond 2^32-1");
 		return nullptr;
 	}
+	if (!scan_placement_ok(scanner, data_in, data_out, numel, err)) return nullptr;
 	if (!use_device(cq_exec->ctx, err)) return nullptr;
 	if (strcmp(scanner->impl.name, "blelloch") != 0) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmented scan needs a blelloch scanner");

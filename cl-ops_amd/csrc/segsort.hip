@@ -38,8 +38,18 @@ __global__ void seg_reset(SegMeta* m) {
 	}
 }
 
-__device__ __forceinline__ u32 seg_end(const u32* off, u32 j, u32 nseg, u32 numel) {
-	return j + 1 < nseg ? off[j + 1] : numel;
+// Segment j = [off[j], off[j+1]) (the last one ends at numel), clamped into
+// [0, numel]: malformed offsets (decreasing, or past numel) give a wrong
+// order but never an access outside the arrays.
+struct SegRange {
+	u32 start, len;
+};
+__device__ __forceinline__ SegRange seg_range(const u32* off, u32 j, u32 nseg, u32 numel) {
+	u32 end = j + 1 < nseg ? off[j + 1] : numel;
+	if (end > numel) end = numel;
+	u32 start = off[j];
+	if (start > end) start = end;
+	return {start, end - start};
 }
 
 // Appends medium and large segment indices (one atomic per wave and list).
@@ -47,7 +57,7 @@ __global__ __launch_bounds__(256) void seg_classify(const u32* __restrict__ off,
 	u32 numel, u32 W, u32 CAP, u32* __restrict__ medium, u32* __restrict__ large, SegMeta* m) {
 	const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
 	u32 len = 0;
-	if (j < nseg) len = seg_end(off, j, nseg, numel) - off[j];
+	if (j < nseg) len = seg_range(off, j, nseg, numel).len;
 	const bool is_med = j < nseg && len > W && len <= CAP;
 	const bool is_lrg = j < nseg && len > CAP;
 	const u64 bm = ballot(is_med), bl = ballot(is_lrg);
@@ -231,8 +241,9 @@ __device__ void sort_key_passes(SegSmem<K, KV, IPT>& s, u32 n, bool carry_sid) {
 // Medium segment: whole segment in LDS.
 template <typename K, bool KV, int IPT>
 __device__ void sort_medium(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
-	const u32 start = a.off[j];
-	const u32 len = seg_end(a.off, j, a.nseg, a.numel) - start;
+	const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
+	const u32 start = sr.start;
+	const u32 len = sr.len;
 	g2s<K, IPT>(s.k, a.kin + start, len);
 	if (KV) g2s<u32, IPT>(s.v, a.vin + start, len);
 	__syncthreads();
@@ -257,13 +268,14 @@ __device__ __forceinline__ u32 lower_bound(const u32* off, u32 nseg, u32 x) {
 template <typename K, bool KV, int IPT>
 __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	constexpr u32 W = SegSmem<K, KV, IPT>::kW;
+	constexpr u32 CAP = SegSmem<K, KV, IPT>::kCap;
 	const u32 lo = c * W;
 	const u32 hi = lo + W;
 	if (threadIdx.x == 0) {
 		const u32 j0 = lower_bound(a.off, a.nseg, lo);
 		const u32 j1 = lower_bound(a.off, a.nseg, hi);
 		s.misc[0] = j0;
-		s.misc[1] = j1 - j0;
+		s.misc[1] = j1 > j0 ? j1 - j0 : 0u;  // offsets that are not monotone: none
 	}
 	__syncthreads();
 	const u32 j0 = s.misc[0];
@@ -274,25 +286,29 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	u32 carry = 0, m = 0;
 	for (u32 r0 = 0; r0 < mall; r0 += kRB) {
 		const u32 r = r0 + threadIdx.x;
-		u32 len = 0;
+		u32 len = 0, start = 0;
 		const u32 j = j0 + r;
 		if (r < mall) {
-			len = seg_end(a.off, j, a.nseg, a.numel) - a.off[j];
-			if (len > W) len = 0;
+			const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
+			start = sr.start;
+			len = sr.len > W ? 0u : sr.len;
 		}
 		u32 tot, cnt;
 		const u32 ex = block_exclusive_sum<kRB>(len, s.rs.wsum, &tot);
 		const u32 exc = block_exclusive_sum<kRB>(len ? 1u : 0u, s.rs.wsum, &cnt);
-		if (len) {
-			s.segj[m + exc] = a.off[j];  // the segment's start in the array
+		if (len && m + exc < W) {
+			s.segj[m + exc] = start;  // the segment's start in the array
 			s.lpos[m + exc] = carry + ex;
 		}
 		carry += tot;
 		m += cnt;
 	}
-	if (threadIdx.x == 0) s.lpos[m] = carry;
+	// Well-formed offsets give m <= W and carry <= CAP (distinct starts in a
+	// window of W elements, each segment at most W long).  Malformed ones can
+	// exceed both: the chunk is then left out, never written past LDS.
+	if (threadIdx.x == 0) s.lpos[m < W ? m : W] = carry;
 	__syncthreads();
-	const u32 n = carry;  // <= CAP
+	const u32 n = (m > W || carry > CAP) ? 0u : carry;
 	if (n == 0) return;
 	// Gather: element i belongs to the included segment r with
 	// lpos[r] <= i < lpos[r+1] (fixed-length binary search, m <= W < 2^13),
@@ -358,8 +374,9 @@ template <typename K, bool KV, int IPT>
 __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 	constexpr u32 CAP = SegSmem<K, KV, IPT>::kCap;
 	constexpr u32 P = sizeof(K);
-	const u32 start = a.off[j];
-	const u32 len = seg_end(a.off, j, a.nseg, a.numel) - start;
+	const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
+	const u32 start = sr.start;
+	const u32 len = sr.len;
 	const u32 tid = threadIdx.x;
 	for (u32 i = tid; i < P * 256; i += kRB) (&s.hist[0][0])[i] = 0;
 	__syncthreads();

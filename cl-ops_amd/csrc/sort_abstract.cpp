@@ -104,12 +104,63 @@ static bool check_device_args(CloSort* s, CCLQueue* cq, CCLBuffer* in, CCLBuffer
 	return use_device(cq->ctx, err);
 }
 
+// True when the byte ranges [a, a + an) and [b, b + bn) share a byte.
+static bool ranges_overlap(const CCLBuffer* a, size_t an, const CCLBuffer* b, size_t bn) {
+	if (!a || !b || !an || !bn) return false;
+	const uintptr_t x = reinterpret_cast<uintptr_t>(a->ptr);
+	const uintptr_t y = reinterpret_cast<uintptr_t>(b->ptr);
+	return x < y + bn && y < x + an;
+}
+
+// Where the output of `bytes` bytes goes relative to the input: 1 = in place
+// (out NULL or the same address), 0 = a disjoint buffer, -1 = a partial
+// overlap, which no algorithm here can sort through.
+static int placement(const CCLBuffer* in, const CCLBuffer* out, size_t bytes) {
+	if (!out || out->ptr == in->ptr) return 1;
+	return ranges_overlap(in, bytes, out, bytes) ? -1 : 0;
+}
+
+// Key-value buffers: keys and values are both sorted in place or both out of
+// place, and no two of the distinct buffers share a byte.  On success, an
+// in-place call has keys_out and vals_out set to NULL.
+static bool check_kv_placement(CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer** keys_out,
+	CCLBuffer** vals_out, size_t kbytes, size_t vbytes, GError** err) {
+	const int pk = placement(keys_in, *keys_out, kbytes);
+	const int pv = placement(vals_in, *vals_out, vbytes);
+	if (pk < 0 || pv < 0 || pk != pv) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"key-value sort: keys and values must both be sorted in place (output NULL or "
+			"equal to the input) or both into buffers disjoint from the inputs");
+		return false;
+	}
+	if (pk == 1) {
+		*keys_out = nullptr;
+		*vals_out = nullptr;
+	}
+	const CCLBuffer* kb[2] = {keys_in, *keys_out};
+	const CCLBuffer* vb[2] = {vals_in, *vals_out};
+	bool clash = false;
+	for (const CCLBuffer* k : kb)
+		for (const CCLBuffer* v : vb) clash = clash || ranges_overlap(k, kbytes, v, vbytes);
+	if (clash) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"key-value sort: key and value buffers overlap");
+		return false;
+	}
+	return true;
+}
+
 CCLEvent* clo_sort_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, size_t lws_max, GError** err) {
 	if (err && *err) return nullptr;
 	if (!check_device_args(sorter, cq_exec, data_in, data_out, numel,
 			sorter ? clo_type_sizeof(sorter->elem_type) : 0, err))
 		return nullptr;
+	if (placement(data_in, data_out, numel * clo_type_sizeof(sorter->elem_type)) < 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"data_out overlaps data_in without being the same buffer");
+		return nullptr;
+	}
 	if (sorter->snip)
 		return snippet_sort(sorter, cq_exec, data_in, nullptr, data_out, nullptr, numel, err);
 	return sorter->impl.sort_with_device_data(sorter, cq_exec, cq_comm, data_in, data_out,
@@ -146,13 +197,14 @@ CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQu
 	if (!check_device_args(sorter, cq_exec, keys_in, keys_out, numel,
 			sorter ? clo_type_sizeof(sorter->elem_type) : 0, err))
 		return nullptr;
-	if (!vals_in || vals_in->size < numel * 4 || (vals_out && vals_out->size < numel * 4) ||
-		((keys_out == nullptr) != (vals_out == nullptr))) {
+	if (!vals_in || vals_in->size < numel * 4 || (vals_out && vals_out->size < numel * 4)) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
-			"KV sort needs uint32 value buffers of numel elements; keys_out and vals_out "
-			"must both be NULL (in place) or both set");
+			"KV sort needs uint32 value buffers of numel elements");
 		return nullptr;
 	}
+	if (!check_kv_placement(keys_in, vals_in, &keys_out, &vals_out,
+			numel * clo_type_sizeof(sorter->elem_type), numel * 4, err))
+		return nullptr;
 	if (sorter->snip && (strcmp(sorter->impl.name, "satradix") == 0 ||
 			strcmp(sorter->impl.name, "gselect") == 0))
 		return snippet_sort(sorter, cq_exec, keys_in, vals_in, keys_out, vals_out, numel, err);
@@ -208,11 +260,19 @@ CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec
 	}
 	if (!offsets || offsets->size < nseg * 4 || nseg > 0xFFFFFFFFull ||
 		(vals_in && (vals_in->size < numel * 4 || (vals_out && vals_out->size < numel * 4))) ||
-		(vals_in && ((keys_out == nullptr) != (vals_out == nullptr))) ||
 		(!vals_in && vals_out)) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
 			"segmented sort needs nseg uint32 offsets and, for key-value data, uint32 value "
-			"buffers of numel elements (keys_out/vals_out both NULL or both set)");
+			"buffers of numel elements");
+		return nullptr;
+	}
+	const size_t kbytes = numel * clo_type_sizeof(sorter->elem_type);
+	if (vals_in) {
+		if (!check_kv_placement(keys_in, vals_in, &keys_out, &vals_out, kbytes, numel * 4, err))
+			return nullptr;
+	} else if (placement(keys_in, keys_out, kbytes) < 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"keys_out overlaps keys_in without being the same buffer");
 		return nullptr;
 	}
 	uint32_t tile = sorter->seg_tile ? sorter->seg_tile : 16u;  // 16: measured best on MI355X

@@ -24,6 +24,20 @@ __host__ __device__ __forceinline__ u64 mix64(u64 z) {
 
 __device__ __forceinline__ u64 U(u64 seed, u64 i) { return mix64(seed + (i + 1) * kGolden); }
 
+// Position whose ascending value position c holds in the "presorted with 1%
+// random swaps" sequence: blocks of 100 positions are paired (block b with
+// b ^ m, m = 1 + U(seed ^ kPairing, 0) % 1023) and each pair exchanges one
+// position of each block, chosen by u = U(seed ^ kPerturb, lower block).
+constexpr u64 kPairing = 0x2545f4914f6cdd1dull;
+__device__ __forceinline__ u64 swapped_position(u64 seed, u64 c) {
+	const u64 m = 1 + U(seed ^ kPairing, 0) % 1023;
+	const u64 b = c / 100, b2 = b ^ m;
+	const u64 lo = b < b2 ? b : b2, hi = b < b2 ? b2 : b;
+	const u64 u = U(seed ^ kPerturb, lo);
+	const u64 x = lo * 100 + u % 100, y = hi * 100 + (u >> 32) % 100;
+	return c == x ? y : (c == y ? x : c);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(256) void util_generate(void* out, u64 n, u64 seed, u64 param,
 	u64 start) {
@@ -53,11 +67,10 @@ __global__ __launch_bounds__(256) void util_generate(void* out, u64 n, u64 seed,
 			static_cast<u64*>(out)[i] = U(seed, c) % param;
 			break;
 		case CLO_GEN_NEARLY_SORTED_U32:
-			static_cast<u32*>(out)[i] = U(seed ^ kPerturb, c) % 100 == 0
-				? static_cast<u32>(U(seed, c) >> 32) : static_cast<u32>(c * param);
+			static_cast<u32*>(out)[i] = static_cast<u32>(swapped_position(seed, c) * param);
 			break;
 		case CLO_GEN_NEARLY_SORTED_U64:
-			static_cast<u64*>(out)[i] = U(seed ^ kPerturb, c) % 100 == 0 ? U(seed, c) : c * param;
+			static_cast<u64*>(out)[i] = swapped_position(seed, c) * param;
 			break;
 		case CLO_GEN_REVERSED_U32:
 			static_cast<u32*>(out)[i] = 0xFFFFFFFFu - static_cast<u32>(c * param);
@@ -115,6 +128,29 @@ __global__ __launch_bounds__(256) void util_check_sorted(const T* k, const u32* 
 		T a = k[i], b = k[i + 1];
 		if (a > b) ++c;
 		else if (v && a == b && !(v[i] < v[i + 1])) ++c;
+	}
+	c = wave_sum(c);
+	if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(bad),
+		static_cast<unsigned long long>(c));
+}
+
+// Pairing and permutation: vout[i] must be an index < n that no other i
+// names (a bitmap of n bits records the indices seen), and kout[i] must be
+// the input key at that index.
+template <typename T>
+__global__ __launch_bounds__(256) void util_check_pairs(const T* kin, const T* kout,
+	const u32* vout, u64 n, u32* seen, u64* bad) {
+	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+	u64 c = 0;
+	for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const u32 v = vout[i];
+		if (v >= n) {
+			++c;
+			continue;
+		}
+		const u32 bit = 1u << (v & 31u);
+		if (atomicOr(&seen[v >> 5], bit) & bit) ++c;
+		if (kout[i] != kin[v]) ++c;
 	}
 	c = wave_sum(c);
 	if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(bad),
@@ -317,6 +353,44 @@ size_t clo_util_check_sorted(CCLQueue* cq, CCLBuffer* keys, CCLBuffer* vals, siz
 		(void)hipFreeAsync(bad, cq->stream);
 		return static_cast<size_t>(-1);
 	}
+	u64 r = 0;
+	if (!read_u64(cq, bad, &r, err)) return static_cast<size_t>(-1);
+	return static_cast<size_t>(r);
+}
+
+size_t clo_util_check_pairs(CCLQueue* cq, CCLBuffer* keys_in, CCLBuffer* keys_out,
+	CCLBuffer* vals_out, size_t n, size_t width, GError** err) {
+	if (!cq || !keys_in || !keys_out || !vals_out || keys_in->size < n * width ||
+		keys_out->size < n * width || vals_out->size < n * 4 || n > 0xFFFFFFFFull) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "invalid pair-check arguments");
+		return static_cast<size_t>(-1);
+	}
+	if (!use_device(cq->ctx, err)) return static_cast<size_t>(-1);
+	u64* bad = nullptr;
+	if (!scratch_u64(cq, &bad, err)) return static_cast<size_t>(-1);
+	u32* seen = nullptr;
+	const size_t words = (n + 31) / 32;
+	if (hip_failed(hipMallocAsync(reinterpret_cast<void**>(&seen), words * 4 + 4, cq->stream),
+			err, "hipMallocAsync") ||
+		hip_failed(hipMemsetAsync(seen, 0, words * 4 + 4, cq->stream), err, "hipMemsetAsync")) {
+		(void)hipFreeAsync(bad, cq->stream);
+		return static_cast<size_t>(-1);
+	}
+	u32 g = grid_for(n);
+	if (g > 2048) g = 2048;
+	const u32* v = static_cast<const u32*>(vals_out->ptr);
+	switch (width) {
+	case 1: util_check_pairs<u8><<<g, 256, 0, cq->stream>>>(static_cast<const u8*>(keys_in->ptr), static_cast<const u8*>(keys_out->ptr), v, n, seen, bad); break;
+	case 2: util_check_pairs<u16><<<g, 256, 0, cq->stream>>>(static_cast<const u16*>(keys_in->ptr), static_cast<const u16*>(keys_out->ptr), v, n, seen, bad); break;
+	case 4: util_check_pairs<u32><<<g, 256, 0, cq->stream>>>(static_cast<const u32*>(keys_in->ptr), static_cast<const u32*>(keys_out->ptr), v, n, seen, bad); break;
+	case 8: util_check_pairs<u64><<<g, 256, 0, cq->stream>>>(static_cast<const u64*>(keys_in->ptr), static_cast<const u64*>(keys_out->ptr), v, n, seen, bad); break;
+	default:
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "width must be 1, 2, 4 or 8");
+		(void)hipFreeAsync(seen, cq->stream);
+		(void)hipFreeAsync(bad, cq->stream);
+		return static_cast<size_t>(-1);
+	}
+	(void)hipFreeAsync(seen, cq->stream);
 	u64 r = 0;
 	if (!read_u64(cq, bad, &r, err)) return static_cast<size_t>(-1);
 	return static_cast<size_t>(r);

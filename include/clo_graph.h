@@ -18,7 +18,14 @@
  *    capturing.  Profiling records are not taken for captured kernels.
  *  - The graph reads and writes the buffers named at capture time; replays
  *    recompute everything from those buffers (data-dependent choices such as
- *    skipped digit passes are made on the device at every replay).
+ *    skipped digit passes are made on the device at every replay).  Caller
+ *    buffers must outlive the graph.
+ *  - The graph holds the device scratch of every sorter/scanner/dispatcher
+ *    used inside the capture: destroying such an object leaves its scratch
+ *    to the graph (freed by clo_graph_destroy), and a later eager call on
+ *    the object that would need larger scratch fails with CLO_ERROR_ARGS
+ *    until the graph is destroyed (calls that fit the scratch run normally
+ *    and may overwrite it; a replay recomputes it).
  */
 #ifndef CLO_GRAPH_H_
 #define CLO_GRAPH_H_

@@ -36,7 +36,9 @@ CloScan* clo_scan_new(const char* type, const char* options,
 void clo_scan_destroy(CloScan* scan);
 
 /* Exclusive scan: data_out[i] = sum_{j<i} (SUM_TYPE) data_in[j], modulo
- * 2^(8*sizeof(SUM_TYPE)).  Asynchronous (returns the last event). */
+ * 2^(8*sizeof(SUM_TYPE)).  Asynchronous (returns the last event).  data_out
+ * may be data_in itself when the element and sum types have the same width
+ * (an in-place scan); any other overlap fails with CLO_ERROR_ARGS. */
 CCLEvent* clo_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* data_in, CCLBuffer* data_out,
 	size_t numel, size_t lws_max, GError** err);

@@ -91,8 +91,11 @@ size_t clo_sort_get_localmem_usage(CloSort* sorter, cl_uint i,
 /* ---- additions (no counterpart in CL_Ops) --------------------------------
  * Key-value sort: keys of the sorter's element type, 32-bit payloads.
  * The result is the stable sort of the pairs by key (payloads of equal keys
- * keep their input order).  keys_out == NULL sorts in place (then vals_out
- * must be NULL too).  Supported by "satradix" (radix), "gselect" (rank) and,
+ * keep their input order).  keys and values are sorted both in place
+ * (output NULL or equal to the input buffer) or both into buffers disjoint
+ * from every input; a mixed call, or buffers that partly overlap, fail with
+ * CLO_ERROR_ARGS before anything is enqueued.  Supported by "satradix"
+ * (radix), "gselect" (rank) and,
  * through a stable rank, none other: bitonic sorts reject KV with
  * CLO_ERROR_ARGS because they are not stable. */
 CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
@@ -107,7 +110,10 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
  * segment [offsets[j], offsets[j+1]) -- the last one ends at numel -- is
  * stable-sorted by key, all segments in one persistent launch.  `offsets` is a
  * device buffer of nseg uint32 segment starts, non-decreasing, offsets[0] = 0.
- * vals_in may be NULL (keys only).  keys_out == NULL sorts in place.
+ * vals_in may be NULL (keys only).  keys_out == NULL sorts in place (the
+ * key-value placement rules of clo_sort_kv_with_device_data apply).
+ * Malformed offsets (decreasing, or beyond numel) are clamped into the
+ * arrays: the order is then unspecified but no access leaves the buffers.
  * Sorter options tune it: segtile=8|16|32 (LDS tile = 256*segtile elements;
  * 8|16 for 64-bit keys), seggrid=<blocks> (0 = three per compute unit). */
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,

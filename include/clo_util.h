@@ -24,9 +24,12 @@ typedef enum {
 	CLO_GEN_ZIPF_U64 = 5,  /* mix64(rank), rank from the CDF table, see below */
 	CLO_GEN_U32_MOD = 6,   /* (U(seed, i) >> 32) % param */
 	CLO_GEN_U64_MOD = 7,   /* U(seed, i) % param */
-	/* Ascending i * param, except that 1% of the positions (those with
-	 * U(seed ^ 0x5bd1e995, i) % 100 == 0) hold a random word (U(seed, i) >> 32
-	 * or U(seed, i)): "presorted with random perturbations". */
+	/* "Presorted with 1% random swaps": position i holds s(i) * param
+	 * (truncated to the width), s a product of disjoint transpositions:
+	 * blocks of 100 positions are paired, block b with b ^ m for
+	 * m = 1 + U(seed ^ 0x2545f4914f6cdd1d, 0) % 1023, and the pair {lo, hi}
+	 * exchanges positions 100 lo + u % 100 and 100 hi + (u >> 32) % 100,
+	 * u = U(seed ^ 0x5bd1e995, lo): 2 positions in every 200 move. */
 	CLO_GEN_NEARLY_SORTED_U32 = 8,
 	CLO_GEN_NEARLY_SORTED_U64 = 9,
 	/* Descending: all-ones minus i * param. */
@@ -68,6 +71,16 @@ cl_ulong clo_util_multiset_hash(CCLQueue* cq, CCLBuffer* buf, size_t n,
  * Returns the number of violations; blocks. */
 size_t clo_util_check_sorted(CCLQueue* cq, CCLBuffer* keys, CCLBuffer* vals,
 	size_t n, size_t width, GError** err);
+
+/* Checks a key-value sort whose input values were the indices 0..n-1:
+ * vals_out must be a permutation of 0..n-1 and keys_out[i] must equal
+ * keys_in[vals_out[i]] (unsigned keys of `width` bytes, n < 2^32).  Returns
+ * the number of violations (out-of-range or repeated indices, mismatched
+ * keys); blocks.  Together with clo_util_check_sorted on (keys_out,
+ * vals_out) a zero result means the output is exactly the stable sort of
+ * the input pairs. */
+size_t clo_util_check_pairs(CCLQueue* cq, CCLBuffer* keys_in, CCLBuffer* keys_out,
+	CCLBuffer* vals_out, size_t n, size_t width, GError** err);
 
 /* Library build description (compiler, target, kernel variants). */
 const char* clo_util_build_info(void);

@@ -72,6 +72,22 @@ static uint64_t orc_U(uint64_t seed, uint64_t i) {
 	return orc_mix64(seed + (i + 1) * 0x9E3779B97F4A7C15ull);
 }
 
+/* "Presorted with 1% random swaps": position c of the ascending sequence
+ * holds the value of position orc_swapped_position(seed, c).  Positions form
+ * blocks of 100; block b is paired with block b ^ m, m = 1 + U(seed ^ S2, 0)
+ * % 1023 (an involution inside each aligned group of 1024 blocks), and each
+ * pair {lo, hi} exchanges one position of each: x = 100 lo + u % 100 with
+ * y = 100 hi + (u >> 32) % 100, u = U(seed ^ S1, lo).  Every position is in
+ * at most one exchange, so 2 positions in 200 move. */
+uint64_t orc_swapped_position(uint64_t seed, uint64_t c) {
+	const uint64_t m = 1 + orc_U(seed ^ 0x2545f4914f6cdd1dull, 0) % 1023;
+	const uint64_t b = c / 100, b2 = b ^ m;
+	const uint64_t lo = b < b2 ? b : b2, hi = b < b2 ? b2 : b;
+	const uint64_t u = orc_U(seed ^ 0x5bd1e995ull, lo);
+	const uint64_t x = lo * 100 + u % 100, y = hi * 100 + (u >> 32) % 100;
+	return c == x ? y : (c == y ? x : c);
+}
+
 void orc_generate(int kind, uint64_t seed, uint64_t param, uint64_t start, size_t n, void* out) {
 	for (size_t i = 0; i < n; ++i) {
 		uint64_t c = i + start;
@@ -83,14 +99,8 @@ void orc_generate(int kind, uint64_t seed, uint64_t param, uint64_t start, size_
 		case 4: ((uint32_t*)out)[i] = (uint32_t)c; break;
 		case 6: ((uint32_t*)out)[i] = (uint32_t)(orc_U(seed, c) >> 32) % (uint32_t)param; break;
 		case 7: ((uint64_t*)out)[i] = orc_U(seed, c) % param; break;
-		case 8:
-			((uint32_t*)out)[i] = orc_U(seed ^ 0x5bd1e995ull, c) % 100 == 0
-				? (uint32_t)(orc_U(seed, c) >> 32) : (uint32_t)(c * param);
-			break;
-		case 9:
-			((uint64_t*)out)[i] = orc_U(seed ^ 0x5bd1e995ull, c) % 100 == 0 ? orc_U(seed, c)
-				: c * param;
-			break;
+		case 8: ((uint32_t*)out)[i] = (uint32_t)(orc_swapped_position(seed, c) * param); break;
+		case 9: ((uint64_t*)out)[i] = orc_swapped_position(seed, c) * param; break;
 		case 10: ((uint32_t*)out)[i] = 0xFFFFFFFFu - (uint32_t)(c * param); break;
 		case 11: ((uint64_t*)out)[i] = ~0ull - c * param; break;
 		default: break;

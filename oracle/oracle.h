@@ -23,6 +23,7 @@ uint64_t orc_mix64(uint64_t z);
 /* kind: 0 U64, 1 U32_HI, 2 U64_MASK, 3 U32_MASK, 4 IOTA_U32, 6 U32_MOD,
  * 7 U64_MOD, 8/9 NEARLY_SORTED_U32/U64, 10/11 REVERSED_U32/U64 (same
  * numbering and formulas as CloGenKind). */
+uint64_t orc_swapped_position(uint64_t seed, uint64_t c);
 void orc_generate(int kind, uint64_t seed, uint64_t param, uint64_t start, size_t n, void* out);
 void orc_zipf_cdf(double* cdf, size_t ranks, double s);
 void orc_generate_zipf(const double* cdf, size_t ranks, uint64_t seed, size_t n, uint64_t* out);

@@ -35,17 +35,29 @@ def test_generator_kinds(oracle):
                           (u >> np.uint64(32)).astype(np.uint32) % np.uint32(1000))
     assert np.array_equal(oracle.generate(oracle.GEN_IOTA_U32, 0, 10, start=3),
                           np.arange(3, 13, dtype=np.uint32))
-    # presorted with 1% perturbed positions; reversed
-    n = 20000
+    # Presorted with 1% random swaps: over whole groups of 1024 blocks of 100
+    # positions the source positions form a permutation made of disjoint
+    # transpositions (an involution), one moved position per block, each
+    # pairing block b with block b ^ m for one m in [1, 1023].
+    n = 2 * 102400
     ns = oracle.generate(oracle.GEN_NEARLY_SORTED_U64, 77, n, param=1000)
-    hit = oracle.generate(oracle.GEN_U64, 77 ^ 0x5bd1e995, n) % np.uint64(100) == 0
-    full = oracle.generate(oracle.GEN_U64, 77, n)
-    idx = np.arange(n, dtype=np.uint64)
-    assert np.array_equal(ns, np.where(hit, full, idx * np.uint64(1000)))
-    assert 0.005 < hit.mean() < 0.015
+    assert np.all(ns % np.uint64(1000) == 0)
+    s = (ns // np.uint64(1000)).astype(np.int64)
+    idx = np.arange(n, dtype=np.int64)
+    assert np.array_equal(np.sort(s), idx)
+    assert np.array_equal(s[s], idx)
+    moved = np.nonzero(s != idx)[0]
+    assert moved.size == n // 100
+    assert np.array_equal(np.bincount(moved // 100, minlength=n // 100), np.ones(n // 100))
+    m = np.unique((moved // 100) ^ (s[moved] // 100))
+    assert m.size == 1 and 1 <= int(m[0]) <= 1023
+    assert np.all(np.diff(np.delete(s, moved)) > 0)
     ns32 = oracle.generate(oracle.GEN_NEARLY_SORTED_U32, 77, n, param=1000)
-    assert np.array_equal(ns32, np.where(hit, (full >> np.uint64(32)).astype(np.uint32),
-                                         (idx * np.uint64(1000)).astype(np.uint32)))
+    assert np.array_equal(ns32, (ns & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    # A prefix and a chunk from `start` agree with the whole sequence.
+    assert np.array_equal(oracle.generate(oracle.GEN_NEARLY_SORTED_U64, 77, 777, 1000, start=5000),
+                          ns[5000:5777])
+    idx = np.arange(n, dtype=np.uint64)
     rv = oracle.generate(oracle.GEN_REVERSED_U32, 0, n, param=7)
     assert np.array_equal(rv, (0xFFFFFFFF - 7 * idx).astype(np.uint32))
     assert np.all(np.diff(rv.astype(np.int64)) < 0)
