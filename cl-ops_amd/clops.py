@@ -160,6 +160,8 @@ def lib():
         "clo_util_build_info": (cp, []),
         "clo_util_count_diff": (sz, [vp, vp, vp, sz, ep]),
         "clo_util_check_pairs": (sz, [vp, vp, vp, vp, sz, sz, ep]),
+        "clo_scan_with_device_data_noalloc": (u32, [vp, vp, vp, vp, sz, ep]),
+        "clo_scan_noalloc_max_numel": (sz, [vp]),
         # dispatcher
         "clo_dispatch_new": (vp, [vp, ep]),
         "clo_dispatch_destroy": (None, [vp]),

@@ -78,9 +78,11 @@ CCLEvent* record_event(CCLQueue* q, const char* name, GError** err) {
 	return e;
 }
 
+thread_local bool g_launch_quiet = false;
+
 Launch::Launch(CCLQueue* q_, const char* name_) : q(q_), name(name_), rec(nullptr) {
 	(void)hipGetLastError();  // clear stale launch errors
-	if (!q->prof || q->capturing) return;
+	if (!q->prof || q->capturing || g_launch_quiet) return;
 	if (!q->prof_filter.empty() &&
 		strncmp(name, q->prof_filter.c_str(), q->prof_filter.size()) != 0)
 		return;

@@ -85,6 +85,11 @@ extern thread_local int g_capture_depth;
 // Activates the context's device for the calling thread.
 bool use_device(CCLContext* ctx, GError** err);
 
+// Set by entry points that must not create anything on the host or device
+// while they run (the no-allocation scan): launches then take no profiling
+// records even on a profiling queue.
+extern thread_local bool g_launch_quiet;
+
 // Kernel launch wrapper: records start/end events when the queue profiles,
 // and converts a launch failure into a GError.
 struct Launch {

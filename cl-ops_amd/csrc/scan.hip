@@ -20,6 +20,7 @@
 
 #include <atomic>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "clo_internal.hpp"
@@ -839,9 +840,18 @@ void blelloch_finalize(CloScan* scanner) {
 	scanner->data = nullptr;
 }
 
-// Launches the scan of numel elements at `in` into `out` on cq_exec.
+// Workspace carved from a caller-provided device region (the no-allocation
+// scan): the single-pass status words first (kWsStatusBytes, zeroed once when
+// the region is set up), then the tile sums, then the segment flags.
+constexpr size_t kWsStatusBytes = ((2 + 2 * static_cast<size_t>(kOnePassMaxTiles)) * 8 + 255) /
+	256 * 256;
+
+// Launches the scan of numel elements at `in` into `out` on cq_exec.  `ws`
+// (device memory) replaces the scanner's own scratch when set; the caller
+// has checked that it is large enough.
 bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, size_t numel,
-	const u32* off, u32 nseg, GError** err, u32 items, void* chain) {
+	const u32* off, u32 nseg, GError** err, u32 items, void* chain,
+	unsigned char* ws = nullptr) {
 	BlellochData* bd = static_cast<BlellochData*>(scanner->data);
 	if (items != 8 && items != 16 && items != 32) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "items per thread must be 8, 16 or 32");
@@ -855,7 +865,11 @@ bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, s
 	const size_t ss = clo_type_sizeof(scanner->sum_type);
 	const size_t acc = ss == 8 ? 8 : 4;
 	const size_t sum_bytes = (tiles * acc + 255) / 256 * 256;
-	if (!bd->tiles.ensure(sum_bytes + tiles * 4, err)) return false;
+	void* tiles_ptr = ws ? static_cast<void*>(ws + kWsStatusBytes) : nullptr;
+	if (!ws) {
+		if (!bd->tiles.ensure(sum_bytes + tiles * 4, err)) return false;
+		tiles_ptr = bd->tiles.ptr;
+	}
 	// Single pass (plain scans): the epoch word, an unused word and (sum
 	// bytes / 4) status words per tile.  Growing reallocates zeroed memory:
 	// epoch 0, no status.
@@ -864,16 +878,20 @@ bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, s
 	// single pass may re-read a slow predecessor tile's input after that tile
 	// has overwritten it with its prefixes.
 	if (!off && tiles <= kOnePassMaxTiles && in != out) {
-		if (!bd->status.ensure((2 + tiles * (acc / 4)) * sizeof(u64), err)) return false;
-		status = static_cast<u64*>(bd->status.ptr);
+		if (ws) {
+			status = reinterpret_cast<u64*>(ws);
+		} else {
+			if (!bd->status.ensure((2 + tiles * (acc / 4)) * sizeof(u64), err)) return false;
+			status = static_cast<u64*>(bd->status.ptr);
+		}
 	}
 	ScanLaunch a;
 	a.in = in;
 	a.out = out;
 	a.n = numel;
 	a.tiles = static_cast<u32>(tiles);
-	a.tsum = bd->tiles.ptr;
-	a.tflag = reinterpret_cast<u32*>(static_cast<char*>(bd->tiles.ptr) + sum_bytes);
+	a.tsum = tiles_ptr;
+	a.tflag = reinterpret_cast<u32*>(static_cast<char*>(tiles_ptr) + sum_bytes);
 	a.off = off;
 	a.nseg = nseg;
 	a.items = items;
@@ -1134,6 +1152,102 @@ size_t clo_scan_get_localmem_usage(CloScan* scanner, cl_uint i, size_t lws_max, 
 void clo_scan_debug_lookback_polls(int polls) {
 	g_lookback_polls.store(polls < 0 ? kLookbackPolls : static_cast<u32>(polls),
 		std::memory_order_relaxed);
+}
+
+}  // extern "C"
+
+// ---- no-allocation scan --------------------------------------------------
+// Workspace of clo_scan_with_device_data_noalloc: a static array that every
+// GPU the library runs on holds until the process ends, each at its own
+// address.  Host code never passes the name `scratch` to a kernel (on the
+// host it is only a stand-in for the symbol): it uses the device address
+// hipGetSymbolAddress returns for the current device, kept per device.
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static unsigned char* g_noalloc_ptr[kNoAllocMaxDevices];
+
+// Clears the single-pass status words (epoch 0, no status published): a
+// kernel rather than a memset, so a first call made under graph capture
+// records no memset node.
+__global__ __launch_bounds__(256) void noalloc_zero(u64* p, u32 words) {
+	for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+		p[i] = 0;
+}
+
+// Largest numel the workspace holds for a sum width: status words, then
+// (acc + 4) bytes per tile of kScanBlock * kScanItems elements (the tile sums
+// rounded up to 256 bytes).
+static size_t noalloc_max_numel(size_t sum_size) {
+	const size_t acc = sum_size == 8 ? 8 : 4;
+	const size_t avail = kNoAllocScratchBytes - kWsStatusBytes - 256;
+	return avail / (acc + 4) * (static_cast<size_t>(kScanBlock) * kScanItems);
+}
+
+extern "C" {
+
+size_t clo_scan_noalloc_max_numel(CloScan* scanner) {
+	return scanner ? noalloc_max_numel(clo_type_sizeof(scanner->sum_type)) : 0;
+}
+
+cl_bool clo_scan_with_device_data_noalloc(CloScan* scanner, CCLQueue* cq_exec,
+	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, GError** err) {
+	if (err && *err) return CL_FALSE;
+	if (!scanner || !cq_exec || !data_in || !data_out) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL scanner, queue or buffer");
+		return CL_FALSE;
+	}
+	if (strcmp(scanner->impl.name, "blelloch") != 0) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "the no-allocation scan needs a blelloch scanner");
+		return CL_FALSE;
+	}
+	const size_t maxn = noalloc_max_numel(clo_type_sizeof(scanner->sum_type));
+	if (numel > maxn) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"numel %zu exceeds the no-allocation scan's limit of %zu elements for %zu-byte "
+			"sums (its workspace is %zu bytes)", numel, maxn,
+			clo_type_sizeof(scanner->sum_type), kNoAllocScratchBytes);
+		return CL_FALSE;
+	}
+	if (data_in->size < numel * clo_type_sizeof(scanner->elem_type) ||
+		data_out->size < numel * clo_type_sizeof(scanner->sum_type)) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "buffers smaller than numel elements");
+		return CL_FALSE;
+	}
+	if (!scan_placement_ok(scanner, data_in, data_out, numel, err)) return CL_FALSE;
+	if (!use_device(cq_exec->ctx, err)) return CL_FALSE;
+	const int dev = cq_exec->ctx->dev;
+	if (dev < 0 || dev >= kNoAllocMaxDevices) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "device index %d out of range", dev);
+		return CL_FALSE;
+	}
+	unsigned char* ws = nullptr;
+	{
+		std::lock_guard<std::mutex> lock(g_noalloc_mu);
+		ws = g_noalloc_ptr[dev];
+		if (!ws) {
+			// Set-up, once per device with that device current: the array's
+			// address on this GPU, and its status words cleared ahead of the
+			// first scan on the same queue.
+			void* p = nullptr;
+			if (hip_failed(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), err,
+					"hipGetSymbolAddress"))
+				return CL_FALSE;
+			ws = static_cast<unsigned char*>(p);
+			noalloc_zero<<<4, 256, 0, cq_exec->stream>>>(reinterpret_cast<u64*>(ws),
+				static_cast<u32>(kWsStatusBytes / 8));
+			if (hip_failed(hipGetLastError(), err, "noalloc_zero")) return CL_FALSE;
+			g_noalloc_ptr[dev] = ws;
+		}
+	}
+	if (numel == 0) return CL_TRUE;
+	g_launch_quiet = true;
+	const bool ok = scan_ptrs(scanner, cq_exec, data_in->ptr, data_out->ptr, numel, nullptr, 0,
+		err, kScanItems, nullptr, ws);
+	g_launch_quiet = false;
+	return ok ? CL_TRUE : CL_FALSE;
 }
 
 }  // extern "C"

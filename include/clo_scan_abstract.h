@@ -67,6 +67,25 @@ const char* clo_scan_get_kernel_name(CloScan* scanner, cl_uint i, GError** err);
 size_t clo_scan_get_localmem_usage(CloScan* scanner, cl_uint i,
 	size_t lws_max, size_t numel, GError** err);
 
+/* No-allocation variant of clo_scan_with_device_data.  No counterpart in
+ * CL_Ops: it is for callers that cannot allocate device memory while they
+ * run (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same result as clo_scan_with_device_data (same kernels), but
+ * its workspace is a 1 MiB static device array that each GPU holds from the
+ * library's load until the process ends, so the call allocates nothing on
+ * the device or the host: no hipMalloc/hipFree, no event (it returns
+ * CL_TRUE/CL_FALSE; wait on the queue), and no profiling records.  numel is
+ * limited to clo_scan_noalloc_max_numel(scanner) (355 M elements with 64-bit
+ * sums, 532 M with 32-bit sums); larger calls fail with CLO_ERROR_ARGS.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (two queues, two host threads) race and must be serialised by
+ * the caller. */
+cl_bool clo_scan_with_device_data_noalloc(CloScan* scanner, CCLQueue* cq_exec,
+	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, GError** err);
+/* Largest numel clo_scan_with_device_data_noalloc accepts for the scanner's
+ * sum type. */
+size_t clo_scan_noalloc_max_numel(CloScan* scanner);
+
 #ifdef __cplusplus
 }
 #endif
