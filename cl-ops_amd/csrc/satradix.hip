@@ -36,6 +36,7 @@
 
 #include "clo_internal.hpp"
 #include "dev_common.hpp"
+#include "radix_tile.hpp"
 
 using namespace clo;
 using namespace clo::dev;
@@ -429,231 +430,10 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 // of LDS instead of 68 KB.  Full tiles (every tile but a ragged last one) run
 // a copy of the body without per-element bounds checks.  Measured on the 2^28
 // u64-key/u32-value sort: 1.326 -> 1.298 ms per downsweep.
-template <typename K, bool KV, int NT, int ITEMS>
-struct DownSmem {
-	static constexpr u32 kTile = NT * ITEMS;
-	static constexpr bool kSplit = sizeof(K) == 8;
-	static constexpr u32 kStageBytes =
-		kTile * (kSplit ? 4u : (sizeof(K) > 4 ? sizeof(K) : (KV ? 4u : sizeof(K))));
-	unsigned char stage[kStageBytes] __attribute__((aligned(16)));
-	u32 whist[NT / kWave][kRadixMax];
-	u32 tstart[kRadixMax];
-	u32 gofs[kRadixMax];
-	u32 wsum[NT / kWave];
-};
-
-// Lanes holding the same digit when every lane of the wave holds an element.
-template <u32 MAXBITS>
-__device__ __forceinline__ u64 match_peers_full(u32 dig, u32 bits) {
-	u32 mlo = 0, mhi = 0;
-#pragma unroll
-	for (u32 bt = 0; bt < MAXBITS; ++bt) {
-		if (bt >= bits) break;
-		const u32 sb = static_cast<u32>(__builtin_amdgcn_sbfe(static_cast<int>(dig),
-			static_cast<int>(bt), 1));
-		const u64 bb = ballot(sb != 0u);
-		mlo = __builtin_amdgcn_bitop3_b32(mlo, static_cast<u32>(bb), sb, 0xF6);
-		mhi = __builtin_amdgcn_bitop3_b32(mhi, static_cast<u32>(bb >> 32), sb, 0xF6);
-	}
-	return ~((static_cast<u64>(mhi) << 32) | mlo);
-}
-
-template <typename K, bool KV, int NT, int ITEMS, int FIXED_BITS, bool FULL>
-__device__ __forceinline__ void downsweep_tile(DownSmem<K, KV, NT, ITEMS>& sm, const Bufs<K>& b,
-	const PassDesc& d, u32 tile, u32 n, u32 bits, u32 goff, u8* __restrict__ ndig) {
-	using S = DownSmem<K, KV, NT, ITEMS>;
-	constexpr u32 kTile = S::kTile;
-	constexpr int NW = NT / kWave;
-	const u32 tid = threadIdx.x;
-	const u32 lane = lane_id();
-	const u32 w = wave_id();
-	const u32 tile_base = tile * kTile;
-	const u32 mask = (1u << bits) - 1u;
-	const u32 shift = d.shift;
-	const K* __restrict__ src = b.k[d.src];
-	K* __restrict__ dst = b.k[d.dst];
-
-	// Load (coalesced: 64 consecutive elements per wave instruction).  A wave
-	// issuing its loads (and later its scatter stores) runs at raised priority
-	// so its memory requests go out ahead of other waves' ranking work
-	// (measured: 2% on the 2^28 u64 KV downsweep).
-	const u32 wbase = tile_base + w * kWave * ITEMS;
-	K k[ITEMS];
-	u32 v[ITEMS];
-	__builtin_amdgcn_s_setprio(3);
-#pragma unroll
-	for (int j = 0; j < ITEMS; ++j) {
-		const u32 idx = wbase + j * kWave + lane;
-		k[j] = (FULL || idx < n) ? src[idx] : K(0);
-	}
-	if (KV) {
-		const u32* __restrict__ vsrc = b.v[d.src];
-#pragma unroll
-		for (int j = 0; j < ITEMS; ++j) {
-			const u32 idx = wbase + j * kWave + lane;
-			v[j] = (FULL || idx < n) ? vsrc[idx] : 0u;
-		}
-	}
-	__builtin_amdgcn_s_setprio(0);
-	__syncthreads();
-
-	// Rank inside the wave.  The lanes holding one digit (found with one ballot
-	// per digit bit) form a peer group; its lowest lane adds the group's size
-	// to the wave's running digit count with one LDS atomic that returns the
-	// previous count.  The atomics of all items issue back to back (a wave's
-	// LDS operations complete in order, so item j sees the counts of items
-	// before it); each lane then takes its group's base from the leader.
-	// Items go in groups of kGroup (8 atomics in flight, 8 extra registers).
-	constexpr int kGroup = ITEMS < 8 ? ITEMS : 8;
-	static_assert(ITEMS % kGroup == 0, "whole groups");
-	constexpr u32 kMatch = FIXED_BITS ? FIXED_BITS : 8;
-	u32 rank[ITEMS];  // (leader lane << 16) | peers below, then the rank
-#pragma unroll
-	for (int j0 = 0; j0 < ITEMS; j0 += kGroup) {
-		u32 got[kGroup];
-#pragma unroll
-		for (int jj = 0; jj < kGroup; ++jj) {
-			const int j = j0 + jj;
-			const bool valid = FULL || (wbase + j * kWave + lane) < n;
-			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
-			const u64 peers = FULL ? match_peers_full<kMatch>(dig, bits)
-				: match_peers<kMatch>(dig, bits, valid);
-			const u32 below = count_below(peers);
-			// An invalid lane has no peers; its leader (lane 63) is never used.
-			const u32 leader = static_cast<u32>(__builtin_ctzll(peers | (1ull << 63)));
-			got[jj] = 0;
-			if (valid && below == 0)
-				got[jj] = atomicAdd(&sm.whist[w][dig], static_cast<u32>(__popcll(peers)));
-			rank[j] = (leader << 16) | below;
-		}
-#pragma unroll
-		for (int jj = 0; jj < kGroup; ++jj) {
-			const int j = j0 + jj;
-			rank[j] = static_cast<u32>(__shfl(static_cast<int>(got[jj]),
-				static_cast<int>(rank[j] >> 16), kWave)) + (rank[j] & 0xFFFFu);
-		}
-	}
-	__syncthreads();
-
-	// Per-digit tile totals and per-wave exclusive offsets.
-	u32 total = 0;
-	if (tid < kRadixMax) {
-#pragma unroll
-		for (int ww = 0; ww < NW; ++ww) {
-			const u32 c = sm.whist[ww][tid];
-			sm.whist[ww][tid] = total;
-			total += c;
-		}
-	}
-	const u32 tstart = block_exclusive_sum<NT>(total, sm.wsum, static_cast<u32*>(nullptr));
-	if (tid < kRadixMax) {
-		sm.tstart[tid] = tstart;
-		sm.gofs[tid] = goff - tstart;
-	}
-	__syncthreads();
-
-	// Stage the keys in digit order in LDS; thread tid then writes elements
-	// tid, tid + NT, ... of the staged tile to their run in the output.
-	const u32 count = FULL ? kTile : ((n - tile_base) < kTile ? (n - tile_base) : kTile);
-	u32 oidx[ITEMS];
-	const u32 nshift = d.nshift;
-	if constexpr (S::kSplit) {
-		u32* s32 = reinterpret_cast<u32*>(sm.stage);
-#pragma unroll
-		for (int j = 0; j < ITEMS; ++j) {
-			const bool valid = FULL || (wbase + j * kWave + lane) < n;
-			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
-			if (valid) {
-				rank[j] += sm.tstart[dig] + sm.whist[w][dig];
-				s32[rank[j]] = static_cast<u32>(static_cast<u64>(k[j]) >> 32);
-			}
-		}
-		__syncthreads();
-		u32 hi[ITEMS];
-#pragma unroll
-		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * NT + tid;
-			hi[r] = (FULL || i < count) ? s32[i] : 0u;
-		}
-		__syncthreads();
-#pragma unroll
-		for (int j = 0; j < ITEMS; ++j) {
-			const bool valid = FULL || (wbase + j * kWave + lane) < n;
-			if (valid) s32[rank[j]] = static_cast<u32>(k[j]);
-		}
-		__syncthreads();
-		__builtin_amdgcn_s_setprio(2);
-#pragma unroll
-		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * NT + tid;
-			if (FULL || i < count) {
-				const K key = static_cast<K>((static_cast<u64>(hi[r]) << 32) | s32[i]);
-				const u32 dig = static_cast<u32>(key >> shift) & mask;
-				const u32 o = sm.gofs[dig] + i;
-				dst[o] = key;
-				if (d.nact) ndig[o] = static_cast<u8>(static_cast<u32>(key >> nshift) & mask);
-				oidx[r] = o;
-			}
-		}
-	} else {
-		K* s_keys = reinterpret_cast<K*>(sm.stage);
-#pragma unroll
-		for (int j = 0; j < ITEMS; ++j) {
-			const bool valid = FULL || (wbase + j * kWave + lane) < n;
-			const u32 dig = static_cast<u32>(k[j] >> shift) & mask;
-			if (valid) {
-				rank[j] += sm.tstart[dig] + sm.whist[w][dig];
-				s_keys[rank[j]] = k[j];
-			}
-		}
-		__syncthreads();
-		__builtin_amdgcn_s_setprio(2);
-		if (d.nact) {
-#pragma unroll
-			for (int r = 0; r < ITEMS; ++r) {
-				const u32 i = r * NT + tid;
-				if (FULL || i < count) {
-					const K key = s_keys[i];
-					const u32 dig = static_cast<u32>(key >> shift) & mask;
-					const u32 o = sm.gofs[dig] + i;
-					dst[o] = key;
-					ndig[o] = static_cast<u8>(static_cast<u32>(key >> nshift) & mask);
-					oidx[r] = o;
-				}
-			}
-		} else {
-#pragma unroll
-			for (int r = 0; r < ITEMS; ++r) {
-				const u32 i = r * NT + tid;
-				if (FULL || i < count) {
-					const K key = s_keys[i];
-					const u32 dig = static_cast<u32>(key >> shift) & mask;
-					const u32 o = sm.gofs[dig] + i;
-					dst[o] = key;
-					oidx[r] = o;
-				}
-			}
-		}
-	}
-	if (KV) {
-		u32* s_vals = reinterpret_cast<u32*>(sm.stage);
-		u32* __restrict__ vdst = b.v[d.dst];
-		__syncthreads();
-#pragma unroll
-		for (int j = 0; j < ITEMS; ++j) {
-			const bool valid = FULL || (wbase + j * kWave + lane) < n;
-			if (valid) s_vals[rank[j]] = v[j];
-		}
-		__syncthreads();
-#pragma unroll
-		for (int r = 0; r < ITEMS; ++r) {
-			const u32 i = r * NT + tid;
-			if (FULL || i < count) vdst[oidx[r]] = s_vals[i];
-		}
-	}
-	__builtin_amdgcn_s_setprio(0);
-}
-
+// The per-tile work lives in radix_tile.hpp (tile_pass), shared with the MSD
+// partition levels.  Measured on the 2^28 u64-key/u32-value sort: split
+// staging of 8-byte keys and the check-free full-tile body take the
+// downsweep from 1.326 to 1.298 ms.
 template <typename K, bool KV, int NT, int ITEMS, int FIXED_BITS>
 __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits_arg,
@@ -661,7 +441,7 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const u32 bits = FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits_arg;
 	constexpr u32 kTile = NT * ITEMS;
 	static_assert(NT >= kRadixMax, "one thread per digit bin");
-	__shared__ DownSmem<K, KV, NT, ITEMS> sm;
+	__shared__ TileSmem<K, KV, NT, ITEMS> sm;
 
 	const PassDesc d = plan->pass[pass];
 	if (!d.active) return;  // uniform: the digit is equal in every key
@@ -701,10 +481,20 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	} else if (tid < kRadixMax) {
 		goff = offsets[static_cast<size_t>(tile) * kRadixMax + tid];
 	}
-	if (tile * kTile + kTile <= n)
-		downsweep_tile<K, KV, NT, ITEMS, FIXED_BITS, true>(sm, b, d, tile, n, bits, goff, ndig);
+	TilePass<K> tp;
+	tp.src = b.k[d.src];
+	tp.vsrc = b.v[d.src];
+	tp.dst = b.k[d.dst];
+	tp.vdst = b.v[d.dst];
+	tp.ndig = d.nact ? ndig : nullptr;
+	tp.base = tile * kTile;
+	tp.count = (n - tp.base) < kTile ? (n - tp.base) : kTile;
+	tp.shift = d.shift;
+	tp.nshift = d.nshift;
+	if (tp.count == kTile)
+		tile_pass<K, KV, NT, ITEMS, FIXED_BITS, true>(sm, tp, bits, goff);
 	else
-		downsweep_tile<K, KV, NT, ITEMS, FIXED_BITS, false>(sm, b, d, tile, n, bits, goff, ndig);
+		tile_pass<K, KV, NT, ITEMS, FIXED_BITS, false>(sm, tp, bits, goff);
 }
 
 // ---------------------------------------------------------------------------

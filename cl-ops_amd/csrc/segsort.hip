@@ -26,14 +26,15 @@ namespace {
 struct SegMeta {
 	u32 n_medium;
 	u32 n_large;
+	u32 n_chunks;
 	u32 head;
-	u32 pad;
 };
 
 __global__ void seg_reset(SegMeta* m) {
 	if (threadIdx.x == 0) {
 		m->n_medium = 0;
 		m->n_large = 0;
+		m->n_chunks = 0;
 		m->head = 0;
 	}
 }
@@ -52,24 +53,51 @@ __device__ __forceinline__ SegRange seg_range(const u32* off, u32 j, u32 nseg, u
 	return {start, end - start};
 }
 
-// Appends medium and large segment indices (one atomic per wave and list).
+// Appends the medium and large segments' indices and the windows of W
+// elements in which small segments start (chunk items), one atomic per wave
+// and list; wrange[window] receives the range [j0, j1) of the segments
+// starting in the window.  Small segments starting in one window are
+// consecutive (a longer segment pushes its successor's start past the
+// window), so the first of them records j0 and the last j1; windows without
+// a small segment never become work items.
 __global__ __launch_bounds__(256) void seg_classify(const u32* __restrict__ off, u32 nseg,
-	u32 numel, u32 W, u32 CAP, u32* __restrict__ medium, u32* __restrict__ large, SegMeta* m) {
+	u32 numel, u32 W, u32 CAP, u32* __restrict__ medium, u32* __restrict__ large,
+	u32* __restrict__ chunks, uint2* __restrict__ wrange, SegMeta* m) {
 	const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-	u32 len = 0;
-	if (j < nseg) len = seg_range(off, j, nseg, numel).len;
-	const bool is_med = j < nseg && len > W && len <= CAP;
-	const bool is_lrg = j < nseg && len > CAP;
-	const u64 bm = ballot(is_med), bl = ballot(is_lrg);
-	u32 basem = 0, basel = 0;
+	SegRange sr = {0u, 0u};
+	if (j < nseg) sr = seg_range(off, j, nseg, numel);
+	const bool is_med = j < nseg && sr.len > W && sr.len <= CAP;
+	const bool is_lrg = j < nseg && sr.len > CAP;
+	const bool is_sml = j < nseg && sr.len <= W;
+	bool head = false;
+	if (is_sml) {
+		const u32 win = sr.start / W;
+		bool first = j == 0, last = j + 1 == nseg;
+		if (!first) {
+			const SegRange pr = seg_range(off, j - 1, nseg, numel);
+			first = pr.len > W || pr.start / W != win;
+		}
+		if (!last) {
+			const SegRange nx = seg_range(off, j + 1, nseg, numel);
+			last = nx.len > W || nx.start / W != win;
+		}
+		if (first) wrange[win].x = j;
+		if (last) wrange[win].y = j + 1;
+		head = first;
+	}
+	const u64 bm = ballot(is_med), bl = ballot(is_lrg), bc = ballot(head);
+	u32 basem = 0, basel = 0, basec = 0;
 	if (lane_id() == 0) {
 		if (bm) basem = atomicAdd(&m->n_medium, static_cast<u32>(__popcll(bm)));
 		if (bl) basel = atomicAdd(&m->n_large, static_cast<u32>(__popcll(bl)));
+		if (bc) basec = atomicAdd(&m->n_chunks, static_cast<u32>(__popcll(bc)));
 	}
 	basem = __shfl(basem, 0, kWave);
 	basel = __shfl(basel, 0, kWave);
+	basec = __shfl(basec, 0, kWave);
 	if (is_med) medium[basem + count_below(bm)] = j;
 	if (is_lrg) large[basel + count_below(bl)] = j;
+	if (head) chunks[basec + count_below(bc)] = sr.start / W;
 }
 
 template <typename K>
@@ -83,10 +111,11 @@ struct SegArgs {
 	const u32* off;
 	const u32* medium;
 	const u32* large;
+	const u32* chunks;    // windows holding small segment starts
+	const uint2* wrange;  // per window: segments [x, y) starting in it
 	SegMeta* meta;
 	u32 nseg;
 	u32 numel;
-	u32 nchunks;
 };
 
 template <typename K, bool KV, int IPT>
@@ -253,29 +282,15 @@ __device__ void sort_medium(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) 
 	__syncthreads();
 }
 
-// First segment index with offsets[j] >= x.
-__device__ __forceinline__ u32 lower_bound(const u32* off, u32 nseg, u32 x) {
-	u32 lo = 0, hi = nseg;
-	while (lo < hi) {
-		const u32 mid = (lo + hi) >> 1;
-		if (off[mid] < x) lo = mid + 1;
-		else hi = mid;
-	}
-	return lo;
-}
-
 // Chunk of small segments starting in [c*W, (c+1)*W).
 template <typename K, bool KV, int IPT>
 __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	constexpr u32 W = SegSmem<K, KV, IPT>::kW;
 	constexpr u32 CAP = SegSmem<K, KV, IPT>::kCap;
-	const u32 lo = c * W;
-	const u32 hi = lo + W;
 	if (threadIdx.x == 0) {
-		const u32 j0 = lower_bound(a.off, a.nseg, lo);
-		const u32 j1 = lower_bound(a.off, a.nseg, hi);
-		s.misc[0] = j0;
-		s.misc[1] = j1 > j0 ? j1 - j0 : 0u;  // offsets that are not monotone: none
+		const uint2 r = a.wrange[c];
+		s.misc[0] = r.x;
+		s.misc[1] = r.y > r.x ? r.y - r.x : 0u;  // offsets that are not monotone: none
 	}
 	__syncthreads();
 	const u32 j0 = s.misc[0];
@@ -541,7 +556,7 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 	__shared__ SegSmem<K, KV, IPT> s;
 	const u32 nl = a.meta->n_large;
 	const u32 nm = a.meta->n_medium;
-	const u32 total = nl + nm + a.nchunks;
+	const u32 total = nl + nm + a.meta->n_chunks;
 	for (;;) {
 		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head, 1u);
 		__syncthreads();
@@ -550,7 +565,7 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 		if (item >= total) break;
 		if (item < nl) sort_large<K, KV, IPT>(a, s, a.large[item]);
 		else if (item < nl + nm) sort_medium<K, KV, IPT>(a, s, a.medium[item - nl]);
-		else sort_chunk<K, KV, IPT>(a, s, item - nl - nm);
+		else sort_chunk<K, KV, IPT>(a, s, a.chunks[item - nl - nm]);
 	}
 }
 
@@ -566,7 +581,6 @@ bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, GError
 	constexpr u32 CAP = kRB * IPT;
 	constexpr u32 W = CAP / 2;
 	SegArgs<K> a = a0;
-	a.nchunks = static_cast<u32>(div_ceil(a.numel, W));
 	hipStream_t st = q->stream;
 	{
 		Launch L(q, "segsort_reset");
@@ -577,7 +591,8 @@ bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, GError
 		Launch L(q, "segsort_classify");
 		const u32 g = static_cast<u32>(div_ceil(a.nseg, 256));
 		seg_classify<<<g ? g : 1, 256, 0, st>>>(a.off, a.nseg, a.numel, W, CAP,
-			const_cast<u32*>(a.medium), const_cast<u32*>(a.large), a.meta);
+			const_cast<u32*>(a.medium), const_cast<u32*>(a.large), const_cast<u32*>(a.chunks),
+			const_cast<uint2*>(a.wrange), a.meta);
 		if (!L.done(err)) return false;
 	}
 	{
@@ -603,7 +618,11 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	SegData* sd = &static_cast<clo_segsort_cache*>(*cache)->d;
 	const size_t ks = clo_type_sizeof(sorter->elem_type);
 	if (numel == 0 || nseg == 0) return record_event(cq, "segsort_done", err);
-	if (!sd->lists.ensure(2 * nseg * sizeof(u32), err) || !sd->meta.ensure(sizeof(SegMeta), err) ||
+	// medium, large and chunk lists (nseg entries each), then [j0, j1) per
+	// window of the smallest tile's W = 1024 elements.
+	const size_t nwin = numel / 1024 + 2;
+	if (!sd->lists.ensure(3 * nseg * sizeof(u32) + 64 + nwin * sizeof(uint2), err) ||
+		!sd->meta.ensure(sizeof(SegMeta), err) ||
 		!sd->kscr.ensure(numel * ks, err) || (vin && !sd->vscr.ensure(numel * 4, err)))
 		return nullptr;
 	if (grid == 0) {
@@ -628,10 +647,12 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	a.off = static_cast<const u32*>(offsets->ptr);                                           \
 	a.medium = lists;                                                                        \
 	a.large = lists + nseg;                                                                  \
+	a.chunks = lists + 2 * nseg;                                                             \
+	a.wrange = reinterpret_cast<const uint2*>(lists + (3 * nseg + 3) / 4 * 4 + 4);          \
 	a.meta = static_cast<SegMeta*>(sd->meta.ptr);                                            \
 	a.nseg = static_cast<u32>(nseg);                                                         \
 	a.numel = static_cast<u32>(numel);                                                       \
-	a.nchunks = 0;
+
 	bool ok = false;
 	if (ks == 4) {
 		SEG_ARGS(u32)

@@ -141,6 +141,14 @@ def test_segmented_scan(clops, oracle, ctx, queue, elem, sumt, nseg, max_len):
     assert np.array_equal(bo.read(queue, sumt, n), _ref_segscan(oracle, a, off, n, sumt))
 
 
+def _mix64(x):
+    z = x.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
 def test_config4_shape(clops, oracle, ctx, queue):
     """32768 log-uniform segments in [1, 65536]: device checks + a bit-exact sample."""
     nseg = 32768
@@ -158,6 +166,16 @@ def test_config4_shape(clops, oracle, ctx, queue):
     got = kb.read(queue, np.uint32, n)
     k = oracle.generate(oracle.GEN_U32_HI, 0xC4B, n)
     ends = np.append(off[1:], n)
+    # Every segment: non-decreasing inside, and the same multiset as its input
+    # (per-segment sums of the splitmix64 finalizer of each key, mod 2^64).
+    inner = np.ones(max(n - 1, 0), dtype=bool)
+    inner[off[1:][off[1:] > 0] - 1] = False
+    assert not np.any((got[1:] < got[:-1]) & inner)
+    nz = lens > 0
+    with np.errstate(over="ignore"):
+        hs_in = np.add.reduceat(_mix64(k), off[nz])
+        hs_out = np.add.reduceat(_mix64(got), off[nz])
+    assert np.array_equal(hs_in, hs_out)
     for j in list(range(0, nseg, 97)) + [int(np.argmax(lens))]:
         s0, e0 = int(off[j]), int(ends[j])
         assert np.array_equal(got[s0:e0], np.sort(k[s0:e0])), j
