@@ -133,23 +133,25 @@ def headline(args, ctx):
     q.finish()
     s = clops.Sorter(ctx, "satradix", np.uint64)
 
-    def step():
-        s.sort_kv_device(q, kin, vin, kout, vout, n)
+    def step(qq):
+        s.sort_kv_device(qq, kin, vin, kout, vout, n)
 
-    # Timed region: events around the dominant kernel's launches only (8 of
-    # the ~40 launches of a sort), so the per-kernel timing adds little to the
-    # step time.  A second pass with events around every kernel gives the
-    # per-kernel breakdown for the log.
+    # Timed region 1 (value): the steps with no per-kernel events, only the
+    # markers around the region.  Timed region 2 (roofline): the same steps
+    # with HIP events around the dominant kernel's launches on the exec
+    # stream (8 of the ~40 launches of a sort).  Events around kernels cost
+    # time of their own (0.2-0.3 ms per step here), so the un-instrumented
+    # region gives the throughput.  A third pass with events around every
+    # kernel gives the per-kernel breakdown for the log.
+    q_plain = clops.Queue(ctx)
+    ms = time_steps(q_plain, lambda: step(q_plain), args.steps, args.warmup)
     q.prof_filter("satradix_downsweep")
-    ms = time_steps(q, step, args.steps, args.warmup)
+    ms_ds = time_steps(q, lambda: step(q), args.steps, 1)
     cnt, tot = q.prof("satradix_downsweep")
     q.prof_filter("")
-    ms_all = time_steps(q, step, args.steps, 1)
-    q_plain = clops.Queue(ctx)
-    ms_plain = time_steps(q_plain, lambda: s.sort_kv_device(q_plain, kin, vin, kout, vout, n),
-                          args.steps, 1)
-    log("  same steps with events around every kernel: %.3f ms/step; without events: "
-        "%.3f ms/step" % (ms_all, ms_plain))
+    ms_all = time_steps(q, lambda: step(q), args.steps, 1)
+    log("  same steps with events around the downsweeps: %.3f ms/step; around every kernel: "
+        "%.3f ms/step; without events: %.3f ms/step" % (ms_ds, ms_all, ms))
     for name in q.prof_names():
         c, t = q.prof(name)
         log("  kernel %-22s launches/step %5.1f  avg %.4f ms  total/step %.4f ms"
@@ -171,8 +173,8 @@ def headline(args, ctx):
     if bad != 0 or pairs != 0 or not same:
         raise SystemExit("benchmark output failed verification")
     return {
-        "ms": ms, "n": n, "passes": passes, "scatter_ms": scatter_ms, "achieved": achieved,
-        "bytes_per_launch": bytes_per_launch,
+        "ms": ms, "ms_downsweep_events": ms_ds, "n": n, "passes": passes,
+        "scatter_ms": scatter_ms, "achieved": achieved, "bytes_per_launch": bytes_per_launch,
     }
 
 
@@ -705,6 +707,9 @@ def main():
             "traffic": traffic_from_profiles(),
             "kernel": "satradix_downsweep",
             "kernel_avg_ms": round(h["scatter_ms"], 4),
+            "timed_region": "%d steps with HIP events around every satradix_downsweep launch "
+                            "(%.4f ms/step; value comes from the same steps without events)"
+                            % (args.steps, h["ms_downsweep_events"]),
             "bytes_per_launch": h["bytes_per_launch"],
         },
         "cpu_baseline": cpu,
