@@ -307,6 +307,107 @@ def test_mt19937_known_answer(oracle):
     assert int(oracle.mt19937(5489, 10000)[-1]) == 4123659995
 
 
+def _xs64_unstep(s):
+    """Inverse of the xorshift64 state update: y = x ^ (x << a) is undone by
+    x = y ^ (y << a) ^ (y << 2a) ^ ..., likewise for right shifts; the three
+    shifts are undone in reverse order."""
+    def unshl(y, a):
+        x, t = y, y
+        while True:
+            t = (t << a) & M64
+            if not t:
+                return x
+            x ^= t
+
+    def unshr(y, a):
+        x, t = y, y
+        while True:
+            t >>= a
+            if not t:
+                return x
+            x ^= t
+    return unshl(unshr(unshl(s, 4), 35), 21)
+
+
+def test_xorshift64_output_word(oracle):
+    """The draw is convert_uint of the new state, i.e. its least significant
+    32 bits (OpenCL 1.2 6.2.3.3, clo_rng_xorshift64.cl:52).  States are chosen
+    through the inverse update so that the new state is a given value: new
+    states equal in the low half give equal draws whatever the high half, and
+    the draw is that low half.  Taking the high half, or any other window,
+    fails."""
+    def draw(state):
+        st = np.array([state], dtype=np.uint64).view(np.uint8)
+        out = oracle.rng_next("xorshift64", st)
+        return out, int(st.view(np.uint64)[0])
+
+    rng = np.random.default_rng(0x64)
+    for s in [int(x) for x in rng.integers(0, 1 << 63, 16, dtype=np.int64)] + [1, M64]:
+        assert draw(_xs64_unstep(s))[1] == s  # the inverse is right
+    for lo in (0, 1, 0x80000000, 0xFFFFFFFF, 0x12345678):
+        outs = set()
+        for hi in (0, 1, 0xFFFFFFFF, 0xDEADBEEF):
+            target = (hi << 32) | lo
+            out, new = draw(_xs64_unstep(target))
+            assert new == target
+            outs.add(out)
+        assert outs == {lo}, (hex(lo), outs)
+
+
+def test_seed_conversion_single_bits(oracle):
+    """Seeds with one bit set, expected words derived by hand from the
+    conversions (clo_rng_xorshift128.cl:30: words = seed >> 0, 16, 32, 46
+    masked to 32 bits; clo_rng_tauslcg.cl:35: as_uint4 of (seed, seed), low
+    half first).  Any other shift moves a bit out of its expected place."""
+    xs128 = {
+        0: [1, 0, 0, 0],
+        15: [1 << 15, 0, 0, 0],
+        16: [1 << 16, 1, 0, 0],
+        31: [1 << 31, 1 << 15, 0, 0],
+        32: [0, 1 << 16, 1, 0],
+        45: [0, 1 << 29, 1 << 13, 0],
+        46: [0, 1 << 30, 1 << 14, 1],
+        47: [0, 1 << 31, 1 << 15, 2],
+        48: [0, 0, 1 << 16, 4],
+        63: [0, 0, 1 << 31, 1 << 17],
+    }
+    for k, want in xs128.items():
+        got = [int(x) for x in oracle.rng_from_seed("xorshift128", 1 << k).view(np.uint32)]
+        assert got == want, k
+    taus = {
+        0x0000000100000002: [2, 1, 2, 1],
+        1 << 63: [0, 1 << 31, 0, 1 << 31],
+        0xFFFFFFFF: [0xFFFFFFFF, 0, 0xFFFFFFFF, 0],
+    }
+    for s, want in taus.items():
+        got = [int(x) for x in oracle.rng_from_seed("tauslcg", s).view(np.uint32)]
+        assert got == want, hex(s)
+
+
+def test_xs1_hash_inverts(oracle):
+    """XS1 (clo_rng_init.cl:30-33) on 64-bit values is a bijection made of
+    x ^ (x >> 16) and a multiply by 0x45d9f3b mod 2^64.  Undoing those steps,
+    x ^ (x >> 16) by x ^ (x >> 16) ^ (x >> 32) ^ (x >> 48) and the multiply by
+    the constant's inverse mod 2^64, must give back gid + seed for every state
+    the oracle wrote; a different shift or constant does not invert."""
+    cinv = pow(0x45D9F3B, -1, 1 << 64)
+
+    def unxs(y):
+        return y ^ (y >> 16) ^ (y >> 32) ^ (y >> 48)
+
+    def inverse(h):
+        h = unxs(h)
+        h = unxs((h * cinv) & M64)
+        return unxs((h * cinv) & M64)
+
+    for seed in (0, 1, 0xFFFF, 1 << 40, M64 - 100):
+        st = oracle.rng_init_dev_gid("lcg", 64, seed, 2).view(np.uint64)
+        for gid in range(64):
+            assert inverse(int(st[gid])) == (gid + seed) & M64, (seed, gid)
+    # XS1(0) = 0.
+    assert int(oracle.rng_init_dev_gid("lcg", 1, 0, 2).view(np.uint64)[0]) == 0
+
+
 def test_seed_conversions(oracle):
     s = 0x0123456789ABCDEF
     xs = oracle.rng_from_seed("xorshift128", s).view(np.uint32)
