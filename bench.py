@@ -479,6 +479,49 @@ def suite_segmented(ctx, q, baseline_stride=16):
                           "timed, scaled by %d/%d)" % (baseline_stride, nseg, idx.size),
                 "ms": ms_base, "speedup_graph": ms_base / ms_graph})
     g.close()
+    res.extend(segmented_tuning(ctx, q, kin, kout, sout, offb, nseg, n))
+    return res
+
+
+def segmented_tuning(ctx, q, kin, kout, kout2, offb, nseg, n):
+    """The exhaustive tuner over the segmented tile x grid configurations on
+    this workload (the library's defaults are its choice), then the learned
+    per-segment work classes: the segment cost model is trained on work-class
+    timings (every length bin up to the tile's CAP, each feasible class) and
+    its class table is timed against the length thresholds alone, outputs
+    compared word for word."""
+    d = clops.Dispatch(ctx)
+    t = d.tune_segmented(q, kin, kout, offb, nseg, n, 4, reps=3)
+    fixed = clops.Dispatch.fixed(clops.OP_SEGSORT, n, 4)
+    best = int(np.argmin(np.where(t > 0, t, np.inf)))
+    res = [{"config": "segmented tuner over tile x grid (%d keys)" % n,
+            "times_ms": {d.config_name(c): round(float(t[c]), 4) for c in range(clops.N_CONFIGS)
+                         if t[c] > 0},
+            "tuner_best": d.config_name(best), "default": d.config_name(fixed),
+            "default_vs_best": round(float(t[fixed] / t[best]), 4)}]
+    X, y = [], []
+    for b in range(clops.SEG_BINS):
+        L = clops.Dispatch.seg_bin_length(b)
+        if L == 0 or L > 256 * 16:
+            continue
+        tc = d.tune_segment_class(q, fixed, 4, L, reps=2)
+        for c in range(3):
+            if tc[c] > 0:
+                X.append(clops.Dispatch.segment_row(L, 4, fixed, c))
+                y.append(np.log2(tc[c]))
+    m = clops.Mlp(ctx, clops.SEG_INPUTS, seed=0x5E)
+    loss = m.train(q, np.array(X, dtype=np.float32), np.array(y, dtype=np.float32), 3000, 3e-3)
+    d.set_segment_model(m)
+    classes = d.segment_classes(q, fixed, 4, 0.05)
+    ms_fixed = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout, offb, nseg, n, 4), 5, 2)
+    ms_cls = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout2, offb, nseg, n, 4,
+                                                   classes=classes), 5, 2)
+    diff = clops.count_diff(q, kout, kout2, n)
+    res.append({"config": "learned per-segment work classes (%s)" % d.config_name(fixed),
+                "classes": "".join(map(str, classes)), "train_rows": len(y),
+                "train_loss": round(loss, 4), "ms_thresholds": ms_fixed, "ms_learned": ms_cls,
+                "differing_words": int(diff)})
+    q.gc()
     return res
 
 
@@ -525,8 +568,9 @@ class RequestData:
         self.cdf = clops.Buffer.from_array(ctx, cdfh)
         self.ranks = cdfh.size
 
-    def fill(self, n, width, dist, seed):
-        q, b = self.q, self.inb
+    def fill(self, n, width, dist, seed, buf=None, q=None):
+        q = q or self.q
+        b = buf or self.inb
         w8 = width == 8
         step = max(((1 << (8 * width)) - 1) // max(n, 1), 1)
         if dist == 0:
@@ -545,8 +589,206 @@ class RequestData:
             clops._call(fn, q.ptr, b.ptr, n, self.cdf.ptr, self.ranks, seed)
 
 
-def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=8000, lr=2e-3,
-                  margin=0.02):
+def _collect(d, q, data, reqs, reps):
+    """Features and exhaustive tuner timings (min of reps timed runs) of
+    every request."""
+    feats, times = [], []
+    for (op, n, width, dist, s) in reqs:
+        data.fill(n, width, dist, s)
+        feats.append(d.features(q, op, data.inb, n, width))
+        times.append(d.tune(q, op, data.inb, data.outb, n, width, reps=reps))
+    q.gc()
+    return np.array(feats), np.array(times)
+
+
+def _rows(reqs, feats, times):
+    """Model rows and targets: log2 of each timed configuration's time over
+    the fixed configuration's time for the same request."""
+    X, y = [], []
+    for (op, n, width, _, _), f, t in zip(reqs, feats, times):
+        t_fixed = t[clops.Dispatch.fixed(op, n, width)]
+        for c in range(clops.N_CONFIGS):
+            if t[c] > 0:
+                X.append(clops.Dispatch.model_row(f, c))
+                y.append(np.log2(t[c] / t_fixed))
+    return np.array(X, dtype=np.float32), np.array(y, dtype=np.float32)
+
+
+def _policy(ctx, q, reqs, feats, times, steps, lr, margin, seed):
+    """A dispatcher whose table and model learn from (reqs, feats, times)."""
+    d = clops.Dispatch(ctx)
+    for (op, n, width, _, _), f, t in zip(reqs, feats, times):
+        d.table_add(op, n, width, f, t)
+    X, y = _rows(reqs, feats, times)
+    m = clops.Mlp(ctx, clops.MODEL_INPUTS, seed=seed)
+    t0 = time.perf_counter()
+    loss = m.train(q, X, y, steps, lr)
+    d.set_model(m)
+    d.set_margin(margin)
+    return d, m, loss, int(y.size), time.perf_counter() - t0
+
+
+def _windows(reqs, pool_bytes, max_count=256):
+    """Consecutive requests packed into windows whose inputs fit one pool
+    (256-byte aligned offsets)."""
+    out, cur, used = [], [], 0
+    for j, r in enumerate(reqs):
+        nb = (r[1] * r[2] + 255) // 256 * 256
+        if cur and (used + nb > pool_bytes or len(cur) == max_count):
+            out.append(cur)
+            cur, used = [], 0
+        cur.append((j, used))
+        used += nb
+    if cur:
+        out.append(cur)
+    return out
+
+
+def run_stream(d, data, reqs, pools, windows, mode, q_exec, q_side, choices=None):
+    """Runs the request stream through two pools of device memory: while the
+    exec queue sorts or scans window w from one pool, the side queue fills
+    window w+1 into the other and, under the model policy, computes its
+    features (one batched launch) and the model's choices (one inference
+    launch).  The exec queue waits for a window's fill with an event, the side
+    queue for the exec queue to be done with a pool before refilling it.
+    mode: "fixed", "model", or "verify" (runs choices[j] and the fixed
+    configuration where they differ and counts differing words).  Returns
+    (host wall s, exec-queue ms from first to last work without the drains
+    that release events every 64 windows, choices, diffs)."""
+    chosen = np.zeros(len(reqs), dtype=np.int32)
+    done = [None, None]
+    diffs = 0
+
+    def prepare(w):
+        slot = w % 2
+        if done[slot] is not None:
+            q_side.wait_for(done[slot])
+        views, ops, ns, ws = [], [], [], []
+        for j, offset in windows[w]:
+            op, n, width, dist, seed = reqs[j]
+            v = pools[slot].view(offset, n * width)
+            data.fill(n, width, dist, seed, buf=v, q=q_side)
+            views.append(v)
+            ops.append(op)
+            ns.append(n)
+            ws.append(width)
+        if mode == "model":
+            f = d.features_batch(q_side, ops, views, ns, ws)
+            cfgs = d.choose(q_side, clops.POLICY_MODEL, ops, ns, ws, f)
+        elif mode == "fixed":
+            cfgs = [clops.Dispatch.fixed(o, n, w_) for o, n, w_ in zip(ops, ns, ws)]
+        else:
+            cfgs = [choices[j] for j, _ in windows[w]]
+        return views, cfgs, q_side.marker()
+
+    q_exec.finish()
+    q_side.finish()
+    t0 = time.perf_counter()
+    ms = 0.0
+    start = q_exec.marker()
+    prep = prepare(0)
+    for w in range(len(windows)):
+        views, cfgs, ready = prep
+        q_exec.wait_for(ready)
+        for (j, _), v, c in zip(windows[w], views, cfgs):
+            op, n, width = reqs[j][:3]
+            chosen[j] = c
+            if mode == "verify":
+                fx = clops.Dispatch.fixed(op, n, width)
+                if c != fx:
+                    d.run(q_exec, int(c), op, v, data.outb, n, width)
+                    d.run(q_exec, fx, op, v, data.out2, n, width)
+                    diffs += clops.count_diff(q_exec, data.outb, data.out2, n * width // 4)
+            else:
+                d.run(q_exec, int(c), op, v, data.outb, n, width)
+        done[w % 2] = q_exec.marker()
+        if w % 64 == 63:
+            # Release the queues' completed events.  Waiting first means no
+            # event still to be waited on is released: the pools are free.
+            # The exec time of the windows so far is taken before, and the
+            # drain itself is left out of it.
+            q_exec.finish()
+            q_side.finish()
+            ms += q_exec.elapsed_ms(start, done[w % 2])
+            q_exec.gc()
+            q_side.gc()
+            done = [None, None]
+            start = q_exec.marker()
+        if w + 1 < len(windows):
+            prep = prepare(w + 1)
+        if w % 200 == 199:
+            log("  %s stream: window %d/%d, %.0f s" % (mode, w + 1, len(windows),
+                                                     time.perf_counter() - t0))
+    end = q_exec.marker()
+    q_exec.finish()
+    wall = time.perf_counter() - t0
+    ms += q_exec.elapsed_ms(start, end)
+    q_exec.gc()
+    q_side.gc()
+    return wall, ms, chosen, diffs
+
+
+def dispatch_full(ctx, d, data, count, seed=0xC5, pool_bytes=8 << 30):
+    """The whole request stream, no exhaustive tuning: once under the fixed
+    configuration and once under the model policy (both timed end to end,
+    fills and the model's batched features and inference included, see
+    run_stream), then an untimed pass that runs the model's choice and the
+    fixed configuration wherever they differ and compares the outputs word
+    for word."""
+    reqs = request_stream(seed, count, 28)
+    pools = [clops.Buffer(ctx, pool_bytes) for _ in range(2)]
+    windows = _windows(reqs, pool_bytes)
+    q_exec, q_side = clops.Queue(ctx), clops.Queue(ctx)
+    res = {}
+    for mode in ("fixed", "model"):
+        wall, ms, chosen, _ = run_stream(d, data, reqs, pools, windows, mode, q_exec, q_side)
+        res[mode] = (wall, ms)
+        if mode == "model":
+            choices = chosen
+        log("  full stream %s: %.1f s wall, %.1f ms exec queue" % (mode, wall, ms))
+    fixed = np.array([clops.Dispatch.fixed(r[0], r[1], r[2]) for r in reqs])
+    wall_v, _, _, diffs = run_stream(d, data, reqs, pools, windows, "verify", q_exec, q_side,
+                                     choices=choices)
+    for p in pools:
+        p.close()
+    return {
+        "full_stream_requests": count,
+        "full_stream_keys": int(sum(r[1] for r in reqs)),
+        "full_stream_windows": len(windows),
+        "full_stream_wall_s": {"fixed": round(res["fixed"][0], 2),
+                               "model": round(res["model"][0], 2)},
+        "full_stream_exec_ms": {"fixed": round(res["fixed"][1], 1),
+                                "model": round(res["model"][1], 1)},
+        "full_stream_speedup_vs_fixed": round(res["fixed"][1] / res["model"][1], 4),
+        "full_stream_speedup_vs_fixed_wall": round(res["fixed"][0] / res["model"][0], 4),
+        "full_stream_model_changed": int(np.count_nonzero(choices != fixed)),
+        "full_stream_differing_words_vs_fixed": int(diffs),
+        "full_stream_verify_s": round(wall_v, 1),
+        "full_stream_timing": "end to end per policy: windows of up to 256 requests in two "
+                              "%d GiB pools; fills and the model policy's batched features "
+                              "and inference run on a side queue overlapped with the exec "
+                              "queue's sorts and scans; exec ms from the first to the last "
+                              "work on the exec queue" % (pool_bytes >> 30),
+    }
+
+
+def _warm_configs(d, q, data):
+    """Every configuration once at the largest request size of each width,
+    so no timed run later grows a sorter's scratch."""
+    for width in (4, 8):
+        n = 1 << 28
+        data.fill(n, width, 0, 1)
+        for c in range(clops.N_CONFIGS):
+            for op in (clops.OP_SORT, clops.OP_SCAN):
+                if d.feasible(c, op, n, width):
+                    d.run(q, c, op, data.inb, data.outb, n, width)
+                    d.run(q, c, op, data.inb, data.out2, n, width)
+        q.finish()
+        q.gc()
+
+
+def dispatch_eval(ctx, n_train=2000, n_eval=1500, n_full=100000, n_sparse=200,
+                  train_max_log2=28, steps=8000, lr=2e-3, margin=0.02):
     """Learned dispatch on the mixed request stream.
 
     Training: n_train requests (seed 0xC50, sizes up to 2^train_max_log2),
@@ -554,43 +796,34 @@ def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=8000, 
     runs); the cost model is trained on the GPU on those timings, as log2 of
     each configuration's time over the fixed configuration's time for the
     same request (the choice only needs one request's configurations in
-    order).  The model policy leaves the fixed configuration only for a
-    predicted gain above `margin` (log2).
-    Evaluation: the first n_eval requests of the stream of seed 0xC5 (sizes
-    2^10..2^28); each is tuned exhaustively (ground truth), and the model's,
-    the tuner table's and the fixed configuration's choices are priced with
-    those same timings.  Every chosen output is compared word for word with
-    the fixed configuration's output."""
+    order), and the tuner table keeps their per-bucket means.  The model
+    policy leaves the fixed configuration only for a predicted gain above
+    `margin` (log2).
+    Regret: the first n_eval requests of the stream of seed 0xC5 (sizes
+    2^10..2^28), each tuned exhaustively (min of 3 timed runs: the ground
+    truth); the model's, the table's and the fixed configuration's choices
+    are priced with those timings.  Every chosen output is compared word for
+    word with the fixed configuration's output.
+    Sparse ablation: a table and a model that learn from the first n_sparse
+    training requests only (most buckets then have no entry, and the table
+    falls back to the fixed configuration there), priced the same way.
+    Full stream: all n_full requests of the seed-0xC5 stream run under the
+    fixed configuration and under the model policy (dispatch_full)."""
     q = clops.Queue(ctx)
-    d = clops.Dispatch(ctx)
     data = RequestData(ctx, q, 1 << 28)
     t0 = time.perf_counter()
-    X, y = [], []
-    for (op, n, width, dist, s) in request_stream(0xC50, n_train, train_max_log2):
-        data.fill(n, width, dist, s)
-        f = d.features(q, op, data.inb, n, width)
-        t = d.tune(q, op, data.inb, data.outb, n, width, reps=2)
-        d.table_add(op, n, width, f, t)
-        t_fixed = t[clops.Dispatch.fixed(op, n, width)]
-        for c in range(clops.N_CONFIGS):
-            if t[c] > 0:
-                X.append(clops.Dispatch.model_row(f, c))
-                y.append(np.log2(t[c] / t_fixed))
+    train = request_stream(0xC50, n_train, train_max_log2)
+    tf, tt = _collect(clops.Dispatch(ctx), q, data, train, 2)
     t_collect = time.perf_counter() - t0
-    X, y = np.array(X, dtype=np.float32), np.array(y, dtype=np.float32)
-    m = clops.Mlp(ctx, clops.MODEL_INPUTS, seed=0xC5)
-    t0 = time.perf_counter()
-    loss = m.train(q, X, y, steps, lr)
-    t_train = time.perf_counter() - t0
-    d.set_model(m)
-    d.set_margin(margin)
+    d, m, loss, n_rows, t_train = _policy(ctx, q, train, tf, tt, steps, lr, margin, 0xC5)
+    ds, ms_, loss_s, n_rows_s, _ = _policy(ctx, q, train[:n_sparse], tf[:n_sparse],
+                                           tt[:n_sparse], steps, lr, margin, 0xC6)
+    log("dispatch: trained on %d timings (%.1f s collect, %.2f s train), sparse on %d"
+        % (n_rows, t_collect, t_train, n_rows_s))
     reqs = request_stream(0xC5, n_eval, 28)
-    feats, times = [], []
-    for (op, n, width, dist, s) in reqs:
-        data.fill(n, width, dist, s)
-        feats.append(d.features(q, op, data.inb, n, width))
-        times.append(d.tune(q, op, data.inb, data.outb, n, width, reps=3))
-    feats, times = np.array(feats), np.array(times)
+    t0 = time.perf_counter()
+    feats, times = _collect(d, q, data, reqs, 3)
+    t_eval = time.perf_counter() - t0
     ops = [r[0] for r in reqs]
     ns = [r[1] for r in reqs]
     ws = [r[2] for r in reqs]
@@ -599,10 +832,14 @@ def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=8000, 
     t_choose = (time.perf_counter() - t0) / len(reqs)
     table = d.choose(q, clops.POLICY_TABLE, ops, ns, ws, feats)
     fixed = d.choose(q, clops.POLICY_FIXED, ops, ns, ws, feats)
+    model_s = ds.choose(q, clops.POLICY_MODEL, ops, ns, ws, feats)
+    table_s = ds.choose(q, clops.POLICY_TABLE, ops, ns, ws, feats)
     best = np.array([int(np.argmin(np.where(t > 0, t, np.inf))) for t in times])
     rows = np.arange(len(reqs))
     tot = {k: float(times[rows, v].sum()) for k, v in
-           (("model", model), ("table", table), ("fixed", fixed), ("best", best))}
+           (("model", model), ("table", table), ("fixed", fixed), ("best", best),
+            ("sparse_model", model_s), ("sparse_table", table_s))}
+    sparse_hits = sum(ds.table_choose(op, n, w, f) >= 0 for op, n, w, f in zip(ops, ns, ws, feats))
     # bit-exact: chosen configuration vs the fixed one, word for word
     diffs = 0
     for r, (op, n, width, dist, s) in enumerate(reqs):
@@ -612,23 +849,33 @@ def dispatch_eval(ctx, n_train=2700, n_eval=300, train_max_log2=28, steps=8000, 
         d.run(q, int(model[r]), op, data.inb, data.outb, n, width)
         d.run(q, int(fixed[r]), op, data.inb, data.out2, n, width)
         diffs += clops.count_diff(q, data.outb, data.out2, n * width // 4)
-    return {
-        "train_requests": n_train, "train_timings": int(y.size),
+    q.gc()
+    out = {
+        "train_requests": n_train, "train_timings": n_rows,
         "train_collect_s": round(t_collect, 1), "train_s": round(t_train, 2),
         "train_loss": round(loss, 4), "train_target": "log2(time / fixed-config time)",
-        "margin_log2": margin, "eval_requests": n_eval,
+        "margin_log2": margin, "eval_requests": n_eval, "eval_tune_s": round(t_eval, 1),
         "eval_keys": int(sum(ns)),
         "regret_model": round(tot["model"] / tot["best"] - 1, 4),
         "regret_table": round(tot["table"] / tot["best"] - 1, 4),
         "regret_fixed": round(tot["fixed"] / tot["best"] - 1, 4),
         "speedup_model_vs_fixed": round(tot["fixed"] / tot["model"], 4),
         "model_matches_best": round(float(np.mean(model == best)), 3),
+        "regret_sparse_table": round(tot["sparse_table"] / tot["best"] - 1, 4),
+        "regret_sparse_model": round(tot["sparse_model"] / tot["best"] - 1, 4),
+        "sparse": "table and model from the first %d training requests (%d timings); the sparse "
+                  "table has an entry for %d of the %d evaluation requests' buckets"
+                  % (n_sparse, n_rows_s, sparse_hits, n_eval),
         "choose_us_per_request": round(t_choose * 1e6, 1),
         "eval_ms": {k: round(v, 3) for k, v in tot.items()},
         "differing_words_vs_fixed": int(diffs),
         "sample": "first %d requests of the seed-0xC5 stream (sizes 2^10..2^28), exhaustive "
                   "tuner ground truth (min of 3 timed runs per configuration)" % n_eval,
     }
+    if n_full:
+        _warm_configs(d, q, data)
+        out.update(dispatch_full(ctx, d, data, n_full))
+    return out
 
 
 def copy_probe(ctx, nbytes):
@@ -653,8 +900,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--suite", action="store_true")
     ap.add_argument("--no-dispatch", action="store_true")
-    ap.add_argument("--dispatch-train", type=int, default=2700)
-    ap.add_argument("--dispatch-eval", type=int, default=300)
+    ap.add_argument("--dispatch-train", type=int, default=2000)
+    ap.add_argument("--dispatch-eval", type=int, default=1500)
+    ap.add_argument("--dispatch-full", type=int, default=100000,
+                    help="requests of the stream run under fixed and model policies (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -669,7 +918,7 @@ def main():
         suite(ctx, 0 if args.no_cpu_baseline else (args.cpu_threads or min(os.cpu_count() or 1, 16)))
     disp = None
     if not args.no_dispatch:
-        disp = dispatch_eval(ctx, args.dispatch_train, args.dispatch_eval)
+        disp = dispatch_eval(ctx, args.dispatch_train, args.dispatch_eval, args.dispatch_full)
         log("dispatch: " + json.dumps(disp))
     cpu = None
     if not args.no_cpu_baseline:

@@ -92,8 +92,12 @@ def lib():
         "ccl_context_destroy": (None, [vp]),
         "ccl_queue_new": (vp, [vp, vp, u64, ep]),
         "ccl_queue_finish": (u32, [vp, ep]),
+        "ccl_queue_gc": (None, [vp]),
         "ccl_queue_destroy": (None, [vp]),
         "ccl_queue_enqueue_marker": (vp, [vp, ep]),
+        "ccl_enqueue_barrier": (vp, [vp, vp, ep]),
+        "ccl_event_wait_list_add": (None, [vp, vp, vp]),
+        "ccl_event_wait_list_clear": (None, [vp]),
         "ccl_queue_get_stream": (vp, [vp]),
         "ccl_buffer_new": (vp, [vp, u64, sz, vp, ep]),
         "ccl_buffer_destroy": (None, [vp]),
@@ -179,6 +183,16 @@ def lib():
         "clo_dispatch_set_model": (u32, [vp, vp, ep]),
         "clo_dispatch_set_margin": (None, [vp, ctypes.c_float]),
         "clo_dispatch_choose": (u32, [vp, vp, i32, sz, vp, vp, vp, vp, vp, ep]),
+        "clo_dispatch_features_segmented": (u32, [vp, vp, vp, vp, sz, sz, sz, vp, ep]),
+        "clo_dispatch_features_batch": (u32, [vp, vp, sz, vp, vp, vp, vp, vp, ep]),
+        "clo_dispatch_run_segmented": (vp, [vp, vp, u32, vp, vp, vp, sz, sz, sz, vp, ep]),
+        "clo_dispatch_tune_segmented": (u32, [vp, vp, vp, vp, vp, sz, sz, sz, u32, vp, ep]),
+        "clo_dispatch_seg_bin": (u32, [sz]),
+        "clo_dispatch_seg_bin_length": (sz, [u32]),
+        "clo_dispatch_tune_segment_class": (u32, [vp, vp, u32, sz, sz, u32, vp, ep]),
+        "clo_dispatch_segment_row": (None, [sz, sz, u32, u32, vp]),
+        "clo_dispatch_set_segment_model": (u32, [vp, vp, ep]),
+        "clo_dispatch_segment_classes": (u32, [vp, vp, u32, sz, ctypes.c_float, vp, ep]),
         # cost-model MLP
         "clo_mlp_new": (vp, [vp, u32, u64, ep]),
         "clo_mlp_destroy": (None, [vp]),
@@ -255,8 +269,19 @@ class Queue:
     def finish(self):
         _call(self.L.ccl_queue_finish, self.ptr)
 
+    def gc(self):
+        """Waits, then releases the events recorded on the queue so far."""
+        self.L.ccl_queue_gc(self.ptr)
+
     def marker(self):
         return _call(self.L.ccl_queue_enqueue_marker, self.ptr)
+
+    def wait_for(self, *events):
+        """Later work on this queue waits for the events (of any queue)."""
+        ewl = ctypes.c_void_p(None)
+        for e in events:
+            self.L.ccl_event_wait_list_add(ctypes.byref(ewl), e, None)
+        return _call(self.L.ccl_enqueue_barrier, self.ptr, ctypes.byref(ewl))
 
     def elapsed_ms(self, e0, e1):
         return _call(self.L.ccl_event_elapsed_ms, e0, e1)
@@ -364,10 +389,11 @@ class Mlp:
             pass
 
 
-OP_SORT, OP_SCAN = 0, 1
+OP_SORT, OP_SCAN, OP_SEGSORT = 0, 1, 2
 POLICY_FIXED, POLICY_TABLE, POLICY_MODEL = 0, 1, 2
-N_CONFIGS, N_FEATURES = 9, 8
+N_CONFIGS, N_FEATURES = 24, 10
 MODEL_INPUTS = N_FEATURES + N_CONFIGS
+SEG_BINS, SEG_INPUTS = 64, 7
 
 
 def _vp(a):
@@ -375,13 +401,15 @@ def _vp(a):
 
 
 class Dispatch:
-    """Configuration choice for sort/scan requests (clo_dispatch_*)."""
+    """Configuration choice for sort, scan and segmented sort requests
+    (clo_dispatch_*), and the per-segment work class."""
 
     def __init__(self, ctx):
         self.L = lib()
         self.ctx = ctx
         self.ptr = _call(self.L.clo_dispatch_new, ctx.ptr)
         self.model = None
+        self.seg_model = None
 
     @staticmethod
     def config_name(c):
@@ -416,6 +444,69 @@ class Dispatch:
         _call(self.L.clo_dispatch_tune, self.ptr, q.ptr, op, bin_.ptr, bout.ptr, n, width, reps,
               _vp(t))
         return t
+
+    def features_batch(self, q, ops, bufs, ns, widths):
+        count = len(bufs)
+        f = np.empty((count, N_FEATURES), dtype=np.float32)
+        if count == 0:
+            return f
+        o = np.ascontiguousarray(ops, dtype=np.int32)
+        p = (ctypes.c_void_p * count)(*[b.ptr for b in bufs])
+        n = np.ascontiguousarray(ns, dtype=np.uint64)
+        w = np.ascontiguousarray(widths, dtype=np.uint64)
+        _call(self.L.clo_dispatch_features_batch, self.ptr, q.ptr, count, _vp(o), p, _vp(n),
+              _vp(w), _vp(f))
+        return f
+
+    def features_segmented(self, q, buf, off, nseg, n, width):
+        f = np.empty(N_FEATURES, dtype=np.float32)
+        _call(self.L.clo_dispatch_features_segmented, self.ptr, q.ptr, buf.ptr, off.ptr, nseg, n,
+              width, _vp(f))
+        return f
+
+    def run_segmented(self, q, cfg, bin_, bout, off, nseg, n, width, classes=None):
+        c = None
+        if classes is not None:
+            c = np.ascontiguousarray(classes, dtype=np.uint8)
+            assert c.size == SEG_BINS
+        return _call(self.L.clo_dispatch_run_segmented, self.ptr, q.ptr, cfg, bin_.ptr, bout.ptr,
+                     off.ptr, nseg, n, width, _vp(c) if c is not None else None)
+
+    def tune_segmented(self, q, bin_, bout, off, nseg, n, width, reps=3):
+        t = np.empty(N_CONFIGS, dtype=np.float32)
+        _call(self.L.clo_dispatch_tune_segmented, self.ptr, q.ptr, bin_.ptr, bout.ptr, off.ptr,
+              nseg, n, width, reps, _vp(t))
+        return t
+
+    @staticmethod
+    def seg_bin(length):
+        return lib().clo_dispatch_seg_bin(length)
+
+    @staticmethod
+    def seg_bin_length(b):
+        return lib().clo_dispatch_seg_bin_length(b)
+
+    def tune_segment_class(self, q, cfg, width, length, reps=3):
+        t = np.empty(3, dtype=np.float32)
+        _call(self.L.clo_dispatch_tune_segment_class, self.ptr, q.ptr, cfg, width, length, reps,
+              _vp(t))
+        return t
+
+    @staticmethod
+    def segment_row(length, width, cfg, cls):
+        row = np.empty(SEG_INPUTS, dtype=np.float32)
+        lib().clo_dispatch_segment_row(length, width, cfg, cls, _vp(row))
+        return row
+
+    def set_segment_model(self, mlp):
+        self.seg_model = mlp
+        _call(self.L.clo_dispatch_set_segment_model, self.ptr, mlp.ptr if mlp else None)
+
+    def segment_classes(self, q, cfg, width, log2_margin=0.0):
+        c = np.empty(SEG_BINS, dtype=np.uint8)
+        _call(self.L.clo_dispatch_segment_classes, self.ptr, q.ptr if q else None, cfg, width,
+              log2_margin, _vp(c))
+        return c
 
     def table_add(self, op, n, width, features, times):
         f, t = _f32(features), _f32(times)

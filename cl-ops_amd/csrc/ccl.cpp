@@ -349,6 +349,21 @@ cl_bool ccl_queue_finish(CCLQueue* cq, GError** err) {
 		? CL_FALSE : CL_TRUE;
 }
 
+void ccl_queue_gc(CCLQueue* cq) {
+	if (!cq || cq->capturing) return;
+	(void)hipSetDevice(cq->ctx->dev);
+	size_t keep = 0;
+	for (CCLEvent* e : cq->events) {
+		if (e->ev && hipEventQuery(e->ev) != hipSuccess) {
+			cq->events[keep++] = e;  // still pending (or failed): keep it
+			continue;
+		}
+		if (e->ev) (void)hipEventDestroy(e->ev);
+		delete e;
+	}
+	cq->events.resize(keep);
+}
+
 void ccl_queue_destroy(CCLQueue* cq) {
 	if (!cq) return;
 	(void)hipSetDevice(cq->ctx->dev);
@@ -366,6 +381,18 @@ void ccl_queue_destroy(CCLQueue* cq) {
 	if (cq->owns_stream) (void)hipStreamDestroy(cq->stream);
 	ccl_context_destroy(cq->ctx);
 	delete cq;
+}
+
+static bool wait_list_on_stream(CCLEventWaitList* ewl, hipStream_t s, GError** err);
+
+CCLEvent* ccl_enqueue_barrier(CCLQueue* cq, CCLEventWaitList* ewl, GError** err) {
+	if (!cq) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL queue");
+		return nullptr;
+	}
+	if (!use_device(cq->ctx, err)) return nullptr;
+	if (!wait_list_on_stream(ewl, cq->stream, err)) return nullptr;
+	return record_event(cq, "barrier", err);
 }
 
 CCLEvent* ccl_queue_enqueue_marker(CCLQueue* cq, GError** err) {

@@ -147,6 +147,13 @@ bool get_key_is_identity(const char* get_key);
 bool type_is_integer(CloType t);
 bool type_is_signed(CloType t);
 
+// Segmented sort defaults: items per thread of the 256-thread tile and
+// resident blocks per CU of the persistent launch, as chosen by the
+// dispatcher's exhaustive tuner on 32768 log-uniform segments of up to 65536
+// uint32 keys.  The dispatcher's fixed segmented configuration is the same.
+constexpr uint32_t kSegTileDefault = 16;
+constexpr uint32_t kSegBlocksPerCuDefault = 3;
+
 }  // namespace clo
 
 // Internal accessors of the sorter shared by the algorithm files.
@@ -160,9 +167,12 @@ struct clo_sort {
 	// Set by satradix: device word with the executed pass count of the last
 	// call, and the queue it ran on.
 	CCLQueue* last_queue;
-	// Segmented sort: tuning (segtile, seggrid options) and its scratch.
+	// Segmented sort: tuning (segtile, seggrid, segclass options) and its
+	// scratch.  seg_cls: least work class per length bin, two bits per bin
+	// (0 chunk, 1 medium, 2 large; all zero = by length alone).
 	uint32_t seg_tile;
 	uint32_t seg_grid;
+	uint64_t seg_cls[2];
 	void* seg_cache;
 	// Run-time compiled kernels for user compare / key snippets (NULL when
 	// the ahead-of-time kernels serve the sorter).

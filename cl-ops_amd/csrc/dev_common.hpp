@@ -27,6 +27,17 @@ __device__ __forceinline__ u32 count_below(u64 mask) {
 
 __device__ __forceinline__ u64 ballot(bool p) { return __ballot(p); }
 
+// Length bin of a segment for the segmented sort's work-class table: four
+// bins per octave, from the leading one and the two bits after it (len 0 and
+// 1 share bin 0), capped at 63.  The same function on host and device.
+__host__ __device__ inline u32 seg_len_bin(u32 len) {
+	if (len < 2) return 0;
+	const u32 lg = 31u - static_cast<u32>(__builtin_clz(len));
+	const u32 sub = lg >= 2 ? (len >> (lg - 2)) & 3u : (len << (2 - lg)) & 3u;
+	const u32 b = 4 * lg + sub;
+	return b < 63 ? b : 63;
+}
+
 // ---- XCD-aware tile mapping -------------------------------------------------
 // MI355X dispatches consecutive blocks round-robin over its 8 XCDs, each with
 // its own L2.  Kernels whose tiles are independent map block b to tile

@@ -27,7 +27,7 @@ using namespace clo::dev;
 namespace {
 
 constexpr int kH = 32;       // hidden units
-constexpr int kMaxIn = 32;   // input features
+constexpr int kMaxIn = 48;   // input features
 constexpr int kRows = 128;   // samples per gradient block
 constexpr int kPad = kH + 1;
 constexpr u32 kMaxParams = kH * kMaxIn + kH + kH * kH + kH + kH + 1;

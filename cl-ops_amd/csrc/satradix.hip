@@ -749,6 +749,26 @@ const char* satradix_init(CloSort* sorter, const char* options, GError** err) {
 				return nullptr;
 			}
 			sorter->seg_grid = static_cast<u32>(g);
+		} else if (o.key == "segclass") {
+			// One digit per length bin from bin 0: the least work class
+			// (0 chunk, 1 medium, 2 large); bins past the string keep 0.
+			const std::string& v = o.value;
+			bool ok = v.size() <= 64;
+			u64 cls[2] = {0, 0};
+			for (size_t i = 0; ok && i < v.size(); ++i) {
+				if (v[i] < '0' || v[i] > '2') {
+					ok = false;
+					break;
+				}
+				cls[i / 32] |= static_cast<u64>(v[i] - '0') << (2 * (i % 32));
+			}
+			if (!ok) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+					"segclass takes up to 64 digits 0, 1 or 2 (one per length bin).");
+				return nullptr;
+			}
+			sorter->seg_cls[0] = cls[0];
+			sorter->seg_cls[1] = cls[1];
 		} else if (o.key.size() >= 4 && strncasecmp(o.key.c_str(), "scan", 4) == 0) {
 			if (o.key.size() == 4) sd->scan_type = o.value;
 		} else {

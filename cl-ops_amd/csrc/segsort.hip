@@ -12,6 +12,13 @@
 //                           over the local segment index
 // with CAP = 256 * IPT elements and W = CAP / 2 (so a chunk never exceeds
 // CAP).  Digit passes whose digit is equal over the whole item are skipped.
+//
+// The work class of each segment can be raised per length: a table of 64
+// length bins (four per octave) gives the least class of every bin, and a
+// segment takes the greater of that and the least class that holds it.  An
+// all-zero table is the rule above.  A small segment raised out of the chunk
+// class stays out of its window's chunk (the chunk gathers only its own
+// segments), so every segment is sorted by exactly one item.
 
 #include <cstring>
 
@@ -53,6 +60,19 @@ __device__ __forceinline__ SegRange seg_range(const u32* off, u32 j, u32 nseg, u
 	return {start, end - start};
 }
 
+// Work class (0 chunk, 1 medium, 2 large): the table's entry for the length
+// bin (two bits per bin, bins 0..31 in lo and 32..63 in hi), raised to the
+// least class that can hold the segment.
+__device__ __forceinline__ u32 seg_class(u64 lo, u64 hi, u32 len, u32 W, u32 CAP) {
+	const u32 b = seg_len_bin(len);
+	u32 c = static_cast<u32>(((b < 32 ? lo : hi) >> (2 * (b & 31))) & 3u);
+	if (c > 2) c = 2;
+	if (len > W && c < 1) c = 1;
+	if (len > CAP) c = 2;
+	if (len == 0) c = 0;
+	return c;
+}
+
 // Appends the medium and large segments' indices and the windows of W
 // elements in which small segments start (chunk items), one atomic per wave
 // and list; wrange[window] receives the range [j0, j1) of the segments
@@ -61,13 +81,16 @@ __device__ __forceinline__ SegRange seg_range(const u32* off, u32 j, u32 nseg, u
 // window), so the first of them records j0 and the last j1; windows without
 // a small segment never become work items.
 __global__ __launch_bounds__(256) void seg_classify(const u32* __restrict__ off, u32 nseg,
-	u32 numel, u32 W, u32 CAP, u32* __restrict__ medium, u32* __restrict__ large,
-	u32* __restrict__ chunks, uint2* __restrict__ wrange, SegMeta* m) {
+	u32 numel, u32 W, u32 CAP, u64 cls_lo, u64 cls_hi, u32* __restrict__ medium,
+	u32* __restrict__ large, u32* __restrict__ chunks, uint2* __restrict__ wrange, SegMeta* m) {
 	const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
 	SegRange sr = {0u, 0u};
 	if (j < nseg) sr = seg_range(off, j, nseg, numel);
-	const bool is_med = j < nseg && sr.len > W && sr.len <= CAP;
-	const bool is_lrg = j < nseg && sr.len > CAP;
+	const u32 cls = j < nseg ? seg_class(cls_lo, cls_hi, sr.len, W, CAP) : 0u;
+	const bool is_med = j < nseg && cls == 1;
+	const bool is_lrg = j < nseg && cls == 2;
+	// Windows are built over every segment of at most W elements, whatever
+	// its class; the chunk leaves out those of another class.
 	const bool is_sml = j < nseg && sr.len <= W;
 	bool head = false;
 	if (is_sml) {
@@ -114,6 +137,7 @@ struct SegArgs {
 	const u32* chunks;    // windows holding small segment starts
 	const uint2* wrange;  // per window: segments [x, y) starting in it
 	SegMeta* meta;
+	u64 cls_lo, cls_hi;   // per-length-bin least work class (seg_class)
 	u32 nseg;
 	u32 numel;
 };
@@ -306,7 +330,8 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 		if (r < mall) {
 			const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
 			start = sr.start;
-			len = sr.len > W ? 0u : sr.len;
+			len = (sr.len > W || seg_class(a.cls_lo, a.cls_hi, sr.len, W, CAP) != 0) ? 0u
+				: sr.len;
 		}
 		u32 tot, cnt;
 		const u32 ex = block_exclusive_sum<kRB>(len, s.rs.wsum, &tot);
@@ -590,8 +615,8 @@ bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, GError
 	{
 		Launch L(q, "segsort_classify");
 		const u32 g = static_cast<u32>(div_ceil(a.nseg, 256));
-		seg_classify<<<g ? g : 1, 256, 0, st>>>(a.off, a.nseg, a.numel, W, CAP,
-			const_cast<u32*>(a.medium), const_cast<u32*>(a.large), const_cast<u32*>(a.chunks),
+		seg_classify<<<g ? g : 1, 256, 0, st>>>(a.off, a.nseg, a.numel, W, CAP, a.cls_lo,
+			a.cls_hi, const_cast<u32*>(a.medium), const_cast<u32*>(a.large), const_cast<u32*>(a.chunks),
 			const_cast<uint2*>(a.wrange), a.meta);
 		if (!L.done(err)) return false;
 	}
@@ -630,7 +655,7 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 		if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
 			cus <= 0)
 			cus = 256;
-		grid = static_cast<u32>(cus) * 3;  // three resident blocks per CU (measured best)
+		grid = static_cast<u32>(cus) * kSegBlocksPerCuDefault;
 	}
 	u32* lists = static_cast<u32*>(sd->lists.ptr);
 	const bool kv = vin != nullptr;
@@ -650,6 +675,8 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	a.chunks = lists + 2 * nseg;                                                             \
 	a.wrange = reinterpret_cast<const uint2*>(lists + (3 * nseg + 3) / 4 * 4 + 4);          \
 	a.meta = static_cast<SegMeta*>(sd->meta.ptr);                                            \
+	a.cls_lo = sorter->seg_cls[0];                                                           \
+	a.cls_hi = sorter->seg_cls[1];                                                           \
 	a.nseg = static_cast<u32>(nseg);                                                         \
 	a.numel = static_cast<u32>(numel);                                                       \
 

@@ -45,6 +45,7 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 	s->last_queue = nullptr;
 	s->seg_tile = 0;
 	s->seg_grid = 0;
+	s->seg_cls[0] = s->seg_cls[1] = 0;
 	s->seg_cache = nullptr;
 	s->snip = nullptr;
 	s->host_cache = nullptr;
@@ -275,7 +276,7 @@ CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec
 			"keys_out overlaps keys_in without being the same buffer");
 		return nullptr;
 	}
-	uint32_t tile = sorter->seg_tile ? sorter->seg_tile : 16u;  // 16: measured best on MI355X
+	uint32_t tile = sorter->seg_tile ? sorter->seg_tile : clo::kSegTileDefault;
 	return clo_satradix_sort_segmented(sorter, cq_exec, keys_in, vals_in, keys_out, vals_out,
 		offsets, nseg, numel, tile, sorter->seg_grid, &sorter->seg_cache, err);
 }

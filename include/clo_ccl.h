@@ -96,6 +96,15 @@ CCLQueue* ccl_queue_new_wrap(CCLContext* ctx, void* hip_stream,
 void* ccl_queue_get_stream(CCLQueue* cq);
 CCLContext* ccl_queue_get_context(CCLQueue* cq, GError** err);
 cl_bool ccl_queue_finish(CCLQueue* cq, GError** err);
+/* Releases the events the queue has recorded that have completed
+ * (cf4ocl's ccl_queue_gc); does not wait.  Those events are invalid
+ * afterwards.  For long-running callers that enqueue many calls on one
+ * queue.  Does nothing while the queue is being captured. */
+void ccl_queue_gc(CCLQueue* cq);
+/* Makes the queue's later work wait for every event in the list (events of
+ * any queue of the same context), without blocking the host; clears the
+ * list.  Returns a marker event. */
+CCLEvent* ccl_enqueue_barrier(CCLQueue* cq, CCLEventWaitList* ewl, GError** err);
 void ccl_queue_destroy(CCLQueue* cq);
 /* Records a marker event on the queue (for caller-side timing). */
 CCLEvent* ccl_queue_enqueue_marker(CCLQueue* cq, GError** err);

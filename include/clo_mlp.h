@@ -20,7 +20,7 @@ extern "C" {
 
 typedef struct clo_mlp CloMlp;
 
-/* New network with n_in inputs (1..32): He-uniform weights drawn from a
+/* New network with n_in inputs (1..48): He-uniform weights drawn from a
  * splitmix64 stream of `seed`, zero biases. */
 CloMlp* clo_mlp_new(CCLContext* ctx, cl_uint n_in, cl_ulong seed, GError** err);
 void clo_mlp_destroy(CloMlp* mlp);

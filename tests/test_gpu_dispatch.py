@@ -75,9 +75,12 @@ def _features_ref(x, op):
         p = c[c > 0] / S
         ents.append(float(-(p * np.log2(p)).sum()))
     ents = np.array(ents)
+    # One segment of n: mean log2(len + 1) / 16 and the share of keys in
+    # segments longer than 4096.
     return np.array([math.log2(n) / 32, w / 8, ents.mean() / 8, ents.max() / 8,
                      np.mean(ents < 1e-6), (a < b).sum() / pairs if pairs else 0,
-                     (a > b).sum() / pairs if pairs else 0, op], dtype=np.float64)
+                     (a > b).sum() / pairs if pairs else 0, op,
+                     math.log2(n + 1) / 16, 1.0 if n > 4096 else 0.0], dtype=np.float64)
 
 
 @pytest.mark.parametrize("width", [4, 8])
@@ -153,7 +156,18 @@ def _synthetic_cost(ops, ns, widths, feats, c):
         return 6 + 0.0011 * ns * w
     if c == 7:
         return 9 + 0.0008 * ns * w
+    if c == 9:
+        return 30 + 0.008 * ns * w
+    if c == 10:
+        return 25 + 0.006 * ns * w
+    if c == 11:
+        return 22 + 0.005 * ns * w
     return 15 + 0.0007 * ns * w
+
+
+def clops_n_features():
+    import clops
+    return clops.N_FEATURES
 
 
 def _requests(rng, count):
@@ -161,7 +175,7 @@ def _requests(rng, count):
     lg = rng.uniform(10, 24, count)
     ns = np.floor(2 ** lg).astype(np.uint64)
     widths = rng.choice([4, 8], count).astype(np.uint64)
-    feats = np.zeros((count, 8), dtype=np.float32)
+    feats = np.zeros((count, clops_n_features()), dtype=np.float32)
     feats[:, 0] = np.log2(ns) / 32
     feats[:, 1] = widths / 8
     feats[:, 2] = rng.uniform(0.3, 1, count)
@@ -169,6 +183,8 @@ def _requests(rng, count):
     feats[:, 5] = rng.uniform(0, 1, count)
     feats[:, 6] = (1 - feats[:, 5]) * rng.uniform(0, 1, count)
     feats[:, 7] = ops
+    feats[:, 8] = np.log2(ns + 1.0) / 16
+    feats[:, 9] = ns > 4096
     return ops, ns, widths, feats
 
 
@@ -217,3 +233,50 @@ def test_learned_choice_beats_fixed_on_known_costs(clops, ctx):
     t_model, t_best, t_fixed = total(choice), total(best), total(fixed)
     assert t_model / t_best - 1 <= 0.05
     assert t_model < t_fixed
+
+
+def test_features_batch_matches_single(clops, oracle, ctx):
+    """One launch over requests of both widths, both ops and every
+    distribution (views into one pool): each row equals the single-request
+    features."""
+    q = clops.Queue(ctx)
+    d = clops.Dispatch(ctx)
+    pool = clops.Buffer(ctx, 1 << 24)
+    views, ops, ns, ws, singles = [], [], [], [], []
+    off = 0
+    for i, (dist, width, n) in enumerate([("uniform", 4, 1), ("few", 8, 3), ("nearly", 4, 4096),
+                                          ("reversed", 8, 123457), ("uniform", 8, 70001),
+                                          ("few", 4, 1000)]):
+        x, _ = _data(clops, oracle, ctx, q, dist, width, n, 40 + i)
+        v = pool.view(off, n * width)
+        v.write(q, x)
+        off += (n * width + 255) // 256 * 256
+        op = clops.OP_SORT if i % 2 else clops.OP_SCAN
+        views.append(v)
+        ops.append(op)
+        ns.append(n)
+        ws.append(width)
+        singles.append(d.features(q, op, v, n, width))
+    f = d.features_batch(q, ops, views, ns, ws)
+    assert np.allclose(f, np.array(singles), rtol=0, atol=1e-6)
+    with pytest.raises(clops.ClopsError):
+        d.features_batch(q, [clops.OP_SEGSORT], views[:1], ns[:1], ws[:1])
+
+
+def test_barrier_orders_queues(clops, oracle, ctx):
+    """Work on one queue waits for an event of another: a fill on the side
+    queue, then a sort on the exec queue that waited for it."""
+    side, ex = clops.Queue(ctx), clops.Queue(ctx)
+    n = 1 << 22
+    b = clops.Buffer(ctx, n * 4)
+    o = clops.Buffer(ctx, n * 4)
+    s = clops.Sorter(ctx, "satradix", np.uint32)
+    for rep in range(3):
+        clops.generate(side, b, n, clops.GEN_U32_HI, 0xBA + rep)
+        ex.wait_for(side.marker())
+        s.sort_device(ex, b, o, n)
+        ex.finish()
+        want = np.sort(oracle.generate(oracle.GEN_U32_HI, 0xBA + rep, n))
+        assert np.array_equal(o.read(ex, np.uint32, n), want), rep
+    side.gc()
+    ex.gc()
