@@ -483,7 +483,30 @@ def suite_segmented(ctx, q, baseline_stride=16):
     return res
 
 
-def segmented_tuning(ctx, q, kin, kout, kout2, offb, nseg, n):
+def seg_tile(cfg):
+    """Items per thread of segmented configuration cfg (12 + 4 t + g - 2)."""
+    return (8, 16, 32)[(cfg - 12) // 4]
+
+
+def seg_tune(ctx):
+    """The segmented tuner and the learned work classes on the 32768
+    log-uniform segments, for 32- and 64-bit keys."""
+    q = clops.Queue(ctx)
+    lens, off = c4_offsets(ctx, q)
+    nseg = lens.size
+    n = int(lens.sum(dtype=np.uint64))
+    offb = clops.Buffer.from_array(ctx, off)
+    res = []
+    for width in (4, 8):
+        bufs = [clops.Buffer(ctx, n * width) for _ in range(3)]
+        clops.generate(q, bufs[0], n, clops.GEN_U32_HI if width == 4 else clops.GEN_U64, 0xC4B)
+        res.extend(segmented_tuning(ctx, q, *bufs, offb, nseg, n, width))
+        for b in bufs:
+            b.close()
+    return res
+
+
+def segmented_tuning(ctx, q, kin, kout, kout2, offb, nseg, n, width=4):
     """The exhaustive tuner over the segmented tile x grid configurations on
     this workload (the library's defaults are its choice), then the learned
     per-segment work classes: the segment cost model is trained on work-class
@@ -491,10 +514,10 @@ def segmented_tuning(ctx, q, kin, kout, kout2, offb, nseg, n):
     its class table is timed against the length thresholds alone, outputs
     compared word for word."""
     d = clops.Dispatch(ctx)
-    t = d.tune_segmented(q, kin, kout, offb, nseg, n, 4, reps=3)
-    fixed = clops.Dispatch.fixed(clops.OP_SEGSORT, n, 4)
+    t = d.tune_segmented(q, kin, kout, offb, nseg, n, width, reps=3)
+    fixed = clops.Dispatch.fixed(clops.OP_SEGSORT, n, width)
     best = int(np.argmin(np.where(t > 0, t, np.inf)))
-    res = [{"config": "segmented tuner over tile x grid (%d keys)" % n,
+    res = [{"config": "segmented tuner over tile x grid (%d keys of %d bytes)" % (n, width),
             "times_ms": {d.config_name(c): round(float(t[c]), 4) for c in range(clops.N_CONFIGS)
                          if t[c] > 0},
             "tuner_best": d.config_name(best), "default": d.config_name(fixed),
@@ -502,21 +525,22 @@ def segmented_tuning(ctx, q, kin, kout, kout2, offb, nseg, n):
     X, y = [], []
     for b in range(clops.SEG_BINS):
         L = clops.Dispatch.seg_bin_length(b)
-        if L == 0 or L > 256 * 16:
+        if L == 0 or L > 256 * seg_tile(fixed):
             continue
-        tc = d.tune_segment_class(q, fixed, 4, L, reps=2)
+        tc = d.tune_segment_class(q, fixed, width, L, reps=2)
         for c in range(3):
             if tc[c] > 0:
-                X.append(clops.Dispatch.segment_row(L, 4, fixed, c))
+                X.append(clops.Dispatch.segment_row(L, width, fixed, c))
                 y.append(np.log2(tc[c]))
     m = clops.Mlp(ctx, clops.SEG_INPUTS, seed=0x5E)
     loss = m.train(q, np.array(X, dtype=np.float32), np.array(y, dtype=np.float32), 3000, 3e-3)
     d.set_segment_model(m)
-    classes = d.segment_classes(q, fixed, 4, 0.05)
-    ms_fixed = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout, offb, nseg, n, 4), 5, 2)
-    ms_cls = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout2, offb, nseg, n, 4,
+    classes = d.segment_classes(q, fixed, width, 0.05)
+    ms_fixed = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout, offb, nseg, n, width),
+                          5, 2)
+    ms_cls = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout2, offb, nseg, n, width,
                                                    classes=classes), 5, 2)
-    diff = clops.count_diff(q, kout, kout2, n)
+    diff = clops.count_diff(q, kout, kout2, n * width // 4)
     res.append({"config": "learned per-segment work classes (%s)" % d.config_name(fixed),
                 "classes": "".join(map(str, classes)), "train_rows": len(y),
                 "train_loss": round(loss, 4), "ms_thresholds": ms_fixed, "ms_learned": ms_cls,
@@ -899,6 +923,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--suite", action="store_true")
+    ap.add_argument("--seg-tune", action="store_true",
+                    help="only the segmented tuner and learned work classes, 32768 segments")
     ap.add_argument("--no-dispatch", action="store_true")
     ap.add_argument("--dispatch-train", type=int, default=2000)
     ap.add_argument("--dispatch-eval", type=int, default=1500)
@@ -912,6 +938,10 @@ def main():
     ctx = clops.Context(0)
     if args.copy_probe:
         copy_probe(ctx, 1 << 31)
+        return
+    if args.seg_tune:
+        for r in seg_tune(ctx):
+            print(json.dumps(r), flush=True)
         return
     h = headline(args, ctx)
     if args.suite:

@@ -148,11 +148,13 @@ bool type_is_integer(CloType t);
 bool type_is_signed(CloType t);
 
 // Segmented sort defaults: items per thread of the 256-thread tile and
-// resident blocks per CU of the persistent launch, as chosen by the
-// dispatcher's exhaustive tuner on 32768 log-uniform segments of up to 65536
-// uint32 keys.  The dispatcher's fixed segmented configuration is the same.
-constexpr uint32_t kSegTileDefault = 16;
-constexpr uint32_t kSegBlocksPerCuDefault = 3;
+// blocks per CU of the persistent launch, as chosen by the dispatcher's
+// exhaustive tuner (clo_dispatch_tune_segmented) on 32768 log-uniform
+// segments of up to 65536 keys: tile 8 for 32-bit keys (2.71 ms against
+// 2.75 for tile 16), tile 16 for 64-bit keys, 4 blocks per CU for both.  The
+// dispatcher's fixed segmented configuration is the same.
+constexpr uint32_t seg_default_tile(size_t key_bytes) { return key_bytes == 8 ? 16u : 8u; }
+constexpr uint32_t kSegBlocksPerCuDefault = 4;
 
 }  // namespace clo
 

@@ -49,18 +49,20 @@ constexpr CfgInfo kCfg[kCfgs] = {
 	{"satradix 4-bit digits", CLO_OP_SORT, "satradix", "radix=16", 0, 0, 0, 0},
 	{"satradix 6-bit digits", CLO_OP_SORT, "satradix", "radix=64", 0, 0, 0, 0},
 	{"satradix 7-bit digits", CLO_OP_SORT, "satradix", "radix=128", 0, 0, 0, 0},
-	SEGCFG(8, 1), SEGCFG(8, 2), SEGCFG(8, 3), SEGCFG(8, 4),
-	SEGCFG(16, 1), SEGCFG(16, 2), SEGCFG(16, 3), SEGCFG(16, 4),
-	SEGCFG(32, 1), SEGCFG(32, 2), SEGCFG(32, 3), SEGCFG(32, 4),
+	SEGCFG(8, 2), SEGCFG(8, 3), SEGCFG(8, 4), SEGCFG(8, 5),
+	SEGCFG(16, 2), SEGCFG(16, 3), SEGCFG(16, 4), SEGCFG(16, 5),
+	SEGCFG(32, 2), SEGCFG(32, 3), SEGCFG(32, 4), SEGCFG(32, 5),
 };
 #undef SEGCFG
 constexpr u32 kSegFirst = 12;  // first segmented configuration
 
 constexpr u32 seg_cfg_index(u32 tile, u32 grid) {
-	return kSegFirst + (tile == 8 ? 0u : tile == 16 ? 4u : 8u) + (grid - 1);
+	return kSegFirst + (tile == 8 ? 0u : tile == 16 ? 4u : 8u) + (grid - 2);
 }
-static_assert(kCfg[seg_cfg_index(kSegTileDefault, kSegBlocksPerCuDefault)].segtile ==
-		kSegTileDefault, "segmented configuration table");
+static_assert(kCfg[seg_cfg_index(seg_default_tile(4), kSegBlocksPerCuDefault)].segtile ==
+		seg_default_tile(4) && kCfg[seg_cfg_index(seg_default_tile(8),
+		kSegBlocksPerCuDefault)].seggrid == kSegBlocksPerCuDefault,
+	"segmented configuration table");
 
 // LDS of the feature statistics (one request per 256-thread block).
 struct FeatSmem {
@@ -390,7 +392,7 @@ cl_bool clo_dispatch_config_feasible(cl_uint cfg, CloOp op, size_t n, size_t wid
 
 cl_uint clo_dispatch_fixed_config(CloOp op, size_t n, size_t width) {
 	if (op == CLO_OP_SCAN) return 7;
-	if (op == CLO_OP_SEGSORT) return seg_cfg_index(kSegTileDefault, kSegBlocksPerCuDefault);
+	if (op == CLO_OP_SEGSORT) return seg_cfg_index(seg_default_tile(width), kSegBlocksPerCuDefault);
 	// satradix's default tile shape for keys only
 	if (n <= (static_cast<size_t>(1) << 20)) return 2;
 	return width == 8 ? 1 : 0;

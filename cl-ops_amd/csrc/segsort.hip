@@ -566,12 +566,13 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 	__syncthreads();
 }
 
-// Blocks per CU the register allocation aims for: three for the default
-// 32-bit keys-only shape (measured best), fewer where the shape's registers
-// and LDS cannot reach three.
+// Blocks per CU the register allocation aims for: five for the 8-item
+// 32-bit keys-only tile, three for the 16-item one, fewer where the shape's
+// registers and LDS cannot reach that.
 template <typename K, bool KV, int IPT>
 constexpr int seg_min_blocks() {
-	return (sizeof(K) == 4 && !KV && IPT <= 16) ? 3
+	return (sizeof(K) == 4 && !KV && IPT <= 8) ? 5
+		: (sizeof(K) == 4 && !KV && IPT <= 16) ? 3
 		: (IPT >= 32 || (KV && sizeof(K) == 8)) ? 1 : 2;
 }
 

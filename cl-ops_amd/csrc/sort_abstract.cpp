@@ -276,7 +276,8 @@ CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec
 			"keys_out overlaps keys_in without being the same buffer");
 		return nullptr;
 	}
-	uint32_t tile = sorter->seg_tile ? sorter->seg_tile : clo::kSegTileDefault;
+	uint32_t tile = sorter->seg_tile ? sorter->seg_tile
+		: clo::seg_default_tile(clo_type_sizeof(sorter->elem_type));
 	return clo_satradix_sort_segmented(sorter, cq_exec, keys_in, vals_in, keys_out, vals_out,
 		offsets, nseg, numel, tile, sorter->seg_grid, &sorter->seg_cache, err);
 }

@@ -23,12 +23,15 @@
  *   9 satradix 4-bit digits  sort     any n (radix=16, default tile shape)
  *  10 satradix 6-bit digits  sort     any n (radix=64)
  *  11 satradix 7-bit digits  sort     any n (radix=128); 8-bit digits are 0-2
- *  12 + 4 t + (g - 1)        segsort  segmented tile 256 x {8, 16, 32}[t]
- *                                     items, g = 1..4 resident blocks per CU;
- *                                     tile 32 for 32-bit keys only
+ *  12 + 4 t + (g - 2)      segsort  segmented tile 256 x {8, 16, 32}[t]
+ *                                     items, g = 2..5 blocks per CU in the
+ *                                     persistent grid; tile 32 for 32-bit
+ *                                     keys only
  * The fixed configuration is what the plain API runs: satradix's default
  * tile shape (2 up to 2^20 keys, else 0 for 32-bit and 1 for 64-bit keys),
- * 7 for scans, and the segmented defaults (tile 16, 3 blocks per CU: 18).
+ * 7 for scans, and the segmented defaults (the tuner's choice on 32768
+ * log-uniform segments: tile 8 for 32-bit keys, 14; tile 16 for 64-bit
+ * keys, 18; both 4 blocks per CU).
  *
  * Features (CLO_DISPATCH_NUM_FEATURES floats, from a GPU kernel over a
  * sample of up to 4096 evenly spaced elements and segments):

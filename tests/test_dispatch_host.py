@@ -39,8 +39,9 @@ def test_config_table_rules(clops):
     # Digit widths are sort configurations of any size.
     for c in (9, 10, 11):
         assert d.feasible(c, clops.OP_SORT, 1 << 28, 8) and not d.feasible(c, clops.OP_SCAN, 10, 4)
-    assert d.fixed(clops.OP_SEGSORT, 1000, 4) == 18
-    assert "256x16, 3/CU" in names[18]
+    fx = d.fixed(clops.OP_SEGSORT, 1000, 4)
+    assert fx in seg and names[fx].startswith("segmented tile 256x")
+    assert names[12] == "segmented tile 256x8, 2/CU" and names[23] == "segmented tile 256x32, 5/CU"
     for op in (clops.OP_SORT, clops.OP_SCAN, clops.OP_SEGSORT):
         for w in (4, 8):
             assert d.feasible(d.fixed(op, 5000, w), op, 5000, w)
@@ -48,7 +49,7 @@ def test_config_table_rules(clops):
 
 def test_segment_rows(clops):
     d = clops.Dispatch
-    r = d.segment_row(1023, 8, 18, 2)
+    r = d.segment_row(1023, 8, 17, 2)  # tile 16, 3 blocks per CU
     assert r.size == clops.SEG_INPUTS
     assert np.allclose(r, [np.log2(1024) / 16, 1.0, 0.5, 0.75, 0, 0, 1])
     f = np.arange(clops.N_FEATURES, dtype=np.float32)
