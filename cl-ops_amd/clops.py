@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libcl_ops.so")
+# CLOPS_LIB selects another build of the same library (the sanitizer build,
+# lib/libcl_ops_san.so); the default is the in-tree build.
+LIB_PATH = os.environ.get("CLOPS_LIB") or os.path.join(HERE, "lib", "libcl_ops.so")
 
 CL_QUEUE_PROFILING_ENABLE = 1 << 1
 
