@@ -447,6 +447,8 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 				++sweeps;
 			}
 		bd->last_sweeps = sweeps;
+		CLO_LOG(1, "sbitonic: numel=%zu (network %u = 2^%u) launches=%u (one per stage and step)",
+			static_cast<size_t>(n), static_cast<unsigned>(npo2), S, sweeps);
 		return true;
 	}
 	// abitonic strategy: stages 1..k in one LDS tile kernel, then per stage the
@@ -482,6 +484,8 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 		}
 	}
 	bd->last_sweeps = sweeps;
+	CLO_LOG(1, "abitonic: numel=%zu (network 2^%u) LDS tile 2^%u, register groups of <= %u steps, "
+		"launches=%u", static_cast<size_t>(n), S, k, qmax, sweeps);
 	return true;
 }
 

@@ -2,9 +2,12 @@
 // and the common module (CL_Ops' src/cl_ops/common/clo_common.c).
 
 #include <atomic>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "clo_internal.hpp"
 
@@ -80,7 +83,36 @@ CCLEvent* record_event(CCLQueue* q, const char* name, GError** err) {
 
 thread_local bool g_launch_quiet = false;
 
+namespace {
+std::atomic<int> g_log_override{-1};
+int env_log_level() {
+	static const int lvl = [] {
+		const char* e = getenv("CLO_LOG");
+		return e ? atoi(e) : 0;
+	}();
+	return lvl;
+}
+}  // namespace
+
+int log_level() {
+	const int o = g_log_override.load(std::memory_order_relaxed);
+	return o >= 0 ? o : env_log_level();
+}
+
+void log_msg(const char* fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof buf, fmt, ap);
+	va_end(ap);
+	fprintf(stderr, "clo: %s\n", buf);
+}
+
+ApiRange::ApiRange(const char* name) { roctxRangePushA(name); }
+ApiRange::~ApiRange() { roctxRangePop(); }
+
 Launch::Launch(CCLQueue* q_, const char* name_) : q(q_), name(name_), rec(nullptr) {
+	CLO_LOG(2, "launch %s", name_);
 	(void)hipGetLastError();  // clear stale launch errors
 	if (!q->prof || q->capturing || g_launch_quiet) return;
 	if (!q->prof_filter.empty() &&
@@ -651,6 +683,9 @@ CloType clo_type_by_name(const char* name, GError** err) {
 
 // clo_common.c:141-198.  Same results as the reference's bit tricks,
 // including nlpo2(0) == 0 and nlpo2(x > 2^31) == 0 (32-bit wrap).
+void clo_set_log_level(int level) { g_log_override.store(level < 0 ? -1 : level); }
+int clo_get_log_level(void) { return log_level(); }
+
 unsigned int clo_nlpo2(unsigned int x) {
 	if ((x & (x - 1)) == 0) return x;
 	if (x > 0x80000000u) return 0u;

@@ -92,6 +92,28 @@ extern thread_local bool g_launch_quiet;
 
 // Kernel launch wrapper: records start/end events when the queue profiles,
 // and converts a launch failure into a GError.
+// Debug log of the library's decisions on stderr ("clo: ..."), as CL_Ops
+// logs them with g_debug (clo_sort_satradix.c:171,192,
+// clo_sort_abitonic.c:398-407, clo_scan_blelloch.c:160): level 1 one line
+// per call with the chosen path and sizes, level 2 also every kernel launch.
+// The level comes from clo_set_log_level, else the CLO_LOG environment
+// variable, else 0 (off).
+int log_level();
+void log_msg(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+#define CLO_LOG(level, ...)                                      \
+	do {                                                         \
+		if (::clo::log_level() >= (level)) ::clo::log_msg(__VA_ARGS__); \
+	} while (0)
+
+// A named range around one public call for tracing tools (roctx, shown by
+// rocprofv3 --marker-trace); a no-op when no tool is attached.
+struct ApiRange {
+	explicit ApiRange(const char* name);
+	~ApiRange();
+	ApiRange(const ApiRange&) = delete;
+	ApiRange& operator=(const ApiRange&) = delete;
+};
+
 struct Launch {
 	CCLQueue* q;
 	const char* name;

@@ -448,6 +448,7 @@ cl_bool clo_dispatch_features(CloDispatch* d, CCLQueue* cq, CloOp op, CCLBuffer*
 cl_bool clo_dispatch_features_batch(CloDispatch* d, CCLQueue* cq, size_t count, const int* ops,
 	CCLBuffer* const* data, const size_t* ns, const size_t* widths, float* features,
 	GError** err) {
+	ApiRange api_range("clo_dispatch_features_batch");
 	if (err && *err) return CL_FALSE;
 	if (!d || !cq || (count && (!ops || !data || !ns || !widths || !features)) ||
 		count > 0x7FFFFFFFu) {
@@ -504,6 +505,7 @@ cl_bool clo_dispatch_features_segmented(CloDispatch* d, CCLQueue* cq, CCLBuffer*
 
 CCLEvent* clo_dispatch_run(CloDispatch* d, CCLQueue* cq, cl_uint cfg, CloOp op, CCLBuffer* in,
 	CCLBuffer* out, size_t n, size_t width, GError** err) {
+	ApiRange api_range("clo_dispatch_run");
 	if (err && *err) return nullptr;
 	if (!d || !cq || !in || !out || !clo_dispatch_config_feasible(cfg, op, n, width)) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "configuration %u not feasible for this request",
@@ -533,6 +535,7 @@ CCLEvent* clo_dispatch_run(CloDispatch* d, CCLQueue* cq, cl_uint cfg, CloOp op, 
 
 cl_bool clo_dispatch_tune(CloDispatch* d, CCLQueue* cq, CloOp op, CCLBuffer* in, CCLBuffer* out,
 	size_t n, size_t width, cl_uint reps, float* times_ms, GError** err) {
+	ApiRange api_range("clo_dispatch_tune");
 	if (err && *err) return CL_FALSE;
 	if (!d || !cq || !times_ms || reps == 0) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "bad tuner arguments");
@@ -554,6 +557,7 @@ cl_bool clo_dispatch_tune(CloDispatch* d, CCLQueue* cq, CloOp op, CCLBuffer* in,
 CCLEvent* clo_dispatch_run_segmented(CloDispatch* d, CCLQueue* cq, cl_uint cfg, CCLBuffer* in,
 	CCLBuffer* out, CCLBuffer* offsets, size_t nseg, size_t n, size_t width,
 	const unsigned char* classes, GError** err) {
+	ApiRange api_range("clo_dispatch_run_segmented");
 	if (err && *err) return nullptr;
 	if (!d || !cq || !in || !out || !offsets ||
 		!clo_dispatch_config_feasible(cfg, CLO_OP_SEGSORT, n, width)) {
@@ -575,6 +579,7 @@ CCLEvent* clo_dispatch_run_segmented(CloDispatch* d, CCLQueue* cq, cl_uint cfg, 
 cl_bool clo_dispatch_tune_segmented(CloDispatch* d, CCLQueue* cq, CCLBuffer* in, CCLBuffer* out,
 	CCLBuffer* offsets, size_t nseg, size_t n, size_t width, cl_uint reps, float* times_ms,
 	GError** err) {
+	ApiRange api_range("clo_dispatch_tune_segmented");
 	if (err && *err) return CL_FALSE;
 	if (!d || !cq || !times_ms || reps == 0) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "bad tuner arguments");
@@ -815,6 +820,7 @@ void clo_dispatch_set_margin(CloDispatch* d, float log2_margin) {
 cl_bool clo_dispatch_choose(CloDispatch* d, CCLQueue* cq, CloPolicy policy, size_t count,
 	const int* ops, const size_t* ns, const size_t* widths, const float* features, int* cfg_out,
 	GError** err) {
+	ApiRange api_range("clo_dispatch_choose");
 	if (err && *err) return CL_FALSE;
 	if (!d || !ops || !ns || !widths || !features || !cfg_out) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "bad choose arguments");
@@ -858,13 +864,17 @@ cl_bool clo_dispatch_choose(CloDispatch* d, CCLQueue* cq, CloPolicy policy, size
 			arg[r] = static_cast<int>(who[i].second);
 		}
 	}
+	size_t moved = 0;
 	for (size_t r = 0; r < count; ++r) {
 		if (arg[r] < 0) continue;
 		// Leave the fixed configuration only for a predicted gain above the margin.
-		cfg_out[r] = best[r] < fixed_pred[r] - d->margin
-			? arg[r] : static_cast<int>(clo_dispatch_fixed_config(static_cast<CloOp>(ops[r]),
-				ns[r], widths[r]));
+		const int fx = static_cast<int>(clo_dispatch_fixed_config(static_cast<CloOp>(ops[r]),
+			ns[r], widths[r]));
+		cfg_out[r] = best[r] < fixed_pred[r] - d->margin ? arg[r] : fx;
+		moved += cfg_out[r] != fx ? 1 : 0;
 	}
+	CLO_LOG(1, "dispatch: model policy over %zu requests (%zu rows, one inference launch): "
+		"%zu leave the fixed configuration", count, who.size(), moved);
 	return CL_TRUE;
 }
 

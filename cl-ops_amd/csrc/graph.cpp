@@ -104,6 +104,7 @@ CloGraph* clo_graph_end_capture(CCLQueue* cq, GError** err) {
 }
 
 CCLEvent* clo_graph_launch(CloGraph* graph, CCLQueue* cq, GError** err) {
+	ApiRange api_range("clo_graph_launch");
 	if (err && *err) return nullptr;
 	if (!graph || !cq) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL graph or queue");

@@ -210,6 +210,7 @@ static std::string strip(const char* s) {
 
 CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type, void* seeds, size_t seeds_count,
 	cl_ulong main_seed, const char* hash, CCLContext* ctx, CCLQueue* cq, GError** err) {
+	ApiRange api_range("clo_rng_new");
 	if (err && *err) return nullptr;
 	if (!ctx) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "clo_rng_new: NULL context");
@@ -347,6 +348,7 @@ size_t clo_rng_get_size(CloRng* rng) {
 
 CCLEvent* clo_rng_fill(CloRng* rng, CCLQueue* cq, CCLBuffer* out, cl_uint draws, cl_uint bits,
 	cl_uint maxint, GError** err) {
+	ApiRange api_range("clo_rng_fill");
 	if (err && *err) return nullptr;
 	if (!rng || !cq || !out) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL rng, queue or buffer");
@@ -362,6 +364,8 @@ CCLEvent* clo_rng_fill(CloRng* rng, CCLQueue* cq, CCLBuffer* out, cl_uint draws,
 		return nullptr;
 	}
 	if (!use_device(cq->ctx, err)) return nullptr;
+	CLO_LOG(1, "rng fill: generator %d, %llu work-items x %u draws, %s", rng->gen,
+		static_cast<unsigned long long>(G), draws, maxint ? "next %% maxint" : "next >> (32 - bits)");
 	if (draws == 0) return record_event(cq, "clo_rng_fill", err);
 	const u32 shift = 32 - bits;
 	const u32 g = static_cast<u32>((G + 255) / 256);

@@ -552,6 +552,9 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	const u32 nblk = static_cast<u32>(div_ceil(tiles, kScanTiles));
 	const size_t plan_bytes = (sizeof(SortPlan) + 255) / 256 * 256;
 	const size_t andor_bytes = (2 * static_cast<size_t>(tiles) * sizeof(K) + 255) / 256 * 256;
+	CLO_LOG(1, "satradix: numel=%zu key=%zu B%s radix=%u (bits_in_digit=%u) tile=%dx%d tiles=%u "
+		"passes<=%u%s", n, sizeof(K), KV ? " +u32 values" : "", 1u << bits, bits, NT, ITEMS, tiles,
+		passes, in_place ? " in place" : "");
 	if (!sd->aux_keys.ensure(n * sizeof(K), err)) return false;
 	if (KV && !sd->aux_vals.ensure(n * sizeof(u32), err)) return false;
 	if (!sd->counts.ensure(static_cast<size_t>(tiles) * kRadixMax * sizeof(u32), err)) return false;

@@ -898,6 +898,12 @@ bool scan_ptrs(CloScan* scanner, CCLQueue* cq_exec, const void* in, void* out, s
 	a.chain = chain;
 	a.status = status;
 	a.q = cq_exec;
+	CLO_LOG(1, "blelloch: numel=%zu elem=%zu B sum=%zu B tiles=%zu of 256x%u %s%s", numel,
+		clo_type_sizeof(scanner->elem_type), ss, tiles, items,
+		off ? "segmented (reduce-then-scan)"
+			: status ? "single pass (decoupled look-back) if plain unsigned, else reduce-then-scan"
+					 : "reduce-then-scan",
+		ws ? ", caller workspace" : "");
 	bool ok = false;
 	switch (scanner->elem_type) {
 	case CLO_CHAR: ok = launch_for_in<int8_t>(ss, a, err); break;
@@ -1063,6 +1069,7 @@ static bool scan_placement_ok(CloScan* sc, CCLBuffer* in, CCLBuffer* out, size_t
 
 CCLEvent* clo_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, size_t lws_max, GError** err) {
+	ApiRange api_range("clo_scan_with_device_data");
 	if (err && *err) return nullptr;
 	if (!scanner || !cq_exec || !data_in || !data_out) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "NULL scanner, queue or buffer");
@@ -1082,6 +1089,7 @@ CCLEvent* clo_scan_with_device_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueu
 CCLEvent* clo_scan_segmented_with_device_data(CloScan* scanner, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* data_in, CCLBuffer* data_out, CCLBuffer* offsets, size_t nseg,
 	size_t numel, GError** err) {
+	ApiRange api_range("clo_scan_segmented_with_device_data");
 	(void)cq_comm;
 	if (err && *err) return nullptr;
 	if (!scanner || !cq_exec || !data_in || !data_out || !offsets) {
@@ -1107,6 +1115,7 @@ CCLEvent* clo_scan_segmented_with_device_data(CloScan* scanner, CCLQueue* cq_exe
 
 cl_bool clo_scan_with_host_data(CloScan* scanner, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	void* data_in, void* data_out, size_t numel, size_t lws_max, GError** err) {
+	ApiRange api_range("clo_scan_with_host_data");
 	(void)lws_max;
 	if (err && *err) return CL_FALSE;
 	if (!scanner || !data_in || !data_out) {

@@ -660,6 +660,9 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	}
 	u32* lists = static_cast<u32*>(sd->lists.ptr);
 	const bool kv = vin != nullptr;
+	CLO_LOG(1, "segmented sort: nseg=%zu numel=%zu key=%zu B%s tile=256x%u (CAP %u) grid=%u "
+		"classes=%s", nseg, numel, ks, kv ? " +u32 values" : "", tile_items, 256 * tile_items, grid,
+		(sorter->seg_cls[0] | sorter->seg_cls[1]) ? "per length bin" : "by length");
 	const bool inplace = kout == nullptr || kout->ptr == kin->ptr;
 #define SEG_ARGS(K)                                                                          \
 	SegArgs<K> a;                                                                            \

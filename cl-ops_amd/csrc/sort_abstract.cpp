@@ -153,6 +153,7 @@ static bool check_kv_placement(CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer
 
 CCLEvent* clo_sort_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	CCLBuffer* data_in, CCLBuffer* data_out, size_t numel, size_t lws_max, GError** err) {
+	ApiRange api_range("clo_sort_with_device_data");
 	if (err && *err) return nullptr;
 	if (!check_device_args(sorter, cq_exec, data_in, data_out, numel,
 			sorter ? clo_type_sizeof(sorter->elem_type) : 0, err))
@@ -170,6 +171,7 @@ CCLEvent* clo_sort_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue
 
 cl_bool clo_sort_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	void* data_in, void* data_out, size_t numel, size_t lws_max, GError** err) {
+	ApiRange api_range("clo_sort_with_host_data");
 	(void)lws_max;
 	if (err && *err) return CL_FALSE;
 	if (!sorter || !data_in || !data_out) {
@@ -193,6 +195,7 @@ cl_bool clo_sort_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq
 CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out, CCLBuffer* vals_out,
 	size_t numel, size_t lws_max, GError** err) {
+	ApiRange api_range("clo_sort_kv_with_device_data");
 	(void)lws_max;
 	if (err && *err) return nullptr;
 	if (!check_device_args(sorter, cq_exec, keys_in, keys_out, numel,
@@ -223,6 +226,7 @@ CCLEvent* clo_sort_kv_with_device_data(CloSort* sorter, CCLQueue* cq_exec, CCLQu
 
 cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue* cq_comm,
 	void* keys, void* vals, size_t numel, size_t lws_max, GError** err) {
+	ApiRange api_range("clo_sort_kv_with_host_data");
 	(void)lws_max;
 	if (err && *err) return CL_FALSE;
 	if (!sorter || !keys || !vals) {
@@ -244,6 +248,7 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec, CCLQueue*
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
 	CCLBuffer* vals_out, CCLBuffer* offsets, size_t nseg, size_t numel, GError** err) {
+	ApiRange api_range("clo_sort_segmented_with_device_data");
 	(void)cq_comm;
 	if (err && *err) return nullptr;
 	if (!check_device_args(sorter, cq_exec, keys_in, keys_out, numel,

@@ -66,6 +66,14 @@ unsigned int clo_ones32(unsigned int x);
 unsigned int clo_tzc(int x);
 unsigned int clo_sum(unsigned int x);
 
+/* Debug log of the library's decisions on stderr, one "clo: ..." line per
+ * call (the chosen kernel path and its sizes), as CL_Ops logs them with
+ * g_debug (clo_sort_satradix.c:171,192): 0 off, 1 per call, 2 also every
+ * kernel launch; -1 restores the default, the CLO_LOG environment variable
+ * (0 when unset).  No counterpart function in CL_Ops (an addition). */
+void clo_set_log_level(int level);
+int clo_get_log_level(void);
+
 /* clo_common.c:206-208: a print handler that discards its input. */
 void clo_print_to_null(const gchar* string);
 

@@ -116,3 +116,15 @@ def test_no_context_without_device(clops):
         assert e.domain == L.ccl_error_quark()
     else:
         raise AssertionError("device 4096 should not exist: %r" % ctx)
+
+
+def test_log_level_setter(clops):
+    L = clops.lib()
+    L.clo_set_log_level.argtypes = [ctypes.c_int]
+    L.clo_get_log_level.restype = ctypes.c_int
+    L.clo_set_log_level(2)
+    assert L.clo_get_log_level() == 2
+    L.clo_set_log_level(0)
+    assert L.clo_get_log_level() == 0
+    L.clo_set_log_level(-1)  # back to the CLO_LOG environment variable
+    assert L.clo_get_log_level() == int(os.environ.get("CLO_LOG", "0") or 0)

@@ -106,3 +106,42 @@ def test_device_generators_match_oracle(clops, oracle, ctx, queue):
         b = clops.Buffer(ctx, n * np.dtype(dt).itemsize)
         clops.generate(queue, b, n, kind, 0xABC, param, 5)
         assert np.array_equal(b.read(queue, dt, n), oracle.generate(kind, 0xABC, n, param, 5))
+
+
+def test_debug_log_lines(clops, oracle, ctx, capfd):
+    """Level 1 logs one line per call with the chosen path and sizes, level 2
+    also every kernel launch; level 0 is silent."""
+    import ctypes
+    L = clops.lib()
+    L.clo_set_log_level.argtypes = [ctypes.c_int]
+    q = clops.Queue(ctx)
+    n = 100000
+    k = oracle.generate(oracle.GEN_U64, 3, n)
+    b, o = clops.Buffer.from_array(ctx, k), clops.Buffer(ctx, n * 8)
+    s = clops.Sorter(ctx, "satradix", np.uint64)
+    sc = clops.Scanner(ctx, np.uint32, np.uint32)
+    a = clops.Buffer.from_array(ctx, np.arange(n, dtype=np.uint32))
+    capfd.readouterr()
+    L.clo_set_log_level(1)
+    try:
+        s.sort_device(q, b, o, n)
+        sc.scan_device(q, a, o, n)
+        q.finish()
+    finally:
+        L.clo_set_log_level(0)
+    err = capfd.readouterr().err
+    assert "clo: satradix: numel=100000 key=8 B radix=256 (bits_in_digit=8)" in err, err
+    assert "clo: blelloch: numel=100000 elem=4 B sum=4 B" in err, err
+    assert "clo: launch" not in err
+    L.clo_set_log_level(2)
+    try:
+        s.sort_device(q, b, o, n)
+        q.finish()
+    finally:
+        L.clo_set_log_level(0)
+    err = capfd.readouterr().err
+    assert "clo: launch satradix_downsweep" in err, err
+    s.sort_device(q, b, o, n)
+    q.finish()
+    assert "clo:" not in capfd.readouterr().err
+    assert np.array_equal(o.read(q, np.uint64, n), np.sort(k))

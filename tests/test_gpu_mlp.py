@@ -14,7 +14,7 @@ def _case(n_in, rows, seed):
 
 
 @pytest.mark.parametrize("n_in,rows", [(1, 1), (5, 100), (17, 128), (17, 129), (32, 1000),
-                                       (12, 10007)])
+                                       (12, 10007), (34, 500), (48, 257)])
 def test_gradient_matches_fp64_oracle(clops, oracle, ctx, queue, n_in, rows):
     m = clops.Mlp(ctx, n_in, seed=n_in * 31 + rows)
     X, t = _case(n_in, rows, rows)
@@ -81,6 +81,6 @@ def test_mlp_argument_errors(clops, ctx, queue):
     with pytest.raises(clops.ClopsError):
         clops.Mlp(ctx, 0)
     with pytest.raises(clops.ClopsError):
-        clops.Mlp(ctx, 33)
+        clops.Mlp(ctx, 49)  # inputs 1..48
     with pytest.raises(clops.ClopsError):
         clops.Mlp(ctx, path="/nonexistent/w.clomlp")
