@@ -105,6 +105,10 @@ void log_msg(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 		if (::clo::log_level() >= (level)) ::clo::log_msg(__VA_ARGS__); \
 	} while (0)
 
+// Enqueues on cq the deferred DEV_GID seed initialisation of every RNG on
+// cq's device that has not filled yet (rng.hip).
+bool rng_init_pending_on(CCLQueue* cq, GError** err);
+
 // A named range around one public call for tracing tools (roctx, shown by
 // rocprofv3 --marker-trace); a no-op when no tool is attached.
 struct ApiRange {

@@ -48,6 +48,9 @@ cl_bool clo_graph_begin_capture(CCLQueue* cq, GError** err) {
 		return CL_FALSE;
 	}
 	if (!use_device(cq->ctx, err)) return CL_FALSE;
+	// RNG seeds whose initialisation waits for their first fill are
+	// initialised now, outside the graph: a replay must not re-seed them.
+	if (!rng_init_pending_on(cq, err)) return CL_FALSE;
 	// Thread-local mode: allocation and synchronisation calls on this thread
 	// fail while the capture is open instead of silently escaping the graph.
 	if (hip_failed(hipStreamBeginCapture(cq->stream, hipStreamCaptureModeThreadLocal), err,

@@ -2,8 +2,11 @@
 // kernels: device seed initialisation (clo_rng_init.cl:27-60) and the
 // register-resident multi-draw fill.
 
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "clo_internal.hpp"
 #include "clo_rng_device.h"
@@ -46,12 +49,18 @@ __global__ __launch_bounds__(256) void rng_init_kernel(NS_STATE* seeds, u64 coun
 }
 
 // out[r * G + gid] for r < draws; the state is read once and written once.
-template <typename S, unsigned int (*STEP)(S&)>
+// INIT: the first fill of DEV_GID seeds, whose initialisation
+// (clo_rng_init.cl:47-60) was deferred to here: the state comes from
+// CONV(hash(gid + main_seed)) instead of memory, so the states are written
+// once instead of written, read and written again.
+template <typename S, unsigned int (*STEP)(S&), S (*CONV)(unsigned long long), bool INIT>
 __global__ __launch_bounds__(256) void rng_fill_kernel(S* __restrict__ states, u32* __restrict__ out,
-	u64 G, u32 draws, u32 shift, u32 maxint) {
+	u64 G, u32 draws, u32 shift, u32 maxint, u64 main_seed, u32 hash) {
 	const u64 gid = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (gid >= G) return;
-	S s = states[gid];
+	S s;
+	if constexpr (INIT) s = CONV(apply_hash(gid + main_seed, hash));
+	else s = states[gid];
 	u32* o = out + gid;
 	if (maxint == 0) {
 		for (u32 r = 0; r < draws; ++r) {
@@ -119,7 +128,33 @@ struct clo_rng {
 	CCLBuffer* seeds;
 	size_t count;
 	std::string src;
+	// DEV_GID seeds with a predefined hash are initialised by the first fill
+	// (or when the seeds are first read, or a graph capture begins on the
+	// device): until then `pending` and the seeds buffer holds no states.
+	bool pending = false;
+	u64 init_seed = 0;
+	u32 init_hash = 0;
 };
+
+namespace {
+std::mutex g_pending_mu;
+std::vector<CloRng*> g_pending;  // RNGs whose DEV_GID initialisation is deferred
+
+void pending_remove(CloRng* r) {
+	std::lock_guard<std::mutex> lk(g_pending_mu);
+	g_pending.erase(std::remove(g_pending.begin(), g_pending.end(), r), g_pending.end());
+}
+template <typename S, unsigned int (*STEP)(S&), S (*CONV)(unsigned long long)>
+void launch_fill(bool init, u32 g, hipStream_t s, void* st, u32* o, u64 G, u32 draws, u32 shift,
+	u32 maxint, const CloRng* rng) {
+	if (init)
+		rng_fill_kernel<S, STEP, CONV, true><<<g, 256, 0, s>>>(static_cast<S*>(st), o, G, draws,
+			shift, maxint, rng->init_seed, rng->init_hash);
+	else
+		rng_fill_kernel<S, STEP, CONV, false><<<g, 256, 0, s>>>(static_cast<S*>(st), o, G, draws,
+			shift, maxint, 0, 0);
+}
+}  // namespace
 
 extern "C" {
 
@@ -183,6 +218,33 @@ static bool launch_init(int gen, void* seeds, u64 count, u64 main_seed, u32 hash
 	}
 	return L.done(err);
 }
+
+}  // extern "C"
+
+namespace clo {
+
+// Deferred DEV_GID initialisations of RNGs on cq's device, enqueued on cq
+// (a graph capture must not replay an initialisation; clo_graph_begin_capture
+// calls this first).
+bool rng_init_pending_on(CCLQueue* cq, GError** err) {
+	std::vector<CloRng*> todo;
+	{
+		std::lock_guard<std::mutex> lk(g_pending_mu);
+		for (CloRng* r : g_pending)
+			if (r->seeds->ctx->dev == cq->ctx->dev) todo.push_back(r);
+	}
+	for (CloRng* r : todo) {
+		if (!launch_init(r->gen, r->seeds->ptr, r->count, r->init_seed, r->init_hash, cq, err))
+			return false;
+		r->pending = false;
+		pending_remove(r);
+	}
+	return true;
+}
+
+}  // namespace clo
+
+extern "C" {
 
 // DEV_GID initialisation for a seed hash other than the predefined ones:
 // clo_rng_init.cl's kernel with `#define CLO_RNG_HASH(x) <hash>` prepended
@@ -263,8 +325,24 @@ CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type, void* seeds, siz
 			void* args[] = {&p, &cnt, &ms};
 			launched = rtc_launch(iq, "clo_rng_init", init_fn, seeds_count, args, err);
 		} else if (buf) {
-			launched = launch_init(gen, buf->ptr, seeds_count, main_seed,
-				static_cast<u32>(hid), iq, err);
+			// Predefined hash: the initialisation is deferred to the first
+			// fill, which computes the states instead of reading them.
+			CloRng* r = new clo_rng();
+			r->gen = gen;
+			r->seeds = buf;
+			r->count = seeds_count;
+			r->src = gen_source(gen);
+			r->pending = true;
+			r->init_seed = main_seed;
+			r->init_hash = static_cast<u32>(hid);
+			{
+				std::lock_guard<std::mutex> lk(g_pending_mu);
+				g_pending.push_back(r);
+			}
+			if (own) ccl_queue_destroy(own);
+			CLO_LOG(1, "rng new: %s, %zu work-items, DEV_GID seeds (hash %d) deferred to the "
+				"first fill", kInfos[gen].name, seeds_count, hid);
+			return r;
 		}
 		if (!launched) {
 			if (buf) ccl_buffer_destroy(buf);
@@ -336,12 +414,33 @@ CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type, void* seeds, siz
 
 void clo_rng_destroy(CloRng* rng) {
 	if (!rng) return;
+	if (rng->pending) pending_remove(rng);
 	ccl_buffer_destroy(rng->seeds);
 	delete rng;
 }
 
 const char* clo_rng_get_source(CloRng* rng) { return rng ? rng->src.c_str() : nullptr; }
-CCLBuffer* clo_rng_get_device_seeds(CloRng* rng) { return rng ? rng->seeds : nullptr; }
+
+CCLBuffer* clo_rng_get_device_seeds(CloRng* rng) {
+	if (!rng) return nullptr;
+	if (rng->pending) {
+		// The caller may read the buffer at once: initialise it now, blocking.
+		GError* e = nullptr;
+		CCLQueue* q = ccl_queue_new(rng->seeds->ctx, nullptr, 0, &e);
+		if (q && launch_init(rng->gen, rng->seeds->ptr, rng->count, rng->init_seed,
+				rng->init_hash, q, &e) && ccl_queue_finish(q, &e)) {
+			rng->pending = false;
+			pending_remove(rng);
+		}
+		if (q) ccl_queue_destroy(q);
+		if (e) {
+			fprintf(stderr, "clo_rng_get_device_seeds: %s\n", e->message ? e->message : "error");
+			clo_error_free(e);
+			return nullptr;
+		}
+	}
+	return rng->seeds;
+}
 size_t clo_rng_get_size(CloRng* rng) {
 	return rng ? rng->count * kInfos[rng->gen].seed_size : 0;
 }
@@ -372,39 +471,42 @@ CCLEvent* clo_rng_fill(CloRng* rng, CCLQueue* cq, CCLBuffer* out, cl_uint draws,
 	void* st = rng->seeds->ptr;
 	u32* o = static_cast<u32*>(out->ptr);
 	hipStream_t s = cq->stream;
-	Launch L(cq, "clo_rng_fill");
+	const bool init = rng->pending;
+	Launch L(cq, init ? "clo_rng_fill_init" : "clo_rng_fill");
 	switch (rng->gen) {
 	case kLcg:
-		rng_fill_kernel<clo_rng_lcg::clo_statetype, clo_rng_lcg::clo_rng_step><<<g, 256, 0, s>>>(
-			static_cast<clo_rng_lcg::clo_statetype*>(st), o, G, draws, shift, maxint);
+		launch_fill<clo_rng_lcg::clo_statetype, clo_rng_lcg::clo_rng_step,
+			clo_rng_lcg::clo_ulong2statetype>(init, g, s, st, o, G, draws, shift, maxint, rng);
 		break;
 	case kXs64:
-		rng_fill_kernel<clo_rng_xorshift64::clo_statetype, clo_rng_xorshift64::clo_rng_step>
-			<<<g, 256, 0, s>>>(static_cast<clo_rng_xorshift64::clo_statetype*>(st), o, G, draws,
-			shift, maxint);
+		launch_fill<clo_rng_xorshift64::clo_statetype, clo_rng_xorshift64::clo_rng_step,
+			clo_rng_xorshift64::clo_ulong2statetype>(init, g, s, st, o, G, draws, shift, maxint,
+			rng);
 		break;
 	case kXs128:
-		rng_fill_kernel<clo_rng_xorshift128::clo_statetype, clo_rng_xorshift128::clo_rng_step>
-			<<<g, 256, 0, s>>>(static_cast<clo_rng_xorshift128::clo_statetype*>(st), o, G, draws,
-			shift, maxint);
+		launch_fill<clo_rng_xorshift128::clo_statetype, clo_rng_xorshift128::clo_rng_step,
+			clo_rng_xorshift128::clo_ulong2statetype>(init, g, s, st, o, G, draws, shift, maxint,
+			rng);
 		break;
 	case kMwc:
-		rng_fill_kernel<clo_rng_mwc64x::clo_statetype, clo_rng_mwc64x::clo_rng_step>
-			<<<g, 256, 0, s>>>(static_cast<clo_rng_mwc64x::clo_statetype*>(st), o, G, draws,
-			shift, maxint);
+		launch_fill<clo_rng_mwc64x::clo_statetype, clo_rng_mwc64x::clo_rng_step,
+			clo_rng_mwc64x::clo_ulong2statetype>(init, g, s, st, o, G, draws, shift, maxint, rng);
 		break;
 	case kPm:
-		rng_fill_kernel<clo_rng_parkmiller::clo_statetype, clo_rng_parkmiller::clo_rng_step>
-			<<<g, 256, 0, s>>>(static_cast<clo_rng_parkmiller::clo_statetype*>(st), o, G, draws,
-			shift, maxint);
+		launch_fill<clo_rng_parkmiller::clo_statetype, clo_rng_parkmiller::clo_rng_step,
+			clo_rng_parkmiller::clo_ulong2statetype>(init, g, s, st, o, G, draws, shift, maxint,
+			rng);
 		break;
 	case kTaus:
-		rng_fill_kernel<clo_rng_tauslcg::clo_statetype, clo_rng_tauslcg::clo_rng_step>
-			<<<g, 256, 0, s>>>(static_cast<clo_rng_tauslcg::clo_statetype*>(st), o, G, draws,
-			shift, maxint);
+		launch_fill<clo_rng_tauslcg::clo_statetype, clo_rng_tauslcg::clo_rng_step,
+			clo_rng_tauslcg::clo_ulong2statetype>(init, g, s, st, o, G, draws, shift, maxint, rng);
 		break;
 	}
 	if (!L.done(err)) return nullptr;
+	if (init) {
+		rng->pending = false;
+		pending_remove(rng);
+	}
 	return record_event(cq, "clo_rng_fill", err);
 }
 

@@ -52,7 +52,14 @@ typedef enum clo_rng_seed_type {
  *            (uint32) main_seed (GLib g_rand_new_with_seed/g_rand_int).
  *  EXT_DEV : `seeds` is a CCLBuffer* of seeds_count states (referenced).
  *  EXT_HOST: `seeds` is a host array of seeds_count states (copied).
- * HOST_MT and EXT_HOST block; DEV_GID is asynchronous on `cq`. */
+ * HOST_MT and EXT_HOST block; DEV_GID is asynchronous on `cq`.
+ * DEV_GID with a predefined hash defers the initialisation: the first
+ * clo_rng_fill computes the states from gid instead of reading them (the
+ * same draws and final states; the state array is written once instead of
+ * written, read and written again).  clo_rng_get_device_seeds before the
+ * first fill initialises them first (blocking), and so does
+ * clo_graph_begin_capture for every such RNG on the device (on the queue
+ * being captured, before the capture), so a graph never re-seeds. */
 CloRng* clo_rng_new(const char* type, CloRngSeedType seed_type,
 	void* seeds, size_t seeds_count, cl_ulong main_seed,
 	const char* hash, CCLContext* ctx, CCLQueue* cq, GError** err);

@@ -209,3 +209,45 @@ extern "C" __global__ void user_draw(clo_statetype* st, unsigned int* out, unsig
             want[g, k] = oracle.rng_next(gen, s) % nmod
     assert np.array_equal(got, want)
     hip.hipModuleUnload(mod)
+
+
+@pytest.mark.parametrize("gen", GENS)
+@pytest.mark.parametrize("hash_name,hash_id", [(None, 0), ("XS1(x)", 2)])
+def test_dev_gid_first_fill_computes_seeds(clops, oracle, ctx, queue, gen, hash_name, hash_id):
+    """DEV_GID seeds are computed by the first fill (never read from memory
+    before it): the draws and the states after it equal the oracle's from
+    the seeded states, and a second fill continues the stream."""
+    G, draws = 5003, 7
+    r = clops.Rng(ctx, gen, clops.SEED_DEV_GID, G, main_seed=77, hash=hash_name, q=queue)
+    st = oracle.rng_init_dev_gid(gen, G, 77, hash_id)
+    out = clops.Buffer(ctx, G * draws * 4)
+    for rep in range(2):
+        r.fill(queue, out, draws)
+        ref = oracle.rng_fill(gen, st, draws)  # advances st in place
+        assert np.array_equal(out.read(queue, np.uint32, G * draws), ref), rep
+    assert np.array_equal(r.read_states(queue), st)
+
+
+def test_dev_gid_seeds_initialised_before_capture(clops, oracle, ctx):
+    """A graph captured over the first fill of DEV_GID seeds does not
+    re-seed on replay: the seeds are initialised when the capture begins,
+    and every replay continues the stream."""
+    q = clops.Queue(ctx)
+    G, draws = 4096, 4
+    r = clops.Rng(ctx, "xorshift128", clops.SEED_DEV_GID, G, main_seed=3, q=q)
+    st = oracle.rng_init_dev_gid("xorshift128", G, 3, 0)
+    out = clops.Buffer(ctx, G * draws * 4)
+    g = clops.Graph(q)
+    g.begin()
+    try:
+        r.fill(q, out, draws)
+    finally:
+        g.end()
+    q.finish()
+    for rep in range(3):
+        g.launch()
+        q.finish()
+        ref = oracle.rng_fill("xorshift128", st, draws)
+        assert np.array_equal(out.read(q, np.uint32, G * draws), ref), rep
+    g.close()
+    assert np.array_equal(r.read_states(q), st)
