@@ -916,13 +916,51 @@ def dispatch_eval(ctx, n_train=2000, n_eval=1500, n_full=100000, n_sparse=200,
 
 
 def copy_probe(ctx, nbytes):
+    """Bandwidth ceilings: util_copy (the traffic counters' calibration
+    kernel), util_stream_copy (16 KiB tiles, four 16-byte loads in flight per
+    thread, XCD-contiguous) over `nbytes`, and util_scatter_runs, the write
+    pattern of one uniform radix pass over 2^28 (u64, u32) pairs in runs of
+    32, 64 and 128 pairs.  Each: 3 untimed launches, then 10 between two
+    markers; one JSON line per probe on stdout."""
     q = clops.Queue(ctx)
+    L = clops.lib()
     a = clops.Buffer(ctx, nbytes)
     b = clops.Buffer(ctx, nbytes)
-    for _ in range(3):
-        clops._call(clops.lib().clo_util_copy, q.ptr, b.ptr, a.ptr, nbytes)
-    q.finish()
-    log("copy probe: 3 x util_copy of %d bytes" % nbytes)
+    clops.generate(q, a, nbytes // 8, clops.GEN_U64, 1)
+    res = []
+
+    def timed(fn, name, nb):
+        for _ in range(3):
+            fn()
+        q.finish()
+        e0 = q.marker()
+        for _ in range(10):
+            fn()
+        e1 = q.marker()
+        q.finish()
+        ms = q.elapsed_ms(e0, e1) / 10
+        res.append({"probe": name, "ms": round(ms, 4), "bytes": nb,
+                    "GB/s": round(nb / ms / 1e6, 1), "frac_of_8TBs": round(nb / ms / 1e6 / 8000, 4)})
+
+    timed(lambda: clops._call(L.clo_util_copy, q.ptr, b.ptr, a.ptr, nbytes),
+          "util_copy %d bytes" % nbytes, 2 * nbytes)
+    timed(lambda: clops._call(L.clo_util_stream_copy, q.ptr, b.ptr, a.ptr, nbytes),
+          "util_stream_copy %d bytes" % nbytes, 2 * nbytes)
+    a.close()
+    b.close()
+    n = 1 << 28
+    ks, vs = clops.Buffer(ctx, n * 8), clops.Buffer(ctx, n * 4)
+    kd, vd = clops.Buffer(ctx, n * 8), clops.Buffer(ctx, n * 4)
+    clops.generate(q, ks, n, clops.GEN_U64, 2)
+    clops.generate(q, vs, n, clops.GEN_IOTA_U32, 0)
+    for run in (32, 64, 128):
+        timed(lambda: clops._call(L.clo_util_scatter_runs, q.ptr, kd.ptr, vd.ptr, ks.ptr, vs.ptr,
+                                  n, run),
+              "util_scatter_runs run=%d, 2^28 u64/u32 pairs" % run, n * 24)
+    # The scatter is a permutation: every pair lands once.
+    assert clops.multiset_hash(q, kd, n, 8) == clops.multiset_hash(q, ks, n, 8)
+    for r in res:
+        print(json.dumps(r), flush=True)
 
 
 def main():

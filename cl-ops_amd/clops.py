@@ -161,6 +161,8 @@ def lib():
         "clo_util_generate_zipf_u32": (vp, [vp, vp, sz, vp, sz, u64, ep]),
         "clo_util_zipf_cdf": (None, [vp, sz, d]),
         "clo_util_copy": (vp, [vp, vp, vp, sz, ep]),
+        "clo_util_stream_copy": (vp, [vp, vp, vp, sz, ep]),
+        "clo_util_scatter_runs": (vp, [vp, vp, vp, vp, vp, sz, u32, ep]),
         "clo_util_multiset_hash": (u64, [vp, vp, sz, sz, ep]),
         "clo_util_check_sorted": (sz, [vp, vp, vp, sz, sz, ep]),
         "clo_util_build_info": (cp, []),

@@ -108,6 +108,57 @@ __global__ __launch_bounds__(256) void util_copy(uint4* dst, const uint4* src, u
 	if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
 }
 
+// Streaming copy for the bandwidth ceiling: each block moves one 16 KiB
+// tile, four 16-byte loads in flight per thread before its stores, tiles
+// XCD-contiguous.
+constexpr u32 kStreamPer = 4;
+constexpr u32 kStreamTile = 256 * kStreamPer;  // uint4 per tile
+
+__global__ __launch_bounds__(256) void util_stream_copy(uint4* __restrict__ dst,
+	const uint4* __restrict__ src, u32 tiles) {
+	const u32 t = xcd_tile(tiles);
+	if (t >= tiles) return;
+	const u64 base = static_cast<u64>(t) * kStreamTile + threadIdx.x;
+	uint4 v[kStreamPer];
+#pragma unroll
+	for (u32 j = 0; j < kStreamPer; ++j) v[j] = src[base + j * 256];
+#pragma unroll
+	for (u32 j = 0; j < kStreamPer; ++j) dst[base + j * 256] = v[j];
+}
+
+// The write pattern of one uniform 8-bit radix pass over (u64 key, u32
+// value) pairs with nothing else: tile t of 8192 pairs is read contiguously
+// and written as 8192 / run runs of `run` pairs, run d of tile t to
+// d * (n / runs_per_tile) + t * run (every bucket receives `run` pairs per
+// tile, as uniform keys give), consecutive lanes writing consecutive pairs.
+constexpr u32 kProbeTile = 8192;
+
+__global__ __launch_bounds__(256) void util_scatter_runs(u64* __restrict__ kdst,
+	u32* __restrict__ vdst, const u64* __restrict__ ksrc, const u32* __restrict__ vsrc, u32 tiles,
+	u32 run) {
+	const u32 t = xcd_tile(tiles);
+	if (t >= tiles) return;
+	const u64 n = static_cast<u64>(tiles) * kProbeTile;
+	const u32 runs = kProbeTile / run;
+	const u64 bucket = n / runs;
+	constexpr u32 kPer = kProbeTile / 256;
+	u64 k[kPer];
+	u32 v[kPer];
+	const u64 base = static_cast<u64>(t) * kProbeTile;
+#pragma unroll
+	for (u32 j = 0; j < kPer; ++j) {
+		k[j] = ksrc[base + j * 256 + threadIdx.x];
+		v[j] = vsrc[base + j * 256 + threadIdx.x];
+	}
+#pragma unroll
+	for (u32 j = 0; j < kPer; ++j) {
+		const u32 i = j * 256 + threadIdx.x;
+		const u64 o = (i / run) * bucket + static_cast<u64>(t) * run + (i % run);
+		kdst[o] = k[j];
+		vdst[o] = v[j];
+	}
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void util_hash(const T* x, u64 n, u64* acc) {
 	const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
@@ -288,6 +339,47 @@ CCLEvent* clo_util_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src, size_t byt
 		static_cast<const unsigned char*>(src->ptr) + n16 * 16, ntail);
 	if (!L.done(err)) return nullptr;
 	return record_event(cq, "util_copy", err);
+}
+
+CCLEvent* clo_util_stream_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src, size_t bytes,
+	GError** err) {
+	const size_t tile_bytes = static_cast<size_t>(kStreamTile) * 16;
+	if (!cq || !dst || !src || dst->size < bytes || src->size < bytes || bytes % tile_bytes ||
+		bytes / tile_bytes > 0xFFFFFFF0u ||
+		(reinterpret_cast<uintptr_t>(dst->ptr) | reinterpret_cast<uintptr_t>(src->ptr)) % 16) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "stream copy takes 16 KiB multiples, aligned");
+		return nullptr;
+	}
+	if (!use_device(cq->ctx, err)) return nullptr;
+	const u32 tiles = static_cast<u32>(bytes / tile_bytes);
+	if (tiles) {
+		Launch L(cq, "util_stream_copy");
+		util_stream_copy<<<xcd_grid(tiles), 256, 0, cq->stream>>>(static_cast<uint4*>(dst->ptr),
+			static_cast<const uint4*>(src->ptr), tiles);
+		if (!L.done(err)) return nullptr;
+	}
+	return record_event(cq, "util_stream_copy", err);
+}
+
+CCLEvent* clo_util_scatter_runs(CCLQueue* cq, CCLBuffer* kdst, CCLBuffer* vdst, CCLBuffer* ksrc,
+	CCLBuffer* vsrc, size_t n, cl_uint run, GError** err) {
+	if (!cq || !kdst || !vdst || !ksrc || !vsrc || n % kProbeTile || n / kProbeTile > 0xFFFFFFF0u ||
+		(run != 32 && run != 64 && run != 128) || kdst->size < n * 8 || ksrc->size < n * 8 ||
+		vdst->size < n * 4 || vsrc->size < n * 4) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
+			"scatter probe takes n a multiple of 8192 pairs and runs of 32, 64 or 128");
+		return nullptr;
+	}
+	if (!use_device(cq->ctx, err)) return nullptr;
+	const u32 tiles = static_cast<u32>(n / kProbeTile);
+	if (tiles) {
+		Launch L(cq, "util_scatter_runs");
+		util_scatter_runs<<<xcd_grid(tiles), 256, 0, cq->stream>>>(static_cast<u64*>(kdst->ptr),
+			static_cast<u32*>(vdst->ptr), static_cast<const u64*>(ksrc->ptr),
+			static_cast<const u32*>(vsrc->ptr), tiles, run);
+		if (!L.done(err)) return nullptr;
+	}
+	return record_event(cq, "util_scatter_runs", err);
 }
 
 static bool scratch_u64(CCLQueue* cq, u64** p, GError** err) {

@@ -60,6 +60,19 @@ void clo_util_zipf_cdf(double* cdf, size_t ranks, double s);
 CCLEvent* clo_util_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src,
 	size_t bytes, GError** err);
 
+/* Bandwidth probes (benchmark utilities).  clo_util_stream_copy: `bytes`
+ * (a multiple of 16 KiB) copied by blocks of one 16 KiB tile each, four
+ * 16-byte loads in flight per thread, tiles XCD-contiguous: the streaming
+ * ceiling.  clo_util_scatter_runs: the write pattern of one uniform 8-bit
+ * radix pass over n (u64 key, u32 value) pairs (n a multiple of 8192) and
+ * nothing else: tile t of 8192 pairs read contiguously and written as runs
+ * of `run` (32, 64 or 128) pairs, run d to d * (n * run / 8192) + t * run;
+ * 32 is what uniform keys give an 8192-key tile with 256 digits. */
+CCLEvent* clo_util_stream_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src,
+	size_t bytes, GError** err);
+CCLEvent* clo_util_scatter_runs(CCLQueue* cq, CCLBuffer* kdst, CCLBuffer* vdst,
+	CCLBuffer* ksrc, CCLBuffer* vsrc, size_t n, cl_uint run, GError** err);
+
 /* Order-independent 64-bit multiset hash of n elements of `width` bytes
  * (sum of mix64(x) mod 2^64), computed on the device; blocks. */
 cl_ulong clo_util_multiset_hash(CCLQueue* cq, CCLBuffer* buf, size_t n,
