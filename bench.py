@@ -876,7 +876,13 @@ def dispatch_eval(ctx, n_train=2000, n_eval=1500, n_full=100000, n_sparse=200,
     tot = {k: float(times[rows, v].sum()) for k, v in
            (("model", model), ("table", table), ("fixed", fixed), ("best", best),
             ("sparse_model", model_s), ("sparse_table", table_s))}
-    sparse_hits = sum(ds.table_choose(op, n, w, f) >= 0 for op, n, w, f in zip(ops, ns, ws, feats))
+    hit = np.array([ds.table_choose(op, n, w, f) >= 0 for op, n, w, f in zip(ops, ns, ws, feats)])
+    sparse_hits = int(hit.sum())
+    # Held-out buckets: the evaluation requests whose bucket the sparse table
+    # never saw (the table gives the fixed configuration there).
+    ho = ~hit
+    ho_tot = {k: float(times[rows[ho], v[ho]].sum()) for k, v in
+              (("table", table_s), ("model", model_s), ("best", best))}
     # bit-exact: chosen configuration vs the fixed one, word for word
     diffs = 0
     for r, (op, n, width, dist, s) in enumerate(reqs):
@@ -900,6 +906,10 @@ def dispatch_eval(ctx, n_train=2000, n_eval=1500, n_full=100000, n_sparse=200,
         "model_matches_best": round(float(np.mean(model == best)), 3),
         "regret_sparse_table": round(tot["sparse_table"] / tot["best"] - 1, 4),
         "regret_sparse_model": round(tot["sparse_model"] / tot["best"] - 1, 4),
+        "regret_heldout_buckets": {
+            "requests": int(ho.sum()),
+            "table": round(ho_tot["table"] / ho_tot["best"] - 1, 4) if ho.any() else None,
+            "model": round(ho_tot["model"] / ho_tot["best"] - 1, 4) if ho.any() else None},
         "sparse": "table and model from the first %d training requests (%d timings); the sparse "
                   "table has an entry for %d of the %d evaluation requests' buckets"
                   % (n_sparse, n_rows_s, sparse_hits, n_eval),
