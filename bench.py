@@ -930,7 +930,7 @@ def copy_probe(ctx, nbytes):
     kernel), util_stream_copy (16 KiB tiles, four 16-byte loads in flight per
     thread, XCD-contiguous) over `nbytes`, and util_scatter_runs, the write
     pattern of one uniform radix pass over 2^28 (u64, u32) pairs in runs of
-    32, 64 and 128 pairs.  Each: 3 untimed launches, then 10 between two
+    4 to 128 pairs (32: 8-bit digits on 8192-pair tiles; 4: 11-bit digits).  Each: 3 untimed launches, then 10 between two
     markers; one JSON line per probe on stdout."""
     q = clops.Queue(ctx)
     L = clops.lib()
@@ -963,7 +963,7 @@ def copy_probe(ctx, nbytes):
     kd, vd = clops.Buffer(ctx, n * 8), clops.Buffer(ctx, n * 4)
     clops.generate(q, ks, n, clops.GEN_U64, 2)
     clops.generate(q, vs, n, clops.GEN_IOTA_U32, 0)
-    for run in (32, 64, 128):
+    for run in (4, 8, 16, 32, 64, 128):
         timed(lambda: clops._call(L.clo_util_scatter_runs, q.ptr, kd.ptr, vd.ptr, ks.ptr, vs.ptr,
                                   n, run),
               "util_scatter_runs run=%d, 2^28 u64/u32 pairs" % run, n * 24)

@@ -364,10 +364,10 @@ CCLEvent* clo_util_stream_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src, siz
 CCLEvent* clo_util_scatter_runs(CCLQueue* cq, CCLBuffer* kdst, CCLBuffer* vdst, CCLBuffer* ksrc,
 	CCLBuffer* vsrc, size_t n, cl_uint run, GError** err) {
 	if (!cq || !kdst || !vdst || !ksrc || !vsrc || n % kProbeTile || n / kProbeTile > 0xFFFFFFF0u ||
-		(run != 32 && run != 64 && run != 128) || kdst->size < n * 8 || ksrc->size < n * 8 ||
+		(run != 4 && run != 8 && run != 16 && run != 32 && run != 64 && run != 128) || kdst->size < n * 8 || ksrc->size < n * 8 ||
 		vdst->size < n * 4 || vsrc->size < n * 4) {
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS,
-			"scatter probe takes n a multiple of 8192 pairs and runs of 32, 64 or 128");
+			"scatter probe takes n a multiple of 8192 pairs and runs of 4 to 128 pairs (powers of two)");
 		return nullptr;
 	}
 	if (!use_device(cq->ctx, err)) return nullptr;

@@ -66,8 +66,9 @@ CCLEvent* clo_util_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src,
  * ceiling.  clo_util_scatter_runs: the write pattern of one uniform 8-bit
  * radix pass over n (u64 key, u32 value) pairs (n a multiple of 8192) and
  * nothing else: tile t of 8192 pairs read contiguously and written as runs
- * of `run` (32, 64 or 128) pairs, run d to d * (n * run / 8192) + t * run;
- * 32 is what uniform keys give an 8192-key tile with 256 digits. */
+ * of `run` (4, 8, 16, 32, 64 or 128) pairs, run d to d * (n * run / 8192) +
+ * t * run; 32 is what uniform keys give an 8192-key tile with 256 digits, 4
+ * what they would give with 2048 (11-bit digits). */
 CCLEvent* clo_util_stream_copy(CCLQueue* cq, CCLBuffer* dst, CCLBuffer* src,
 	size_t bytes, GError** err);
 CCLEvent* clo_util_scatter_runs(CCLQueue* cq, CCLBuffer* kdst, CCLBuffer* vdst,

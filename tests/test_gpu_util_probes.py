@@ -18,7 +18,7 @@ def test_stream_copy(clops, oracle, ctx):
         clops._call(clops.lib().clo_util_stream_copy, q.ptr, b.ptr, a.ptr, nb - 16)
 
 
-@pytest.mark.parametrize("run", [32, 64, 128])
+@pytest.mark.parametrize("run", [4, 8, 32, 128])
 def test_scatter_runs_layout(clops, oracle, ctx, run):
     q = clops.Queue(ctx)
     tiles = 24
