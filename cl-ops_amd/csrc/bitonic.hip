@@ -29,14 +29,18 @@ namespace {
 
 constexpr int kBitBlock = 256;
 
-// Tiles of the register-tail kernel: 2^11 .. 2^13 keys, LDS sized to the
-// tile so that small tiles keep many blocks resident per CU.
-constexpr u32 kTileLogMin = 11, kTileLogMax = 13;
-// Default tile per key width (measured on MI355X at n = 2^16..2^24 with 16
-// keys per thread: 2^13 best up to 4-byte keys, 2^12 for 8-byte keys).
+// Tiles of the register-tail kernel: 2^11 .. 2^15 keys (2^14 for 8-byte
+// keys: 136-139 KB of LDS with the padding), LDS sized to the tile so that
+// small tiles keep many blocks resident per CU.
+constexpr u32 kTileLogMin = 11;
+constexpr u32 tile_log_cap(size_t key_bytes) { return key_bytes >= 8 ? 14u : 15u; }
+// Default tile per key width, measured on MI355X (abitonic 2^24 keys):
+// 4-byte keys 2^14 keys with 32 per thread (1.149 ms; 2^13 x 16: 1.194,
+// 2^15 x 32: 1.262), 8-byte keys 2^12 with 16 per thread (2.228 ms; 2^13:
+// 2.239, 2^14: 2.487), 1- and 2-byte keys 2^13 with 16.
 template <typename T>
 struct TileMax {
-	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : 13;
+	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : sizeof(T) == 4 ? 14 : 13;
 };
 // The plain LDS kernels (one synchronised step at a time) take tiles of up
 // to 2^11 keys.
@@ -171,32 +175,36 @@ __device__ __forceinline__ void cmpswap(T& a, T& b) {
 	b = mx;
 }
 
-// Half-cleaner of distance H on the thread's keys; `valid` keys are real
-// (positions >= valid are virtual +inf: a comparator reaching one does nothing).
+// Half-cleaner of distance H on the thread's keys.  Positions past the
+// array hold the largest key (all ones after the order mask): every
+// comparator of the all-ascending network puts its minimum at the lower
+// position, so those stay above every real key and no comparator needs a
+// bounds check.  (Per-comparator checks made the compiler index the key array
+// dynamically, which put it in scratch memory.)
 template <typename T, int R, int H>
-__device__ __forceinline__ void reg_half(T (&x)[R], u32 valid) {
+__device__ __forceinline__ void reg_half(T (&x)[R]) {
 #pragma unroll
 	for (int j = 0; j < R; ++j)
-		if (!(j & H) && static_cast<u32>(j | H) < valid) cmpswap(x[j], x[j | H]);
+		if (!(j & H)) cmpswap(x[j], x[j | H]);
 }
 
 // Flip step of blocks of B positions: j pairs with j ^ (B - 1).
 template <typename T, int R, int B>
-__device__ __forceinline__ void reg_flip(T (&x)[R], u32 valid) {
+__device__ __forceinline__ void reg_flip(T (&x)[R]) {
 #pragma unroll
 	for (int j = 0; j < R; ++j)
-		if (!(j & (B / 2)) && static_cast<u32>(j ^ (B - 1)) < valid) cmpswap(x[j], x[j ^ (B - 1)]);
+		if (!(j & (B / 2))) cmpswap(x[j], x[j ^ (B - 1)]);
 }
 
 // Steps t = min(t_hi, TT)..1 of stage s (the step t == s is the flip).
 template <typename T, int R, int TT>
-__device__ __forceinline__ void reg_tail(T (&x)[R], u32 s, u32 t_hi, u32 valid) {
+__device__ __forceinline__ void reg_tail(T (&x)[R], u32 s, u32 t_hi) {
 	if constexpr (TT >= 1) {
 		if (t_hi >= TT) {
-			if (s == TT) reg_flip<T, R, (1 << TT)>(x, valid);
-			else reg_half<T, R, (1 << (TT - 1))>(x, valid);
+			if (s == TT) reg_flip<T, R, (1 << TT)>(x);
+			else reg_half<T, R, (1 << (TT - 1))>(x);
 		}
-		reg_tail<T, R, TT - 1>(x, s, t_hi, valid);
+		reg_tail<T, R, TT - 1>(x, s, t_hi);
 	}
 }
 
@@ -287,7 +295,7 @@ __global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_reg(
 	if (owner) {
 #pragma unroll
 		for (int j = 0; j < R; ++j)
-			x[j] = static_cast<u32>(j) < valid ? s_t[bpad(p0 + j)] : T(0);
+			x[j] = static_cast<u32>(j) < valid ? s_t[bpad(p0 + j)] : static_cast<T>(~T(0));
 	}
 	for (u32 s = s_first; s <= s_last; ++s) {
 		const u32 t_top = s < k ? s : k;
@@ -317,7 +325,7 @@ __global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_reg(
 				for (int j = 0; j < R; ++j) x[j] = s_t[bpad(p0 + j)];
 			}
 		}
-		if (owner) reg_tail<T, R, LR>(x, s, t_top < LR ? t_top : LR, valid);
+		if (owner) reg_tail<T, R, LR>(x, s, t_top < LR ? t_top : LR);
 	}
 	__syncthreads();  // every thread has read its keys of the last LDS step
 	if (owner) {
@@ -340,8 +348,9 @@ __global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_reg(
 }
 
 // Keys per thread held in registers by the tile kernels: 0 runs the plain
-// LDS kernel (tiles of 2^11), 16 or 32 the register-tail kernel.
-constexpr int kDefaultRegKeys = 16;
+// LDS kernel (tiles of 2^11), 16 or 32 the register-tail kernel; the default
+// is 32 for 4-byte keys and 16 otherwise (see TileMax).
+constexpr int default_regkeys(size_t key_bytes) { return key_bytes == 4 ? 32 : 16; }
 // Global steps fused per launch when the maxps option is not given: 5 (2^5
 // keys per thread) for keys of up to 4 bytes, 4 for 8-byte keys (measured on
 // 2^24 keys: u32 1.34 -> 1.29 ms with 5; 6 steps, and 5 for u64, were slower).
@@ -356,13 +365,13 @@ struct BitonicData {
 	u32 last_sweeps = 0;
 };
 
-// log2 of the LDS tile: the maxsfs option (at most 2^13 keys) or the
+// log2 of the LDS tile: the maxsfs option (at most 2^15 keys, 2^14 for 8-byte keys) or the
 // default of the key width.
-inline int regkeys_of(const BitonicData* bd) {
-	return bd->regkeys >= 0 ? bd->regkeys : kDefaultRegKeys;
+inline int regkeys_of(const BitonicData* bd, size_t key_bytes) {
+	return bd->regkeys >= 0 ? bd->regkeys : default_regkeys(key_bytes);
 }
-inline u32 tile_log(const BitonicData* bd, u32 dflt) {
-	const u32 cap = regkeys_of(bd) == 0 ? kPlainTileLog : kTileLogMax;
+inline u32 tile_log(const BitonicData* bd, u32 dflt, size_t key_bytes) {
+	const u32 cap = regkeys_of(bd, key_bytes) == 0 ? kPlainTileLog : tile_log_cap(key_bytes);
 	if (bd->maxsfs == 0xFFFFFFFFu) return dflt < cap ? dflt : cap;
 	return bd->maxsfs < cap ? bd->maxsfs : cap;
 }
@@ -408,7 +417,7 @@ template <typename T>
 bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_first,
 	u32 s_last, T xm, const char* nm, GError** err) {
 	const u32 grid = static_cast<u32>((n + (1ull << k) - 1) >> k);
-	const int r = regkeys_of(bd);
+	const int r = regkeys_of(bd, sizeof(T));
 	const u32 lt = k < kTileLogMin ? kTileLogMin : k;  // LDS tile of the instance
 	Launch L(q, nm);
 #define CLO_TILE_REG(R_, LT_)                                                                    \
@@ -417,10 +426,13 @@ bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_f
 	if (k >= 5 && r != 0) {
 		if (r == 16 && lt == 11) CLO_TILE_REG(16, 11);
 		else if (r == 16 && lt == 12) CLO_TILE_REG(16, 12);
-		else if (r == 16) CLO_TILE_REG(16, 13);
+		else if (r == 16 && lt == 13) CLO_TILE_REG(16, 13);
+		else if (r == 16 && lt == 14) CLO_TILE_REG(16, 14);
 		else if (lt == 11) CLO_TILE_REG(32, 11);
 		else if (lt == 12) CLO_TILE_REG(32, 12);
-		else CLO_TILE_REG(32, 13);
+		else if (lt == 13) CLO_TILE_REG(32, 13);
+		else if (lt == 14) CLO_TILE_REG(32, 14);
+		else if constexpr (sizeof(T) < 8) CLO_TILE_REG(32, 15);  // 2^15 keys: 1024 x 32
 	}
 #undef CLO_TILE_REG
 	else
@@ -454,7 +466,7 @@ bool run_network(CloSort* sorter, BitonicData* bd, CCLQueue* q, T* d, u64 n, GEr
 	// abitonic strategy: stages 1..k in one LDS tile kernel, then per stage the
 	// global steps in register groups of up to maxps steps (at least minps when
 	// that many remain) and the last k steps in one LDS kernel.
-	u32 k = tile_log(bd, TileMax<T>::kLog);
+	u32 k = tile_log(bd, TileMax<T>::kLog, sizeof(T));
 	if (k > S) k = S;
 	if (k >= 1) {
 		if (!launch_tile<T>(q, bd, d, n, k, 1, k, xm, CLO_SORT_ABITONIC_KNAME_TILESORT, err))
@@ -603,8 +615,9 @@ size_t abitonic_localmem(CloSort* sorter, cl_uint i, size_t, size_t, GError** er
 	const size_t es = clo_type_sizeof(sorter->elem_type);
 	if (i < 4 || i > 5) return 0;  // register-only kernels
 	const BitonicData* bd = static_cast<const BitonicData*>(sorter->data);
-	const u32 k = tile_log(bd, es >= 8 ? TileMax<u64>::kLog : TileMax<u32>::kLog);
-	if (k < 5 || regkeys_of(bd) == 0) return kPlainTileMax * es;
+	const u32 k = tile_log(bd, es >= 8 ? TileMax<u64>::kLog : es == 4 ? TileMax<u32>::kLog
+		: TileMax<u16>::kLog, es);
+	if (k < 5 || regkeys_of(bd, es) == 0) return kPlainTileMax * es;
 	return bpad(1u << (k < kTileLogMin ? kTileLogMin : k)) * es;
 }
 
