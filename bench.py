@@ -549,10 +549,16 @@ def segmented_tuning(ctx, q, kin, kout, kout2, offb, nseg, n, width=4):
     loss = m.train(q, np.array(X, dtype=np.float32), np.array(y, dtype=np.float32), 3000, 3e-3)
     d.set_segment_model(m)
     classes = d.segment_classes(q, fixed, width, 0.05)
-    ms_fixed = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout, offb, nseg, n, width),
-                          5, 2)
-    ms_cls = time_steps(q, lambda: d.run_segmented(q, fixed, kin, kout2, offb, nseg, n, width,
-                                                   classes=classes), 5, 2)
+    # Interleaved, least of 7 each: a run measured first would otherwise
+    # carry the warm-up of both.
+    run_f = lambda: d.run_segmented(q, fixed, kin, kout, offb, nseg, n, width)
+    run_c = lambda: d.run_segmented(q, fixed, kin, kout2, offb, nseg, n, width, classes=classes)
+    run_f()
+    run_c()
+    ms_fixed = ms_cls = float("inf")
+    for _ in range(7):
+        ms_fixed = min(ms_fixed, time_steps(q, run_f, 1, 0))
+        ms_cls = min(ms_cls, time_steps(q, run_c, 1, 0))
     diff = clops.count_diff(q, kout, kout2, n * width // 4)
     res.append({"config": "learned per-segment work classes (%s)" % d.config_name(fixed),
                 "classes": "".join(map(str, classes)), "train_rows": len(y),

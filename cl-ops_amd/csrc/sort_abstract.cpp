@@ -282,7 +282,7 @@ CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec
 		return nullptr;
 	}
 	uint32_t tile = sorter->seg_tile ? sorter->seg_tile
-		: clo::seg_default_tile(clo_type_sizeof(sorter->elem_type));
+		: clo::seg_default_tile(clo_type_sizeof(sorter->elem_type), vals_in != nullptr);
 	return clo_satradix_sort_segmented(sorter, cq_exec, keys_in, vals_in, keys_out, vals_out,
 		offsets, nseg, numel, tile, sorter->seg_grid, &sorter->seg_cache, err);
 }

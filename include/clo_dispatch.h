@@ -29,9 +29,8 @@
  *                                     keys only
  * The fixed configuration is what the plain API runs: satradix's default
  * tile shape (2 up to 2^20 keys, else 0 for 32-bit and 1 for 64-bit keys),
- * 7 for scans, and the segmented defaults (the tuner's choice on 32768
- * log-uniform segments: tile 8 for 32-bit keys, 14; tile 16 for 64-bit
- * keys, 18; both 4 blocks per CU).
+ * 7 for scans, and the segmented default (the tuner's choice on 32768
+ * log-uniform segments: tile 16, 4 blocks per CU, 18).
  *
  * Features (CLO_DISPATCH_NUM_FEATURES floats, from a GPU kernel over a
  * sample of up to 4096 evenly spaced elements and segments):

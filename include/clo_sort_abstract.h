@@ -115,7 +115,7 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
  * Malformed offsets (decreasing, or beyond numel) are clamped into the
  * arrays: the order is then unspecified but no access leaves the buffers.
  * Sorter options tune it: segtile=8|16|32 (LDS tile = 256*segtile elements;
- * 8|16 for 64-bit keys; default 8 for 32-bit and 16 for 64-bit keys),
+ * 8|16 for 64-bit keys; default 16, and 8 for 64-bit key-value sorts),
  * seggrid=<blocks> (0 = four per compute unit), segclass=<up to 64 digits 0|1|2> (the least work class of each
  * segment length bin, see clo_dispatch.h; default all 0).  No option changes
  * the result. */

@@ -169,15 +169,19 @@ def test_segment_class_timings(clops, ctx):
         assert all(x == -1 for x, f in zip(t, feasible) if not f)
 
 
-def _time_best(q, fn, reps):
-    fn()
+def _time_best_pair(q, fa, fb, reps):
+    """Least times of fa and fb over reps interleaved runs each (a run
+    measured first would otherwise carry the warm-up of both)."""
+    fa()
+    fb()
     q.finish()
-    best = math.inf
+    best = [math.inf, math.inf]
     for _ in range(reps):
-        t0 = time.perf_counter()
-        fn()
-        q.finish()
-        best = min(best, time.perf_counter() - t0)
+        for i, fn in enumerate((fa, fb)):
+            t0 = time.perf_counter()
+            fn()
+            q.finish()
+            best[i] = min(best[i], time.perf_counter() - t0)
     return best
 
 
@@ -222,9 +226,9 @@ def test_learned_segment_classes(clops, oracle, ctx):
     clops.generate(q, kb, n, clops.GEN_U32_HI, 0x5D8)
     ob = clops.Buffer.from_array(ctx, off)
     o1, o2 = clops.Buffer(ctx, n * 4), clops.Buffer(ctx, n * 4)
-    t_fixed = _time_best(q, lambda: d.run_segmented(q, cfg, kb, o1, ob, nseg, n, 4), 7)
-    t_model = _time_best(q, lambda: d.run_segmented(q, cfg, kb, o2, ob, nseg, n, 4,
-                                                    classes=classes), 7)
+    t_fixed, t_model = _time_best_pair(
+        q, lambda: d.run_segmented(q, cfg, kb, o1, ob, nseg, n, 4),
+        lambda: d.run_segmented(q, cfg, kb, o2, ob, nseg, n, 4, classes=classes), 9)
     assert clops.count_diff(q, o1, o2, n) == 0
     k = kb.read(q, np.uint32, n)
     got = o2.read(q, np.uint32, n)
