@@ -8,7 +8,9 @@ digit pass): algorithmic bytes per launch = n * 2 * (8 + 4) divided by its
 average launch duration measured with HIP events on the exec stream over the
 timed region.  `cpu_baseline` times the oracle's reference-structured satradix
 (4-bit digits, 256-element blocks, blocks spread over host threads) on a
-bounded sample of the same workload.
+bounded sample of the same workload.  `config.copy_ceiling_gbs` is a
+streaming copy of the 2 GiB key array timed in the same run; the whole sort
+and the downsweep are also given as fractions of it.
 
 `dispatch` reports the learned dispatcher on the mixed sort/scan request
 stream: regret against the exhaustive tuner and speedup over the fixed
@@ -172,9 +174,23 @@ def headline(args, ctx):
            same, total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
     if bad != 0 or pairs != 0 or not same:
         raise SystemExit("benchmark output failed verification")
+    # Measured copy ceiling of this GPU: util_stream_copy of the 2 GiB key
+    # array into the (already verified) output buffer, 3 untimed + 10 timed.
+    nb = n * 8 // 16384 * 16384
+    L = clops.lib()
+
+    def copy():
+        clops._call(L.clo_util_stream_copy, q_plain.ptr, kout.ptr, kin.ptr, nb)
+
+    copy_gbs = None
+    if nb:
+        copy_ms = time_steps(q_plain, copy, 10, 3)
+        copy_gbs = 2 * nb / (copy_ms * 1e-3) / 1e9
+        log("stream copy %d bytes: %.4f ms, %.1f GB/s" % (nb, copy_ms, copy_gbs))
     return {
         "ms": ms, "ms_downsweep_events": ms_ds, "n": n, "passes": passes,
         "scatter_ms": scatter_ms, "achieved": achieved, "bytes_per_launch": bytes_per_launch,
+        "copy_gbs": copy_gbs,
     }
 
 
@@ -1053,6 +1069,10 @@ def main():
             "passes_executed": h["passes"],
             "whole_sort_hbm_frac": round((n * 8 + h["passes"] * h["bytes_per_launch"]) /
                                          (h["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "copy_ceiling_gbs": h["copy_gbs"] and round(h["copy_gbs"], 1),
+            "whole_sort_of_copy_ceiling": h["copy_gbs"] and round(
+                (n * 8 + h["passes"] * h["bytes_per_launch"]) / (h["ms"] * 1e-3) / 1e9 /
+                h["copy_gbs"], 4),
         },
         "roofline": {
             "bound": "hbm",
@@ -1067,6 +1087,7 @@ def main():
                             "(%.4f ms/step; value comes from the same steps without events)"
                             % (args.steps, h["ms_downsweep_events"]),
             "bytes_per_launch": h["bytes_per_launch"],
+            "frac_of_copy_ceiling": h["copy_gbs"] and round(h["achieved"] / h["copy_gbs"], 4),
         },
         "cpu_baseline": cpu,
         "dispatch": disp,
