@@ -8,18 +8,22 @@
 //
 // Kernel plan per call (all on cq_exec, no host synchronisation, no
 // inter-block waiting):
-//   satradix_upsweep    per tile (4096 or 8192 keys): digit counts; the first
-//                       pass reads the keys and also takes their bitwise
-//                       AND/OR, later passes read one digit byte per key
-//   satradix_plan       one block: which digits vary between keys (the other
-//                       passes are skipped) and the buffers of each pass
+//   satradix_upsweep    first pass, per tile (4096 to 16384 keys): 16-bit
+//                       digit counts from the keys, and their bitwise AND/OR
+//   satradix_upsweep_digits  later passes, one wave per tile: counts of the
+//                       digit bytes the previous pass wrote (1 byte per key)
+//   satradix_plan       one block, only for sorts of at most 256 tiles: which
+//                       digits vary between keys (the other passes are
+//                       skipped) and the buffers of each pass
 //   satradix_reduce     per 256 tiles and quarter of the digits: sums of the
-//                       tiles' digit counts
+//                       tiles' digit counts (first pass: also the group's
+//                       AND/OR)
 //   satradix_scan       per 256 tiles and quarter of the digits: tile counts
 //                       -> global output offsets (each block derives its
-//                       bases from all group sums); sorts of at most 256
-//                       tiles skip reduce and scan: each downsweep block sums
-//                       the tile counts itself
+//                       bases from all group sums; in the first pass it
+//                       derives the plan from the group AND/OR); sorts of at
+//                       most 256 tiles skip reduce and scan: each downsweep
+//                       block sums the tile counts itself
 //   satradix_downsweep  per tile: wave64 ballot ranking, LDS-staged scatter
 //                       to the offsets (contiguous runs per digit), and the
 //                       next executed pass's digit byte of every key
@@ -78,18 +82,14 @@ struct Bufs {
 	u32* v[3];
 };
 
-// Digit counts of one tile of NT*ITEMS keys.  The first pass reads the
-// input and also records the tile's bitwise AND and OR of the keys, from which
-// the plan kernel finds the digits that are equal in every key.  Later passes
-// count the digit bytes of their keys (`digits`, written by the previous
-// executed pass's downsweep): 1 byte per key read instead of sizeof(K).
+// Digit counts of one tile of NT*ITEMS keys for the first pass, which reads
+// the input and also records the tile's bitwise AND and OR of the keys, from
+// which the plan finds the digits that are equal in every key.
 template <typename K, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
-	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits,
-	u32* __restrict__ counts, K* __restrict__ andor, u32 first, const u8* __restrict__ digits) {
+__global__ __launch_bounds__(NT) void satradix_upsweep(const K* __restrict__ src, u32 n,
+	u32 tiles, u32 bits, u16* __restrict__ counts, K* __restrict__ andor) {
 	constexpr u32 kTile = NT * ITEMS;
 	constexpr int NW = NT / kWave;
-	static_assert(ITEMS % 4 == 0, "digit bytes are read as words");
 	static_assert(NT >= kRadixMax, "one thread per digit bin");
 	// Four copies of each wave's histogram, lane l adding to copy l % 4:
 	// lanes holding the same digit collide on one counter a quarter as often.
@@ -98,16 +98,6 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 	// (one copy: 0.66, two: 0.44) on Zipf keys, whose top key is 11.5%.
 	__shared__ __attribute__((aligned(16))) u32 s_h[NW][4 * kRadixMax];
 	__shared__ K s_red[2 * NW];
-	u32 shift = 0;
-	u32 dsrc = 0;
-	const K* __restrict__ src = b.k[kIn];
-	if (!first) {
-		const PassDesc d = plan->pass[pass];
-		if (!d.active) return;
-		shift = d.shift;
-		src = b.k[d.src];
-		dsrc = d.dsrc;
-	}
 	const u32 tile = xcd_tile(tiles);
 	if (tile >= tiles) return;
 	const u32 tid = threadIdx.x;
@@ -117,36 +107,6 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 		reinterpret_cast<uint4*>(s_h[w])[i] = make_uint4(0u, 0u, 0u, 0u);
 	__syncthreads();
 	const u32 copy = lane & 3u;
-	if (dsrc) {
-		const u32 base = tile * kTile;
-		if (base + kTile <= n) {
-			// kTile is a multiple of 4 and the array is 16-byte aligned.
-			const u32* __restrict__ dw = reinterpret_cast<const u32*>(digits + base);
-			u32 x[ITEMS / 4];
-#pragma unroll
-			for (int j = 0; j < ITEMS / 4; ++j) x[j] = dw[j * NT + tid];
-#pragma unroll
-			for (int j = 0; j < ITEMS / 4; ++j) {
-#pragma unroll
-				for (int q = 0; q < 4; ++q)
-					atomicAdd(&s_h[w][((x[j] >> (8 * q)) & 0xFFu) * 4 + copy], 1u);
-			}
-		} else {
-			for (u32 i = base + tid; i < n; i += NT)
-				atomicAdd(&s_h[w][digits[i] * 4u + copy], 1u);
-		}
-		__syncthreads();
-		if (tid < kRadixMax) {
-			u32 c = 0;
-#pragma unroll
-			for (int ww = 0; ww < NW; ++ww) {
-				const uint4 h = reinterpret_cast<const uint4*>(s_h[ww])[tid];
-				c += h.x + h.y + h.z + h.w;
-			}
-			counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
-		}
-		return;
-	}
 	const u32 mask = (1u << bits) - 1u;
 	const u32 wbase = tile * kTile + w * kWave * ITEMS;
 	K k[ITEMS];
@@ -159,21 +119,19 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 #pragma unroll
 	for (int j = 0; j < ITEMS; ++j) {
 		if (wbase + j * kWave + lane < n) {
-			atomicAdd(&s_h[w][(static_cast<u32>(k[j] >> shift) & mask) * 4 + copy], 1u);
+			atomicAdd(&s_h[w][(static_cast<u32>(k[j]) & mask) * 4 + copy], 1u);
 			ka &= k[j];
 			ko |= k[j];
 		}
 	}
-	if (first) {
 #pragma unroll
-		for (int off = kWave / 2; off > 0; off >>= 1) {
-			ka &= __shfl_xor(ka, off, kWave);
-			ko |= __shfl_xor(ko, off, kWave);
-		}
-		if (lane == 0) {
-			s_red[w] = ka;
-			s_red[NW + w] = ko;
-		}
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		ka &= __shfl_xor(ka, off, kWave);
+		ko |= __shfl_xor(ko, off, kWave);
+	}
+	if (lane == 0) {
+		s_red[w] = ka;
+		s_red[NW + w] = ko;
 	}
 	__syncthreads();
 	if (tid < kRadixMax) {
@@ -183,9 +141,9 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 			const uint4 h = reinterpret_cast<const uint4*>(s_h[ww])[tid];
 			c += h.x + h.y + h.z + h.w;
 		}
-		counts[static_cast<size_t>(tile) * kRadixMax + tid] = c;
+		counts[static_cast<size_t>(tile) * kRadixMax + tid] = static_cast<u16>(c);
 	}
-	if (first && tid == 0) {
+	if (tid == 0) {
 		K a = ~K(0), o = K(0);
 #pragma unroll
 		for (int ww = 0; ww < NW; ++ww) {
@@ -197,9 +155,132 @@ __global__ __launch_bounds__(NT) void satradix_upsweep(Bufs<K> b,
 	}
 }
 
+// Digit counts of one tile for the passes after the first: one wave per tile
+// of TILE keys.  A pass whose previous executed pass wrote digit bytes
+// (dsrc) reads those, 1 byte per key (TILE / 256 words per lane, all loads in
+// flight); otherwise (the first executed pass is not pass 0) it reads the
+// keys.  The wave keeps kDigitCopies copies of its histogram, lane l adding to
+// copy l % kDigitCopies.  Measured alone on 2^28 digit bytes in 8192-digit
+// tiles: 0.059 ms per pass, against 0.070 ms for four 64-lane waves per tile
+// with four copies each, and 0.043 ms for reading the bytes alone; 2 and 16
+// copies, copy-major layouts and two waves per tile were slower.  Inside the
+// 2^28-pair sort (after a downsweep) it takes 0.083 ms against about 0.085.
+constexpr int kDigitCopies = 8;
+
+template <typename K, int TILE>
+__global__ __launch_bounds__(kWave) void satradix_upsweep_digits(Bufs<K> b,
+	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits,
+	u16* __restrict__ counts, const u8* __restrict__ digits) {
+	constexpr u32 C = kDigitCopies;
+	constexpr int WPL = TILE / 4 / kWave;  // digit words per lane
+	static_assert(TILE % (4 * kWave) == 0, "whole digit words per lane");
+	__shared__ __attribute__((aligned(16))) u32 s_h[C * kRadixMax];
+	const PassDesc d = plan->pass[pass];
+	if (!d.active) return;
+	const u32 tile = xcd_tile(tiles);
+	if (tile >= tiles) return;
+	const u32 lane = threadIdx.x;
+	const u32 base = tile * static_cast<u32>(TILE);
+	const u32 copy = lane % C;
+	if (d.dsrc && base + TILE <= n) {
+		// TILE is a multiple of 4 and the digit array is 16-byte aligned.
+		const u32* __restrict__ dw = reinterpret_cast<const u32*>(digits + base);
+		u32 x[WPL];
+#pragma unroll
+		for (int j = 0; j < WPL; ++j) x[j] = dw[j * kWave + lane];
+#pragma unroll
+		for (u32 i = lane; i < C * kRadixMax / 4; i += kWave)
+			reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+		for (int j = 0; j < WPL; ++j) {
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				atomicAdd(&s_h[((x[j] >> (8 * q)) & 0xFFu) * C + copy], 1u);
+		}
+	} else {
+#pragma unroll
+		for (u32 i = lane; i < C * kRadixMax / 4; i += kWave)
+			reinterpret_cast<uint4*>(s_h)[i] = make_uint4(0u, 0u, 0u, 0u);
+		const u32 end = (n - base) < static_cast<u32>(TILE) ? n : base + TILE;
+		if (d.dsrc) {
+			for (u32 i = base + lane; i < end; i += kWave) atomicAdd(&s_h[digits[i] * C + copy], 1u);
+		} else {
+			const K* __restrict__ src = b.k[d.src];
+			const u32 mask = (1u << bits) - 1u;
+			for (u32 i = base + lane; i < end; i += kWave)
+				atomicAdd(&s_h[(static_cast<u32>(src[i] >> d.shift) & mask) * C + copy], 1u);
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (u32 dg = lane; dg < kRadixMax; dg += kWave) {
+		u32 c = 0;
+#pragma unroll
+		for (u32 k = 0; k < C; k += 4) {
+			const uint4 h = reinterpret_cast<const uint4*>(s_h)[(dg * C + k) / 4];
+			c += h.x + h.y + h.z + h.w;
+		}
+		counts[static_cast<size_t>(tile) * kRadixMax + dg] = static_cast<u16>(c);
+	}
+}
+
+// The pass plan from the bits that differ between keys, written by one full
+// wave: lane p describes pass p (kMaxPasses == kWave).  Executed pass k (k-th
+// active one) reads the buffer pass k-1 wrote and writes kAux or kIn/kOut in
+// turn, so that an out-of-place sort ends in kOut; the digit-byte hand-over
+// links each executed pass with the next one.
+static_assert(kMaxPasses == kWave, "one lane per pass");
+
+__device__ __forceinline__ u32 plan_dst(u32 k, u32 executed, u32 in_place) {
+	if (in_place) return (k % 2 == 0) ? kAux : kIn;
+	return ((executed - 1 - k) % 2 == 0) ? kOut : kAux;  // the last one lands in kOut
+}
+
+template <typename K>
+__device__ void plan_write(SortPlan* __restrict__ plan, K diff, u32 passes, u32 bits,
+	u32 in_place) {
+	const u32 p = lane_id();
+	const u32 mask = (1u << bits) - 1u;
+	const bool act = p < passes && (static_cast<u32>(diff >> (p * bits)) & mask) != 0u;
+	const u64 am = ballot(act);
+	const u32 executed = static_cast<u32>(__popcll(am));
+	const u32 k = count_below(am);  // executed passes before pass p
+	PassDesc d = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+	if (act) {
+		d.active = 1;
+		d.shift = p * bits;
+		d.src = k == 0 ? kIn : plan_dst(k - 1, executed, in_place);
+		d.dst = plan_dst(k, executed, in_place);
+		d.dsrc = k > 0 ? 1u : 0u;
+		const u64 later = p + 1 < 64 ? (am >> (p + 1)) : 0ull;
+		if (later) {
+			d.nact = 1;
+			d.nshift = (p + 1 + static_cast<u32>(__builtin_ctzll(later))) * bits;
+		}
+	}
+	plan->pass[p] = d;
+	if (p == 0) {
+		plan->executed = executed;
+		plan->copy_active = 0;
+		plan->copy_src = 0;
+		plan->copy_dst = 0;
+		if (in_place && (executed % 2 == 1)) {
+			plan->copy_active = 1;
+			plan->copy_src = kAux;
+			plan->copy_dst = kIn;
+		} else if (!in_place && executed == 0) {
+			plan->copy_active = 1;
+			plan->copy_src = kIn;
+			plan->copy_dst = kOut;
+		}
+	}
+}
+
 // Which digit passes run (a digit equal in every key needs no pass), and the
 // source and destination buffer of each executed pass.  One block of
-// kPlanThreads; each thread keeps kPlanUnroll tile loads in flight.
+// kPlanThreads; each thread keeps kPlanUnroll tile loads in flight.  Launched
+// only for sorts of at most kScanTiles tiles: larger ones derive the plan in
+// the first pass's counter scan from per-group AND/OR.
 constexpr int kPlanThreads = 1024;
 constexpr u32 kPlanUnroll = 8;
 
@@ -234,54 +315,14 @@ __global__ __launch_bounds__(kPlanThreads) void satradix_plan(SortPlan* __restri
 		s_red[kPW + wave_id()] = o;
 	}
 	__syncthreads();
-	if (tid != 0) return;
+	if (wave_id() != 0) return;
 	a = ~K(0);
 	o = K(0);
 	for (int ww = 0; ww < kPW; ++ww) {
 		a &= s_red[ww];
 		o |= s_red[kPW + ww];
 	}
-	const K diff = a ^ o;  // bits that differ between keys
-	const u32 mask = (1u << bits) - 1u;
-	u32 executed = 0;
-	for (u32 p = 0; p < passes; ++p) executed += (static_cast<u32>(diff >> (p * bits)) & mask) ? 1u : 0u;
-	u32 k = 0;
-	u32 cur = kIn;
-	u32 prev = kMaxPasses;  // previous executed pass
-	for (u32 p = 0; p < kMaxPasses; ++p) {
-		PassDesc d = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-		if (p < passes && (static_cast<u32>(diff >> (p * bits)) & mask)) {
-			u32 dst;
-			if (in_place) dst = (k % 2 == 0) ? kAux : kIn;
-			else dst = ((executed - 1 - k) % 2 == 0) ? kOut : kAux;  // last one lands in kOut
-			d.active = 1;
-			d.shift = p * bits;
-			d.src = cur;
-			d.dst = dst;
-			d.dsrc = prev < kMaxPasses ? 1u : 0u;
-			cur = dst;
-			if (prev < kMaxPasses) {
-				plan->pass[prev].nact = 1;
-				plan->pass[prev].nshift = p * bits;
-			}
-			prev = p;
-			++k;
-		}
-		plan->pass[p] = d;
-	}
-	plan->executed = executed;
-	plan->copy_active = 0;
-	plan->copy_src = 0;
-	plan->copy_dst = 0;
-	if (in_place && (executed % 2 == 1)) {
-		plan->copy_active = 1;
-		plan->copy_src = kAux;
-		plan->copy_dst = kIn;
-	} else if (!in_place && executed == 0) {
-		plan->copy_active = 1;
-		plan->copy_src = kIn;
-		plan->copy_dst = kOut;
-	}
+	plan_write<K>(plan, a ^ o, passes, bits, in_place);
 }
 
 // Counter scan.  Tiles form groups of kScanTiles, handled by blocks of
@@ -307,24 +348,47 @@ constexpr u32 kDigitSplit = 4;
 
 // Loads this thread's PER count rows (digit d) from tile t0 on.
 template <u32 PER>
-__device__ __forceinline__ void load_part(const u32* __restrict__ counts, u32 t0, u32 tiles,
+__device__ __forceinline__ void load_part(const u16* __restrict__ counts, u32 t0, u32 tiles,
 	u32 d, u32 (&c)[PER]) {
 #pragma unroll
 	for (u32 i = 0; i < PER; ++i)
 		c[i] = (t0 + i < tiles) ? counts[static_cast<size_t>(t0 + i) * kRadixMax + d] : 0u;
 }
 
-// partial[g][d] = sum of counts[t][d] over the tiles of group g.
-template <u32 DSPLIT>
+// partial[g][d] = sum of counts[t][d] over the tiles of group g.  In the first
+// pass (first = 1, before any plan exists) the blocks of the first digit
+// quarter also reduce their group's per-tile AND/OR into gandor[2g], [2g+1],
+// from which the counter scan derives the plan.
+template <typename K, u32 DSPLIT>
 __global__ __launch_bounds__(kGroupThreads) void satradix_reduce(const SortPlan* __restrict__ plan,
-	u32 pass, const u32* __restrict__ counts, u32 tiles, u32* __restrict__ partial) {
+	u32 pass, const u16* __restrict__ counts, u32 tiles, u32* __restrict__ partial,
+	const K* __restrict__ andor, K* __restrict__ gandor, u32 first) {
 	using S = GroupShape<DSPLIT>;
 	__shared__ u32 s_q[S::kParts][S::kDigits];
-	if (!plan->pass[pass].active) return;
+	__shared__ K s_red[2 * (kGroupThreads / kWave)];
+	if (!first && !plan->pass[pass].active) return;
 	const u32 dl = threadIdx.x % S::kDigits, q = threadIdx.x / S::kDigits;
 	const u32 d = blockIdx.y * S::kDigits + dl;
 	u32 c[S::kPer];
 	load_part(counts, blockIdx.x * kScanTiles + q * S::kPer, tiles, d, c);
+	if (first && blockIdx.y == 0) {
+		// kScanTiles <= kGroupThreads: one tile's AND/OR per thread.
+		const u32 t = blockIdx.x * kScanTiles + threadIdx.x;
+		K a = ~K(0), o = K(0);
+		if (threadIdx.x < kScanTiles && t < tiles) {
+			a = andor[2 * static_cast<size_t>(t)];
+			o = andor[2 * static_cast<size_t>(t) + 1];
+		}
+#pragma unroll
+		for (int off = kWave / 2; off > 0; off >>= 1) {
+			a &= __shfl_xor(a, off, kWave);
+			o |= __shfl_xor(o, off, kWave);
+		}
+		if (lane_id() == 0) {
+			s_red[wave_id()] = a;
+			s_red[kGroupThreads / kWave + wave_id()] = o;
+		}
+	}
 	u32 s = 0;
 #pragma unroll
 	for (u32 i = 0; i < S::kPer; ++i) s += c[i];
@@ -336,16 +400,31 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_reduce(const SortPlan*
 		for (u32 qq = 0; qq < S::kParts; ++qq) tot += s_q[qq][dl];
 		partial[blockIdx.x * kRadixMax + d] = tot;
 	}
+	if (first && blockIdx.y == 0 && threadIdx.x == 0) {
+		K a = ~K(0), o = K(0);
+#pragma unroll
+		for (u32 w = 0; w < kGroupThreads / kWave; ++w) {
+			a &= s_red[w];
+			o |= s_red[kGroupThreads / kWave + w];
+		}
+		gandor[2 * blockIdx.x] = a;
+		gandor[2 * blockIdx.x + 1] = o;
+	}
 }
 
-// Turns the tile counts into global output offsets, in place:
-// counts[t][d] <- (#keys with a smaller digit) + (#keys with digit d in tiles < t).
+// Turns the tile counts into global output offsets:
+// offs[t][d] = (#keys with a smaller digit) + (#keys with digit d in tiles < t).
 // Each block first derives, from the group sums of all groups, its digits'
 // bases: (#keys with a smaller digit) + (#keys with digit d in earlier
-// groups); no separate launch scans the group sums.
-template <u32 DSPLIT>
-__global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* __restrict__ plan,
-	u32 pass, u32* __restrict__ counts, u32 tiles, const u32* __restrict__ partial) {
+// groups); no separate launch scans the group sums.  In the first pass
+// (first = 1) every block derives from the group AND/OR whether the pass
+// runs, and block (0, 0) writes the plan of the whole sort, which every later
+// kernel reads.
+template <typename K, u32 DSPLIT>
+__global__ __launch_bounds__(kGroupThreads) void satradix_scan(SortPlan* __restrict__ plan,
+	u32 pass, const u16* __restrict__ counts, u32* __restrict__ offs, u32 tiles,
+	const u32* __restrict__ partial, const K* __restrict__ gandor, u32 first, u32 passes,
+	u32 bits, u32 in_place) {
 	using S = GroupShape<DSPLIT>;
 	constexpr u32 kGQ = kGroupThreads / kRadixMax;  // threads per digit over the group sums
 	__shared__ u32 s_q[S::kParts][S::kDigits];
@@ -353,8 +432,31 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 	__shared__ u32 s_pre[kGQ][kRadixMax];
 	__shared__ u32 s_base[kRadixMax];
 	__shared__ u32 s_wsum[kGroupThreads / kWave];
-	if (!plan->pass[pass].active) return;
 	const u32 tid = threadIdx.x;
+	if (first) {
+		// AND/OR over the groups, by wave 0: at most 2^32 / 4096 / 256 groups.
+		if (wave_id() == 0) {
+			K a = ~K(0), o = K(0);
+			for (u32 g = lane_id(); g < gridDim.x; g += kWave) {
+				a &= gandor[2 * g];
+				o |= gandor[2 * g + 1];
+			}
+#pragma unroll
+			for (int off = kWave / 2; off > 0; off >>= 1) {
+				a &= __shfl_xor(a, off, kWave);
+				o |= __shfl_xor(o, off, kWave);
+			}
+			const K diff = a ^ o;
+			if (blockIdx.x == 0 && blockIdx.y == 0) plan_write<K>(plan, diff, passes, bits, in_place);
+			if (lane_id() == 0) s_wsum[0] = (static_cast<u32>(diff) & ((1u << bits) - 1u)) ? 1u : 0u;
+		}
+		__syncthreads();
+		const u32 active = s_wsum[0];
+		__syncthreads();
+		if (!active) return;
+	} else if (!plan->pass[pass].active) {
+		return;
+	}
 	const u32 dl = tid % S::kDigits, q = tid / S::kDigits;
 	const u32 d = blockIdx.y * S::kDigits + dl;
 	const u32 t0 = blockIdx.x * kScanTiles + q * S::kPer;
@@ -407,7 +509,7 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 	for (u32 qq = 0; qq < q; ++qq) run += s_q[qq][dl];
 #pragma unroll
 	for (u32 i = 0; i < S::kPer; ++i) {
-		if (t0 + i < tiles) counts[static_cast<size_t>(t0 + i) * kRadixMax + d] = run;
+		if (t0 + i < tiles) offs[static_cast<size_t>(t0 + i) * kRadixMax + d] = run;
 		run += c[i];
 	}
 }
@@ -421,9 +523,10 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 // each element the number of equal-digit elements before it (stability).
 // FIXED_BITS = 8 compiles the digit-match loop for the default 8-bit digits;
 // 0 takes the digit width from `bits` (radix option).
-// fused_scan (sorts of at most kScanTiles tiles): `offsets` holds the
-// upsweep's tile counts and every block derives its own tile's offsets from
-// them, in place of a separate counter-scan launch.
+// fused_scan (sorts of at most kScanTiles tiles): every block derives its own
+// tile's offsets from the upsweep's tile counts (`counts`), in place of a
+// separate counter-scan launch; otherwise `offs` holds the counter scan's
+// offsets.
 //
 // Staging: 8-byte keys pass through LDS as two 32-bit halves (high halves,
 // then low halves) in a buffer of 4 bytes per key, so the block needs 36 KB
@@ -437,7 +540,8 @@ __global__ __launch_bounds__(kGroupThreads) void satradix_scan(const SortPlan* _
 template <typename K, bool KV, int NT, int ITEMS, int FIXED_BITS>
 __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	const SortPlan* __restrict__ plan, u32 pass, u32 n, u32 tiles, u32 bits_arg,
-	const u32* __restrict__ offsets, u8* __restrict__ ndig, u32 fused_scan) {
+	const u32* __restrict__ offs, const u16* __restrict__ counts, u8* __restrict__ ndig,
+	u32 fused_scan) {
 	const u32 bits = FIXED_BITS ? static_cast<u32>(FIXED_BITS) : bits_arg;
 	constexpr u32 kTile = NT * ITEMS;
 	static_assert(NT >= kRadixMax, "one thread per digit bin");
@@ -455,14 +559,14 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 	if (fused_scan) {
 		// Offset of digit d = (#keys with a smaller digit) + (#keys with digit
 		// d in earlier tiles), from the counts of all tiles (<= kScanTiles
-		// rows of 1 KB, read row by row, 16 rows in flight).
+		// rows of 512 B, read row by row, 16 rows in flight).
 		u32 pre = 0, tot = 0;
 		if (tid < kRadixMax) {
 			u32 t2 = 0;
 			for (; t2 + 16 <= tiles; t2 += 16) {
 				u32 c[16];
 #pragma unroll
-				for (u32 j = 0; j < 16; ++j) c[j] = offsets[static_cast<size_t>(t2 + j) * kRadixMax + tid];
+				for (u32 j = 0; j < 16; ++j) c[j] = counts[static_cast<size_t>(t2 + j) * kRadixMax + tid];
 #pragma unroll
 				for (u32 j = 0; j < 16; ++j) {
 					tot += c[j];
@@ -470,7 +574,7 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 				}
 			}
 			for (; t2 < tiles; ++t2) {
-				const u32 c = offsets[static_cast<size_t>(t2) * kRadixMax + tid];
+				const u32 c = counts[static_cast<size_t>(t2) * kRadixMax + tid];
 				tot += c;
 				pre += (t2 < tile) ? c : 0u;
 			}
@@ -479,7 +583,7 @@ __global__ __launch_bounds__(NT) void satradix_downsweep(Bufs<K> b,
 			static_cast<u32*>(nullptr));
 		goff = tid < kRadixMax ? base + pre : 0u;
 	} else if (tid < kRadixMax) {
-		goff = offsets[static_cast<size_t>(tile) * kRadixMax + tid];
+		goff = offs[static_cast<size_t>(tile) * kRadixMax + tid];
 	}
 	TilePass<K> tp;
 	tp.src = b.k[d.src];
@@ -532,8 +636,9 @@ struct SatradixData {
 	std::string scan_type = "blelloch";
 	Scratch aux_keys;
 	Scratch aux_vals;
-	Scratch counts;  // tiles x 256 digit counts, then offsets
-	Scratch meta;    // plan + per-tile AND/OR + partial sums
+	Scratch counts;  // tiles x 256 digit counts (16-bit: a tile holds at most 16384 keys)
+	Scratch offs;    // tiles x 256 output offsets (the counter scan's result)
+	Scratch meta;    // plan + per-tile AND/OR + partial sums + per-group AND/OR
 	Scratch digits;  // next pass's digit byte per key
 	CCLQueue* last_queue = nullptr;
 };
@@ -557,8 +662,13 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 		passes, in_place ? " in place" : "");
 	if (!sd->aux_keys.ensure(n * sizeof(K), err)) return false;
 	if (KV && !sd->aux_vals.ensure(n * sizeof(u32), err)) return false;
-	if (!sd->counts.ensure(static_cast<size_t>(tiles) * kRadixMax * sizeof(u32), err)) return false;
-	if (!sd->meta.ensure(plan_bytes + andor_bytes + static_cast<size_t>(nblk) * kRadixMax * 4, err))
+	const bool fused = nblk == 1;
+	const size_t gandor_bytes = (2 * static_cast<size_t>(nblk) * sizeof(K) + 255) / 256 * 256;
+	if (!sd->counts.ensure(static_cast<size_t>(tiles) * kRadixMax * sizeof(u16), err)) return false;
+	if (!fused && !sd->offs.ensure(static_cast<size_t>(tiles) * kRadixMax * sizeof(u32), err))
+		return false;
+	if (!sd->meta.ensure(plan_bytes + andor_bytes + static_cast<size_t>(nblk) * kRadixMax * 4 +
+			gandor_bytes, err))
 		return false;
 	if (passes > 1 && !sd->digits.ensure((n + 15) / 16 * 16, err)) return false;
 	u8* digits = passes > 1 ? static_cast<u8*>(sd->digits.ptr) : nullptr;
@@ -566,7 +676,10 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	SortPlan* plan = reinterpret_cast<SortPlan*>(mb);
 	K* andor = reinterpret_cast<K*>(mb + plan_bytes);
 	u32* partial = reinterpret_cast<u32*>(mb + plan_bytes + andor_bytes);
-	u32* counts = static_cast<u32*>(sd->counts.ptr);
+	K* gandor = reinterpret_cast<K*>(mb + plan_bytes + andor_bytes +
+		static_cast<size_t>(nblk) * kRadixMax * 4);
+	u16* counts = static_cast<u16*>(sd->counts.ptr);
+	u32* offs = fused ? nullptr : static_cast<u32*>(sd->offs.ptr);
 
 	Bufs<K> b;
 	b.k[kIn] = kin;
@@ -581,11 +694,11 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	const u32 grid = xcd_grid(tiles);
 	{
 		Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
-		satradix_upsweep<K, NT, ITEMS><<<grid, NT, 0, st>>>(b, plan, 0, nn, tiles, bits, counts,
-			andor, 1, digits);
+		satradix_upsweep<K, NT, ITEMS><<<grid, NT, 0, st>>>(kin, nn, tiles, bits, counts, andor);
 		if (!L.done(err)) return false;
 	}
-	{
+	if (fused) {
+		// One group of tiles: no counter scan to derive the plan in.
 		Launch L(q, CLO_SORT_SATRADIX_KNAME_PLAN);
 		satradix_plan<K><<<1, kPlanThreads, 0, st>>>(plan, andor, tiles, passes, bits, in_place ? 1u : 0u);
 		if (!L.done(err)) return false;
@@ -593,21 +706,21 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 	for (u32 p = 0; p < passes; ++p) {
 		if (p > 0) {
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_UPSWEEP);
-			satradix_upsweep<K, NT, ITEMS><<<grid, NT, 0, st>>>(b, plan, p, nn, tiles, bits,
-				counts, andor, 0, digits);
+			satradix_upsweep_digits<K, NT * ITEMS><<<grid, kWave, 0, st>>>(b, plan, p, nn, tiles,
+				bits, counts, digits);
 			if (!L.done(err)) return false;
 		}
-		const bool fused = nblk == 1;
+		const u32 first = p == 0 ? 1u : 0u;
 		if (!fused) {
 			{
 				Launch L(q, CLO_SORT_SATRADIX_KNAME_REDUCE);
-				satradix_reduce<kDigitSplit><<<dim3(nblk, kDigitSplit), kGroupThreads, 0, st>>>(plan, p,
-					counts, tiles, partial);
+				satradix_reduce<K, kDigitSplit><<<dim3(nblk, kDigitSplit), kGroupThreads, 0, st>>>(plan,
+					p, counts, tiles, partial, andor, gandor, first);
 				if (!L.done(err)) return false;
 			}
 			Launch L(q, CLO_SORT_SATRADIX_KNAME_SCAN);
-			satradix_scan<kDigitSplit><<<dim3(nblk, kDigitSplit), kGroupThreads, 0, st>>>(plan, p, counts,
-				tiles, partial);
+			satradix_scan<K, kDigitSplit><<<dim3(nblk, kDigitSplit), kGroupThreads, 0, st>>>(plan, p,
+				counts, offs, tiles, partial, gandor, first, passes, bits, in_place ? 1u : 0u);
 			if (!L.done(err)) return false;
 		}
 		{
@@ -615,13 +728,13 @@ bool launch_all(SatradixData* sd, CCLQueue* q, K* kin, u32* vin, K* kout, u32* v
 			if constexpr (NT == 256 && ITEMS == 32) {
 				if (bits != 8) {
 					satradix_downsweep<K, KV, NT, ITEMS, 0><<<grid, NT, 0, st>>>(b, plan, p, nn,
-						tiles, bits, counts, digits, fused ? 1u : 0u);
+						tiles, bits, offs, counts, digits, fused ? 1u : 0u);
 					if (!L.done(err)) return false;
 					continue;
 				}
 			}
 			satradix_downsweep<K, KV, NT, ITEMS, 8><<<grid, NT, 0, st>>>(b, plan, p, nn, tiles,
-				bits, counts, digits, fused ? 1u : 0u);
+				bits, offs, counts, digits, fused ? 1u : 0u);
 			if (!L.done(err)) return false;
 		}
 	}
