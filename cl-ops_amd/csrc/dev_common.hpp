@@ -119,11 +119,27 @@ __device__ __forceinline__ u64 match_peers(u32 dig, u32 bits, bool valid) {
 		if (bt >= bits) break;
 		const u32 sb = static_cast<u32>(__builtin_amdgcn_sbfe(static_cast<int>(dig),
 			static_cast<int>(bt), 1));
-		const u64 bb = ballot(sb != 0u);
+		const u64 bb = ballot(static_cast<int>(sb) < 0);
 		mlo = __builtin_amdgcn_bitop3_b32(mlo, static_cast<u32>(bb), sb, 0xF6);
 		mhi = __builtin_amdgcn_bitop3_b32(mhi, static_cast<u32>(bb >> 32), sb, 0xF6);
 	}
 	return pv & ~((static_cast<u64>(mhi) << 32) | mlo);
+}
+
+// Lanes holding the same digit when every lane of the wave holds an element.
+template <u32 MAXBITS>
+__device__ __forceinline__ u64 match_peers_full(u32 dig, u32 bits) {
+	u32 mlo = 0, mhi = 0;
+#pragma unroll
+	for (u32 bt = 0; bt < MAXBITS; ++bt) {
+		if (bt >= bits) break;
+		const u32 sb = static_cast<u32>(__builtin_amdgcn_sbfe(static_cast<int>(dig),
+			static_cast<int>(bt), 1));
+		const u64 bb = ballot(static_cast<int>(sb) < 0);
+		mlo = __builtin_amdgcn_bitop3_b32(mlo, static_cast<u32>(bb), sb, 0xF6);
+		mhi = __builtin_amdgcn_bitop3_b32(mhi, static_cast<u32>(bb >> 32), sb, 0xF6);
+	}
+	return ~((static_cast<u64>(mhi) << 32) | mlo);
 }
 
 // ---- look-back status words -----------------------------------------------

@@ -23,7 +23,8 @@ struct RankSmem {
 // the item's position when the block's valid items are ordered by digit,
 // ties in element order; sm.tstart[d] / sm.total[d] hold the block's digit
 // starts and counts.  Contains barriers: call from uniform control flow.
-template <int IPT>
+// FULL: every item of the block is valid (`valid` is not read).
+template <int IPT, bool FULL = false>
 __device__ __forceinline__ void block_rank(const u32 (&dig)[IPT], const bool (&valid)[IPT],
 	u32 (&pos)[IPT], u32 bits, RankSmem& sm) {
 	const u32 tid = threadIdx.x;
@@ -34,10 +35,10 @@ __device__ __forceinline__ void block_rank(const u32 (&dig)[IPT], const bool (&v
 #pragma unroll
 	for (int j = 0; j < IPT; ++j) {
 		const u32 d = dig[j];
-		const u64 peers = match_peers<8>(d, bits, valid[j]);
+		const u64 peers = FULL ? match_peers_full<8>(d, bits) : match_peers<8>(d, bits, valid[j]);
 		const u32 below = count_below(peers);
 		u32 base = 0;
-		if (valid[j]) {
+		if (FULL || valid[j]) {
 			base = sm.whist[w][d];
 			if (below == 0) sm.whist[w][d] = base + static_cast<u32>(__popcll(peers));
 		}
@@ -57,7 +58,7 @@ __device__ __forceinline__ void block_rank(const u32 (&dig)[IPT], const bool (&v
 	__syncthreads();
 #pragma unroll
 	for (int j = 0; j < IPT; ++j)
-		if (valid[j]) pos[j] += sm.tstart[dig[j]] + sm.whist[w][dig[j]];
+		if (FULL || valid[j]) pos[j] += sm.tstart[dig[j]] + sm.whist[w][dig[j]];
 }
 
 }  // namespace dev

@@ -27,22 +27,6 @@ struct TileSmem {
 	u32 wsum[NT / kWave];
 };
 
-// Lanes holding the same digit when every lane of the wave holds an element.
-template <u32 MAXBITS>
-__device__ __forceinline__ u64 match_peers_full(u32 dig, u32 bits) {
-	u32 mlo = 0, mhi = 0;
-#pragma unroll
-	for (u32 bt = 0; bt < MAXBITS; ++bt) {
-		if (bt >= bits) break;
-		const u32 sb = static_cast<u32>(__builtin_amdgcn_sbfe(static_cast<int>(dig),
-			static_cast<int>(bt), 1));
-		const u64 bb = ballot(sb != 0u);
-		mlo = __builtin_amdgcn_bitop3_b32(mlo, static_cast<u32>(bb), sb, 0xF6);
-		mhi = __builtin_amdgcn_bitop3_b32(mhi, static_cast<u32>(bb >> 32), sb, 0xF6);
-	}
-	return ~((static_cast<u64>(mhi) << 32) | mlo);
-}
-
 // Where a tile pass reads and writes.  Keys [base, base + count) of src
 // (count <= NT * ITEMS) go to dst, the element of digit d at rank i within
 // the tile's keys of that digit to goff(d) + i, where thread d < 256 holds

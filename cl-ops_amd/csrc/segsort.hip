@@ -153,6 +153,7 @@ struct SegSmem {
 	u32 segj[kW];
 	u32 hist[sizeof(K)][256];
 	u32 base[256];
+	u32 gofs[256];
 	u32 red[2 * kRBWaves];
 	u32 misc[4];
 	RankSmem rs;
@@ -409,6 +410,59 @@ __device__ void sort_chunk(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 c) {
 	__syncthreads();
 }
 
+// One chunk of a large segment in one digit pass: stable rank in the block,
+// staged in digit order in LDS, then written to the pass's destination run
+// of each digit (s.base: the segment's next free position per digit, advanced
+// by the chunk's digit counts).  FULL: cnt == CAP, no per-element checks.
+template <typename K, bool KV, int IPT, bool FULL>
+__device__ __forceinline__ void large_chunk(SegSmem<K, KV, IPT>& s, const K (&k)[IPT],
+	const u32 (&v)[IPT], u32 cnt, u32 shift, K* __restrict__ dst, u32* __restrict__ vdst) {
+	const u32 tid = threadIdx.x;
+	const u32 lane = lane_id();
+	const u32 w = wave_id();
+	u32 dig[IPT];
+	bool valid[IPT];
+	u32 pos[IPT];
+#pragma unroll
+	for (int jj = 0; jj < IPT; ++jj) {
+		const u32 e = w * kWave * IPT + jj * kWave + lane;
+		valid[jj] = FULL || e < cnt;
+		dig[jj] = digit_of(k[jj], shift);
+	}
+	block_rank<IPT, FULL>(dig, valid, pos, 8, s.rs);
+	// Output position of staged element i of digit d: gofs[d] + i.
+	s.gofs[tid] = s.base[tid] - s.rs.tstart[tid];
+#pragma unroll
+	for (int jj = 0; jj < IPT; ++jj)
+		if (FULL || valid[jj]) s.k[pos[jj]] = k[jj];
+	__syncthreads();
+	u32 gidx[IPT];
+#pragma unroll
+	for (int r = 0; r < IPT; ++r) {
+		const u32 i = r * kRB + tid;
+		if (FULL || i < cnt) {
+			const K key = s.k[i];
+			gidx[r] = s.gofs[digit_of(key, shift)] + i;
+			dst[gidx[r]] = key;
+		}
+	}
+	if (KV) {
+		__syncthreads();
+#pragma unroll
+		for (int jj = 0; jj < IPT; ++jj)
+			if (FULL || valid[jj]) s.v[pos[jj]] = v[jj];
+		__syncthreads();
+#pragma unroll
+		for (int r = 0; r < IPT; ++r) {
+			const u32 i = r * kRB + tid;
+			if (FULL || i < cnt) vdst[gidx[r]] = s.v[i];
+		}
+	}
+	__syncthreads();
+	s.base[tid] += s.rs.total[tid];
+	__syncthreads();
+}
+
 // Large segment: LSD passes through global memory.
 template <typename K, bool KV, int IPT>
 __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
@@ -499,16 +553,10 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 			const u32 cnt = (len - c0) < CAP ? (len - c0) : CAP;
 			K k[IPT];
 			u32 v[IPT];
-			u32 dig[IPT];
-			bool valid[IPT];
-			u32 pos[IPT];
 #pragma unroll
 			for (int jj = 0; jj < IPT; ++jj) {
-				const u32 e = w * kWave * IPT + jj * kWave + lane;
-				valid[jj] = e < cnt;
 				k[jj] = kn[jj];
 				if (KV) v[jj] = vn[jj];
-				dig[jj] = digit_of(k[jj], shift);
 			}
 			const u32 c1 = c0 + CAP;
 			if (c1 < len) {
@@ -520,37 +568,15 @@ __device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
 					if (KV) vn[jj] = e < cnt1 ? vsrc[c1 + e] : 0u;
 				}
 			}
-			block_rank<IPT>(dig, valid, pos, 8, s.rs);
-#pragma unroll
-			for (int jj = 0; jj < IPT; ++jj)
-				if (valid[jj]) s.k[pos[jj]] = k[jj];
-			__syncthreads();
-			u32 gidx[IPT];
-#pragma unroll
-			for (int r = 0; r < IPT; ++r) {
-				const u32 i = r * kRB + tid;
-				if (i < cnt) {
-					const K key = s.k[i];
-					const u32 d = digit_of(key, shift);
-					gidx[r] = s.base[d] + (i - s.rs.tstart[d]);
-					dst[gidx[r]] = key;
+			// The check-free body for full chunks only for keys alone: with
+			// values the second copy of the body made the kernel spill.
+			if constexpr (!KV) {
+				if (cnt == CAP) {
+					large_chunk<K, KV, IPT, true>(s, k, v, cnt, shift, dst, vdst);
+					continue;
 				}
 			}
-			if (KV) {
-				__syncthreads();
-#pragma unroll
-				for (int jj = 0; jj < IPT; ++jj)
-					if (valid[jj]) s.v[pos[jj]] = v[jj];
-				__syncthreads();
-#pragma unroll
-				for (int r = 0; r < IPT; ++r) {
-					const u32 i = r * kRB + tid;
-					if (i < cnt) vdst[gidx[r]] = s.v[i];
-				}
-			}
-			__syncthreads();
-			s.base[tid] += s.rs.total[tid];
-			__syncthreads();
+			large_chunk<K, KV, IPT, false>(s, k, v, cnt, shift, dst, vdst);
 		}
 		src = dst;
 		vsrc = vdst;
@@ -586,7 +612,8 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 	for (;;) {
 		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head, 1u);
 		__syncthreads();
-		const u32 item = s.misc[3];
+		// Uniform over the block: scalar loads and addresses downstream.
+		const u32 item = __builtin_amdgcn_readfirstlane(s.misc[3]);
 		__syncthreads();
 		if (item >= total) break;
 		if (item < nl) sort_large<K, KV, IPT>(a, s, a.large[item]);
