@@ -174,18 +174,20 @@ bool type_is_integer(CloType t);
 bool type_is_signed(CloType t);
 
 // Segmented sort defaults: items per thread of the 256-thread tile and
-// blocks per CU of the persistent launch.  Keys only, from the dispatcher's
-// exhaustive tuner (clo_dispatch_tune_segmented) on 32768 log-uniform
-// segments of up to 65536 keys, best mean over three runs: tile 16 with 4
-// blocks per CU for 32- and 64-bit keys (32-bit: 2.78 ms; tile 8 within 0.3%).
-// Key-value sorts, measured on the same segments: tile 16 for 32-bit keys
-// (4.29 ms against 5.49 with tile 8), tile 8 for 64-bit keys (11.9 ms
-// against 16.6: tile 16 spills registers).  The dispatcher's fixed
+// blocks per CU of the persistent launch, measured on 32768 log-uniform
+// segments of up to 65536 keys.  Keys only: tile 8 (32-bit keys: 2.58 ms
+// against 2.65-2.71 with tile 16; 64-bit keys, whose medium and large
+// segments take the MSD bucketing kernel: 3.87 ms against 5.13).  Key-value
+// sorts: tile 16 for 32-bit keys (4.28 ms against 5.05 with tile 8), tile 8
+// for 64-bit keys (tile 16 spills registers).  The dispatcher's fixed
 // segmented configuration (keys only) is the same.
 constexpr uint32_t seg_default_tile(size_t key_bytes, bool kv = false) {
-	return (key_bytes == 8 && kv) ? 8u : 16u;
+	return (key_bytes == 4 && kv) ? 16u : 8u;
 }
 constexpr uint32_t kSegBlocksPerCuDefault = 4;
+// 64-bit key-value sorts: at most two blocks per CU (three resident blocks,
+// each with a large segment's keys and values in flight, measured 16% slower).
+constexpr uint32_t kSegBlocksPerCuKv64 = 2;
 
 }  // namespace clo
 

@@ -34,7 +34,9 @@ struct SegMeta {
 	u32 n_medium;
 	u32 n_large;
 	u32 n_chunks;
-	u32 head;
+	u32 head;      // work queue of the digit-pass kernel
+	u32 head_msd;  // work queue of the keys-only bucketing kernel
+	u32 n_fb;      // large keys-only segments left to the digit passes
 };
 
 __global__ void seg_reset(SegMeta* m) {
@@ -43,6 +45,8 @@ __global__ void seg_reset(SegMeta* m) {
 		m->n_large = 0;
 		m->n_chunks = 0;
 		m->head = 0;
+		m->head_msd = 0;
+		m->n_fb = 0;
 	}
 }
 
@@ -136,6 +140,8 @@ struct SegArgs {
 	const u32* large;
 	const u32* chunks;    // windows holding small segment starts
 	const uint2* wrange;  // per window: segments [x, y) starting in it
+	uint2* fb;            // keys alone: (start, len) of large segments for the digit passes
+	u32 fb_mode;          // digit-pass kernel after the bucketing kernel (keys alone)
 	SegMeta* meta;
 	u64 cls_lo, cls_hi;   // per-length-bin least work class (seg_class)
 	u32 nseg;
@@ -148,9 +154,17 @@ struct SegSmem {
 	static constexpr u32 kW = kCap / 2;
 	K k[kCap];
 	u32 v[KV ? kCap : 1];
-	u16 sid[kCap];
-	u32 lpos[kW + 1];
-	u32 segj[kW];
+	union {
+		struct {  // chunk items
+			u16 sid[kCap];
+			u32 lpos[kW + 1];
+			u32 segj[kW];
+		};
+		struct {  // keys-only tiles
+			u32 mcnt[kCap];          // bucket counts, then cursors
+			uint2 mstack[kCap / 32]; // pending buckets of more than 32 keys
+		};
+	};
 	u32 hist[sizeof(K)][256];
 	u32 base[256];
 	u32 gofs[256];
@@ -289,6 +303,165 @@ __device__ void sort_key_passes(SegSmem<K, KV, IPT>& s, u32 n, bool carry_sid) {
 	for (u32 p = 0; p < sizeof(K); ++p) {
 		if (((diff >> (8 * p)) & 0xFF) == 0) continue;  // uniform: every key shares the digit
 		lds_pass<K, KV, IPT>(s, n, 8 * p, false, carry_sid);
+	}
+}
+
+
+// ---- keys-only tiles: unstable MSD bucketing ------------------------------
+// For keys alone the order of equal keys cannot be seen (they are equal bit
+// patterns), so a tile may be sorted by any method.  One bucketing level
+// over s.k[base, base + m) takes the range's minimum and maximum, puts key x
+// in bucket (x - min) >> shift, with shift such that the key range spans
+// kCap buckets (one key per bucket on average for keys spread over their
+// range), scatters the keys to their buckets with LDS atomic cursors and
+// insertion-sorts every bucket of 2..kMsdMaxBucket keys.  Larger buckets go
+// on a stack in LDS and get a level of their own; each level narrows the key
+// range by a factor kCap, so every range ends sorted, and a range of equal
+// keys ends at once.  A level costs a few VALU operations per key, where a
+// stable digit pass costs about 50 (wave ballots over 8 digit bits).
+constexpr u32 kMsdMaxBucket = 32;
+
+template <typename K>
+__device__ __forceinline__ u32 top_bit(K x) {
+	return sizeof(K) == 8 ? 63u - static_cast<u32>(__builtin_clzll(static_cast<u64>(x)))
+		: 31u - static_cast<u32>(__builtin_clz(static_cast<u32>(x)));
+}
+
+// One level over s.k[base, base + m), m <= kCap.  Contains barriers: call from
+// uniform control flow.  Pushes the buckets of more than kMsdMaxBucket keys
+// onto s.mstack (count in s.misc[2]).
+template <typename K, int IPT>
+__device__ __forceinline__ void msd_level(SegSmem<K, false, IPT>& s, u32 base, u32 m) {
+	using S = SegSmem<K, false, IPT>;
+	constexpr u32 NB = S::kCap;  // buckets (a power of two)
+	constexpr u32 WB = static_cast<u32>(__builtin_ctz(NB));  // bucket bits
+	constexpr u32 BPT = NB / kRB;  // buckets per thread in the scan
+	static_assert((NB & (NB - 1)) == 0 && BPT % 4 == 0, "power-of-two tile, 16-byte rows");
+	const u32 tid = threadIdx.x;
+	const u32 lane = lane_id();
+	const u32 w = wave_id();
+	K k[IPT];
+	bool valid[IPT];
+	K lo = ~K(0), hi = K(0);
+#pragma unroll
+	for (int j = 0; j < IPT; ++j) {
+		const u32 e = j * kRB + tid;
+		valid[j] = e < m;
+		k[j] = valid[j] ? s.k[base + e] : K(0);
+		if (valid[j]) {
+			lo = k[j] < lo ? k[j] : lo;
+			hi = k[j] > hi ? k[j] : hi;
+		}
+	}
+	// Block min/max; every thread zeroes its BPT counters meanwhile.
+#pragma unroll
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		const K l2 = __shfl_xor(lo, off, kWave);
+		const K h2 = __shfl_xor(hi, off, kWave);
+		lo = l2 < lo ? l2 : lo;
+		hi = h2 > hi ? h2 : hi;
+	}
+	K* red = reinterpret_cast<K*>(s.hist);
+	if (lane == 0) {
+		red[w] = lo;
+		red[kRBWaves + w] = hi;
+	}
+#pragma unroll
+	for (u32 q = 0; q < BPT / 4; ++q)
+		reinterpret_cast<uint4*>(s.mcnt)[tid * (BPT / 4) + q] = make_uint4(0u, 0u, 0u, 0u);
+	__syncthreads();
+	lo = ~K(0);
+	hi = K(0);
+#pragma unroll
+	for (int ww = 0; ww < kRBWaves; ++ww) {
+		lo = red[ww] < lo ? red[ww] : lo;
+		hi = red[kRBWaves + ww] > hi ? red[kRBWaves + ww] : hi;
+	}
+	__syncthreads();  // red is read before the next level writes it
+	if (hi == lo) return;  // all keys equal (uniform over the block)
+	// Bucket of key x: (x - lo) >> shift, below NB for every key of the range
+	// and non-decreasing in x.
+	const u32 hb = top_bit(static_cast<K>(hi - lo));
+	const u32 shift = hb + 1 >= WB ? hb + 1 - WB : 0u;
+	const bool exact = shift == 0;  // one key value per bucket
+#pragma unroll
+	for (int j = 0; j < IPT; ++j)
+		if (valid[j]) atomicAdd(&s.mcnt[static_cast<u32>((k[j] - lo) >> shift)], 1u);
+	__syncthreads();
+	// Thread tid scans buckets [tid * BPT, (tid + 1) * BPT).
+	u32 c[BPT];
+#pragma unroll
+	for (u32 q = 0; q < BPT / 4; ++q) {
+		const uint4 x = reinterpret_cast<const uint4*>(s.mcnt)[tid * (BPT / 4) + q];
+		c[4 * q] = x.x;
+		c[4 * q + 1] = x.y;
+		c[4 * q + 2] = x.z;
+		c[4 * q + 3] = x.w;
+	}
+	u32 tot = 0;
+#pragma unroll
+	for (u32 i = 0; i < BPT; ++i) tot += c[i];
+	u32 run = base + block_exclusive_sum<kRB>(tot, s.rs.wsum, static_cast<u32*>(nullptr));
+#pragma unroll
+	for (u32 i = 0; i < BPT; ++i) {
+		const u32 x = c[i];
+		c[i] = run;
+		run += x;
+	}
+#pragma unroll
+	for (u32 q = 0; q < BPT / 4; ++q)
+		reinterpret_cast<uint4*>(s.mcnt)[tid * (BPT / 4) + q] =
+			make_uint4(c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]);
+	__syncthreads();
+	// Scatter (every key is in registers): the bucket's cursor, then +1.
+#pragma unroll
+	for (int j = 0; j < IPT; ++j) {
+		if (valid[j]) {
+			const u32 p = atomicAdd(&s.mcnt[static_cast<u32>((k[j] - lo) >> shift)], 1u);
+			s.k[p] = k[j];
+		}
+	}
+	__syncthreads();
+	if (exact) return;
+	// Cursors are now bucket ends; bucket b is [end(b-1), end(b)).
+	for (u32 b = tid; b < NB; b += kRB) {
+		const u32 e = s.mcnt[b];
+		const u32 b0 = b == 0 ? base : s.mcnt[b - 1];
+		if (e - b0 > kMsdMaxBucket) {
+			const u32 slot = atomicAdd(&s.misc[2], 1u);
+			s.mstack[slot] = make_uint2(b0, e - b0);
+			continue;
+		}
+		for (u32 i = b0 + 1; i < e; ++i) {
+			const K x = s.k[i];
+			u32 jj = i;
+			while (jj > b0) {
+				const K y = s.k[jj - 1];
+				if (!(x < y)) break;
+				s.k[jj] = y;
+				--jj;
+			}
+			s.k[jj] = x;
+		}
+	}
+	__syncthreads();
+}
+
+// Sorts s.k[0, n), n <= kCap, keys alone.  The stack holds disjoint ranges
+// of more than kMsdMaxBucket keys, so it never exceeds kCap / 33 entries.
+template <typename K, int IPT>
+__device__ __forceinline__ void msd_tile_sort(SegSmem<K, false, IPT>& s, u32 n) {
+	if (threadIdx.x == 0) s.misc[2] = 0;
+	__syncthreads();
+	msd_level<K, IPT>(s, 0, n);
+	for (;;) {
+		__syncthreads();
+		const u32 sz = s.misc[2];
+		if (sz == 0) break;
+		const uint2 r = s.mstack[sz - 1];
+		__syncthreads();
+		if (threadIdx.x == 0) s.misc[2] = sz - 1;
+		msd_level<K, IPT>(s, r.x, r.y);
 	}
 }
 
@@ -463,14 +636,186 @@ __device__ __forceinline__ void large_chunk(SegSmem<K, KV, IPT>& s, const K (&k)
 	__syncthreads();
 }
 
+
+// Large segment, keys alone: one unstable partition of the segment into 256
+// buckets by (key - min) >> shift (the segment's key range over 256 buckets)
+// into the scratch array, staged per chunk of CAP keys through LDS so that
+// each bucket's keys are written as one run; then windows of whole buckets,
+// each at most CAP keys, sorted in LDS by msd_tile_sort and written to the
+// output.  A window holds the buckets whose start lies in one range of CAP/2
+// positions, so with every bucket at most CAP/2 keys no window exceeds CAP.
+// Returns false, having written nothing, when a bucket is larger (keys
+// concentrated in a small part of their range): the segment is then left to
+// the stable digit passes.
+template <typename K, int IPT>
+__device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, SegSmem<K, false, IPT>& s,
+	u32 start, u32 len) {
+	using S = SegSmem<K, false, IPT>;
+	constexpr u32 CAP = S::kCap;
+	constexpr u32 HALF = CAP / 2;
+	const u32 tid = threadIdx.x;
+	const u32 lane = lane_id();
+	const u32 w = wave_id();
+	u32* hcnt = s.hist[0];   // bucket sizes
+	u32* ccnt = s.hist[2];   // per chunk: bucket counts, then cursors
+	u32* wbeg = s.hist[3];   // window starts (at most one per bucket)
+	const K* __restrict__ src = a.kin + start;
+	// Pass 1: minimum and maximum.
+	K lo = ~K(0), hi = K(0);
+	for (u32 c0 = 0; c0 < len; c0 += kBatch * kRB) {
+		K t[kBatch];
+#pragma unroll
+		for (int r = 0; r < kBatch; ++r) {
+			const u32 i = c0 + r * kRB + tid;
+			t[r] = i < len ? src[i] : src[0];
+		}
+#pragma unroll
+		for (int r = 0; r < kBatch; ++r) {
+			lo = t[r] < lo ? t[r] : lo;
+			hi = t[r] > hi ? t[r] : hi;
+		}
+	}
+#pragma unroll
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		const K l2 = __shfl_xor(lo, off, kWave);
+		const K h2 = __shfl_xor(hi, off, kWave);
+		lo = l2 < lo ? l2 : lo;
+		hi = h2 > hi ? h2 : hi;
+	}
+	K* red = reinterpret_cast<K*>(s.hist[1]);
+	if (lane == 0) {
+		red[w] = lo;
+		red[kRBWaves + w] = hi;
+	}
+	hcnt[tid] = 0;
+	ccnt[tid] = 0;
+	if (tid == 0) s.misc[2] = 0;
+	__syncthreads();
+	lo = ~K(0);
+	hi = K(0);
+#pragma unroll
+	for (int ww = 0; ww < kRBWaves; ++ww) {
+		lo = red[ww] < lo ? red[ww] : lo;
+		hi = red[kRBWaves + ww] > hi ? red[kRBWaves + ww] : hi;
+	}
+	if (hi == lo) {  // uniform over the block: every key equal
+		if (a.kout + start != src) g2g<K, IPT>(a.kout + start, src, len);
+		return true;
+	}
+	const u32 hb = top_bit(static_cast<K>(hi - lo));
+	const u32 shift = hb + 1 >= 8 ? hb + 1 - 8 : 0u;
+	// Pass 2: bucket sizes.
+	for (u32 c0 = 0; c0 < len; c0 += kBatch * kRB) {
+		K t[kBatch];
+#pragma unroll
+		for (int r = 0; r < kBatch; ++r) {
+			const u32 i = c0 + r * kRB + tid;
+			t[r] = i < len ? src[i] : K(0);
+		}
+#pragma unroll
+		for (int r = 0; r < kBatch; ++r)
+			if (c0 + r * kRB + tid < len)
+				atomicAdd(&hcnt[static_cast<u32>((t[r] - lo) >> shift)], 1u);
+	}
+	__syncthreads();
+	const u32 c = hcnt[tid];
+	if (c > HALF) atomicOr(&s.misc[2], 1u);
+	const u32 b0 = block_exclusive_sum<kRB>(c, s.rs.wsum, static_cast<u32*>(nullptr));
+	if (s.misc[2] != 0) return false;  // read after the barriers above: uniform
+	s.base[tid] = b0;  // next free position of the bucket in the scratch array
+	hcnt[tid] = b0;    // bucket starts
+	__syncthreads();
+	{
+		// Bucket tid opens a window when its start lies in another range of
+		// HALF positions than its predecessor's.
+		const u32 prev = tid == 0 ? 0u : hcnt[tid - 1];
+		const bool op = tid == 0 || (b0 / HALF) != (prev / HALF);
+		u32 nw = 0;
+		const u32 idx = block_exclusive_sum<kRB>(op ? 1u : 0u, s.rs.wsum, &nw);
+		if (op) wbeg[idx] = b0;
+		if (tid == 0) s.misc[0] = nw;
+	}
+	K* __restrict__ scr = a.kscr + start;
+	// Pass 3: partition, chunk by chunk (the next chunk's loads in flight).
+	K kn[IPT];
+#pragma unroll
+	for (int jj = 0; jj < IPT; ++jj) {
+		const u32 e = jj * kRB + tid;
+		kn[jj] = e < len ? src[e] : K(0);
+	}
+	for (u32 c0 = 0; c0 < len; c0 += CAP) {
+		const u32 cnt = (len - c0) < CAP ? (len - c0) : CAP;
+		K k[IPT];
+#pragma unroll
+		for (int jj = 0; jj < IPT; ++jj) k[jj] = kn[jj];
+		const u32 c1 = c0 + CAP;
+		if (c1 < len) {
+			const u32 cnt1 = (len - c1) < CAP ? (len - c1) : CAP;
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj) {
+				const u32 e = jj * kRB + tid;
+				kn[jj] = e < cnt1 ? src[c1 + e] : K(0);
+			}
+		}
+#pragma unroll
+		for (int jj = 0; jj < IPT; ++jj)
+			if (jj * kRB + tid < cnt) atomicAdd(&ccnt[static_cast<u32>((k[jj] - lo) >> shift)], 1u);
+		__syncthreads();
+		const u32 cc = ccnt[tid];
+		const u32 cs = block_exclusive_sum<kRB>(cc, s.rs.wsum, static_cast<u32*>(nullptr));
+		ccnt[tid] = cs;                 // chunk cursor of bucket tid
+		s.gofs[tid] = s.base[tid] - cs;  // staged element i of bucket d goes to gofs[d] + i
+		__syncthreads();
+#pragma unroll
+		for (int jj = 0; jj < IPT; ++jj)
+			if (jj * kRB + tid < cnt)
+				s.k[atomicAdd(&ccnt[static_cast<u32>((k[jj] - lo) >> shift)], 1u)] = k[jj];
+		__syncthreads();
+#pragma unroll
+		for (int r = 0; r < IPT; ++r) {
+			const u32 i = r * kRB + tid;
+			if (i < cnt) {
+				const K key = s.k[i];
+				scr[s.gofs[static_cast<u32>((key - lo) >> shift)] + i] = key;
+			}
+		}
+		__syncthreads();
+		s.base[tid] += cc;
+		ccnt[tid] = 0;
+		__syncthreads();
+	}
+	// Windows: load, sort in LDS, write to the output.
+	const u32 nw = s.misc[0];
+	K* __restrict__ out = a.kout + start;
+	for (u32 wi = 0; wi < nw; ++wi) {
+		const u32 p0 = wbeg[wi];
+		const u32 n = (wi + 1 < nw ? wbeg[wi + 1] : len) - p0;
+		if (n == 0) continue;  // uniform: wbeg is read by every thread
+		g2s<K, IPT>(s.k, scr + p0, n);
+		__syncthreads();
+		if (n > 1) msd_tile_sort<K, IPT>(s, n);
+		s2g<K, IPT>(out + p0, s.k, n);
+		__syncthreads();
+	}
+	return true;
+}
+
+// Medium segment, keys alone: whole segment in LDS, msd_tile_sort.
+template <typename K, int IPT>
+__device__ __forceinline__ void sort_medium_msd(const SegArgs<K>& a, SegSmem<K, false, IPT>& s,
+	u32 start, u32 len) {
+	g2s<K, IPT>(s.k, a.kin + start, len);
+	__syncthreads();
+	if (len > 1) msd_tile_sort<K, IPT>(s, len);
+	s2g<K, IPT>(a.kout + start, s.k, len);
+	__syncthreads();
+}
+
 // Large segment: LSD passes through global memory.
 template <typename K, bool KV, int IPT>
-__device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 j) {
+__device__ void sort_large(const SegArgs<K>& a, SegSmem<K, KV, IPT>& s, u32 start, u32 len) {
 	constexpr u32 CAP = SegSmem<K, KV, IPT>::kCap;
 	constexpr u32 P = sizeof(K);
-	const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
-	const u32 start = sr.start;
-	const u32 len = sr.len;
 	const u32 tid = threadIdx.x;
 	for (u32 i = tid; i < P * 256; i += kRB) (&s.hist[0][0])[i] = 0;
 	__syncthreads();
@@ -606,8 +951,12 @@ template <typename K, bool KV, int IPT>
 __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_persistent(
 	SegArgs<K> a) {
 	__shared__ SegSmem<K, KV, IPT> s;
-	const u32 nl = a.meta->n_large;
-	const u32 nm = a.meta->n_medium;
+	// fb_mode (keys alone, after seg_msd_persistent): the large and medium
+	// segments are done except those in the fallback list, which sit in the
+	// output array and are sorted there in place; then the chunk items.
+	const bool fb = !KV && a.fb_mode;
+	const u32 nl = fb ? a.meta->n_fb : a.meta->n_large;
+	const u32 nm = fb ? 0u : a.meta->n_medium;
 	const u32 total = nl + nm + a.meta->n_chunks;
 	for (;;) {
 		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head, 1u);
@@ -616,9 +965,64 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 		const u32 item = __builtin_amdgcn_readfirstlane(s.misc[3]);
 		__syncthreads();
 		if (item >= total) break;
-		if (item < nl) sort_large<K, KV, IPT>(a, s, a.large[item]);
-		else if (item < nl + nm) sort_medium<K, KV, IPT>(a, s, a.medium[item - nl]);
-		else sort_chunk<K, KV, IPT>(a, s, a.chunks[item - nl - nm]);
+		if (item < nl) {
+			if constexpr (KV) {
+				const SegRange sr = seg_range(a.off, a.large[item], a.nseg, a.numel);
+				sort_large<K, KV, IPT>(a, s, __builtin_amdgcn_readfirstlane(sr.start),
+					__builtin_amdgcn_readfirstlane(sr.len));
+			} else {
+				SegArgs<K> b = a;
+				u32 start, len;
+				if (a.fb_mode) {
+					const uint2 r = a.fb[item];
+					start = r.x;
+					len = r.y;
+					b.kin = a.kout;
+				} else {
+					const SegRange sr = seg_range(a.off, a.large[item], a.nseg, a.numel);
+					start = sr.start;
+					len = sr.len;
+				}
+				sort_large<K, KV, IPT>(b, s, __builtin_amdgcn_readfirstlane(start),
+					__builtin_amdgcn_readfirstlane(len));
+			}
+		} else if (item < nl + nm) {
+			sort_medium<K, KV, IPT>(a, s, a.medium[item - nl]);
+		} else {
+			sort_chunk<K, KV, IPT>(a, s, a.chunks[item - nl - nm]);
+		}
+	}
+}
+
+// Keys alone: the large and medium segments, sorted by unstable MSD
+// bucketing (the order of equal keys cannot be seen).  A large segment whose
+// keys crowd into part of their range is copied to the output unsorted and
+// listed for seg_sort_persistent (fb_mode), which runs the stable digit
+// passes on it in place.
+template <typename K, int IPT>
+__global__ __launch_bounds__(kRB, 3) void seg_msd_persistent(SegArgs<K> a) {
+	__shared__ SegSmem<K, false, IPT> s;
+	const u32 nl = a.meta->n_large;
+	const u32 total = nl + a.meta->n_medium;
+	for (;;) {
+		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head_msd, 1u);
+		__syncthreads();
+		const u32 item = __builtin_amdgcn_readfirstlane(s.misc[3]);
+		__syncthreads();
+		if (item >= total) break;
+		const u32 j = item < nl ? a.large[item] : a.medium[item - nl];
+		const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
+		const u32 start = __builtin_amdgcn_readfirstlane(sr.start);
+		const u32 len = __builtin_amdgcn_readfirstlane(sr.len);
+		if (item >= nl) {
+			sort_medium_msd<K, IPT>(a, s, start, len);
+			continue;
+		}
+		if (sort_large_msd<K, IPT>(a, s, start, len)) continue;
+		__syncthreads();
+		if (a.kout != a.kin) g2g<K, IPT>(a.kout + start, a.kin + start, len);
+		if (threadIdx.x == 0) a.fb[atomicAdd(&a.meta->n_fb, 1u)] = make_uint2(start, len);
+		__syncthreads();
 	}
 }
 
@@ -648,6 +1052,14 @@ bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, GError
 			const_cast<uint2*>(a.wrange), a.meta);
 		if (!L.done(err)) return false;
 	}
+	if constexpr (!KV && sizeof(K) == 8) {
+		// 64-bit keys alone: large and medium segments by MSD bucketing
+		// first.  (For 32-bit keys the four digit passes measured faster.)
+		Launch L(q, "segsort_msd");
+		seg_msd_persistent<K, IPT><<<grid, kRB, 0, st>>>(a);
+		if (!L.done(err)) return false;
+		a.fb_mode = 1;
+	}
 	{
 		Launch L(q, "segsort_persistent");
 		seg_sort_persistent<K, KV, IPT><<<grid, kRB, 0, st>>>(a);
@@ -674,7 +1086,7 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	// medium, large and chunk lists (nseg entries each), then [j0, j1) per
 	// window of the smallest tile's W = 1024 elements.
 	const size_t nwin = numel / 1024 + 2;
-	if (!sd->lists.ensure(3 * nseg * sizeof(u32) + 64 + nwin * sizeof(uint2), err) ||
+	if (!sd->lists.ensure(3 * nseg * sizeof(u32) + 64 + (nwin + nseg) * sizeof(uint2), err) ||
 		!sd->meta.ensure(sizeof(SegMeta), err) ||
 		!sd->kscr.ensure(numel * ks, err) || (vin && !sd->vscr.ensure(numel * 4, err)))
 		return nullptr;
@@ -683,7 +1095,7 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 		if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
 			cus <= 0)
 			cus = 256;
-		grid = static_cast<u32>(cus) * kSegBlocksPerCuDefault;
+		grid = static_cast<u32>(cus) * ((vin && ks == 8) ? kSegBlocksPerCuKv64 : kSegBlocksPerCuDefault);
 	}
 	u32* lists = static_cast<u32*>(sd->lists.ptr);
 	const bool kv = vin != nullptr;
@@ -705,6 +1117,8 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	a.large = lists + nseg;                                                                  \
 	a.chunks = lists + 2 * nseg;                                                             \
 	a.wrange = reinterpret_cast<const uint2*>(lists + (3 * nseg + 3) / 4 * 4 + 4);          \
+	a.fb = reinterpret_cast<uint2*>(lists + (3 * nseg + 3) / 4 * 4 + 4) + nwin;             \
+	a.fb_mode = 0;                                                                           \
 	a.meta = static_cast<SegMeta*>(sd->meta.ptr);                                            \
 	a.cls_lo = sorter->seg_cls[0];                                                           \
 	a.cls_hi = sorter->seg_cls[1];                                                           \

@@ -187,3 +187,70 @@ def test_config4_shape(clops, oracle, ctx, queue):
     for j in range(0, nseg, 89):
         s0, e0 = int(off[j]), int(ends[j])
         assert np.array_equal(res[s0:e0], oracle.scan(got[s0:e0], np.uint32)), j
+
+
+def _keys_for(oracle, dist, n, dtype, seed):
+    """Keys-only distributions for the unstable MSD bucketing paths: spread,
+    constant, two-valued, clustered with outliers (large buckets that need
+    further levels or the digit-pass fallback), many duplicates, heavy head,
+    ordered and reversed."""
+    bits = np.dtype(dtype).itemsize * 8
+    u = oracle.generate(oracle.GEN_U64, seed, n)
+    top = np.uint64((1 << bits) - 1) if bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    if dist == "uniform":
+        k = u
+    elif dist == "constant":
+        k = np.full(n, 0x1234567, dtype=np.uint64)
+    elif dist == "two":
+        k = np.where(u & np.uint64(1), top, np.uint64(0))
+    elif dist == "clustered":
+        # 99% of keys in [2^20, 2^20 + 4096), the rest anywhere
+        k = np.where(u % np.uint64(100) == 0, u, np.uint64(1 << 20) + (u >> np.uint64(40)) % np.uint64(4096))
+    elif dist == "dups":
+        k = u % np.uint64(97)
+    elif dist == "head":
+        # geometric-like: key = number of trailing zero bits of u (0..63)
+        k = np.zeros(n, dtype=np.uint64)
+        x = u.copy()
+        for _ in range(63):
+            m = (x & np.uint64(1)) == 0
+            k += m.astype(np.uint64)
+            x = np.where(m, x >> np.uint64(1), np.uint64(1))
+    elif dist == "sorted":
+        k = np.arange(n, dtype=np.uint64) * np.uint64(3)
+    elif dist == "reversed":
+        k = (np.uint64(n) - np.arange(n, dtype=np.uint64)) << np.uint64(bits - 24 if bits > 24 else 0)
+    else:
+        raise ValueError(dist)
+    return (k & top).astype(dtype)
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64])
+@pytest.mark.parametrize("dist", ["uniform", "constant", "two", "clustered", "dups", "head",
+                                  "sorted", "reversed"])
+@pytest.mark.parametrize("out_of_place", [False, True])
+def test_segmented_keys_only_distributions(clops, oracle, ctx, queue, tile, dtype, dist,
+                                           out_of_place):
+    """Keys alone take the MSD bucketing kernel for medium and large segments
+    (and the digit passes for large segments whose keys crowd into part of
+    their range): every distribution against the oracle's per-segment sort."""
+    if dtype == np.uint64 and tile == 32:
+        pytest.skip("64-bit keys take tiles of 8 or 16 items")
+    cap = 256 * tile
+    lens = np.array([0, 1, 5, cap // 2 + 3, cap - 1, cap, cap + 1, 3 * cap + 5, 70000, 2,
+                     cap * 2, 17, 40000], dtype=np.uint32)
+    off = _offsets(lens)
+    nseg, n = lens.size, int(lens.sum(dtype=np.uint64))
+    k = _keys_for(oracle, dist, n, dtype, 0xD15 + tile)
+    s = clops.Sorter(ctx, "satradix", dtype, options="segtile=%d" % tile)
+    kb = clops.Buffer.from_array(ctx, k)
+    ob = clops.Buffer.from_array(ctx, off)
+    ko = clops.Buffer(ctx, n * np.dtype(dtype).itemsize) if out_of_place else None
+    s.sort_segmented_device(queue, kb, ob, nseg, n, kout=ko)
+    queue.finish()
+    rk, _ = _ref_sort(oracle, k, None, off, n)
+    got = (ko if out_of_place else kb).read(queue, dtype, n)
+    assert np.array_equal(got, rk)
+    if out_of_place:
+        assert np.array_equal(kb.read(queue, dtype, n), k)  # input untouched
