@@ -175,14 +175,16 @@ bool type_is_signed(CloType t);
 
 // Segmented sort defaults: items per thread of the 256-thread tile and
 // blocks per CU of the persistent launch, measured on 32768 log-uniform
-// segments of up to 65536 keys.  Keys only: tile 8 (32-bit keys: 2.58 ms
-// against 2.65-2.71 with tile 16; 64-bit keys, whose medium and large
-// segments take the MSD bucketing kernel: 3.87 ms against 5.13).  Key-value
-// sorts: tile 16 for 32-bit keys (4.28 ms against 5.05 with tile 8), tile 8
-// for 64-bit keys (tile 16 spills registers).  The dispatcher's fixed
+// segments of up to 65536 keys.  32-bit keys: tile 16, from the dispatcher's
+// exhaustive tuner (clo_dispatch_tune_segmented; keys alone 2.52-2.78 ms
+// depending on the box, tile 8 within 3%; with values 4.28 ms against 5.05
+// for tile 8).  64-bit keys: tile 8, for keys alone (whose medium and large
+// segments take the MSD bucketing kernel: 3.87 ms against 5.13 for tile 16)
+// and with values (tile 16 spills registers).  The dispatcher's fixed
 // segmented configuration (keys only) is the same.
 constexpr uint32_t seg_default_tile(size_t key_bytes, bool kv = false) {
-	return (key_bytes == 4 && kv) ? 16u : 8u;
+	(void)kv;
+	return key_bytes == 8 ? 8u : 16u;
 }
 constexpr uint32_t kSegBlocksPerCuDefault = 4;
 // 64-bit key-value sorts: at most two blocks per CU (three resident blocks,
