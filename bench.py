@@ -512,6 +512,31 @@ def suite_segmented(ctx, q, baseline_stride=16):
     return res
 
 
+def segmented_only(ctx, width, steps, warmup):
+    """The configuration-4 segmented sort alone (keys of `width` bytes, out of
+    place, library defaults), timed and checked for per-segment order."""
+    q = clops.Queue(ctx)
+    lens, off = c4_offsets(ctx, q)
+    nseg = lens.size
+    n = int(lens.sum(dtype=np.uint64))
+    dt = np.uint32 if width == 4 else np.uint64
+    kin = clops.Buffer(ctx, n * width)
+    kout = clops.Buffer(ctx, n * width)
+    clops.generate(q, kin, n, clops.GEN_U32_HI if width == 4 else clops.GEN_U64, 0xC4B)
+    offb = clops.Buffer.from_array(ctx, off)
+    s = clops.Sorter(ctx, "satradix", dt)
+    ms = time_steps(q, lambda: s.sort_segmented_device(q, kin, offb, nseg, n, kout=kout),
+                    steps, warmup)
+    ko = kout.read(q, dt, n)
+    inner = np.ones(n - 1, dtype=bool)
+    inner[off[1:] - 1] = False
+    bad = int(np.count_nonzero((ko[1:] < ko[:-1]) & inner))
+    return {"config": "segmented sort, %d segments, %d keys of %d bytes" % (nseg, n, width),
+            "ms": ms, "Gkeys/s": n / ms / 1e6, "unsorted_pairs": bad,
+            "multiset_equal": clops.multiset_hash(q, kin, n, width) ==
+            clops.multiset_hash(q, kout, n, width)}
+
+
 def seg_tile(cfg):
     """Items per thread of segmented configuration cfg (12 + 4 t + g - 2)."""
     return (8, 16, 32)[(cfg - 12) // 4]
@@ -1018,6 +1043,9 @@ def main():
     ap.add_argument("--suite", action="store_true")
     ap.add_argument("--seg-tune", action="store_true",
                     help="only the segmented tuner and learned work classes, 32768 segments")
+    ap.add_argument("--segmented", type=int, default=0, metavar="KEY_BYTES",
+                    help="only the configuration-4 segmented sort, keys alone of 4 or 8 bytes "
+                         "(for profilers)")
     ap.add_argument("--no-dispatch", action="store_true")
     ap.add_argument("--dispatch-train", type=int, default=2000)
     ap.add_argument("--dispatch-eval", type=int, default=1500)
@@ -1035,6 +1063,9 @@ def main():
     if args.seg_tune:
         for r in seg_tune(ctx):
             print(json.dumps(r), flush=True)
+        return
+    if args.segmented:
+        print(json.dumps(segmented_only(ctx, args.segmented, args.steps, args.warmup)), flush=True)
         return
     h = headline(args, ctx)
     if args.suite:

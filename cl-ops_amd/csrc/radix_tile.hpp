@@ -1,7 +1,6 @@
 // One stable digit pass over one tile of consecutive keys: wave64-ballot
 // ranking, LDS staging in digit order and a scatter of contiguous runs to
-// per-digit output offsets.  Shared by the satradix downsweep and the MSD
-// partition levels (msd.hip).
+// per-digit output offsets (the satradix downsweep).
 #pragma once
 
 #include "dev_common.hpp"
