@@ -175,16 +175,15 @@ bool type_is_signed(CloType t);
 
 // Segmented sort defaults: items per thread of the 256-thread tile and
 // blocks per CU of the persistent launch, measured on 32768 log-uniform
-// segments of up to 65536 keys.  32-bit keys: tile 16, from the dispatcher's
-// exhaustive tuner (clo_dispatch_tune_segmented; keys alone 2.52-2.78 ms
-// depending on the box, tile 8 within 3%; with values 4.28 ms against 5.05
-// for tile 8).  64-bit keys: tile 8, for keys alone (whose medium and large
-// segments take the MSD bucketing kernel: 3.87 ms against 5.13 for tile 16)
-// and with values (tile 16 spills registers).  The dispatcher's fixed
-// segmented configuration (keys only) is the same.
+// segments of up to 65536 keys.  Keys alone (medium and large segments on the
+// MSD bucketing kernel): tile 8, 2.30 ms for 32-bit keys and 3.40 ms for
+// 64-bit keys, against 2.52-2.71 and 6.5 ms on the digit passes (tile 16,
+// the tuner's best there).  Key-value sorts (digit passes): tile 16 for
+// 32-bit keys (4.28 ms against 5.05 for tile 8), tile 8 for 64-bit keys
+// (tile 16 spills registers).  The dispatcher's fixed segmented
+// configuration (keys only) is the same.
 constexpr uint32_t seg_default_tile(size_t key_bytes, bool kv = false) {
-	(void)kv;
-	return key_bytes == 8 ? 8u : 16u;
+	return (kv && key_bytes == 4) ? 16u : 8u;
 }
 constexpr uint32_t kSegBlocksPerCuDefault = 4;
 // 64-bit key-value sorts: at most two blocks per CU (three resident blocks,
@@ -210,6 +209,9 @@ struct clo_sort {
 	uint32_t seg_tile;
 	uint32_t seg_grid;
 	uint64_t seg_cls[2];
+	// Keys-only segmented sorts: 1 = MSD bucketing kernel for medium and large
+	// segments, 0 = digit passes only, -1 = default (1).
+	int32_t seg_msd;
 	void* seg_cache;
 	// Run-time compiled kernels for user compare / key snippets (NULL when
 	// the ahead-of-time kernels serve the sorter).

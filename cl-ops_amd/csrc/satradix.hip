@@ -865,6 +865,13 @@ const char* satradix_init(CloSort* sorter, const char* options, GError** err) {
 				return nullptr;
 			}
 			sorter->seg_grid = static_cast<u32>(g);
+		} else if (o.key == "segmsd") {
+			const long m = atol(o.value.c_str());
+			if (o.value.empty() || (m != 0 && m != 1)) {
+				set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmsd must be 0 or 1.");
+				return nullptr;
+			}
+			sorter->seg_msd = static_cast<int32_t>(m);
 		} else if (o.key == "segclass") {
 			// One digit per length bin from bin 0: the least work class
 			// (0 chunk, 1 medium, 2 large); bins past the string keep 0.

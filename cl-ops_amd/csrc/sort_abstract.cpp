@@ -46,6 +46,7 @@ CloSort* clo_sort_new(const char* type, const char* options, CCLContext* ctx,
 	s->seg_tile = 0;
 	s->seg_grid = 0;
 	s->seg_cls[0] = s->seg_cls[1] = 0;
+	s->seg_msd = -1;
 	s->seg_cache = nullptr;
 	s->snip = nullptr;
 	s->host_cache = nullptr;

@@ -230,11 +230,13 @@ def _keys_for(oracle, dist, n, dtype, seed):
 @pytest.mark.parametrize("dist", ["uniform", "constant", "two", "clustered", "dups", "head",
                                   "sorted", "reversed"])
 @pytest.mark.parametrize("out_of_place", [False, True])
+@pytest.mark.parametrize("msd", [0, 1])
 def test_segmented_keys_only_distributions(clops, oracle, ctx, queue, tile, dtype, dist,
-                                           out_of_place):
-    """Keys alone take the MSD bucketing kernel for medium and large segments
-    (and the digit passes for large segments whose keys crowd into part of
-    their range): every distribution against the oracle's per-segment sort."""
+                                           out_of_place, msd):
+    """Keys alone, with the MSD bucketing kernel for medium and large segments
+    (segmsd=1; large segments whose keys crowd into part of their range go on
+    to the digit passes) or the digit passes only (segmsd=0): every
+    distribution against the oracle's per-segment sort."""
     if dtype == np.uint64 and tile == 32:
         pytest.skip("64-bit keys take tiles of 8 or 16 items")
     cap = 256 * tile
@@ -243,7 +245,7 @@ def test_segmented_keys_only_distributions(clops, oracle, ctx, queue, tile, dtyp
     off = _offsets(lens)
     nseg, n = lens.size, int(lens.sum(dtype=np.uint64))
     k = _keys_for(oracle, dist, n, dtype, 0xD15 + tile)
-    s = clops.Sorter(ctx, "satradix", dtype, options="segtile=%d" % tile)
+    s = clops.Sorter(ctx, "satradix", dtype, options="segtile=%d,segmsd=%d" % (tile, msd))
     kb = clops.Buffer.from_array(ctx, k)
     ob = clops.Buffer.from_array(ctx, off)
     ko = clops.Buffer(ctx, n * np.dtype(dtype).itemsize) if out_of_place else None
