@@ -512,9 +512,10 @@ def suite_segmented(ctx, q, baseline_stride=16):
     return res
 
 
-def segmented_only(ctx, width, steps, warmup):
+def segmented_only(ctx, width, steps, warmup, options=None):
     """The configuration-4 segmented sort alone (keys of `width` bytes, out of
-    place, library defaults), timed and checked for per-segment order."""
+    place, library defaults or the given sorter options), timed and checked
+    for per-segment order."""
     q = clops.Queue(ctx)
     lens, off = c4_offsets(ctx, q)
     nseg = lens.size
@@ -524,14 +525,15 @@ def segmented_only(ctx, width, steps, warmup):
     kout = clops.Buffer(ctx, n * width)
     clops.generate(q, kin, n, clops.GEN_U32_HI if width == 4 else clops.GEN_U64, 0xC4B)
     offb = clops.Buffer.from_array(ctx, off)
-    s = clops.Sorter(ctx, "satradix", dt)
+    s = clops.Sorter(ctx, "satradix", dt, options=options)
     ms = time_steps(q, lambda: s.sort_segmented_device(q, kin, offb, nseg, n, kout=kout),
                     steps, warmup)
     ko = kout.read(q, dt, n)
     inner = np.ones(n - 1, dtype=bool)
     inner[off[1:] - 1] = False
     bad = int(np.count_nonzero((ko[1:] < ko[:-1]) & inner))
-    return {"config": "segmented sort, %d segments, %d keys of %d bytes" % (nseg, n, width),
+    return {"config": "segmented sort, %d segments, %d keys of %d bytes%s" % (
+                nseg, n, width, ", options " + options if options else ""),
             "ms": ms, "Gkeys/s": n / ms / 1e6, "unsorted_pairs": bad,
             "multiset_equal": clops.multiset_hash(q, kin, n, width) ==
             clops.multiset_hash(q, kout, n, width)}
@@ -1046,6 +1048,8 @@ def main():
     ap.add_argument("--segmented", type=int, default=0, metavar="KEY_BYTES",
                     help="only the configuration-4 segmented sort, keys alone of 4 or 8 bytes "
                          "(for profilers)")
+    ap.add_argument("--seg-options", default=None,
+                    help="sorter options for --segmented (e.g. segmsd=0)")
     ap.add_argument("--no-dispatch", action="store_true")
     ap.add_argument("--dispatch-train", type=int, default=2000)
     ap.add_argument("--dispatch-eval", type=int, default=1500)
@@ -1065,7 +1069,8 @@ def main():
             print(json.dumps(r), flush=True)
         return
     if args.segmented:
-        print(json.dumps(segmented_only(ctx, args.segmented, args.steps, args.warmup)), flush=True)
+        print(json.dumps(segmented_only(ctx, args.segmented, args.steps, args.warmup,
+                                        args.seg_options)), flush=True)
         return
     h = headline(args, ctx)
     if args.suite:
