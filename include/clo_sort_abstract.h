@@ -115,10 +115,14 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
  * Malformed offsets (decreasing, or beyond numel) are clamped into the
  * arrays: the order is then unspecified but no access leaves the buffers.
  * Sorter options tune it: segtile=8|16|32 (LDS tile = 256*segtile elements;
- * 8|16 for 64-bit keys; default 16, and 8 for 64-bit key-value sorts),
- * seggrid=<blocks> (0 = four per compute unit), segclass=<up to 64 digits 0|1|2> (the least work class of each
- * segment length bin, see clo_dispatch.h; default all 0).  No option changes
- * the result. */
+ * 8|16 for 64-bit keys; default 8, and 16 for 32-bit key-value sorts),
+ * seggrid=<blocks> (0 = four per compute unit, two for 64-bit key-value
+ * sorts), segclass=<up to 64 digits 0|1|2> (the least work class of each
+ * segment length bin, see clo_dispatch.h; default all 0), segmsd=0|1 (keys
+ * alone: 1, the default, sorts medium and large segments by MSD bucketing,
+ * whose order of equal keys is not the input order but whose output words
+ * are the same, since equal keys are equal bit patterns; 0: stable digit
+ * passes only).  No option changes the result. */
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
 	CCLBuffer* vals_out, CCLBuffer* offsets, size_t nseg, size_t numel, GError** err);
