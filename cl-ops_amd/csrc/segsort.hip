@@ -154,17 +154,9 @@ struct SegSmem {
 	static constexpr u32 kW = kCap / 2;
 	K k[kCap];
 	u32 v[KV ? kCap : 1];
-	union {
-		struct {  // chunk items
-			u16 sid[kCap];
-			u32 lpos[kW + 1];
-			u32 segj[kW];
-		};
-		struct {  // keys-only tiles
-			u32 mcnt[kCap];          // bucket counts, then cursors
-			uint2 mstack[kCap / 32]; // pending buckets of more than 32 keys
-		};
-	};
+	u16 sid[kCap];     // chunk items: each key's segment in the window,
+	u32 lpos[kW + 1];  // each segment's start in the tile
+	u32 segj[kW];      // and in the array
 	u32 hist[sizeof(K)][256];
 	u32 base[256];
 	u32 gofs[256];
@@ -327,12 +319,77 @@ __device__ __forceinline__ u32 top_bit(K x) {
 		: 31u - static_cast<u32>(__builtin_clz(static_cast<u32>(x)));
 }
 
+// LDS of the keys-only bucketing kernel.  k holds the tile; mcnt the bucket
+// counts, then cursors; mstack the ranges waiting for a level of their own.
+// Large segments use hist/base/gofs (bucket sizes and starts, window starts,
+// chunk cursors), chunk items sid/lpos/segj (gather and write-out) and
+// smin/sshift (each small segment's minimum key and bucket shift).
+template <typename K, int IPT>
+struct MsdSmem {
+	static constexpr u32 kCap = kRB * IPT;
+	static constexpr u32 kW = kCap / 2;
+	K k[kCap];
+	u32 mcnt[kCap];
+	uint2 mstack[kCap / 32];
+	union {
+		struct {  // large segments
+			u32 hist[4][256];
+			u32 base[256];
+			u32 gofs[256];
+		};
+		struct {  // chunk items
+			u16 sid[kCap];
+			u32 lpos[kW + 1];
+			u32 segj[kW];
+			K smin[kW];
+			u8 sshift[kW];
+		};
+	};
+	K red[2 * kRBWaves];
+	u32 misc[4];
+	u32 wsum[kRBWaves];
+};
+
+// Sorts k[b0, b0 + m), 2 <= m <= kMsdMaxBucket, by one thread: a 4-input
+// sorting network in registers for m <= 4 (absent keys as the largest
+// value), insertion otherwise.
+template <typename K>
+__device__ __forceinline__ void msd_small_bucket(K* k, u32 b0, u32 m) {
+	if (m <= 4) {
+		K x0 = k[b0], x1 = k[b0 + 1];
+		K x2 = m > 2 ? k[b0 + 2] : ~K(0);
+		K x3 = m > 3 ? k[b0 + 3] : ~K(0);
+		K t;
+		t = x0 < x1 ? x0 : x1; x1 = x0 < x1 ? x1 : x0; x0 = t;
+		t = x2 < x3 ? x2 : x3; x3 = x2 < x3 ? x3 : x2; x2 = t;
+		t = x0 < x2 ? x0 : x2; x2 = x0 < x2 ? x2 : x0; x0 = t;
+		t = x1 < x3 ? x1 : x3; x3 = x1 < x3 ? x3 : x1; x1 = t;
+		t = x1 < x2 ? x1 : x2; x2 = x1 < x2 ? x2 : x1; x1 = t;
+		k[b0] = x0;
+		k[b0 + 1] = x1;
+		if (m > 2) k[b0 + 2] = x2;
+		if (m > 3) k[b0 + 3] = x3;
+		return;
+	}
+	for (u32 i = b0 + 1; i < b0 + m; ++i) {
+		const K x = k[i];
+		u32 jj = i;
+		while (jj > b0) {
+			const K y = k[jj - 1];
+			if (!(x < y)) break;
+			k[jj] = y;
+			--jj;
+		}
+		k[jj] = x;
+	}
+}
+
 // One level over s.k[base, base + m), m <= kCap.  Contains barriers: call from
 // uniform control flow.  Pushes the buckets of more than kMsdMaxBucket keys
 // onto s.mstack (count in s.misc[2]).
 template <typename K, int IPT>
-__device__ __forceinline__ void msd_level(SegSmem<K, false, IPT>& s, u32 base, u32 m) {
-	using S = SegSmem<K, false, IPT>;
+__device__ __forceinline__ void msd_level(MsdSmem<K, IPT>& s, u32 base, u32 m) {
+	using S = MsdSmem<K, IPT>;
 	constexpr u32 NB = S::kCap;  // buckets (a power of two)
 	constexpr u32 WB = static_cast<u32>(__builtin_ctz(NB));  // bucket bits
 	constexpr u32 BPT = NB / kRB;  // buckets per thread in the scan
@@ -361,7 +418,7 @@ __device__ __forceinline__ void msd_level(SegSmem<K, false, IPT>& s, u32 base, u
 		lo = l2 < lo ? l2 : lo;
 		hi = h2 > hi ? h2 : hi;
 	}
-	K* red = reinterpret_cast<K*>(s.hist);
+	K* red = s.red;
 	if (lane == 0) {
 		red[w] = lo;
 		red[kRBWaves + w] = hi;
@@ -401,7 +458,7 @@ __device__ __forceinline__ void msd_level(SegSmem<K, false, IPT>& s, u32 base, u
 	u32 tot = 0;
 #pragma unroll
 	for (u32 i = 0; i < BPT; ++i) tot += c[i];
-	u32 run = base + block_exclusive_sum<kRB>(tot, s.rs.wsum, static_cast<u32*>(nullptr));
+	u32 run = base + block_exclusive_sum<kRB>(tot, s.wsum, static_cast<u32*>(nullptr));
 #pragma unroll
 	for (u32 i = 0; i < BPT; ++i) {
 		const u32 x = c[i];
@@ -434,35 +491,7 @@ __device__ __forceinline__ void msd_level(SegSmem<K, false, IPT>& s, u32 base, u
 			s.mstack[slot] = make_uint2(b0, m);
 			continue;
 		}
-		if (m <= 4) {
-			// Most buckets: the keys in registers, a 4-input sorting network
-			// (absent keys as the largest value), written back.
-			K x0 = s.k[b0], x1 = s.k[b0 + 1];
-			K x2 = m > 2 ? s.k[b0 + 2] : ~K(0);
-			K x3 = m > 3 ? s.k[b0 + 3] : ~K(0);
-			K t;
-			t = x0 < x1 ? x0 : x1; x1 = x0 < x1 ? x1 : x0; x0 = t;
-			t = x2 < x3 ? x2 : x3; x3 = x2 < x3 ? x3 : x2; x2 = t;
-			t = x0 < x2 ? x0 : x2; x2 = x0 < x2 ? x2 : x0; x0 = t;
-			t = x1 < x3 ? x1 : x3; x3 = x1 < x3 ? x3 : x1; x1 = t;
-			t = x1 < x2 ? x1 : x2; x2 = x1 < x2 ? x2 : x1; x1 = t;
-			s.k[b0] = x0;
-			s.k[b0 + 1] = x1;
-			if (m > 2) s.k[b0 + 2] = x2;
-			if (m > 3) s.k[b0 + 3] = x3;
-			continue;
-		}
-		for (u32 i = b0 + 1; i < e; ++i) {
-			const K x = s.k[i];
-			u32 jj = i;
-			while (jj > b0) {
-				const K y = s.k[jj - 1];
-				if (!(x < y)) break;
-				s.k[jj] = y;
-				--jj;
-			}
-			s.k[jj] = x;
-		}
+		msd_small_bucket<K>(s.k, b0, m);
 	}
 	__syncthreads();
 }
@@ -470,7 +499,7 @@ __device__ __forceinline__ void msd_level(SegSmem<K, false, IPT>& s, u32 base, u
 // Sorts s.k[0, n), n <= kCap, keys alone.  The stack holds disjoint ranges
 // of more than kMsdMaxBucket keys, so it never exceeds kCap / 33 entries.
 template <typename K, int IPT>
-__device__ __forceinline__ void msd_tile_sort(SegSmem<K, false, IPT>& s, u32 n) {
+__device__ __forceinline__ void msd_tile_sort(MsdSmem<K, IPT>& s, u32 n) {
 	if (threadIdx.x == 0) s.misc[2] = 0;
 	__syncthreads();
 	msd_level<K, IPT>(s, 0, n);
@@ -668,9 +697,9 @@ __device__ __forceinline__ void large_chunk(SegSmem<K, KV, IPT>& s, const K (&k)
 // concentrated in a small part of their range): the segment is then left to
 // the stable digit passes.
 template <typename K, int IPT>
-__device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, SegSmem<K, false, IPT>& s,
+__device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, IPT>& s,
 	u32 start, u32 len) {
-	using S = SegSmem<K, false, IPT>;
+	using S = MsdSmem<K, IPT>;
 	constexpr u32 CAP = S::kCap;
 	constexpr u32 HALF = CAP / 2;
 	const u32 tid = threadIdx.x;
@@ -708,7 +737,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, SegSmem<K, f
 		lo = l2 < lo ? l2 : lo;
 		hi = h2 > hi ? h2 : hi;
 	}
-	K* red = reinterpret_cast<K*>(s.hist[1]);
+	K* red = s.red;
 	if (lane == 0) {
 		red[w] = lo;
 		red[kRBWaves + w] = hi;
@@ -755,7 +784,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, SegSmem<K, f
 		if (hcnt[tid] > HALF) atomicOr(&s.misc[2], 1u);
 	}
 	const u32 c = hcnt[tid];
-	const u32 b0 = block_exclusive_sum<kRB>(c, s.rs.wsum, static_cast<u32*>(nullptr));
+	const u32 b0 = block_exclusive_sum<kRB>(c, s.wsum, static_cast<u32*>(nullptr));
 	if (s.misc[2] != 0) return false;  // read after the barriers above: uniform
 	s.base[tid] = b0;  // next free position of the bucket in the scratch array
 	hcnt[tid] = b0;    // bucket starts
@@ -766,7 +795,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, SegSmem<K, f
 		const u32 prev = tid == 0 ? 0u : hcnt[tid - 1];
 		const bool op = tid == 0 || (b0 / HALF) != (prev / HALF);
 		u32 nw = 0;
-		const u32 idx = block_exclusive_sum<kRB>(op ? 1u : 0u, s.rs.wsum, &nw);
+		const u32 idx = block_exclusive_sum<kRB>(op ? 1u : 0u, s.wsum, &nw);
 		if (op) wbeg[idx] = b0;
 		if (tid == 0) s.misc[0] = nw;
 	}
@@ -804,7 +833,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, SegSmem<K, f
 			if (jj * kRB + tid < cnt) atomicAdd(&ccnt[static_cast<u32>((k[jj] - bbase) >> shift)], 1u);
 		__syncthreads();
 		const u32 cc = ccnt[tid];
-		const u32 cs = block_exclusive_sum<kRB>(cc, s.rs.wsum, static_cast<u32*>(nullptr));
+		const u32 cs = block_exclusive_sum<kRB>(cc, s.wsum, static_cast<u32*>(nullptr));
 		ccnt[tid] = cs;                 // chunk cursor of bucket tid
 		s.gofs[tid] = s.base[tid] - cs;  // staged element i of bucket d goes to gofs[d] + i
 		__syncthreads();
@@ -844,12 +873,194 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, SegSmem<K, f
 
 // Medium segment, keys alone: whole segment in LDS, msd_tile_sort.
 template <typename K, int IPT>
-__device__ __forceinline__ void sort_medium_msd(const SegArgs<K>& a, SegSmem<K, false, IPT>& s,
+__device__ __forceinline__ void sort_medium_msd(const SegArgs<K>& a, MsdSmem<K, IPT>& s,
 	u32 start, u32 len) {
 	g2s<K, IPT>(s.k, a.kin + start, len);
 	__syncthreads();
 	if (len > 1) msd_tile_sort<K, IPT>(s, len);
 	s2g<K, IPT>(a.kout + start, s.k, len);
+	__syncthreads();
+}
+
+// Chunk of small segments, keys alone: the segments starting in
+// [c*W, (c+1)*W) gathered into one tile as in sort_chunk, then one segmented
+// bucketing level: segment r (at tile positions [lpos[r], lpos[r+1]), n_r
+// keys) owns buckets lpos[r] + ((x - min_r) >> shift_r), at most n_r of them,
+// so every key stays inside its segment's range and one pass of counts,
+// cursors and small-bucket sorts orders every segment at once; buckets over
+// kMsdMaxBucket keys get levels of their own as in msd_tile_sort.
+template <typename K, int IPT>
+__device__ __forceinline__ void sort_chunk_msd(const SegArgs<K>& a, MsdSmem<K, IPT>& s, u32 c) {
+	using S = MsdSmem<K, IPT>;
+	constexpr u32 W = S::kW;
+	constexpr u32 CAP = S::kCap;
+	const u32 tid = threadIdx.x;
+	if (tid == 0) {
+		const uint2 r = a.wrange[c];
+		s.misc[0] = r.x;
+		s.misc[1] = r.y > r.x ? r.y - r.x : 0u;  // offsets that are not monotone: none
+	}
+	__syncthreads();
+	const u32 j0 = s.misc[0];
+	const u32 mall = s.misc[1];
+	// The window's non-empty small segments: segj[r] = start in the array,
+	// lpos[r] = start in the tile (at most W of them, CAP keys in all).
+	u32 carry = 0, m = 0;
+	for (u32 r0 = 0; r0 < mall; r0 += kRB) {
+		const u32 r = r0 + tid;
+		u32 len = 0, start = 0;
+		const u32 j = j0 + r;
+		if (r < mall) {
+			const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
+			start = sr.start;
+			len = (sr.len > W || seg_class(a.cls_lo, a.cls_hi, sr.len, W, CAP) != 0) ? 0u
+				: sr.len;
+		}
+		u32 tot, cnt;
+		const u32 ex = block_exclusive_sum<kRB>(len, s.wsum, &tot);
+		const u32 exc = block_exclusive_sum<kRB>(len ? 1u : 0u, s.wsum, &cnt);
+		if (len && m + exc < W) {
+			s.segj[m + exc] = start;
+			s.lpos[m + exc] = carry + ex;
+		}
+		carry += tot;
+		m += cnt;
+	}
+	if (tid == 0) s.lpos[m < W ? m : W] = carry;
+	__syncthreads();
+	const u32 n = (m > W || carry > CAP) ? 0u : carry;
+	if (n == 0) return;
+	// Gather (segment of element i by binary search over lpos) and each
+	// segment's minimum and maximum (the maximum kept in mcnt, unused yet).
+	K* smax = reinterpret_cast<K*>(s.mcnt);  // W keys fit in CAP words
+	for (u32 r = tid; r < m; r += kRB) {
+		s.smin[r] = ~K(0);
+		smax[r] = K(0);
+	}
+	__syncthreads();
+	constexpr int B = IPT < kBatch ? IPT : kBatch;
+#pragma unroll
+	for (int r0 = 0; r0 < IPT; r0 += B) {
+		u32 gidx[B];
+		u32 gsid[B];
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const u32 i = (r0 + r) * kRB + tid;
+			u32 l = 0;
+#pragma unroll
+			for (int b = 12; b >= 0; --b) {
+				const u32 cc = l + (1u << b);
+				if (cc < m && s.lpos[cc] <= i) l = cc;
+			}
+			gsid[r] = l;
+			gidx[r] = s.segj[l] + (i - s.lpos[l]);
+		}
+		K tk[B];
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const bool in = (r0 + r) * kRB + tid < n;
+			tk[r] = in ? a.kin[gidx[r]] : K(0);
+		}
+#pragma unroll
+		for (int r = 0; r < B; ++r) {
+			const u32 i = (r0 + r) * kRB + tid;
+			if (i < n) {
+				s.k[i] = tk[r];
+				s.sid[i] = static_cast<u16>(gsid[r]);
+				atomicMin(&s.smin[gsid[r]], tk[r]);
+				atomicMax(&smax[gsid[r]], tk[r]);
+			}
+		}
+	}
+	__syncthreads();
+	// Bucket shift of segment r: (max - min) >> shift below the largest power
+	// of two not above its length.
+	for (u32 r = tid; r < m; r += kRB) {
+		const u32 nr = s.lpos[r + 1] - s.lpos[r];
+		const K range = smax[r] - s.smin[r];
+		const u32 lb = 31u - static_cast<u32>(__builtin_clz(nr));  // nr >= 1
+		u32 sh = 0;
+		if (range != 0) {
+			const u32 hb = top_bit(range);
+			sh = hb + 1 > lb ? hb + 1 - lb : 0u;
+		}
+		s.sshift[r] = static_cast<u8>(sh);
+	}
+	__syncthreads();
+	for (u32 i = tid; i < n; i += kRB) s.mcnt[i] = 0;
+	if (tid == 0) s.misc[2] = 0;
+	__syncthreads();
+	// Counts, cursors, scatter: every key stays in its segment's range.
+	K k[IPT];
+	u32 bk[IPT];
+#pragma unroll
+	for (int j = 0; j < IPT; ++j) {
+		const u32 i = j * kRB + tid;
+		bk[j] = 0;
+		if (i < n) {
+			k[j] = s.k[i];
+			const u32 r = s.sid[i];
+			bk[j] = s.lpos[r] + static_cast<u32>((k[j] - s.smin[r]) >> s.sshift[r]);
+			atomicAdd(&s.mcnt[bk[j]], 1u);
+		}
+	}
+	__syncthreads();
+	{
+		constexpr u32 BPT = CAP / kRB;  // buckets per thread (= IPT)
+		u32 cnt[BPT];
+		u32 tot = 0;
+#pragma unroll
+		for (u32 q = 0; q < BPT; ++q) {
+			const u32 b = tid * BPT + q;
+			cnt[q] = b < n ? s.mcnt[b] : 0u;
+			tot += cnt[q];
+		}
+		u32 run = block_exclusive_sum<kRB>(tot, s.wsum, static_cast<u32*>(nullptr));
+#pragma unroll
+		for (u32 q = 0; q < BPT; ++q) {
+			const u32 b = tid * BPT + q;
+			if (b < n) s.mcnt[b] = run;
+			run += cnt[q];
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < IPT; ++j)
+		if (j * kRB + tid < n) s.k[atomicAdd(&s.mcnt[bk[j]], 1u)] = k[j];
+	__syncthreads();
+	// Buckets of 2..4 keys by a sorting network, up to kMsdMaxBucket by
+	// insertion, larger ones on the stack.
+	for (u32 b = tid; b < n; b += kRB) {
+		const u32 e = s.mcnt[b];
+		const u32 b0 = b == 0 ? 0u : s.mcnt[b - 1];
+		const u32 mb = e - b0;
+		if (mb < 2) continue;
+		if (mb > kMsdMaxBucket) {
+			const u32 slot = atomicAdd(&s.misc[2], 1u);
+			s.mstack[slot] = make_uint2(b0, mb);
+			continue;
+		}
+		msd_small_bucket<K>(s.k, b0, mb);
+	}
+	__syncthreads();
+	for (;;) {
+		const u32 sz = s.misc[2];
+		if (sz == 0) break;
+		const uint2 r = s.mstack[sz - 1];
+		__syncthreads();
+		if (tid == 0) s.misc[2] = sz - 1;
+		msd_level<K, IPT>(s, r.x, r.y);
+		__syncthreads();
+	}
+	// Write out: element i of segment r goes to segj[r] + (i - lpos[r]).
+#pragma unroll
+	for (int j = 0; j < IPT; ++j) {
+		const u32 i = j * kRB + tid;
+		if (i < n) {
+			const u32 r = s.sid[i];
+			a.kout[s.segj[r] + (i - s.lpos[r])] = s.k[i];
+		}
+	}
 	__syncthreads();
 }
 
@@ -993,13 +1204,13 @@ template <typename K, bool KV, int IPT>
 __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_persistent(
 	SegArgs<K> a) {
 	__shared__ SegSmem<K, KV, IPT> s;
-	// fb_mode (keys alone, after seg_msd_persistent): the large and medium
-	// segments are done except those in the fallback list, which sit in the
-	// output array and are sorted there in place; then the chunk items.
+	// fb_mode (keys alone, after seg_msd_persistent): every segment is done
+	// except those in the fallback list, which sit in the output array and
+	// are sorted there in place.
 	const bool fb = !KV && a.fb_mode;
 	const u32 nl = fb ? a.meta->n_fb : a.meta->n_large;
 	const u32 nm = fb ? 0u : a.meta->n_medium;
-	const u32 total = nl + nm + a.meta->n_chunks;
+	const u32 total = nl + nm + (fb ? 0u : a.meta->n_chunks);
 	for (;;) {
 		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head, 1u);
 		__syncthreads();
@@ -1036,22 +1247,27 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 	}
 }
 
-// Keys alone: the large and medium segments, sorted by unstable MSD
-// bucketing (the order of equal keys cannot be seen).  A large segment whose
-// keys crowd into part of their range is copied to the output unsorted and
-// listed for seg_sort_persistent (fb_mode), which runs the stable digit
-// passes on it in place.
+// Keys alone: every segment, sorted by unstable MSD bucketing (the order of
+// equal keys cannot be seen): large segments, then medium ones, then the
+// chunk items.  A large segment whose keys crowd into part of their range is
+// copied to the output unsorted and listed for seg_sort_persistent (fb_mode),
+// which runs the stable digit passes on it in place.
 template <typename K, int IPT>
 __global__ __launch_bounds__(kRB, 3) void seg_msd_persistent(SegArgs<K> a) {
-	__shared__ SegSmem<K, false, IPT> s;
+	__shared__ MsdSmem<K, IPT> s;
 	const u32 nl = a.meta->n_large;
-	const u32 total = nl + a.meta->n_medium;
+	const u32 nlm = nl + a.meta->n_medium;
+	const u32 total = nlm + a.meta->n_chunks;
 	for (;;) {
 		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head_msd, 1u);
 		__syncthreads();
 		const u32 item = __builtin_amdgcn_readfirstlane(s.misc[3]);
 		__syncthreads();
 		if (item >= total) break;
+		if (item >= nlm) {
+			sort_chunk_msd<K, IPT>(a, s, a.chunks[item - nlm]);
+			continue;
+		}
 		const u32 j = item < nl ? a.large[item] : a.medium[item - nl];
 		const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
 		const u32 start = __builtin_amdgcn_readfirstlane(sr.start);
