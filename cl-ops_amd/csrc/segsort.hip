@@ -784,16 +784,28 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 		if (hcnt[tid] > HALF) atomicOr(&s.misc[2], 1u);
 	}
 	const u32 c = hcnt[tid];
+	u32 cmax = c;
+#pragma unroll
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		const u32 o = __shfl_xor(cmax, off, kWave);
+		cmax = o > cmax ? o : cmax;
+	}
+	if (tid == 0) s.misc[1] = 0;
 	const u32 b0 = block_exclusive_sum<kRB>(c, s.wsum, static_cast<u32*>(nullptr));
 	if (s.misc[2] != 0) return false;  // read after the barriers above: uniform
+	if (lane == 0) atomicMax(&s.misc[1], cmax);
 	s.base[tid] = b0;  // next free position of the bucket in the scratch array
 	hcnt[tid] = b0;    // bucket starts
 	__syncthreads();
 	{
 		// Bucket tid opens a window when its start lies in another range of
-		// HALF positions than its predecessor's.
+		// span = CAP - (largest bucket) positions than its predecessor's: a
+		// window's buckets start within span positions of each other and the
+		// last is at most CAP - span keys, so the window holds fewer than CAP
+		// keys (span >= CAP/2, as no bucket exceeds CAP/2).
+		const u32 span = CAP - s.misc[1];
 		const u32 prev = tid == 0 ? 0u : hcnt[tid - 1];
-		const bool op = tid == 0 || (b0 / HALF) != (prev / HALF);
+		const bool op = tid == 0 || (b0 / span) != (prev / span);
 		u32 nw = 0;
 		const u32 idx = block_exclusive_sum<kRB>(op ? 1u : 0u, s.wsum, &nw);
 		if (op) wbeg[idx] = b0;
@@ -855,14 +867,35 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 		ccnt[tid] = 0;
 		__syncthreads();
 	}
-	// Windows: load, sort in LDS, write to the output.
+	// Windows: load (the next window's keys in flight during the sort), sort
+	// in LDS, write to the output.
 	const u32 nw = s.misc[0];
 	K* __restrict__ out = a.kout + start;
+	u32 p1 = wbeg[0];
+	u32 n1 = (nw > 1 ? wbeg[1] : len) - p1;
+	K kw[IPT];
+#pragma unroll
+	for (int jj = 0; jj < IPT; ++jj) {
+		const u32 e = jj * kRB + tid;
+		kw[jj] = e < n1 ? scr[p1 + e] : K(0);
+	}
 	for (u32 wi = 0; wi < nw; ++wi) {
-		const u32 p0 = wbeg[wi];
-		const u32 n = (wi + 1 < nw ? wbeg[wi + 1] : len) - p0;
-		if (n == 0) continue;  // uniform: wbeg is read by every thread
-		g2s<K, IPT>(s.k, scr + p0, n);
+		const u32 p0 = p1;
+		const u32 n = n1;
+#pragma unroll
+		for (int jj = 0; jj < IPT; ++jj) {
+			const u32 e = jj * kRB + tid;
+			if (e < n) s.k[e] = kw[jj];
+		}
+		if (wi + 1 < nw) {
+			p1 = wbeg[wi + 1];
+			n1 = (wi + 2 < nw ? wbeg[wi + 2] : len) - p1;
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj) {
+				const u32 e = jj * kRB + tid;
+				kw[jj] = e < n1 ? scr[p1 + e] : K(0);
+			}
+		}
 		__syncthreads();
 		if (n > 1) msd_tile_sort<K, IPT>(s, n);
 		s2g<K, IPT>(out + p0, s.k, n);
