@@ -306,8 +306,9 @@ __device__ void sort_key_passes(SegSmem<K, KV, IPT>& s, u32 n, bool carry_sid) {
 // in bucket (x - min) >> shift, with shift such that the key range spans
 // kCap buckets (one key per bucket on average for keys spread over their
 // range), scatters the keys to their buckets with LDS atomic cursors and
-// insertion-sorts every bucket of 2..kMsdMaxBucket keys.  Larger buckets go
-// on a stack in LDS and get a level of their own; each level narrows the key
+// moves every key of a bucket of 2..kMsdMaxBucket keys to its rank in the
+// bucket (msd_place).  Larger buckets go on a stack in LDS and get a level
+// of their own; each level narrows the key
 // range by a factor kCap, so every range ends sorted, and a range of equal
 // keys ends at once.  A level costs a few VALU operations per key, where a
 // stable digit pass costs about 50 (wave ballots over 8 digit bits).
@@ -350,38 +351,59 @@ struct MsdSmem {
 	u32 wsum[kRBWaves];
 };
 
-// Sorts k[b0, b0 + m), 2 <= m <= kMsdMaxBucket, by one thread: a 4-input
-// sorting network in registers for m <= 4 (absent keys as the largest
-// value), insertion otherwise.
-template <typename K>
-__device__ __forceinline__ void msd_small_bucket(K* k, u32 b0, u32 m) {
-	if (m <= 4) {
-		K x0 = k[b0], x1 = k[b0 + 1];
-		K x2 = m > 2 ? k[b0 + 2] : ~K(0);
-		K x3 = m > 3 ? k[b0 + 3] : ~K(0);
-		K t;
-		t = x0 < x1 ? x0 : x1; x1 = x0 < x1 ? x1 : x0; x0 = t;
-		t = x2 < x3 ? x2 : x3; x3 = x2 < x3 ? x3 : x2; x2 = t;
-		t = x0 < x2 ? x0 : x2; x2 = x0 < x2 ? x2 : x0; x0 = t;
-		t = x1 < x3 ? x1 : x3; x3 = x1 < x3 ? x3 : x1; x1 = t;
-		t = x1 < x2 ? x1 : x2; x2 = x1 < x2 ? x2 : x1; x1 = t;
-		k[b0] = x0;
-		k[b0 + 1] = x1;
-		if (m > 2) k[b0 + 2] = x2;
-		if (m > 3) k[b0 + 3] = x3;
-		return;
-	}
-	for (u32 i = b0 + 1; i < b0 + m; ++i) {
-		const K x = k[i];
-		u32 jj = i;
-		while (jj > b0) {
-			const K y = k[jj - 1];
-			if (!(x < y)) break;
-			k[jj] = y;
-			--jj;
+// After a level's scatter over s.k[base, base + m) (s.mcnt[b] = end of
+// bucket b, which starts at s.mcnt[b - 1], or at base for b = 0;
+// bucket_of(x, i) = bucket of key x in slot i): the key in every slot of a
+// bucket of 2..kMsdMaxBucket keys counts the keys of its bucket that precede
+// it (smaller, or equal in an earlier slot) and moves to that rank; a larger
+// bucket goes on the stack, pushed by its first slot.  Work per key, not per
+// bucket: neighbouring lanes take neighbouring slots, mostly of one bucket,
+// and loop as long as the largest bucket among them, four independent LDS
+// loads a step, whatever the number of buckets.  (One thread per bucket,
+// sorting it by a network or by insertion, measured slower: 32768 segments
+// of log-uniform lengths up to 65536, 32-bit keys, 1.83 ms against 1.61.)  Contains barriers: call from
+// uniform control flow.
+template <typename K, int IPT, typename BucketOf>
+__device__ __forceinline__ void msd_place(MsdSmem<K, IPT>& s, u32 base, u32 m,
+	BucketOf bucket_of) {
+	constexpr u32 kNone = ~0u;
+	K x[IPT];
+	u32 np[IPT];
+#pragma unroll
+	for (int j = 0; j < IPT; ++j) {
+		const u32 i = base + j * kRB + threadIdx.x;
+		np[j] = kNone;
+		if (i >= base + m) continue;
+		x[j] = s.k[i];
+		const u32 b = bucket_of(x[j], i);
+		const u32 e = s.mcnt[b];
+		const u32 b0 = b == 0 ? base : s.mcnt[b - 1];
+		const u32 mb = e - b0;
+		if (mb > kMsdMaxBucket) {
+			if (i == b0) {
+				const u32 slot = atomicAdd(&s.misc[2], 1u);
+				s.mstack[slot] = make_uint2(b0, mb);
+			}
+		} else if (mb > 1) {
+			u32 r = b0;
+			// Slots t >= e are not counted; their loads stay inside the LDS
+			// block (s.k is followed by s.mcnt, and t < e + 3 <= kCap + 3).
+			for (u32 t = b0; t < e; t += 4) {
+				K y[4];
+#pragma unroll
+				for (int u = 0; u < 4; ++u) y[u] = s.k[t + u];
+#pragma unroll
+				for (int u = 0; u < 4; ++u)
+					r += (t + u < e && (y[u] < x[j] || (y[u] == x[j] && t + u < i))) ? 1u : 0u;
+			}
+			if (r != i) np[j] = r;
 		}
-		k[jj] = x;
 	}
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < IPT; ++j)
+		if (np[j] != kNone) s.k[np[j]] = x[j];
+	__syncthreads();
 }
 
 // One level over s.k[base, base + m), m <= kCap.  Contains barriers: call from
@@ -480,20 +502,9 @@ __device__ __forceinline__ void msd_level(MsdSmem<K, IPT>& s, u32 base, u32 m) {
 	}
 	__syncthreads();
 	if (exact) return;
-	// Cursors are now bucket ends; bucket b is [end(b-1), end(b)).
-	for (u32 b = tid; b < NB; b += kRB) {
-		const u32 e = s.mcnt[b];
-		const u32 b0 = b == 0 ? base : s.mcnt[b - 1];
-		const u32 m = e - b0;
-		if (m < 2) continue;
-		if (m > kMsdMaxBucket) {
-			const u32 slot = atomicAdd(&s.misc[2], 1u);
-			s.mstack[slot] = make_uint2(b0, m);
-			continue;
-		}
-		msd_small_bucket<K>(s.k, b0, m);
-	}
-	__syncthreads();
+	// Cursors are now bucket ends.
+	msd_place<K, IPT>(s, base, m,
+		[lo, shift](K x, u32) { return static_cast<u32>((x - lo) >> shift); });
 }
 
 // Sorts s.k[0, n), n <= kCap, keys alone.  The stack holds disjoint ranges
@@ -815,8 +826,9 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 	// Pass 3: partition, chunk by chunk (the next chunk's loads in flight),
 	// each chunk staged in LDS by bucket so that every bucket's keys go out
 	// as one run.  (Every key taking its slot from the bucket's LDS cursor and
-	// being stored straight to the scratch array measured slower: C4 2.30 ->
-	// 2.40 ms, 65536-key segments 1.54 -> 2.45 ms.)
+	// being stored straight to the scratch array measured slower: 32768
+	// segments of log-uniform lengths 2.30 -> 2.40 ms, 65536-key segments
+	// 1.54 -> 2.45 ms.)
 	u32* ccnt = s.hist[2];  // per chunk: bucket counts, then cursors
 	ccnt[tid] = 0;
 	K kn[IPT];
@@ -1061,21 +1073,12 @@ __device__ __forceinline__ void sort_chunk_msd(const SegArgs<K>& a, MsdSmem<K, I
 	for (int j = 0; j < IPT; ++j)
 		if (j * kRB + tid < n) s.k[atomicAdd(&s.mcnt[bk[j]], 1u)] = k[j];
 	__syncthreads();
-	// Buckets of 2..4 keys by a sorting network, up to kMsdMaxBucket by
-	// insertion, larger ones on the stack.
-	for (u32 b = tid; b < n; b += kRB) {
-		const u32 e = s.mcnt[b];
-		const u32 b0 = b == 0 ? 0u : s.mcnt[b - 1];
-		const u32 mb = e - b0;
-		if (mb < 2) continue;
-		if (mb > kMsdMaxBucket) {
-			const u32 slot = atomicAdd(&s.misc[2], 1u);
-			s.mstack[slot] = make_uint2(b0, mb);
-			continue;
-		}
-		msd_small_bucket<K>(s.k, b0, mb);
-	}
-	__syncthreads();
+	// Every key to its rank in its bucket (slot i stays in the segment of
+	// slot i); buckets over kMsdMaxBucket keys on the stack.
+	msd_place<K, IPT>(s, 0u, n, [&s](K x, u32 i) {
+		const u32 r = s.sid[i];
+		return s.lpos[r] + static_cast<u32>((x - s.smin[r]) >> s.sshift[r]);
+	});
 	for (;;) {
 		const u32 sz = s.misc[2];
 		if (sz == 0) break;
@@ -1395,8 +1398,9 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 		"classes=%s", nseg, numel, ks, kv ? " +u32 values" : "", tile_items, 256 * tile_items, grid,
 		(sorter->seg_cls[0] | sorter->seg_cls[1]) ? "per length bin" : "by length");
 	const bool inplace = kout == nullptr || kout->ptr == kin->ptr;
-	// MSD bucketing for keys alone (configuration 4, tile 8: 32-bit keys 2.30
-	// ms against 2.52-2.71 on the digit passes, 64-bit keys 3.40 against 6.5).
+	// MSD bucketing for keys alone (32768 segments of log-uniform lengths up to
+	// 65536, tile 8: 32-bit keys 1.6 ms against 2.6 on the digit passes, 64-bit
+	// keys 2.2 against 7.5).
 	const bool msd = !kv && sorter->seg_msd != 0;
 #define SEG_ARGS(K)                                                                          \
 	SegArgs<K> a;                                                                            \

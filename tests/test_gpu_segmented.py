@@ -6,7 +6,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _lengths(oracle, nseg, max_len, seed):
-    # log-uniform lengths in [1, max_len] (configuration 4's distribution)
+    # log-uniform lengths in [1, max_len] (the distribution of the 32768-segment benchmark workload)
     u = oracle.generate(oracle.GEN_U64, seed, nseg)
     x = (u >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
     return np.clip(np.floor(np.exp(x * np.log(max_len))), 1, max_len).astype(np.uint32)
