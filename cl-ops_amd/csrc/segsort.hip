@@ -30,6 +30,18 @@ using namespace clo::dev;
 
 namespace {
 
+// Large segments are listed in index order, and again by length class c:
+// lengths in (CAP * 2^c, CAP * 2^(c+1)], the last class everything longer;
+// class 0 also takes the segments of at most CAP keys that a work-class table
+// sends to the large path.  The bucketing kernel takes the classes longest
+// first, so the work left when its queue runs dry is a few short segments,
+// not a long one that started last (32768 segments of log-uniform lengths up
+// to 65536, 32-bit keys: 1.54 -> 1.50 ms).  The digit-pass kernel keeps index
+// order: its passes over a large segment go through global memory, and all
+// the longest segments at once hold more of it than the caches do (key-value
+// sorts of the same workload 4.27 -> 5.27 ms longest first).
+constexpr u32 kSegLenClasses = 8;
+
 struct SegMeta {
 	u32 n_medium;
 	u32 n_large;
@@ -37,10 +49,36 @@ struct SegMeta {
 	u32 head;      // work queue of the digit-pass kernel
 	u32 head_msd;  // work queue of the keys-only bucketing kernel
 	u32 n_fb;      // large keys-only segments left to the digit passes
+	u32 n_cls[kSegLenClasses];  // large segments per length class
 };
 
+// Where each length class's list starts in the large list: class c holds at
+// most numel / (CAP * 2^c) segments (each is longer than CAP * 2^c) and
+// class 0 at most nseg, so the lists take at most nseg + 2 * numel / CAP + 7
+// entries.  Returns the total.
+struct SegClassBase {
+	u32 b[kSegLenClasses];
+};
+inline size_t seg_class_layout(size_t nseg, size_t numel, size_t cap, SegClassBase* base) {
+	size_t t = 0;
+	for (u32 c = 0; c < kSegLenClasses; ++c) {
+		base->b[c] = static_cast<u32>(t);
+		const size_t m = numel / (cap << c) + 1;
+		t += c == 0 ? nseg : (m < nseg ? m : nseg);
+	}
+	return t;
+}
+
+__device__ __forceinline__ u32 seg_len_class(u32 len, u32 CAP) {
+	const u32 q = (len - 1) / CAP;  // len >= 1
+	if (q == 0) return 0;
+	const u32 c = 31u - static_cast<u32>(__builtin_clz(q));
+	return c < kSegLenClasses - 1 ? c : kSegLenClasses - 1;
+}
+
 __global__ void seg_reset(SegMeta* m) {
-	if (threadIdx.x == 0) {
+	const u32 t = threadIdx.x;
+	if (t == 0) {
 		m->n_medium = 0;
 		m->n_large = 0;
 		m->n_chunks = 0;
@@ -48,6 +86,7 @@ __global__ void seg_reset(SegMeta* m) {
 		m->head_msd = 0;
 		m->n_fb = 0;
 	}
+	if (t < kSegLenClasses) m->n_cls[t] = 0;
 }
 
 // Segment j = [off[j], off[j+1]) (the last one ends at numel), clamped into
@@ -86,7 +125,9 @@ __device__ __forceinline__ u32 seg_class(u64 lo, u64 hi, u32 len, u32 W, u32 CAP
 // a small segment never become work items.
 __global__ __launch_bounds__(256) void seg_classify(const u32* __restrict__ off, u32 nseg,
 	u32 numel, u32 W, u32 CAP, u64 cls_lo, u64 cls_hi, u32* __restrict__ medium,
-	u32* __restrict__ large, u32* __restrict__ chunks, uint2* __restrict__ wrange, SegMeta* m) {
+	u32* __restrict__ large_ix, u32* __restrict__ large, SegClassBase lbase,
+	u32* __restrict__ chunks,
+	uint2* __restrict__ wrange, SegMeta* m) {
 	const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
 	SegRange sr = {0u, 0u};
 	if (j < nseg) sr = seg_range(off, j, nseg, numel);
@@ -123,8 +164,21 @@ __global__ __launch_bounds__(256) void seg_classify(const u32* __restrict__ off,
 	basel = __shfl(basel, 0, kWave);
 	basec = __shfl(basec, 0, kWave);
 	if (is_med) medium[basem + count_below(bm)] = j;
-	if (is_lrg) large[basel + count_below(bl)] = j;
+	if (is_lrg) large_ix[basel + count_below(bl)] = j;
 	if (head) chunks[basec + count_below(bc)] = sr.start / W;
+	if (bl) {  // uniform: the large segments by length class
+		const u32 lc = is_lrg ? seg_len_class(sr.len, CAP) : 0u;
+		u32 pos = 0;
+		for (u32 c = 0; c < kSegLenClasses; ++c) {
+			const u64 b = ballot(is_lrg && lc == c);
+			if (!b) continue;  // uniform
+			u32 base = 0;
+			if (lane_id() == 0) base = atomicAdd(&m->n_cls[c], static_cast<u32>(__popcll(b)));
+			base = __shfl(base, 0, kWave);
+			if (is_lrg && lc == c) pos = lbase.b[c] + base + count_below(b);
+		}
+		if (is_lrg) large[pos] = j;
+	}
 }
 
 template <typename K>
@@ -137,7 +191,9 @@ struct SegArgs {
 	u32* vscr;
 	const u32* off;
 	const u32* medium;
-	const u32* large;
+	const u32* large_ix;  // large segments in index order
+	const u32* large;     // and by length class (SegClassBase)
+	SegClassBase lbase;
 	const u32* chunks;    // windows holding small segment starts
 	const uint2* wrange;  // per window: segments [x, y) starting in it
 	uint2* fb;            // keys alone: (start, len) of large segments for the digit passes
@@ -147,6 +203,30 @@ struct SegArgs {
 	u32 nseg;
 	u32 numel;
 };
+
+// The large segments in work-queue order: the length classes longest first.
+// The class sizes are read once per block (the counters share a cache line
+// with the work-queue heads that every block updates).
+struct LargeOrder {
+	u32 n[kSegLenClasses];
+};
+__device__ __forceinline__ LargeOrder large_order(const SegMeta* m) {
+	LargeOrder o;
+#pragma unroll
+	for (u32 c = 0; c < kSegLenClasses; ++c)
+		o.n[c] = __builtin_amdgcn_readfirstlane(m->n_cls[c]);
+	return o;
+}
+// Large segment number t (below meta->n_large) of the work queue.
+template <typename K>
+__device__ __forceinline__ u32 large_seg(const SegArgs<K>& a, const LargeOrder& o, u32 t) {
+#pragma unroll
+	for (int c = kSegLenClasses - 1; c > 0; --c) {
+		if (t < o.n[c]) return a.large[a.lbase.b[c] + t];
+		t -= o.n[c];
+	}
+	return a.large[a.lbase.b[0] + t];
+}
 
 template <typename K, bool KV, int IPT>
 struct SegSmem {
@@ -313,6 +393,10 @@ __device__ void sort_key_passes(SegSmem<K, KV, IPT>& s, u32 n, bool carry_sid) {
 // keys ends at once.  A level costs a few VALU operations per key, where a
 // stable digit pass costs about 50 (wave ballots over 8 digit bits).
 constexpr u32 kMsdMaxBucket = 32;
+// Large segments of kMsdChunksMin to kMsdChunks tiles are partitioned by top
+// byte in one read (sort_large_msd).
+constexpr u32 kMsdChunksMin = 9;
+constexpr u32 kMsdChunks = 32;
 
 template <typename K>
 __device__ __forceinline__ u32 top_bit(K x) {
@@ -323,7 +407,7 @@ __device__ __forceinline__ u32 top_bit(K x) {
 // LDS of the keys-only bucketing kernel.  k holds the tile; mcnt the bucket
 // counts, then cursors; mstack the ranges waiting for a level of their own.
 // Large segments use hist/base/gofs (bucket sizes and starts, window starts,
-// chunk cursors), chunk items sid/lpos/segj (gather and write-out) and
+// chunk cursors) and co/rsrc/rdst (chunk-major partition), chunk items sid/lpos/segj (gather and write-out) and
 // smin/sshift (each small segment's minimum key and bucket shift).
 template <typename K, int IPT>
 struct MsdSmem {
@@ -337,6 +421,9 @@ struct MsdSmem {
 			u32 hist[4][256];
 			u32 base[256];
 			u32 gofs[256];
+			u16 co[kMsdChunks][256];  // each chunk's bucket starts
+			u32 rsrc[kMsdChunks];     // a window's runs: start in the scratch array
+			u32 rdst[kMsdChunks];     // and in the window
 		};
 		struct {  // chunk items
 			u16 sid[kCap];
@@ -698,47 +785,119 @@ __device__ __forceinline__ void large_chunk(SegSmem<K, KV, IPT>& s, const K (&k)
 
 
 // Large segment, keys alone: one unstable partition of the segment into 256
-// buckets by (key - min) >> shift (the segment's key range over 256 buckets)
-// into the scratch array, staged per chunk of CAP keys through LDS so that
-// each bucket's keys are written as one run; then windows of whole buckets,
-// each at most CAP keys, sorted in LDS by msd_tile_sort and written to the
-// output.  A window holds the buckets whose start lies in one range of CAP/2
-// positions, so with every bucket at most CAP/2 keys no window exceeds CAP.
-// Returns false, having written nothing, when a bucket is larger (keys
-// concentrated in a small part of their range): the segment is then left to
-// the stable digit passes.
+// buckets, then windows of whole buckets, each under CAP keys, sorted in LDS
+// by msd_tile_sort and written to the output.  The buckets are the top byte
+// of the key or, when a top-byte bucket holds more than CAP/2 keys, the
+// segment's key range (minimum to maximum) over 256 buckets.  A window holds
+// the buckets whose start lies in one range of span = CAP - (largest bucket)
+// positions, so it holds fewer than CAP keys.
+//
+// A segment of kMsdChunksMin to kMsdChunks chunks (of CAP keys) is read
+// once: each chunk is grouped by top-byte bucket in LDS and written back to
+// its own place in the scratch array (whole cache lines), its bucket starts
+// kept in s.co, and a window gathers one run from every chunk.  Otherwise
+// (other lengths, or range buckets) a pass counts the buckets, each chunk's
+// buckets are then stored to the bucket's place in the segment, and windows
+// read one contiguous range.  Storing the runs of about CAP/256 keys that
+// every chunk has per bucket costs more than reading them back once the
+// segments in flight outgrow the caches (65536-key segments, 32-bit keys:
+// 1.17 ms stored, 1.02 ms read); for a few chunks the runs still merge in
+// the cache and the gather's run search is the larger cost (4096-key
+// segments: 1.07 ms stored, 1.24 ms read).
+//
+// Returns false, having written nothing to the output, when a range bucket
+// is over CAP/2 keys (keys concentrated in a small part of their range): the
+// segment is then left to the stable digit passes.
 template <typename K, int IPT>
 __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, IPT>& s,
 	u32 start, u32 len) {
 	using S = MsdSmem<K, IPT>;
 	constexpr u32 CAP = S::kCap;
 	constexpr u32 HALF = CAP / 2;
+	constexpr u32 TS = sizeof(K) * 8 - 8;  // top-byte shift
 	const u32 tid = threadIdx.x;
 	const u32 lane = lane_id();
 	const u32 w = wave_id();
-	u32* hcnt = s.hist[0];   // bucket sizes
-	u32* wbeg = s.hist[3];   // window starts (at most one per bucket)
+	u32* hcnt = s.hist[0];  // bucket sizes, then starts
+	u32* wbk = s.hist[1];   // first bucket of each window
+	u32* ccnt = s.hist[2];  // per chunk: bucket counts, then cursors
+	u32* wbeg = s.hist[3];  // window starts (at most one per bucket)
 	const K* __restrict__ src = a.kin + start;
-	// Pass 1: minimum, maximum, and the bucket sizes by the top byte of the
-	// key (the buckets when no top-byte bucket exceeds CAP/2, which saves the
-	// second read for keys spread over the whole key range).
-	constexpr u32 TS = sizeof(K) * 8 - 8;
+	K* __restrict__ scr = a.kscr + start;
+	K* __restrict__ out = a.kout + start;
+	const u32 nch = len / CAP + (len % CAP != 0 ? 1u : 0u);
+	const bool cm = nch >= kMsdChunksMin && nch <= kMsdChunks;  // chunk-major (uniform)
 	hcnt[tid] = 0;
+	ccnt[tid] = 0;
 	if (tid == 0) s.misc[2] = 0;
 	__syncthreads();
 	K lo = ~K(0), hi = K(0);
-	for (u32 c0 = 0; c0 < len; c0 += kBatch * kRB) {
-		K t[kBatch];
+	if (cm) {
+		// Pass A, chunk by chunk (the next chunk's loads in flight).
+		K kn[IPT];
 #pragma unroll
-		for (int r = 0; r < kBatch; ++r) {
-			const u32 i = c0 + r * kRB + tid;
-			t[r] = i < len ? src[i] : src[0];
+		for (int jj = 0; jj < IPT; ++jj) {
+			const u32 e = jj * kRB + tid;
+			kn[jj] = e < len ? src[e] : K(0);
 		}
+		for (u32 c = 0; c < nch; ++c) {
+			const u32 c0 = c * CAP;
+			const u32 cnt = (len - c0) < CAP ? (len - c0) : CAP;
+			K k[IPT];
 #pragma unroll
-		for (int r = 0; r < kBatch; ++r) {
-			lo = t[r] < lo ? t[r] : lo;
-			hi = t[r] > hi ? t[r] : hi;
-			if (c0 + r * kRB + tid < len) atomicAdd(&hcnt[static_cast<u32>(t[r] >> TS)], 1u);
+			for (int jj = 0; jj < IPT; ++jj) k[jj] = kn[jj];
+			if (c + 1 < nch) {
+				const u32 c1 = c0 + CAP;
+				const u32 cnt1 = (len - c1) < CAP ? (len - c1) : CAP;
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj) {
+					const u32 e = jj * kRB + tid;
+					kn[jj] = e < cnt1 ? src[c1 + e] : K(0);
+				}
+			}
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj) {
+				if (jj * kRB + tid < cnt) {
+					lo = k[jj] < lo ? k[jj] : lo;
+					hi = k[jj] > hi ? k[jj] : hi;
+					atomicAdd(&ccnt[static_cast<u32>(k[jj] >> TS)], 1u);
+				}
+			}
+			__syncthreads();
+			const u32 cc = ccnt[tid];
+			const u32 cs = block_exclusive_sum<kRB>(cc, s.wsum, static_cast<u32*>(nullptr));
+			ccnt[tid] = cs;
+			hcnt[tid] += cc;
+			s.co[c][tid] = static_cast<u16>(cs);
+			__syncthreads();
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj)
+				if (jj * kRB + tid < cnt)
+					s.k[atomicAdd(&ccnt[static_cast<u32>(k[jj] >> TS)], 1u)] = k[jj];
+			__syncthreads();
+#pragma unroll
+			for (int r = 0; r < IPT; ++r) {
+				const u32 i = r * kRB + tid;
+				if (i < cnt) scr[c0 + i] = s.k[i];
+			}
+			ccnt[tid] = 0;
+			__syncthreads();
+		}
+	} else {
+		// Pass 1: minimum, maximum and the top-byte bucket sizes.
+		for (u32 c0 = 0; c0 < len; c0 += kBatch * kRB) {
+			K t[kBatch];
+#pragma unroll
+			for (int r = 0; r < kBatch; ++r) {
+				const u32 i = c0 + r * kRB + tid;
+				t[r] = i < len ? src[i] : src[0];
+			}
+#pragma unroll
+			for (int r = 0; r < kBatch; ++r) {
+				lo = t[r] < lo ? t[r] : lo;
+				hi = t[r] > hi ? t[r] : hi;
+				if (c0 + r * kRB + tid < len) atomicAdd(&hcnt[static_cast<u32>(t[r] >> TS)], 1u);
+			}
 		}
 	}
 #pragma unroll
@@ -763,14 +922,15 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 		hi = red[kRBWaves + ww] > hi ? red[kRBWaves + ww] : hi;
 	}
 	if (hi == lo) {  // uniform over the block: every key equal
-		if (a.kout + start != src) g2g<K, IPT>(a.kout + start, src, len);
+		if (out != src) g2g<K, IPT>(out, src, len);
 		return true;
 	}
 	__syncthreads();
 	// Bucket of key x: (x - bbase) >> shift, below 256 and non-decreasing in x.
 	K bbase = K(0);
 	u32 shift = TS;
-	if (s.misc[2] != 0) {  // uniform: read after the barrier
+	const bool top = s.misc[2] == 0;  // uniform: read after the barrier
+	if (!top) {
 		// Pass 2: the segment's key range over 256 buckets.
 		const u32 hb = top_bit(static_cast<K>(hi - lo));
 		bbase = lo;
@@ -819,96 +979,151 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 		const bool op = tid == 0 || (b0 / span) != (prev / span);
 		u32 nw = 0;
 		const u32 idx = block_exclusive_sum<kRB>(op ? 1u : 0u, s.wsum, &nw);
-		if (op) wbeg[idx] = b0;
+		if (op) {
+			wbeg[idx] = b0;
+			wbk[idx] = tid;
+		}
 		if (tid == 0) s.misc[0] = nw;
 	}
-	K* __restrict__ scr = a.kscr + start;
-	// Pass 3: partition, chunk by chunk (the next chunk's loads in flight),
-	// each chunk staged in LDS by bucket so that every bucket's keys go out
-	// as one run.  (Every key taking its slot from the bucket's LDS cursor and
-	// being stored straight to the scratch array measured slower: 32768
-	// segments of log-uniform lengths 2.30 -> 2.40 ms, 65536-key segments
-	// 1.54 -> 2.45 ms.)
-	u32* ccnt = s.hist[2];  // per chunk: bucket counts, then cursors
-	ccnt[tid] = 0;
-	K kn[IPT];
-#pragma unroll
-	for (int jj = 0; jj < IPT; ++jj) {
-		const u32 e = jj * kRB + tid;
-		kn[jj] = e < len ? src[e] : K(0);
-	}
-	__syncthreads();
-	for (u32 c0 = 0; c0 < len; c0 += CAP) {
-		const u32 cnt = (len - c0) < CAP ? (len - c0) : CAP;
-		K k[IPT];
-#pragma unroll
-		for (int jj = 0; jj < IPT; ++jj) k[jj] = kn[jj];
-		const u32 c1 = c0 + CAP;
-		if (c1 < len) {
-			const u32 cnt1 = (len - c1) < CAP ? (len - c1) : CAP;
-#pragma unroll
-			for (int jj = 0; jj < IPT; ++jj) {
-				const u32 e = jj * kRB + tid;
-				kn[jj] = e < cnt1 ? src[c1 + e] : K(0);
-			}
-		}
-#pragma unroll
-		for (int jj = 0; jj < IPT; ++jj)
-			if (jj * kRB + tid < cnt) atomicAdd(&ccnt[static_cast<u32>((k[jj] - bbase) >> shift)], 1u);
-		__syncthreads();
-		const u32 cc = ccnt[tid];
-		const u32 cs = block_exclusive_sum<kRB>(cc, s.wsum, static_cast<u32*>(nullptr));
-		ccnt[tid] = cs;                 // chunk cursor of bucket tid
-		s.gofs[tid] = s.base[tid] - cs;  // staged element i of bucket d goes to gofs[d] + i
-		__syncthreads();
-#pragma unroll
-		for (int jj = 0; jj < IPT; ++jj)
-			if (jj * kRB + tid < cnt)
-				s.k[atomicAdd(&ccnt[static_cast<u32>((k[jj] - bbase) >> shift)], 1u)] = k[jj];
-		__syncthreads();
-#pragma unroll
-		for (int r = 0; r < IPT; ++r) {
-			const u32 i = r * kRB + tid;
-			if (i < cnt) {
-				const K key = s.k[i];
-				scr[s.gofs[static_cast<u32>((key - bbase) >> shift)] + i] = key;
-			}
-		}
-		__syncthreads();
-		s.base[tid] += cc;
+	const bool gather = cm && top;  // windows gather their runs from the chunks
+	if (!gather) {
+		// Pass 3: partition, chunk by chunk (the next chunk's loads in
+		// flight), each chunk staged in LDS by bucket so that every bucket's
+		// keys go out as one run.  (Every key taking its slot from the
+		// bucket's LDS cursor and being stored straight to the scratch array
+		// measured slower: 32768 segments of log-uniform lengths 2.30 -> 2.40
+		// ms, 65536-key segments 1.54 -> 2.45 ms.)
 		ccnt[tid] = 0;
+		K kn[IPT];
+#pragma unroll
+		for (int jj = 0; jj < IPT; ++jj) {
+			const u32 e = jj * kRB + tid;
+			kn[jj] = e < len ? src[e] : K(0);
+		}
+		__syncthreads();
+		for (u32 c0 = 0; c0 < len; c0 += CAP) {
+			const u32 cnt = (len - c0) < CAP ? (len - c0) : CAP;
+			K k[IPT];
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj) k[jj] = kn[jj];
+			const u32 c1 = c0 + CAP;
+			if (c1 < len) {
+				const u32 cnt1 = (len - c1) < CAP ? (len - c1) : CAP;
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj) {
+					const u32 e = jj * kRB + tid;
+					kn[jj] = e < cnt1 ? src[c1 + e] : K(0);
+				}
+			}
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj)
+				if (jj * kRB + tid < cnt)
+					atomicAdd(&ccnt[static_cast<u32>((k[jj] - bbase) >> shift)], 1u);
+			__syncthreads();
+			const u32 cc = ccnt[tid];
+			const u32 cs = block_exclusive_sum<kRB>(cc, s.wsum, static_cast<u32*>(nullptr));
+			ccnt[tid] = cs;                 // chunk cursor of bucket tid
+			s.gofs[tid] = s.base[tid] - cs;  // staged element i of bucket d goes to gofs[d] + i
+			__syncthreads();
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj)
+				if (jj * kRB + tid < cnt)
+					s.k[atomicAdd(&ccnt[static_cast<u32>((k[jj] - bbase) >> shift)], 1u)] = k[jj];
+			__syncthreads();
+#pragma unroll
+			for (int r = 0; r < IPT; ++r) {
+				const u32 i = r * kRB + tid;
+				if (i < cnt) {
+					const K key = s.k[i];
+					scr[s.gofs[static_cast<u32>((key - bbase) >> shift)] + i] = key;
+				}
+			}
+			__syncthreads();
+			s.base[tid] += cc;
+			ccnt[tid] = 0;
+			__syncthreads();
+		}
+	} else {
 		__syncthreads();
 	}
 	// Windows: load (the next window's keys in flight during the sort), sort
-	// in LDS, write to the output.
+	// in LDS, write to the output.  Gathered: window wi is the run of its
+	// buckets [wbk[wi], wbk[wi + 1]) in every chunk, run c starting at
+	// s.rsrc[c] in the scratch array and at s.rdst[c] in the window.
 	const u32 nw = s.misc[0];
-	K* __restrict__ out = a.kout + start;
-	u32 p1 = wbeg[0];
-	u32 n1 = (nw > 1 ? wbeg[1] : len) - p1;
-	K kw[IPT];
+	auto runs = [&](u32 wi) {  // wave 0 builds window wi's run table
+		if (w != 0) return;
+		const u32 bl = wbk[wi];
+		const u32 bh = wi + 1 < nw ? wbk[wi + 1] : 256u;
+		u32 ln = 0, rs = 0;
+		if (lane < nch) {
+			const u32 cntc = (len - lane * CAP) < CAP ? (len - lane * CAP) : CAP;
+			rs = s.co[lane][bl];
+			ln = (bh < 256u ? static_cast<u32>(s.co[lane][bh]) : cntc) - rs;
+		}
+		u32 inc = ln;
 #pragma unroll
-	for (int jj = 0; jj < IPT; ++jj) {
-		const u32 e = jj * kRB + tid;
-		kw[jj] = e < n1 ? scr[p1 + e] : K(0);
-	}
+		for (u32 d = 1; d < kMsdChunks; d <<= 1) {
+			const u32 t = __shfl_up(inc, d, kWave);
+			if (lane >= d) inc += t;
+		}
+		if (lane < nch) {
+			s.rsrc[lane] = lane * CAP + rs;
+			s.rdst[lane] = inc - ln;
+		}
+	};
+	K kw[IPT];
+	auto load = [&](u32 wi) {  // window wi's keys into kw
+		const u32 p = wbeg[wi];
+		const u32 n = (wi + 1 < nw ? wbeg[wi + 1] : len) - p;
+		if (gather) {
+			// Run of element e: the last c < nch with rdst[c] <= e, by a
+			// binary search taken one step at a time for all IPT elements
+			// (IPT independent LDS loads a step).
+			u32 l[IPT];
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj) l[jj] = 0;
+#pragma unroll
+			for (u32 b = kMsdChunks / 2; b > 0; b >>= 1) {
+				u32 v[IPT];
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj) {
+					const u32 c = l[jj] + b;
+					v[jj] = s.rdst[c < nch ? c : 0u];
+				}
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj)
+					if (l[jj] + b < nch && v[jj] <= jj * kRB + tid) l[jj] += b;
+			}
+			u32 src_i[IPT];
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj)
+				src_i[jj] = s.rsrc[l[jj]] + (jj * kRB + tid - s.rdst[l[jj]]);
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj)
+				kw[jj] = jj * kRB + tid < n ? scr[src_i[jj]] : K(0);
+		} else {
+#pragma unroll
+			for (int jj = 0; jj < IPT; ++jj) {
+				const u32 e = jj * kRB + tid;
+				kw[jj] = e < n ? scr[p + e] : K(0);
+			}
+		}
+	};
+	if (gather) runs(0);
+	__syncthreads();
+	load(0);
 	for (u32 wi = 0; wi < nw; ++wi) {
-		const u32 p0 = p1;
-		const u32 n = n1;
+		const u32 p0 = wbeg[wi];
+		const u32 n = (wi + 1 < nw ? wbeg[wi + 1] : len) - p0;
 #pragma unroll
 		for (int jj = 0; jj < IPT; ++jj) {
 			const u32 e = jj * kRB + tid;
 			if (e < n) s.k[e] = kw[jj];
 		}
-		if (wi + 1 < nw) {
-			p1 = wbeg[wi + 1];
-			n1 = (wi + 2 < nw ? wbeg[wi + 2] : len) - p1;
-#pragma unroll
-			for (int jj = 0; jj < IPT; ++jj) {
-				const u32 e = jj * kRB + tid;
-				kw[jj] = e < n1 ? scr[p1 + e] : K(0);
-			}
-		}
+		if (gather && wi + 1 < nw) runs(wi + 1);
 		__syncthreads();
+		if (wi + 1 < nw) load(wi + 1);
 		if (n > 1) msd_tile_sort<K, IPT>(s, n);
 		s2g<K, IPT>(out + p0, s.k, n);
 		__syncthreads();
@@ -1256,7 +1471,7 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 		if (item >= total) break;
 		if (item < nl) {
 			if constexpr (KV) {
-				const SegRange sr = seg_range(a.off, a.large[item], a.nseg, a.numel);
+				const SegRange sr = seg_range(a.off, a.large_ix[item], a.nseg, a.numel);
 				sort_large<K, KV, IPT>(a, s, __builtin_amdgcn_readfirstlane(sr.start),
 					__builtin_amdgcn_readfirstlane(sr.len));
 			} else {
@@ -1268,7 +1483,7 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 					len = r.y;
 					b.kin = a.kout;
 				} else {
-					const SegRange sr = seg_range(a.off, a.large[item], a.nseg, a.numel);
+					const SegRange sr = seg_range(a.off, a.large_ix[item], a.nseg, a.numel);
 					start = sr.start;
 					len = sr.len;
 				}
@@ -1288,9 +1503,19 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 // chunk items.  A large segment whose keys crowd into part of their range is
 // copied to the output unsorted and listed for seg_sort_persistent (fb_mode),
 // which runs the stable digit passes on it in place.
+// Resident blocks per CU the bucketing kernel is compiled for (its register
+// budget): four 256x8 tiles of 32-bit keys fit the LDS of a CU, three of
+// 64-bit keys.
 template <typename K, int IPT>
-__global__ __launch_bounds__(kRB, 3) void seg_msd_persistent(SegArgs<K> a) {
+constexpr int msd_min_blocks() {
+	return IPT <= 8 ? (sizeof(K) == 4 ? 4 : 3) : IPT <= 16 ? 2 : 1;
+}
+
+template <typename K, int IPT>
+__global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persistent(
+	SegArgs<K> a) {
 	__shared__ MsdSmem<K, IPT> s;
+	const LargeOrder lord = large_order(a.meta);
 	const u32 nl = a.meta->n_large;
 	const u32 nlm = nl + a.meta->n_medium;
 	const u32 total = nlm + a.meta->n_chunks;
@@ -1304,7 +1529,7 @@ __global__ __launch_bounds__(kRB, 3) void seg_msd_persistent(SegArgs<K> a) {
 			sort_chunk_msd<K, IPT>(a, s, a.chunks[item - nlm]);
 			continue;
 		}
-		const u32 j = item < nl ? a.large[item] : a.medium[item - nl];
+		const u32 j = item < nl ? large_seg(a, lord, item) : a.medium[item - nl];
 		const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
 		const u32 start = __builtin_amdgcn_readfirstlane(sr.start);
 		const u32 len = __builtin_amdgcn_readfirstlane(sr.len);
@@ -1342,7 +1567,9 @@ bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, bool m
 		Launch L(q, "segsort_classify");
 		const u32 g = static_cast<u32>(div_ceil(a.nseg, 256));
 		seg_classify<<<g ? g : 1, 256, 0, st>>>(a.off, a.nseg, a.numel, W, CAP, a.cls_lo,
-			a.cls_hi, const_cast<u32*>(a.medium), const_cast<u32*>(a.large), const_cast<u32*>(a.chunks),
+			a.cls_hi, const_cast<u32*>(a.medium), const_cast<u32*>(a.large_ix),
+			const_cast<u32*>(a.large), a.lbase,
+			const_cast<u32*>(a.chunks),
 			const_cast<uint2*>(a.wrange), a.meta);
 		if (!L.done(err)) return false;
 	}
@@ -1378,10 +1605,18 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	SegData* sd = &static_cast<clo_segsort_cache*>(*cache)->d;
 	const size_t ks = clo_type_sizeof(sorter->elem_type);
 	if (numel == 0 || nseg == 0) return record_event(cq, "segsort_done", err);
-	// medium, large and chunk lists (nseg entries each), then [j0, j1) per
-	// window of the smallest tile's W = 1024 elements.
+	if (tile_items != 8 && tile_items != 16 && tile_items != 32) {
+		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmented tile items must be 8, 16 or 32");
+		return nullptr;
+	}
+	// medium and large lists (nseg entries each), large lists by length
+	// class, chunk list (nseg entries), then [j0, j1) per window of the
+	// smallest tile's W = 1024 elements and the fallback list (nseg entries).
 	const size_t nwin = numel / 1024 + 2;
-	if (!sd->lists.ensure(3 * nseg * sizeof(u32) + 64 + (nwin + nseg) * sizeof(uint2), err) ||
+	SegClassBase lbase;
+	const size_t nlarge = seg_class_layout(nseg, numel, 256 * static_cast<size_t>(tile_items), &lbase);
+	const size_t nlists = (3 * nseg + nlarge + 3) / 4 * 4 + 4;
+	if (!sd->lists.ensure(nlists * sizeof(u32) + (nwin + nseg) * sizeof(uint2), err) ||
 		!sd->meta.ensure(sizeof(SegMeta), err) ||
 		!sd->kscr.ensure(numel * ks, err) || (vin && !sd->vscr.ensure(numel * 4, err)))
 		return nullptr;
@@ -1413,10 +1648,12 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	a.vscr = kv ? static_cast<u32*>(sd->vscr.ptr) : nullptr;                                  \
 	a.off = static_cast<const u32*>(offsets->ptr);                                           \
 	a.medium = lists;                                                                        \
-	a.large = lists + nseg;                                                                  \
-	a.chunks = lists + 2 * nseg;                                                             \
-	a.wrange = reinterpret_cast<const uint2*>(lists + (3 * nseg + 3) / 4 * 4 + 4);          \
-	a.fb = reinterpret_cast<uint2*>(lists + (3 * nseg + 3) / 4 * 4 + 4) + nwin;             \
+	a.large_ix = lists + nseg;                                                               \
+	a.large = lists + 2 * nseg;                                                              \
+	a.lbase = lbase;                                                                         \
+	a.chunks = lists + 2 * nseg + nlarge;                                                    \
+	a.wrange = reinterpret_cast<const uint2*>(lists + nlists);                               \
+	a.fb = reinterpret_cast<uint2*>(lists + nlists) + nwin;                                  \
 	a.fb_mode = 0;                                                                           \
 	a.meta = static_cast<SegMeta*>(sd->meta.ptr);                                            \
 	a.cls_lo = sorter->seg_cls[0];                                                           \
