@@ -393,9 +393,14 @@ __device__ void sort_key_passes(SegSmem<K, KV, IPT>& s, u32 n, bool carry_sid) {
 // keys ends at once.  A level costs a few VALU operations per key, where a
 // stable digit pass costs about 50 (wave ballots over 8 digit bits).
 constexpr u32 kMsdMaxBucket = 32;
-// Large segments of kMsdChunksMin to kMsdChunks tiles are partitioned by top
-// byte in one read (sort_large_msd).
-constexpr u32 kMsdChunksMin = 9;
+// Large segments of msd_chunks_min<K>() to kMsdChunks tiles are partitioned
+// by top byte in one read (sort_large_msd).  The lower bound is measured:
+// 2^27 32-bit keys in 4096-key segments 1.09 ms from 5 tiles, 1.19 from 2;
+// 64-bit keys 1.49 ms from 2 tiles, 1.55 from 5.
+template <typename K>
+constexpr u32 msd_chunks_min() {
+	return sizeof(K) == 4 ? 5u : 2u;
+}
 constexpr u32 kMsdChunks = 32;
 
 template <typename K>
@@ -792,7 +797,7 @@ __device__ __forceinline__ void large_chunk(SegSmem<K, KV, IPT>& s, const K (&k)
 // the buckets whose start lies in one range of span = CAP - (largest bucket)
 // positions, so it holds fewer than CAP keys.
 //
-// A segment of kMsdChunksMin to kMsdChunks chunks (of CAP keys) is read
+// A segment of msd_chunks_min<K>() to kMsdChunks chunks (of CAP keys) is read
 // once: each chunk is grouped by top-byte bucket in LDS and written back to
 // its own place in the scratch array (whole cache lines), its bucket starts
 // kept in s.co, and a window gathers one run from every chunk.  Otherwise
@@ -826,7 +831,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 	K* __restrict__ scr = a.kscr + start;
 	K* __restrict__ out = a.kout + start;
 	const u32 nch = len / CAP + (len % CAP != 0 ? 1u : 0u);
-	const bool cm = nch >= kMsdChunksMin && nch <= kMsdChunks;  // chunk-major (uniform)
+	const bool cm = nch >= msd_chunks_min<K>() && nch <= kMsdChunks;  // chunk-major (uniform)
 	hcnt[tid] = 0;
 	ccnt[tid] = 0;
 	if (tid == 0) s.misc[2] = 0;
@@ -1085,6 +1090,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 			for (int jj = 0; jj < IPT; ++jj) l[jj] = 0;
 #pragma unroll
 			for (u32 b = kMsdChunks / 2; b > 0; b >>= 1) {
+				if (b >= nch) continue;  // uniform: steps of at least nch find nothing
 				u32 v[IPT];
 #pragma unroll
 				for (int jj = 0; jj < IPT; ++jj) {
@@ -1498,11 +1504,6 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 	}
 }
 
-// Keys alone: every segment, sorted by unstable MSD bucketing (the order of
-// equal keys cannot be seen): large segments, then medium ones, then the
-// chunk items.  A large segment whose keys crowd into part of their range is
-// copied to the output unsorted and listed for seg_sort_persistent (fb_mode),
-// which runs the stable digit passes on it in place.
 // Resident blocks per CU the bucketing kernel is compiled for (its register
 // budget): four 256x8 tiles of 32-bit keys fit the LDS of a CU, three of
 // 64-bit keys.
@@ -1511,6 +1512,12 @@ constexpr int msd_min_blocks() {
 	return IPT <= 8 ? (sizeof(K) == 4 ? 4 : 3) : IPT <= 16 ? 2 : 1;
 }
 
+// Keys alone: every segment, sorted by unstable MSD bucketing (the order of
+// equal keys cannot be seen): large segments, longest first, then medium
+// ones, then the chunk items.  A large segment whose keys crowd into part of
+// their range is copied to the output unsorted and listed for
+// seg_sort_persistent (fb_mode), which runs the stable digit passes on it in
+// place.
 template <typename K, int IPT>
 __global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persistent(
 	SegArgs<K> a) {
@@ -1519,28 +1526,32 @@ __global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persi
 	const u32 nl = a.meta->n_large;
 	const u32 nlm = nl + a.meta->n_medium;
 	const u32 total = nlm + a.meta->n_chunks;
-	for (;;) {
-		if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head_msd, 1u);
-		__syncthreads();
-		const u32 item = __builtin_amdgcn_readfirstlane(s.misc[3]);
-		__syncthreads();
-		if (item >= total) break;
+	if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head_msd, 1u);
+	__syncthreads();
+	u32 item = __builtin_amdgcn_readfirstlane(s.misc[3]);
+	__syncthreads();
+	while (item < total) {
+		// The next ticket is taken now, its round trip under this item's work.
+		u32 ticket = 0;
+		if (threadIdx.x == 0) ticket = atomicAdd(&a.meta->head_msd, 1u);
 		if (item >= nlm) {
 			sort_chunk_msd<K, IPT>(a, s, a.chunks[item - nlm]);
-			continue;
+		} else {
+			const u32 j = item < nl ? large_seg(a, lord, item) : a.medium[item - nl];
+			const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
+			const u32 start = __builtin_amdgcn_readfirstlane(sr.start);
+			const u32 len = __builtin_amdgcn_readfirstlane(sr.len);
+			if (item >= nl) {
+				sort_medium_msd<K, IPT>(a, s, start, len);
+			} else if (!sort_large_msd<K, IPT>(a, s, start, len)) {
+				__syncthreads();
+				if (a.kout != a.kin) g2g<K, IPT>(a.kout + start, a.kin + start, len);
+				if (threadIdx.x == 0) a.fb[atomicAdd(&a.meta->n_fb, 1u)] = make_uint2(start, len);
+			}
 		}
-		const u32 j = item < nl ? large_seg(a, lord, item) : a.medium[item - nl];
-		const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
-		const u32 start = __builtin_amdgcn_readfirstlane(sr.start);
-		const u32 len = __builtin_amdgcn_readfirstlane(sr.len);
-		if (item >= nl) {
-			sort_medium_msd<K, IPT>(a, s, start, len);
-			continue;
-		}
-		if (sort_large_msd<K, IPT>(a, s, start, len)) continue;
+		if (threadIdx.x == 0) s.misc[3] = ticket;
 		__syncthreads();
-		if (a.kout != a.kin) g2g<K, IPT>(a.kout + start, a.kin + start, len);
-		if (threadIdx.x == 0) a.fb[atomicAdd(&a.meta->n_fb, 1u)] = make_uint2(start, len);
+		item = __builtin_amdgcn_readfirstlane(s.misc[3]);
 		__syncthreads();
 	}
 }
