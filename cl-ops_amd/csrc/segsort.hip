@@ -500,9 +500,12 @@ __device__ __forceinline__ void msd_place(MsdSmem<K, IPT>& s, u32 base, u32 m,
 
 // One level over s.k[base, base + m), m <= kCap.  Contains barriers: call from
 // uniform control flow.  Pushes the buckets of more than kMsdMaxBucket keys
-// onto s.mstack (count in s.misc[2]).
-template <typename K, int IPT>
-__device__ __forceinline__ void msd_level(MsdSmem<K, IPT>& s, u32 base, u32 m) {
+// onto s.mstack (count in s.misc[2]).  KNOWN: every key lies in [klo, khi]
+// (klo < khi), which stands in for the range's minimum and maximum and saves
+// their block reduction.
+template <typename K, int IPT, bool KNOWN = false>
+__device__ __forceinline__ void msd_level(MsdSmem<K, IPT>& s, u32 base, u32 m, K klo = K(0),
+	K khi = K(0)) {
 	using S = MsdSmem<K, IPT>;
 	constexpr u32 NB = S::kCap;  // buckets (a power of two)
 	constexpr u32 WB = static_cast<u32>(__builtin_ctz(NB));  // bucket bits
@@ -519,37 +522,46 @@ __device__ __forceinline__ void msd_level(MsdSmem<K, IPT>& s, u32 base, u32 m) {
 		const u32 e = j * kRB + tid;
 		valid[j] = e < m;
 		k[j] = valid[j] ? s.k[base + e] : K(0);
-		if (valid[j]) {
+		if (!KNOWN && valid[j]) {
 			lo = k[j] < lo ? k[j] : lo;
 			hi = k[j] > hi ? k[j] : hi;
 		}
 	}
-	// Block min/max; every thread zeroes its BPT counters meanwhile.
+	if constexpr (KNOWN) {
+		lo = klo;
+		hi = khi;
 #pragma unroll
-	for (int off = kWave / 2; off > 0; off >>= 1) {
-		const K l2 = __shfl_xor(lo, off, kWave);
-		const K h2 = __shfl_xor(hi, off, kWave);
-		lo = l2 < lo ? l2 : lo;
-		hi = h2 > hi ? h2 : hi;
-	}
-	K* red = s.red;
-	if (lane == 0) {
-		red[w] = lo;
-		red[kRBWaves + w] = hi;
-	}
+		for (u32 q = 0; q < BPT / 4; ++q)
+			reinterpret_cast<uint4*>(s.mcnt)[tid * (BPT / 4) + q] = make_uint4(0u, 0u, 0u, 0u);
+		__syncthreads();
+	} else {
+		// Block min/max; every thread zeroes its BPT counters meanwhile.
 #pragma unroll
-	for (u32 q = 0; q < BPT / 4; ++q)
-		reinterpret_cast<uint4*>(s.mcnt)[tid * (BPT / 4) + q] = make_uint4(0u, 0u, 0u, 0u);
-	__syncthreads();
-	lo = ~K(0);
-	hi = K(0);
+		for (int off = kWave / 2; off > 0; off >>= 1) {
+			const K l2 = __shfl_xor(lo, off, kWave);
+			const K h2 = __shfl_xor(hi, off, kWave);
+			lo = l2 < lo ? l2 : lo;
+			hi = h2 > hi ? h2 : hi;
+		}
+		K* red = s.red;
+		if (lane == 0) {
+			red[w] = lo;
+			red[kRBWaves + w] = hi;
+		}
 #pragma unroll
-	for (int ww = 0; ww < kRBWaves; ++ww) {
-		lo = red[ww] < lo ? red[ww] : lo;
-		hi = red[kRBWaves + ww] > hi ? red[kRBWaves + ww] : hi;
+		for (u32 q = 0; q < BPT / 4; ++q)
+			reinterpret_cast<uint4*>(s.mcnt)[tid * (BPT / 4) + q] = make_uint4(0u, 0u, 0u, 0u);
+		__syncthreads();
+		lo = ~K(0);
+		hi = K(0);
+#pragma unroll
+		for (int ww = 0; ww < kRBWaves; ++ww) {
+			lo = red[ww] < lo ? red[ww] : lo;
+			hi = red[kRBWaves + ww] > hi ? red[kRBWaves + ww] : hi;
+		}
+		__syncthreads();  // red is read before the next level writes it
+		if (hi == lo) return;  // all keys equal (uniform over the block)
 	}
-	__syncthreads();  // red is read before the next level writes it
-	if (hi == lo) return;  // all keys equal (uniform over the block)
 	// Bucket of key x: (x - lo) >> shift, below NB for every key of the range
 	// and non-decreasing in x.
 	const u32 hb = top_bit(static_cast<K>(hi - lo));
@@ -601,11 +613,13 @@ __device__ __forceinline__ void msd_level(MsdSmem<K, IPT>& s, u32 base, u32 m) {
 
 // Sorts s.k[0, n), n <= kCap, keys alone.  The stack holds disjoint ranges
 // of more than kMsdMaxBucket keys, so it never exceeds kCap / 33 entries.
-template <typename K, int IPT>
-__device__ __forceinline__ void msd_tile_sort(MsdSmem<K, IPT>& s, u32 n) {
+// KNOWN: every key lies in [klo, khi], klo < khi (msd_level).
+template <typename K, int IPT, bool KNOWN = false>
+__device__ __forceinline__ void msd_tile_sort(MsdSmem<K, IPT>& s, u32 n, K klo = K(0),
+	K khi = K(0)) {
 	if (threadIdx.x == 0) s.misc[2] = 0;
 	__syncthreads();
-	msd_level<K, IPT>(s, 0, n);
+	msd_level<K, IPT, KNOWN>(s, 0, n, klo, khi);
 	for (;;) {
 		__syncthreads();
 		const u32 sz = s.misc[2];
@@ -1130,7 +1144,18 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 		if (gather && wi + 1 < nw) runs(wi + 1);
 		__syncthreads();
 		if (wi + 1 < nw) load(wi + 1);
-		if (n > 1) msd_tile_sort<K, IPT>(s, n);
+		if (n > 1) {
+			if (top) {
+				// The window's top-byte buckets [bl, bh) bound its keys.
+				const u32 bl = wbk[wi];
+				const u32 bh = wi + 1 < nw ? wbk[wi + 1] : 256u;
+				const K wlo = static_cast<K>(bl) << TS;
+				const K whi = bh >= 256u ? ~K(0) : (static_cast<K>(bh) << TS) - 1;
+				msd_tile_sort<K, IPT, true>(s, n, wlo, whi);
+			} else {
+				msd_tile_sort<K, IPT>(s, n);
+			}
+		}
 		s2g<K, IPT>(out + p0, s.k, n);
 		__syncthreads();
 	}
