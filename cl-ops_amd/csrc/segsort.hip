@@ -153,31 +153,51 @@ __global__ __launch_bounds__(256) void seg_classify(const u32* __restrict__ off,
 		if (last) wrange[win].y = j + 1;
 		head = first;
 	}
-	const u64 bm = ballot(is_med), bl = ballot(is_lrg), bc = ballot(head);
-	u32 basem = 0, basel = 0, basec = 0;
+	// One atomic per list and block (the counters share one cache line that
+	// every block updates): per-wave counts, a block total and base per list,
+	// then each wave's share.  Lists: medium, large (index order), chunks,
+	// and large by length class.
+	constexpr u32 kLists = 3 + kSegLenClasses;
+	constexpr u32 kWaves = 256 / kWave;
+	__shared__ u32 s_cnt[kLists][kWaves];
+	const u32 w = wave_id();
+	const u32 lc = is_lrg ? seg_len_class(sr.len, CAP) : 0u;
+	u64 bal[kLists];
+	bal[0] = ballot(is_med);
+	bal[1] = ballot(is_lrg);
+	bal[2] = ballot(head);
+#pragma unroll
+	for (u32 c = 0; c < kSegLenClasses; ++c) bal[3 + c] = ballot(is_lrg && lc == c);
 	if (lane_id() == 0) {
-		if (bm) basem = atomicAdd(&m->n_medium, static_cast<u32>(__popcll(bm)));
-		if (bl) basel = atomicAdd(&m->n_large, static_cast<u32>(__popcll(bl)));
-		if (bc) basec = atomicAdd(&m->n_chunks, static_cast<u32>(__popcll(bc)));
+#pragma unroll
+		for (u32 l = 0; l < kLists; ++l) s_cnt[l][w] = static_cast<u32>(__popcll(bal[l]));
 	}
-	basem = __shfl(basem, 0, kWave);
-	basel = __shfl(basel, 0, kWave);
-	basec = __shfl(basec, 0, kWave);
-	if (is_med) medium[basem + count_below(bm)] = j;
-	if (is_lrg) large_ix[basel + count_below(bl)] = j;
-	if (head) chunks[basec + count_below(bc)] = sr.start / W;
-	if (bl) {  // uniform: the large segments by length class
-		const u32 lc = is_lrg ? seg_len_class(sr.len, CAP) : 0u;
-		u32 pos = 0;
-		for (u32 c = 0; c < kSegLenClasses; ++c) {
-			const u64 b = ballot(is_lrg && lc == c);
-			if (!b) continue;  // uniform
-			u32 base = 0;
-			if (lane_id() == 0) base = atomicAdd(&m->n_cls[c], static_cast<u32>(__popcll(b)));
-			base = __shfl(base, 0, kWave);
-			if (is_lrg && lc == c) pos = lbase.b[c] + base + count_below(b);
+	__syncthreads();
+	if (threadIdx.x < kLists) {
+		const u32 l = threadIdx.x;
+		u32 tot = 0;
+#pragma unroll
+		for (u32 ww = 0; ww < kWaves; ++ww) tot += s_cnt[l][ww];
+		u32* ctr = l == 0 ? &m->n_medium : l == 1 ? &m->n_large : l == 2 ? &m->n_chunks
+			: &m->n_cls[l - 3];
+		u32 run = tot ? atomicAdd(ctr, tot) : 0u;
+#pragma unroll
+		for (u32 ww = 0; ww < kWaves; ++ww) {
+			const u32 c = s_cnt[l][ww];
+			s_cnt[l][ww] = run;
+			run += c;
 		}
-		if (is_lrg) large[pos] = j;
+	}
+	__syncthreads();
+	if (is_med) medium[s_cnt[0][w] + count_below(bal[0])] = j;
+	if (is_lrg) large_ix[s_cnt[1][w] + count_below(bal[1])] = j;
+	if (head) chunks[s_cnt[2][w] + count_below(bal[2])] = sr.start / W;
+	if (is_lrg) {
+		u32 pos = 0;
+#pragma unroll
+		for (u32 c = 0; c < kSegLenClasses; ++c)
+			if (lc == c) pos = lbase.b[c] + s_cnt[3 + c][w] + count_below(bal[3 + c]);
+		large[pos] = j;
 	}
 }
 
