@@ -358,16 +358,70 @@ __device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&x)[IT]) {
 	}
 }
 
+// Thread tid's IT consecutive elements of the tile at `base` (16-byte loads
+// when the whole run is inside the array).
+template <typename In, int IT>
+__device__ __forceinline__ void load_run(const In* __restrict__ in, u64 first, u64 n, In (&x)[IT]) {
+	if (first + IT <= n) {
+		load_vec<In, IT>(in + first, x);
+	} else {
+#pragma unroll
+		for (int j = 0; j < IT; ++j) x[j] = first + j < n ? in[first + j] : In(0);
+	}
+}
+
+// Segment-start bits of thread tid's IT consecutive elements (bit j: element
+// tid * IT + j of the tile), from mark_heads' bitmap.
+template <int IT>
+__device__ __forceinline__ u32 run_heads(const u32* s_head) {
+	static_assert(IT <= 32 && 32 % IT == 0, "a thread's bits inside one bitmap word");
+	const u32 p0 = threadIdx.x * IT;
+	const u32 w = s_head[p0 >> 5] >> (p0 & 31);
+	return IT == 32 ? w : (w & ((1u << IT) - 1u));
+}
+
+// Segmented scans take the blocked layout (thread tid owns elements
+// [tid * IT, (tid + 1) * IT) of the tile), so that a thread's start bits are
+// consecutive bits of the tile's bitmap and its elements stay in registers:
+// no LDS staging of the data, as in the plain vector kernels.
 template <typename In, typename Acc, int IT>
+__device__ __forceinline__ SegPair<Acc> run_pair(const In (&x)[IT], u32 heads) {
+	Acc run = 0;
+#pragma unroll
+	for (int j = 0; j < IT; ++j) {
+		if ((heads >> j) & 1u) run = 0;
+		run += static_cast<Acc>(x[j]);
+	}
+	return {heads != 0u ? 1u : 0u, run};
+}
+
+template <typename In, typename Acc, bool SEG, int IT>
 __global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce_vec(const In* __restrict__ in,
-	u64 n, u32 tiles, Acc* __restrict__ tsum) {
+	u64 n, u32 tiles, Acc* __restrict__ tsum, const u32* __restrict__ off, u32 nseg,
+	u32* __restrict__ tflag) {
 	constexpr u32 kScanTile = kScanBlock * IT;
 	constexpr int PER = 16 / sizeof(In);
 	__shared__ Acc s_wsum[kScanBlock / kWave];
+	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
+	__shared__ SegPair<Acc> s_pw[SEG ? kScanBlock / kWave : 1];
+	__shared__ u32 s_j0;
 	const u32 tile = xcd_tile(tiles);
 	if (tile >= tiles) return;
 	const u64 base = static_cast<u64>(tile) * kScanTile;
 	const u32 tid = threadIdx.x;
+	if constexpr (SEG) {
+		mark_heads<IT>(s_head, &s_j0, off, nseg, base);
+		In x[IT];
+		load_run<In, IT>(in, base + static_cast<u64>(tid) * IT, n, x);
+		SegPair<Acc> tot;
+		block_exclusive_segscan<kScanBlock, Acc>(run_pair<In, Acc, IT>(x, run_heads<IT>(s_head)),
+			s_pw, &tot);
+		if (tid == 0) {
+			tsum[tile] = tot.v;
+			tflag[tile] = tot.f;
+		}
+		return;
+	}
 	Acc s = 0;
 	if (base + kScanTile <= n) {
 		In x[IT];
@@ -388,31 +442,42 @@ __global__ __launch_bounds__(kScanBlock) void blelloch_tile_reduce_vec(const In*
 	if (tid == 0) tsum[tile] = tot;
 }
 
-template <typename In, typename Acc, int IT>
+template <typename In, typename Acc, bool SEG, int IT>
 __global__ __launch_bounds__(kScanBlock) void blelloch_tile_scan_vec(const In* __restrict__ in,
-	Acc* __restrict__ out, u64 n, u32 tiles, const Acc* __restrict__ carry_in) {
+	Acc* __restrict__ out, u64 n, u32 tiles, const Acc* __restrict__ carry_in,
+	const u32* __restrict__ off, u32 nseg) {
 	constexpr u32 kScanTile = kScanBlock * IT;
 	__shared__ Acc s_wsum[kScanBlock / kWave];
+	__shared__ u32 s_head[SEG ? kScanTile / 32 : 1];
+	__shared__ SegPair<Acc> s_pw[SEG ? kScanBlock / kWave : 1];
+	__shared__ u32 s_j0;
 	const u32 tile = xcd_tile_rev(tiles);
 	if (tile >= tiles) return;
 	const u32 tid = threadIdx.x;
 	const u64 first = static_cast<u64>(tile) * kScanTile + static_cast<u64>(tid) * IT;
 	const bool full = first + IT <= n;
+	if (SEG) mark_heads<IT>(s_head, &s_j0, off, nseg, static_cast<u64>(tile) * kScanTile);
 	In x[IT];
-	if (full) {
-		load_vec<In, IT>(in + first, x);
+	load_run<In, IT>(in, first, n, x);
+	Acc run;
+	u32 heads = 0;
+	if constexpr (SEG) {
+		heads = run_heads<IT>(s_head);
+		SegPair<Acc> tot;
+		const SegPair<Acc> ex = block_exclusive_segscan<kScanBlock, Acc>(
+			run_pair<In, Acc, IT>(x, heads), s_pw, &tot);
+		run = ex.f ? ex.v : carry_in[tile] + ex.v;
 	} else {
+		Acc sum = 0;
 #pragma unroll
-		for (int j = 0; j < IT; ++j) x[j] = first + j < n ? in[first + j] : In(0);
+		for (int j = 0; j < IT; ++j) sum += static_cast<Acc>(x[j]);
+		Acc tot;
+		run = carry_in[tile] + block_exclusive_sum<kScanBlock>(sum, s_wsum, &tot);
 	}
-	Acc sum = 0;
-#pragma unroll
-	for (int j = 0; j < IT; ++j) sum += static_cast<Acc>(x[j]);
-	Acc tot;
-	Acc run = carry_in[tile] + block_exclusive_sum<kScanBlock>(sum, s_wsum, &tot);
 	Acc y[IT];
 #pragma unroll
 	for (int j = 0; j < IT; ++j) {
+		if (SEG && ((heads >> j) & 1u)) run = 0;
 		y[j] = run;
 		run += static_cast<Acc>(x[j]);
 	}
@@ -746,9 +811,9 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 	const u32 grid = xcd_grid(a.tiles);
 	const In* in = static_cast<const In*>(a.in);
 	Acc* tsum = static_cast<Acc*>(a.tsum);
-	if constexpr (!SEG && (std::is_same<In, u32>::value || std::is_same<In, u64>::value) &&
+	if constexpr ((std::is_same<In, u32>::value || std::is_same<In, u64>::value) &&
 		sizeof(Acc) == OUTW) {
-		if (a.status && a.tiles <= kOnePassMaxTiles &&
+		if (!SEG && a.status && a.tiles <= kOnePassMaxTiles &&
 			((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0) {
 			Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_ONEPASS);
 			// status[0]: epoch; status[1]: unused; tiles' words from status[2]
@@ -761,19 +826,19 @@ bool launch_scan3(const ScanLaunch& a, GError** err) {
 		if (((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0) {
 			{
 				Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_REDUCE);
-				blelloch_tile_reduce_vec<In, Acc, IT><<<grid, kScanBlock, 0, st>>>(in, a.n,
-					a.tiles, tsum);
+				blelloch_tile_reduce_vec<In, Acc, SEG, IT><<<grid, kScanBlock, 0, st>>>(in, a.n,
+					a.tiles, tsum, a.off, a.nseg, a.tflag);
 				if (!L.done(err)) return false;
 			}
 			{
 				Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_TILES);
-				blelloch_tiles_scan<Acc, false><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles,
+				blelloch_tiles_scan<Acc, SEG><<<1, kTilesBlock, 0, st>>>(tsum, a.tflag, a.tiles,
 					static_cast<Acc*>(a.chain));
 				if (!L.done(err)) return false;
 			}
 			Launch L(a.q, CLO_SCAN_BLELLOCH_KNAME_SCAN);
-			blelloch_tile_scan_vec<In, Acc, IT><<<grid, kScanBlock, 0, st>>>(in,
-				static_cast<Acc*>(a.out), a.n, a.tiles, tsum);
+			blelloch_tile_scan_vec<In, Acc, SEG, IT><<<grid, kScanBlock, 0, st>>>(in,
+				static_cast<Acc*>(a.out), a.n, a.tiles, tsum, a.off, a.nseg);
 			return L.done(err);
 		}
 	}
