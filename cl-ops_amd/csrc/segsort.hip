@@ -1260,6 +1260,7 @@ __device__ __forceinline__ void sort_chunk_msd(const SegArgs<K>& a, MsdSmem<K, I
 			u32 l = 0;
 #pragma unroll
 			for (int b = 12; b >= 0; --b) {
+				if ((1u << b) >= m) continue;  // uniform: no segment that far
 				const u32 cc = l + (1u << b);
 				if (cc < m && s.lpos[cc] <= i) l = cc;
 			}
@@ -1275,9 +1276,29 @@ __device__ __forceinline__ void sort_chunk_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 		for (int r = 0; r < B; ++r) {
 			const u32 i = (r0 + r) * kRB + tid;
-			if (i < n) {
+			const bool in = i < n;
+			if (in) {
 				s.k[i] = tk[r];
 				s.sid[i] = static_cast<u16>(gsid[r]);
+			}
+			// A wave whose elements all lie in one segment (every wave but
+			// the few at segment boundaries, for segments of 64 keys and
+			// more) reduces them first and takes one atomic per bound.
+			const u32 g0 = __builtin_amdgcn_readfirstlane(gsid[r]);
+			if (ballot(in && gsid[r] != g0) == 0 && ballot(in) == ~0ull) {
+				K lo = tk[r], hi = tk[r];
+#pragma unroll
+				for (int off = kWave / 2; off > 0; off >>= 1) {
+					const K l2 = __shfl_xor(lo, off, kWave);
+					const K h2 = __shfl_xor(hi, off, kWave);
+					lo = l2 < lo ? l2 : lo;
+					hi = h2 > hi ? h2 : hi;
+				}
+				if (lane_id() == 0) {
+					atomicMin(&s.smin[g0], lo);
+					atomicMax(&smax[g0], hi);
+				}
+			} else if (in) {
 				atomicMin(&s.smin[gsid[r]], tk[r]);
 				atomicMax(&smax[gsid[r]], tk[r]);
 			}
@@ -1549,6 +1570,9 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 	}
 }
 
+// Medium or chunk items per work ticket of the bucketing kernel.
+constexpr u32 kSmallBatch = 8;
+
 // Resident blocks per CU the bucketing kernel is compiled for (its register
 // budget): four 256x8 tiles of 32-bit keys fit the LDS of a CU, three of
 // 64-bit keys.
@@ -1571,32 +1595,43 @@ __global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persi
 	const u32 nl = a.meta->n_large;
 	const u32 nlm = nl + a.meta->n_medium;
 	const u32 total = nlm + a.meta->n_chunks;
+	// Tickets: one per large segment, then one per kSmallBatch medium or chunk
+	// items (one atomic per item on the one queue counter bounds a queue of
+	// one-tile segments).
+	const u32 tickets = nl + (total - nl + kSmallBatch - 1) / kSmallBatch;
 	if (threadIdx.x == 0) s.misc[3] = atomicAdd(&a.meta->head_msd, 1u);
 	__syncthreads();
-	u32 item = __builtin_amdgcn_readfirstlane(s.misc[3]);
+	u32 t = __builtin_amdgcn_readfirstlane(s.misc[3]);
 	__syncthreads();
-	while (item < total) {
+	while (t < tickets) {
 		// The next ticket is taken now, its round trip under this item's work.
 		u32 ticket = 0;
 		if (threadIdx.x == 0) ticket = atomicAdd(&a.meta->head_msd, 1u);
-		if (item >= nlm) {
-			sort_chunk_msd<K, IPT>(a, s, a.chunks[item - nlm]);
-		} else {
-			const u32 j = item < nl ? large_seg(a, lord, item) : a.medium[item - nl];
-			const SegRange sr = seg_range(a.off, j, a.nseg, a.numel);
+		if (t < nl) {
+			const SegRange sr = seg_range(a.off, large_seg(a, lord, t), a.nseg, a.numel);
 			const u32 start = __builtin_amdgcn_readfirstlane(sr.start);
 			const u32 len = __builtin_amdgcn_readfirstlane(sr.len);
-			if (item >= nl) {
-				sort_medium_msd<K, IPT>(a, s, start, len);
-			} else if (!sort_large_msd<K, IPT>(a, s, start, len)) {
+			if (!sort_large_msd<K, IPT>(a, s, start, len)) {
 				__syncthreads();
 				if (a.kout != a.kin) g2g<K, IPT>(a.kout + start, a.kin + start, len);
 				if (threadIdx.x == 0) a.fb[atomicAdd(&a.meta->n_fb, 1u)] = make_uint2(start, len);
 			}
+		} else {
+			const u32 i0 = nl + (t - nl) * kSmallBatch;
+			const u32 i1 = total - i0 < kSmallBatch ? total : i0 + kSmallBatch;
+			for (u32 item = i0; item < i1; ++item) {
+				if (item >= nlm) {
+					sort_chunk_msd<K, IPT>(a, s, a.chunks[item - nlm]);
+				} else {
+					const SegRange sr = seg_range(a.off, a.medium[item - nl], a.nseg, a.numel);
+					sort_medium_msd<K, IPT>(a, s, __builtin_amdgcn_readfirstlane(sr.start),
+						__builtin_amdgcn_readfirstlane(sr.len));
+				}
+			}
 		}
 		if (threadIdx.x == 0) s.misc[3] = ticket;
 		__syncthreads();
-		item = __builtin_amdgcn_readfirstlane(s.misc[3]);
+		t = __builtin_amdgcn_readfirstlane(s.misc[3]);
 		__syncthreads();
 	}
 }
