@@ -207,6 +207,8 @@ struct SegArgs {
 	const u32* vin;
 	K* kout;   // final destination (== kin for in-place)
 	u32* vout;
+	const u32* kin32;  // composite sorts (K = u64): the 32-bit keys in and out
+	u32* kout32;
 	K* kscr;   // numel-sized scratch for large segments
 	u32* vscr;
 	const u32* off;
@@ -321,6 +323,80 @@ __device__ __forceinline__ void g2g(T* __restrict__ dst, const T* __restrict__ s
 			const u32 i = c0 + r * kRB + threadIdx.x;
 			if (i < n) dst[i] = t[r];
 		}
+	}
+}
+
+// Key-value sorts of 32-bit keys, out of place, on the keys-alone bucketing
+// kernel (COMP): it sorts 64-bit composites, key << 32 | position in the
+// segment.  Composites are unique and ordered as the stable order of the
+// pairs, so the unstable bucketing gives the stable result; the output takes
+// each composite's key and the value at its position.
+template <typename K, bool COMP>
+__device__ __forceinline__ K seg_key(const SegArgs<K>& a, u32 g, u32 pos) {
+	if constexpr (COMP) return (static_cast<K>(a.kin32[g]) << 32) | pos;
+	else return a.kin[g];
+}
+
+// Output element g (of the segment starting at seg0) from its sorted key.
+template <typename K, bool COMP>
+__device__ __forceinline__ void seg_put(const SegArgs<K>& a, u32 g, u32 seg0, K c) {
+	if constexpr (COMP) {
+		a.kout32[g] = static_cast<u32>(c >> 32);
+		a.vout[g] = a.vin[seg0 + static_cast<u32>(c)];
+	} else {
+		a.kout[g] = c;
+	}
+}
+
+// Segment [start, start + len) to the output unchanged (keys, and in COMP
+// the values).
+template <typename K, int IPT, bool COMP>
+__device__ __forceinline__ void seg_copy(const SegArgs<K>& a, u32 start, u32 len) {
+	if constexpr (COMP) {
+		g2g<u32, IPT>(a.kout32 + start, a.kin32 + start, len);
+		g2g<u32, IPT>(a.vout + start, a.vin + start, len);
+	} else {
+		if (a.kout != a.kin) g2g<K, IPT>(a.kout + start, a.kin + start, len);
+	}
+}
+
+// The n <= IPT * kRB keys of segment elements [g0, g0 + n) (positions
+// pos0.. in their segment) into LDS.
+template <typename K, int IPT, bool COMP>
+__device__ __forceinline__ void g2s_in(const SegArgs<K>& a, K* dst, u32 g0, u32 pos0, u32 n) {
+	if constexpr (COMP) {
+		constexpr int B = IPT < kBatch ? IPT : kBatch;
+#pragma unroll
+		for (int r0 = 0; r0 < IPT; r0 += B) {
+			K t[B];
+#pragma unroll
+			for (int r = 0; r < B; ++r) {
+				const u32 i = (r0 + r) * kRB + threadIdx.x;
+				t[r] = i < n ? seg_key<K, COMP>(a, g0 + i, pos0 + i) : K(0);
+			}
+#pragma unroll
+			for (int r = 0; r < B; ++r) {
+				const u32 i = (r0 + r) * kRB + threadIdx.x;
+				if (i < n) dst[i] = t[r];
+			}
+		}
+	} else {
+		g2s<K, IPT>(dst, a.kin + g0, n);
+	}
+}
+
+// Sorted keys src[0, n) to output elements [g0, g0 + n) of the segment
+// starting at seg0.
+template <typename K, int IPT, bool COMP>
+__device__ __forceinline__ void s2g_out(const SegArgs<K>& a, u32 g0, u32 seg0, const K* src, u32 n) {
+	if constexpr (COMP) {
+#pragma unroll
+		for (int r = 0; r < IPT; ++r) {
+			const u32 i = r * kRB + threadIdx.x;
+			if (i < n) seg_put<K, COMP>(a, g0 + i, seg0, src[i]);
+		}
+	} else {
+		s2g<K, IPT>(a.kout + g0, src, n);
 	}
 }
 
@@ -847,7 +923,7 @@ __device__ __forceinline__ void large_chunk(SegSmem<K, KV, IPT>& s, const K (&k)
 // Returns false, having written nothing to the output, when a range bucket
 // is over CAP/2 keys (keys concentrated in a small part of their range): the
 // segment is then left to the stable digit passes.
-template <typename K, int IPT>
+template <typename K, int IPT, bool COMP>
 __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, IPT>& s,
 	u32 start, u32 len) {
 	using S = MsdSmem<K, IPT>;
@@ -861,9 +937,8 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 	u32* wbk = s.hist[1];   // first bucket of each window
 	u32* ccnt = s.hist[2];  // per chunk: bucket counts, then cursors
 	u32* wbeg = s.hist[3];  // window starts (at most one per bucket)
-	const K* __restrict__ src = a.kin + start;
+	auto src = [&](u32 e) { return seg_key<K, COMP>(a, start + e, e); };
 	K* __restrict__ scr = a.kscr + start;
-	K* __restrict__ out = a.kout + start;
 	const u32 nch = len / CAP + (len % CAP != 0 ? 1u : 0u);
 	const bool cm = nch >= msd_chunks_min<K>() && nch <= kMsdChunks;  // chunk-major (uniform)
 	hcnt[tid] = 0;
@@ -877,7 +952,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 		for (int jj = 0; jj < IPT; ++jj) {
 			const u32 e = jj * kRB + tid;
-			kn[jj] = e < len ? src[e] : K(0);
+			kn[jj] = e < len ? src(e) : K(0);
 		}
 		for (u32 c = 0; c < nch; ++c) {
 			const u32 c0 = c * CAP;
@@ -891,7 +966,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 				for (int jj = 0; jj < IPT; ++jj) {
 					const u32 e = jj * kRB + tid;
-					kn[jj] = e < cnt1 ? src[c1 + e] : K(0);
+					kn[jj] = e < cnt1 ? src(c1 + e) : K(0);
 				}
 			}
 #pragma unroll
@@ -929,7 +1004,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 			for (int r = 0; r < kBatch; ++r) {
 				const u32 i = c0 + r * kRB + tid;
-				t[r] = i < len ? src[i] : src[0];
+				t[r] = i < len ? src(i) : src(0);
 			}
 #pragma unroll
 			for (int r = 0; r < kBatch; ++r) {
@@ -961,7 +1036,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 		hi = red[kRBWaves + ww] > hi ? red[kRBWaves + ww] : hi;
 	}
 	if (hi == lo) {  // uniform over the block: every key equal
-		if (out != src) g2g<K, IPT>(out, src, len);
+		seg_copy<K, IPT, COMP>(a, start, len);
 		return true;
 	}
 	__syncthreads();
@@ -983,7 +1058,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 			for (int r = 0; r < kBatch; ++r) {
 				const u32 i = c0 + r * kRB + tid;
-				t[r] = i < len ? src[i] : K(0);
+				t[r] = i < len ? src(i) : K(0);
 			}
 #pragma unroll
 			for (int r = 0; r < kBatch; ++r)
@@ -1037,7 +1112,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 		for (int jj = 0; jj < IPT; ++jj) {
 			const u32 e = jj * kRB + tid;
-			kn[jj] = e < len ? src[e] : K(0);
+			kn[jj] = e < len ? src(e) : K(0);
 		}
 		__syncthreads();
 		for (u32 c0 = 0; c0 < len; c0 += CAP) {
@@ -1051,7 +1126,7 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 				for (int jj = 0; jj < IPT; ++jj) {
 					const u32 e = jj * kRB + tid;
-					kn[jj] = e < cnt1 ? src[c1 + e] : K(0);
+					kn[jj] = e < cnt1 ? src(c1 + e) : K(0);
 				}
 			}
 #pragma unroll
@@ -1176,20 +1251,20 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 				msd_tile_sort<K, IPT>(s, n);
 			}
 		}
-		s2g<K, IPT>(out + p0, s.k, n);
+		s2g_out<K, IPT, COMP>(a, start + p0, start, s.k, n);
 		__syncthreads();
 	}
 	return true;
 }
 
 // Medium segment, keys alone: whole segment in LDS, msd_tile_sort.
-template <typename K, int IPT>
+template <typename K, int IPT, bool COMP>
 __device__ __forceinline__ void sort_medium_msd(const SegArgs<K>& a, MsdSmem<K, IPT>& s,
 	u32 start, u32 len) {
-	g2s<K, IPT>(s.k, a.kin + start, len);
+	g2s_in<K, IPT, COMP>(a, s.k, start, 0, len);
 	__syncthreads();
 	if (len > 1) msd_tile_sort<K, IPT>(s, len);
-	s2g<K, IPT>(a.kout + start, s.k, len);
+	s2g_out<K, IPT, COMP>(a, start, start, s.k, len);
 	__syncthreads();
 }
 
@@ -1200,7 +1275,7 @@ __device__ __forceinline__ void sort_medium_msd(const SegArgs<K>& a, MsdSmem<K, 
 // so every key stays inside its segment's range and one pass of counts,
 // cursors and small-bucket sorts orders every segment at once; buckets over
 // kMsdMaxBucket keys get levels of their own as in msd_tile_sort.
-template <typename K, int IPT>
+template <typename K, int IPT, bool COMP>
 __device__ __forceinline__ void sort_chunk_msd(const SegArgs<K>& a, MsdSmem<K, IPT>& s, u32 c) {
 	using S = MsdSmem<K, IPT>;
 	constexpr u32 W = S::kW;
@@ -1271,7 +1346,7 @@ __device__ __forceinline__ void sort_chunk_msd(const SegArgs<K>& a, MsdSmem<K, I
 #pragma unroll
 		for (int r = 0; r < B; ++r) {
 			const bool in = (r0 + r) * kRB + tid < n;
-			tk[r] = in ? a.kin[gidx[r]] : K(0);
+			tk[r] = in ? seg_key<K, COMP>(a, gidx[r], gidx[r] - s.segj[gsid[r]]) : K(0);
 		}
 #pragma unroll
 		for (int r = 0; r < B; ++r) {
@@ -1381,7 +1456,7 @@ __device__ __forceinline__ void sort_chunk_msd(const SegArgs<K>& a, MsdSmem<K, I
 		const u32 i = j * kRB + tid;
 		if (i < n) {
 			const u32 r = s.sid[i];
-			a.kout[s.segj[r] + (i - s.lpos[r])] = s.k[i];
+			seg_put<K, COMP>(a, s.segj[r] + (i - s.lpos[r]), s.segj[r], s.k[i]);
 		}
 	}
 	__syncthreads();
@@ -1523,14 +1598,17 @@ constexpr int seg_min_blocks() {
 		: (IPT >= 32 || (KV && sizeof(K) == 8)) ? 1 : 2;
 }
 
-template <typename K, bool KV, int IPT>
+// FB (key-value sorts): the fallback list of a composite sort on the
+// bucketing kernel, copied to the output and sorted there in place; a
+// separate instantiation, so that the others keep their register budgets.
+template <typename K, bool KV, int IPT, bool FB = false>
 __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_persistent(
 	SegArgs<K> a) {
 	__shared__ SegSmem<K, KV, IPT> s;
 	// fb_mode (keys alone, after seg_msd_persistent): every segment is done
 	// except those in the fallback list, which sit in the output array and
 	// are sorted there in place.
-	const bool fb = !KV && a.fb_mode;
+	const bool fb = (!KV || FB) && a.fb_mode;
 	const u32 nl = fb ? a.meta->n_fb : a.meta->n_large;
 	const u32 nm = fb ? 0u : a.meta->n_medium;
 	const u32 total = nl + nm + (fb ? 0u : a.meta->n_chunks);
@@ -1542,7 +1620,14 @@ __global__ __launch_bounds__(kRB, (seg_min_blocks<K, KV, IPT>())) void seg_sort_
 		__syncthreads();
 		if (item >= total) break;
 		if (item < nl) {
-			if constexpr (KV) {
+			if constexpr (KV && FB) {
+				const uint2 r = a.fb[item];
+				SegArgs<K> b = a;
+				b.kin = a.kout;
+				b.vin = a.vout;
+				sort_large<K, KV, IPT>(b, s, __builtin_amdgcn_readfirstlane(r.x),
+					__builtin_amdgcn_readfirstlane(r.y));
+			} else if constexpr (KV) {
 				const SegRange sr = seg_range(a.off, a.large_ix[item], a.nseg, a.numel);
 				sort_large<K, KV, IPT>(a, s, __builtin_amdgcn_readfirstlane(sr.start),
 					__builtin_amdgcn_readfirstlane(sr.len));
@@ -1587,7 +1672,7 @@ constexpr int msd_min_blocks() {
 // their range is copied to the output unsorted and listed for
 // seg_sort_persistent (fb_mode), which runs the stable digit passes on it in
 // place.
-template <typename K, int IPT>
+template <typename K, int IPT, bool COMP>
 __global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persistent(
 	SegArgs<K> a) {
 	__shared__ MsdSmem<K, IPT> s;
@@ -1611,9 +1696,9 @@ __global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persi
 			const SegRange sr = seg_range(a.off, large_seg(a, lord, t), a.nseg, a.numel);
 			const u32 start = __builtin_amdgcn_readfirstlane(sr.start);
 			const u32 len = __builtin_amdgcn_readfirstlane(sr.len);
-			if (!sort_large_msd<K, IPT>(a, s, start, len)) {
+			if (!sort_large_msd<K, IPT, COMP>(a, s, start, len)) {
 				__syncthreads();
-				if (a.kout != a.kin) g2g<K, IPT>(a.kout + start, a.kin + start, len);
+				seg_copy<K, IPT, COMP>(a, start, len);
 				if (threadIdx.x == 0) a.fb[atomicAdd(&a.meta->n_fb, 1u)] = make_uint2(start, len);
 			}
 		} else {
@@ -1621,10 +1706,10 @@ __global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persi
 			const u32 i1 = total - i0 < kSmallBatch ? total : i0 + kSmallBatch;
 			for (u32 item = i0; item < i1; ++item) {
 				if (item >= nlm) {
-					sort_chunk_msd<K, IPT>(a, s, a.chunks[item - nlm]);
+					sort_chunk_msd<K, IPT, COMP>(a, s, a.chunks[item - nlm]);
 				} else {
 					const SegRange sr = seg_range(a.off, a.medium[item - nl], a.nseg, a.numel);
-					sort_medium_msd<K, IPT>(a, s, __builtin_amdgcn_readfirstlane(sr.start),
+					sort_medium_msd<K, IPT, COMP>(a, s, __builtin_amdgcn_readfirstlane(sr.start),
 						__builtin_amdgcn_readfirstlane(sr.len));
 				}
 			}
@@ -1668,7 +1753,7 @@ bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, bool m
 		if (msd) {
 			// Keys alone: large and medium segments by MSD bucketing first.
 			Launch L(q, "segsort_msd");
-			seg_msd_persistent<K, IPT><<<grid, kRB, 0, st>>>(a);
+			seg_msd_persistent<K, IPT, false><<<grid, kRB, 0, st>>>(a);
 			if (!L.done(err)) return false;
 			a.fb_mode = 1;
 		}
@@ -1679,6 +1764,42 @@ bool seg_launch(CCLQueue* q, SegData* sd, const SegArgs<K>& a0, u32 grid, bool m
 		if (!L.done(err)) return false;
 	}
 	return true;
+}
+
+// Key-value sort of 32-bit keys out of place: 64-bit composites on the
+// bucketing kernel (tile IPT), then the key-value digit passes (tile 8) on
+// the segments it left, sorted in place in the output.  a64 and a32 share
+// the lists, the metadata and the scratch array.
+template <int IPT>
+bool seg_launch_comp(CCLQueue* q, const SegArgs<u64>& a64, const SegArgs<u32>& a32, u32 grid,
+	GError** err) {
+	constexpr u32 CAP = kRB * IPT;
+	constexpr u32 W = CAP / 2;
+	hipStream_t st = q->stream;
+	{
+		Launch L(q, "segsort_reset");
+		seg_reset<<<1, 64, 0, st>>>(a64.meta);
+		if (!L.done(err)) return false;
+	}
+	{
+		Launch L(q, "segsort_classify");
+		const u32 g = static_cast<u32>(div_ceil(a64.nseg, 256));
+		seg_classify<<<g ? g : 1, 256, 0, st>>>(a64.off, a64.nseg, a64.numel, W, CAP, a64.cls_lo,
+			a64.cls_hi, const_cast<u32*>(a64.medium), const_cast<u32*>(a64.large_ix),
+			const_cast<u32*>(a64.large), a64.lbase, const_cast<u32*>(a64.chunks),
+			const_cast<uint2*>(a64.wrange), a64.meta);
+		if (!L.done(err)) return false;
+	}
+	{
+		Launch L(q, "segsort_msd");
+		seg_msd_persistent<u64, IPT, true><<<grid, kRB, 0, st>>>(a64);
+		if (!L.done(err)) return false;
+	}
+	SegArgs<u32> b = a32;
+	b.fb_mode = 1;
+	Launch L(q, "segsort_persistent");
+	seg_sort_persistent<u32, true, 8, true><<<grid, kRB, 0, st>>>(b);
+	return L.done(err);
 }
 
 }  // namespace
@@ -1700,6 +1821,16 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 		set_error(err, kQuarkClo, CLO_ERROR_ARGS, "segmented tile items must be 8, 16 or 32");
 		return nullptr;
 	}
+	const bool kv = vin != nullptr;
+	const bool inplace = kout == nullptr || kout->ptr == kin->ptr;
+	// Key-value sorts of 32-bit keys out of place take the bucketing kernel on
+	// composites (key << 32 | position), tile 8 unless a segtile option asks
+	// for 16.  In place they keep the digit passes: the values are gathered
+	// from their input positions when the output is written.
+	const bool comp = kv && ks == 4 && !inplace && vout && vout->ptr != vin->ptr &&
+		sorter->seg_msd != 0 && (sorter->seg_tile == 0 || sorter->seg_tile == 8 ||
+			sorter->seg_tile == 16);
+	if (comp) tile_items = sorter->seg_tile ? sorter->seg_tile : 8;
 	// medium and large lists (nseg entries each), large lists by length
 	// class, chunk list (nseg entries), then [j0, j1) per window of the
 	// smallest tile's W = 1024 elements and the fallback list (nseg entries).
@@ -1709,7 +1840,7 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	const size_t nlists = (3 * nseg + nlarge + 3) / 4 * 4 + 4;
 	if (!sd->lists.ensure(nlists * sizeof(u32) + (nwin + nseg) * sizeof(uint2), err) ||
 		!sd->meta.ensure(sizeof(SegMeta), err) ||
-		!sd->kscr.ensure(numel * ks, err) || (vin && !sd->vscr.ensure(numel * 4, err)))
+		!sd->kscr.ensure(numel * (comp ? 8 : ks), err) || (vin && !sd->vscr.ensure(numel * 4, err)))
 		return nullptr;
 	if (grid == 0) {
 		int dev = cq->ctx->dev, cus = 0;
@@ -1719,11 +1850,10 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 		grid = static_cast<u32>(cus) * ((vin && ks == 8) ? kSegBlocksPerCuKv64 : kSegBlocksPerCuDefault);
 	}
 	u32* lists = static_cast<u32*>(sd->lists.ptr);
-	const bool kv = vin != nullptr;
 	CLO_LOG(1, "segmented sort: nseg=%zu numel=%zu key=%zu B%s tile=256x%u (CAP %u) grid=%u "
-		"classes=%s", nseg, numel, ks, kv ? " +u32 values" : "", tile_items, 256 * tile_items, grid,
-		(sorter->seg_cls[0] | sorter->seg_cls[1]) ? "per length bin" : "by length");
-	const bool inplace = kout == nullptr || kout->ptr == kin->ptr;
+		"classes=%s%s", nseg, numel, ks, kv ? " +u32 values" : "", tile_items, 256 * tile_items,
+		grid, (sorter->seg_cls[0] | sorter->seg_cls[1]) ? "per length bin" : "by length",
+		comp ? " composite keys" : "");
 	// MSD bucketing for keys alone (32768 segments of log-uniform lengths up to
 	// 65536, tile 8: 32-bit keys 1.6 ms against 2.6 on the digit passes, 64-bit
 	// keys 2.2 against 7.5).
@@ -1735,6 +1865,8 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	a.kout = inplace ? static_cast<K*>(kin->ptr) : static_cast<K*>(kout->ptr);                \
 	a.vout = kv ? (inplace ? static_cast<u32*>(vin->ptr) : static_cast<u32*>(vout->ptr))     \
 		: nullptr;                                                                           \
+	a.kin32 = nullptr;                                                                       \
+	a.kout32 = nullptr;                                                                      \
 	a.kscr = static_cast<K*>(sd->kscr.ptr);                                                  \
 	a.vscr = kv ? static_cast<u32*>(sd->vscr.ptr) : nullptr;                                  \
 	a.off = static_cast<const u32*>(offsets->ptr);                                           \
@@ -1753,7 +1885,31 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	a.numel = static_cast<u32>(numel);                                                       \
 
 	bool ok = false;
-	if (ks == 4) {
+	if (comp) {
+		SEG_ARGS(u32)
+		SegArgs<u64> c = {};
+		c.vin = a.vin;
+		c.vout = a.vout;
+		c.kin32 = a.kin;
+		c.kout32 = a.kout;
+		c.kscr = static_cast<u64*>(sd->kscr.ptr);
+		c.off = a.off;
+		c.medium = a.medium;
+		c.large_ix = a.large_ix;
+		c.large = a.large;
+		c.lbase = a.lbase;
+		c.chunks = a.chunks;
+		c.wrange = a.wrange;
+		c.fb = a.fb;
+		c.fb_mode = 0;
+		c.meta = a.meta;
+		c.cls_lo = a.cls_lo;
+		c.cls_hi = a.cls_hi;
+		c.nseg = a.nseg;
+		c.numel = a.numel;
+		ok = tile_items == 16 ? seg_launch_comp<16>(cq, c, a, grid, err)
+			: seg_launch_comp<8>(cq, c, a, grid, err);
+	} else if (ks == 4) {
 		SEG_ARGS(u32)
 		switch (tile_items) {
 		case 8: ok = kv ? seg_launch<u32, true, 8>(cq, sd, a, grid, msd, err) : seg_launch<u32, false, 8>(cq, sd, a, grid, msd, err); break;

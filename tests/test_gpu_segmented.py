@@ -256,3 +256,36 @@ def test_segmented_keys_only_distributions(clops, oracle, ctx, queue, tile, dtyp
     assert np.array_equal(got, rk)
     if out_of_place:
         assert np.array_equal(kb.read(queue, dtype, n), k)  # input untouched
+
+
+@pytest.mark.parametrize("tile", [8, 16])
+@pytest.mark.parametrize("dist", ["uniform", "constant", "two", "clustered", "dups", "head",
+                                  "sorted", "reversed"])
+@pytest.mark.parametrize("msd", [0, 1])
+def test_segmented_kv_out_of_place(clops, oracle, ctx, queue, tile, dist, msd):
+    """Key-value, 32-bit keys, out of place: with segmsd=1 the bucketing kernel
+    sorts 64-bit composites (key, position in the segment), and segments it
+    cannot bucket go to the key-value digit passes in the output; with
+    segmsd=0 the digit passes take everything.  Stable order (equal keys keep
+    their input order) and the values that travel with the keys, against the
+    oracle's per-segment stable sort; inputs untouched."""
+    cap = 256 * tile
+    lens = np.array([0, 1, 5, cap // 2 + 3, cap - 1, cap, cap + 1, 3 * cap + 5, 70000, 2,
+                     cap * 2, 17, 40000, 300000], dtype=np.uint32)
+    off = _offsets(lens)
+    nseg, n = lens.size, int(lens.sum(dtype=np.uint64))
+    k = _keys_for(oracle, dist, n, np.uint32, 0xE15 + tile)
+    v = (oracle.generate(oracle.GEN_U64, 0xE16, n) >> np.uint64(32)).astype(np.uint32)
+    s = clops.Sorter(ctx, "satradix", np.uint32, options="segtile=%d,segmsd=%d" % (tile, msd))
+    kb = clops.Buffer.from_array(ctx, k)
+    vb = clops.Buffer.from_array(ctx, v)
+    ob = clops.Buffer.from_array(ctx, off)
+    ko = clops.Buffer(ctx, n * 4)
+    vo = clops.Buffer(ctx, n * 4)
+    s.sort_segmented_device(queue, kb, ob, nseg, n, vin=vb, kout=ko, vout=vo)
+    queue.finish()
+    rk, rv = _ref_sort(oracle, k, v, off, n)
+    assert np.array_equal(ko.read(queue, np.uint32, n), rk)
+    assert np.array_equal(vo.read(queue, np.uint32, n), rv)
+    assert np.array_equal(kb.read(queue, np.uint32, n), k)
+    assert np.array_equal(vb.read(queue, np.uint32, n), v)
