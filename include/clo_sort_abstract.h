@@ -118,11 +118,13 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
  * 8|16 for 64-bit keys; default 8, and 16 for 32-bit key-value sorts),
  * seggrid=<blocks> (0 = four per compute unit, two for 64-bit key-value
  * sorts), segclass=<up to 64 digits 0|1|2> (the least work class of each
- * segment length bin, see clo_dispatch.h; default all 0), segmsd=0|1 (keys
- * alone: 1, the default, sorts medium and large segments by MSD bucketing,
- * whose order of equal keys is not the input order but whose output words
- * are the same, since equal keys are equal bit patterns; 0: stable digit
- * passes only).  No option changes the result. */
+ * segment length bin, see clo_dispatch.h; default all 0), segmsd=0|1 (1,
+ * the default: keys alone are sorted by MSD bucketing, whose order of equal
+ * keys is not the input order but whose output words are the same, since
+ * equal keys are equal bit patterns; key-value sorts of 32-bit keys out of
+ * place are bucketed as 64-bit (key, input position) composites, which keeps
+ * the stable order, with tile 8 unless segtile=16; 0: stable digit passes
+ * only).  No option changes the result. */
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
 	CCLBuffer* vals_out, CCLBuffer* offsets, size_t nseg, size_t numel, GError** err);
