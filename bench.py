@@ -539,6 +539,32 @@ def segmented_only(ctx, width, steps, warmup, options=None):
             clops.multiset_hash(q, kout, n, width)}
 
 
+def segmented_summary(ctx, steps=10, warmup=3):
+    """The 32768-segment workload (log-uniform lengths up to 65536, 1.94e8
+    u32 keys) as the default run reports it: the keys-alone segmented sort
+    (out of place, library defaults, every segment checked for order and the
+    multiset kept) and the segmented scan over the same offsets."""
+    sort = segmented_only(ctx, 4, steps, warmup)
+    q = clops.Queue(ctx)
+    lens, off = c4_offsets(ctx, q)
+    nseg, n = lens.size, int(lens.sum(dtype=np.uint64))
+    kin = clops.Buffer(ctx, n * 4)
+    sout = clops.Buffer(ctx, n * 4)
+    clops.generate(q, kin, n, clops.GEN_U32_HI, 0xC4B)
+    offb = clops.Buffer.from_array(ctx, off)
+    sc = clops.Scanner(ctx, np.uint32, np.uint32)
+    ms_scan = time_steps(q, lambda: sc.scan_segmented_device(q, kin, sout, offb, nseg, n),
+                         steps, warmup)
+    return {"workload": "32768 segments, log-uniform lengths in [1, 65536] from U(0xC4A, j), "
+                        "%d u32 keys U(0xC4B, i) >> 32" % n,
+            "sort_ms": round(sort["ms"], 4), "sort_gkeys_per_s": round(sort["Gkeys/s"], 2),
+            "sort_unsorted_pairs": sort["unsorted_pairs"],
+            "sort_multiset_equal": sort["multiset_equal"],
+            "sort_hbm_frac": round(n * 8 / sort["ms"] / 1e6 / HBM_PEAK_GBS, 4),
+            "scan_ms": round(ms_scan, 4),
+            "scan_hbm_frac": round(n * 8 / ms_scan / 1e6 / HBM_PEAK_GBS, 4)}
+
+
 def seg_tile(cfg):
     """Items per thread of segmented configuration cfg (12 + 4 t + g - 2)."""
     return (8, 16, 32)[(cfg - 12) // 4]
@@ -1051,6 +1077,8 @@ def main():
     ap.add_argument("--seg-options", default=None,
                     help="sorter options for --segmented (e.g. segmsd=0)")
     ap.add_argument("--no-dispatch", action="store_true")
+    ap.add_argument("--no-segmented", action="store_true",
+                    help="leave the 32768-segment sort and scan out of the default line")
     ap.add_argument("--dispatch-train", type=int, default=2000)
     ap.add_argument("--dispatch-eval", type=int, default=1500)
     ap.add_argument("--dispatch-full", type=int, default=100000,
@@ -1079,6 +1107,10 @@ def main():
     if not args.no_dispatch:
         disp = dispatch_eval(ctx, args.dispatch_train, args.dispatch_eval, args.dispatch_full)
         log("dispatch: " + json.dumps(disp))
+    seg = None
+    if not args.no_segmented:
+        seg = segmented_summary(ctx)
+        log("segmented: " + json.dumps(seg))
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -1127,6 +1159,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "dispatch": disp,
+        "segmented": seg,
     }
     print(json.dumps(line), flush=True)
 
