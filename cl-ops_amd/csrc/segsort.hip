@@ -937,8 +937,17 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 	u32* wbk = s.hist[1];   // first bucket of each window
 	u32* ccnt = s.hist[2];  // per chunk: bucket counts, then cursors
 	u32* wbeg = s.hist[3];  // window starts (at most one per bucket)
-	auto src = [&](u32 e) { return seg_key<K, COMP>(a, start + e, e); };
+	// The segment's keys through base pointers.
+	const K* __restrict__ ksrc = nullptr;
+	const u32* __restrict__ ksrc32 = nullptr;
+	if constexpr (COMP) ksrc32 = a.kin32 + start;
+	else ksrc = a.kin + start;
+	auto src = [&](u32 e) -> K {
+		if constexpr (COMP) return (static_cast<K>(ksrc32[e]) << 32) | e;
+		else return ksrc[e];
+	};
 	K* __restrict__ scr = a.kscr + start;
+	K* __restrict__ out = a.kout + start;  // keys alone
 	const u32 nch = len / CAP + (len % CAP != 0 ? 1u : 0u);
 	const bool cm = nch >= msd_chunks_min<K>() && nch <= kMsdChunks;  // chunk-major (uniform)
 	hcnt[tid] = 0;
@@ -1251,7 +1260,11 @@ __device__ __forceinline__ bool sort_large_msd(const SegArgs<K>& a, MsdSmem<K, I
 				msd_tile_sort<K, IPT>(s, n);
 			}
 		}
-		s2g_out<K, IPT, COMP>(a, start + p0, start, s.k, n);
+		// Keys alone store through the segment's own restrict pointer (the
+		// generic store measured 4-6% slower: 1.37 -> 1.42 ms for 32768
+		// segments of log-uniform lengths, 32-bit keys).
+		if constexpr (COMP) s2g_out<K, IPT, COMP>(a, start + p0, start, s.k, n);
+		else s2g<K, IPT>(out + p0, s.k, n);
 		__syncthreads();
 	}
 	return true;
@@ -1704,14 +1717,37 @@ __global__ __launch_bounds__(kRB, (msd_min_blocks<K, IPT>())) void seg_msd_persi
 		} else {
 			const u32 i0 = nl + (t - nl) * kSmallBatch;
 			const u32 i1 = total - i0 < kSmallBatch ? total : i0 + kSmallBatch;
+			// Medium items: the next one's keys are in flight during the
+			// current one's sort.
+			K kw[IPT];
+			u32 nstart = 0, nlen = 0;
+			auto fetch = [&](u32 item) {
+				const SegRange sr = seg_range(a.off, a.medium[item - nl], a.nseg, a.numel);
+				nstart = __builtin_amdgcn_readfirstlane(sr.start);
+				nlen = __builtin_amdgcn_readfirstlane(sr.len);
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj) {
+					const u32 e = jj * kRB + threadIdx.x;
+					kw[jj] = e < nlen ? seg_key<K, COMP>(a, nstart + e, e) : K(0);
+				}
+			};
+			if (i0 < nlm) fetch(i0);
 			for (u32 item = i0; item < i1; ++item) {
 				if (item >= nlm) {
 					sort_chunk_msd<K, IPT, COMP>(a, s, a.chunks[item - nlm]);
-				} else {
-					const SegRange sr = seg_range(a.off, a.medium[item - nl], a.nseg, a.numel);
-					sort_medium_msd<K, IPT, COMP>(a, s, __builtin_amdgcn_readfirstlane(sr.start),
-						__builtin_amdgcn_readfirstlane(sr.len));
+					continue;
 				}
+				const u32 start = nstart, len = nlen;
+#pragma unroll
+				for (int jj = 0; jj < IPT; ++jj) {
+					const u32 e = jj * kRB + threadIdx.x;
+					if (e < len) s.k[e] = kw[jj];
+				}
+				if (item + 1 < i1 && item + 1 < nlm) fetch(item + 1);
+				__syncthreads();
+				if (len > 1) msd_tile_sort<K, IPT>(s, len);
+				s2g_out<K, IPT, COMP>(a, start, start, s.k, len);
+				__syncthreads();
 			}
 		}
 		if (threadIdx.x == 0) s.misc[3] = ticket;
