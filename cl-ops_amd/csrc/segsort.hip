@@ -1859,13 +1859,14 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 	}
 	const bool kv = vin != nullptr;
 	const bool inplace = kout == nullptr || kout->ptr == kin->ptr;
-	// Key-value sorts of 32-bit keys out of place take the bucketing kernel on
-	// composites (key << 32 | position), tile 8 unless a segtile option asks
-	// for 16.  In place they keep the digit passes: the values are gathered
-	// from their input positions when the output is written.
-	const bool comp = kv && ks == 4 && !inplace && vout && vout->ptr != vin->ptr &&
-		sorter->seg_msd != 0 && (sorter->seg_tile == 0 || sorter->seg_tile == 8 ||
-			sorter->seg_tile == 16);
+	// Key-value sorts of 32-bit keys take the bucketing kernel on composites
+	// (key << 32 | position), tile 8 unless a segtile option asks for 16.  The
+	// values are gathered from their input positions when the output is
+	// written; in place they are first copied to the value scratch array, so
+	// that the gather reads values no output has overwritten (a segment's keys
+	// are all read before its output is written).
+	const bool comp = kv && ks == 4 && sorter->seg_msd != 0 &&
+		(sorter->seg_tile == 0 || sorter->seg_tile == 8 || sorter->seg_tile == 16);
 	if (comp) tile_items = sorter->seg_tile ? sorter->seg_tile : 8;
 	// medium and large lists (nseg entries each), large lists by length
 	// class, chunk list (nseg entries), then [j0, j1) per window of the
@@ -1926,6 +1927,14 @@ CCLEvent* clo_satradix_sort_segmented(CloSort* sorter, CCLQueue* cq, CCLBuffer* 
 		SegArgs<u64> c = {};
 		c.vin = a.vin;
 		c.vout = a.vout;
+		if (inplace) {
+			if (hipMemcpyAsync(a.vscr, a.vin, numel * sizeof(u32), hipMemcpyDeviceToDevice,
+					cq->stream) != hipSuccess) {
+				set_error(err, kQuarkClo, CLO_ERROR_LIBRARY, "segmented sort: value copy failed");
+				return nullptr;
+			}
+			c.vin = a.vscr;
+		}
 		c.kin32 = a.kin;
 		c.kout32 = a.kout;
 		c.kscr = static_cast<u64*>(sd->kscr.ptr);

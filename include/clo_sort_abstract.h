@@ -121,9 +121,9 @@ cl_bool clo_sort_kv_with_host_data(CloSort* sorter, CCLQueue* cq_exec,
  * segment length bin, see clo_dispatch.h; default all 0), segmsd=0|1 (1,
  * the default: keys alone are sorted by MSD bucketing, whose order of equal
  * keys is not the input order but whose output words are the same, since
- * equal keys are equal bit patterns; key-value sorts of 32-bit keys out of
- * place are bucketed as 64-bit (key, input position) composites, which keeps
- * the stable order, with tile 8 unless segtile=16; 0: stable digit passes
+ * equal keys are equal bit patterns; key-value sorts of 32-bit keys are
+ * bucketed as 64-bit (key, input position) composites, which keeps the
+ * stable order, with tile 8 unless segtile=16; 0: stable digit passes
  * only).  No option changes the result. */
 CCLEvent* clo_sort_segmented_with_device_data(CloSort* sorter, CCLQueue* cq_exec,
 	CCLQueue* cq_comm, CCLBuffer* keys_in, CCLBuffer* vals_in, CCLBuffer* keys_out,
