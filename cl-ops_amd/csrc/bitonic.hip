@@ -347,6 +347,95 @@ __global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_reg(
 	}
 }
 
+// ---- local half-cleaner passes in registers ------------------------------
+// A local pass of stage s > k (steps t = k..1 on every tile of 2^LOGT keys,
+// all half-cleaners: the stage's flip step is a global one) runs every step
+// in registers.  A thread holding R = 2^LR keys at stride 2^A (positions
+// hi * 2^(A+LR) + m * 2^A + lo, m = 0..R-1) owns every comparator of the
+// LR steps with distances 2^A .. 2^(A+LR-1); the steps go in chunks of LR
+// from the top, each chunk in its own layout, with one LDS round trip
+// between chunks (2 for 2^14 keys at 32 per thread, against 5 for the
+// register-tail kernel's grouped LDS steps).
+
+// Half-cleaners of distances 2^EHI .. 2^ELO (relative to the thread's stride).
+template <typename T, int R, int EHI, int ELO>
+__device__ __forceinline__ void reg_halves(T (&x)[R]) {
+	if constexpr (EHI >= ELO) {
+		reg_half<T, R, (1 << EHI)>(x);
+		reg_halves<T, R, EHI - 1, ELO>(x);
+	}
+}
+
+// Chunk C (steps THI = LOGT - C*LR down to A + 1) and the chunks after it.
+template <typename T, int R, int LR, int LOGT, int C>
+__device__ __forceinline__ void half_chunks(T (&x)[R], T* s_t) {
+	constexpr int THI = LOGT - C * LR;
+	if constexpr (THI >= 1) {
+		constexpr int A = THI - LR > 0 ? THI - LR : 0;
+		const u32 tid = threadIdx.x;
+		const u32 lo = tid & ((1u << A) - 1u);
+		const u32 hi = tid >> A;
+#pragma unroll
+		for (int m = 0; m < R; ++m)
+			x[m] = s_t[bpad((hi << (A + LR)) + (static_cast<u32>(m) << A) + lo)];
+		reg_halves<T, R, THI - 1 - A, 0>(x);
+#pragma unroll
+		for (int m = 0; m < R; ++m)
+			s_t[bpad((hi << (A + LR)) + (static_cast<u32>(m) << A) + lo)] = x[m];
+		__syncthreads();
+		half_chunks<T, R, LR, LOGT, C + 1>(x, s_t);
+	}
+}
+
+// One local pass on tiles of exactly 2^LOGT keys (the last tile may be
+// ragged: positions past the array hold the largest key, as in
+// bitonic_tile_reg, and are never stored).
+template <typename T, int R, int LOGT>
+__global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_half(T* __restrict__ data, u64 n,
+	T xmask) {
+	constexpr u32 NT = (1u << LOGT) / R;
+	constexpr int LR = R == 32 ? 5 : 4;
+	static_assert(R == 16 || R == 32, "16 or 32 keys per thread");
+	static_assert(LOGT >= 2 * LR, "at least two chunks");
+	constexpr u32 tile_n = 1u << LOGT;
+	__shared__ T s_t[bpad(tile_n)];
+	const u64 base = static_cast<u64>(blockIdx.x) << LOGT;
+	if (base >= n) return;
+	const u32 tid = threadIdx.x;
+	T* const tile = data + base;
+	const u64 left = n - base;
+	const u32 cnt = left < tile_n ? static_cast<u32>(left) : tile_n;
+	constexpr u32 E = 16 / sizeof(T);  // keys per 16-byte word
+	const bool vec = cnt == tile_n && (reinterpret_cast<uintptr_t>(tile) & 15) == 0;
+	if (vec) {
+		for (u32 w = tid; w < tile_n / E; w += NT) {
+			const uint4 v = reinterpret_cast<const uint4*>(tile)[w];
+			T e[E];
+			__builtin_memcpy(e, &v, 16);
+#pragma unroll
+			for (u32 i = 0; i < E; ++i) s_t[bpad(w * E + i)] = static_cast<T>(e[i] ^ xmask);
+		}
+	} else {
+		for (u32 i = tid; i < tile_n; i += NT)
+			s_t[bpad(i)] = i < cnt ? static_cast<T>(tile[i] ^ xmask) : static_cast<T>(~T(0));
+	}
+	__syncthreads();
+	T x[R];
+	half_chunks<T, R, LR, LOGT, 0>(x, s_t);
+	if (vec) {
+		for (u32 w = tid; w < tile_n / E; w += NT) {
+			T e[E];
+#pragma unroll
+			for (u32 i = 0; i < E; ++i) e[i] = static_cast<T>(s_t[bpad(w * E + i)] ^ xmask);
+			uint4 v;
+			__builtin_memcpy(&v, e, 16);
+			reinterpret_cast<uint4*>(tile)[w] = v;
+		}
+	} else {
+		for (u32 i = tid; i < cnt; i += NT) tile[i] = static_cast<T>(s_t[bpad(i)] ^ xmask);
+	}
+}
+
 // Keys per thread held in registers by the tile kernels: 0 runs the plain
 // LDS kernel (tiles of 2^11), 16 or 32 the register-tail kernel; the default
 // is 32 for 4-byte keys and 16 otherwise (see TileMax).
@@ -420,6 +509,20 @@ bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_f
 	const int r = regkeys_of(bd, sizeof(T));
 	const u32 lt = k < kTileLogMin ? kTileLogMin : k;  // LDS tile of the instance
 	Launch L(q, nm);
+	if (s_first > k && s_first == s_last && k == lt && r != 0) {
+		// A local pass (half-cleaners only): every step in registers.
+#define CLO_TILE_HALF(R_, LT_)                                                                   \
+	bitonic_tile_half<T, R_, LT_><<<grid, (1u << LT_) / R_, 0, q->stream>>>(d, n, xm)
+		bool done = true;
+		if (r == 16 && lt == 12) CLO_TILE_HALF(16, 12);
+		else if (r == 16 && lt == 13) CLO_TILE_HALF(16, 13);
+		else if (r == 16 && lt == 14) CLO_TILE_HALF(16, 14);
+		else if (r == 32 && lt == 13) CLO_TILE_HALF(32, 13);
+		else if (r == 32 && lt == 14) CLO_TILE_HALF(32, 14);
+		else done = false;
+#undef CLO_TILE_HALF
+		if (done) return L.done(err);
+	}
 #define CLO_TILE_REG(R_, LT_)                                                                    \
 	bitonic_tile_reg<T, R_, LT_><<<grid, (1u << LT_) / R_, 0, q->stream>>>(d, n, k, s_first,   \
 		s_last, xm)

@@ -60,7 +60,7 @@ def test_abitonic_options(clops, oracle, ctx, opts):
 
 
 @pytest.mark.parametrize("regkeys", [0, 16, 32])
-@pytest.mark.parametrize("maxsfs", [5, 9, 11, 12, 13])
+@pytest.mark.parametrize("maxsfs", [5, 9, 11, 12, 13, 14])
 @pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint32, np.int64])
 def test_abitonic_tile_kernels(clops, oracle, ctx, regkeys, maxsfs, dtype):
     """Every tile kernel (plain LDS steps, 16 or 32 register keys per
@@ -69,6 +69,19 @@ def test_abitonic_tile_kernels(clops, oracle, ctx, regkeys, maxsfs, dtype):
     for n in (1 << 15, 40003):
         k = _keys(oracle, dtype, n, 0xBB + n)
         assert np.array_equal(s.sort_host(k), np.sort(k))
+
+
+@pytest.mark.parametrize("dtype", INT_TYPES)
+@pytest.mark.parametrize("desc", [False, True])
+def test_abitonic_local_passes(clops, oracle, ctx, dtype, desc):
+    """Sizes of several default tiles (2^12..2^14 keys), whole and ragged, so
+    every stage above the tile ends in a local pass of half-cleaners, in both
+    orders and for signed keys."""
+    s = clops.Sorter(ctx, "abitonic", dtype, compare="((a) < (b))" if desc else None)
+    for n in (1 << 17, (1 << 16) + 4099):
+        k = _keys(oracle, dtype, n, 0xBC + n)
+        ref = np.sort(k)[::-1] if desc else np.sort(k)
+        assert np.array_equal(s.sort_host(k), ref)
 
 
 def test_abitonic_bad_options(clops, ctx):
