@@ -34,13 +34,15 @@ constexpr int kBitBlock = 256;
 // small tiles keep many blocks resident per CU.
 constexpr u32 kTileLogMin = 11;
 constexpr u32 tile_log_cap(size_t key_bytes) { return key_bytes >= 8 ? 14u : 15u; }
-// Default tile per key width, measured on MI355X (abitonic 2^24 keys):
-// 4-byte keys 2^14 keys with 32 per thread (1.149 ms; 2^13 x 16: 1.194,
-// 2^15 x 32: 1.262), 8-byte keys 2^12 with 16 per thread (2.228 ms; 2^13:
-// 2.239, 2^14: 2.487), 1- and 2-byte keys 2^13 with 16.
+// Default tile per key width, measured on MI355X (abitonic 2^24 keys, tile
+// sort and local passes in registers): 4-byte keys 2^14 keys with 32 per
+// thread (0.750 ms; 2^14 x 16: 0.785, 2^13 x 32: 0.841, 2^15 x 32: 1.246),
+// 8-byte keys 2^12 with 16 per thread (1.850 ms; 2^13 x 16: 1.908, 2^13 x 32:
+// 2.082, 2^14 x 16: 2.167), 2-byte keys 2^14 with 32 (0.606 ms; 2^13 x 16:
+// 0.707, 2^14 x 16: 0.672), and 1-byte keys as 2-byte ones.
 template <typename T>
 struct TileMax {
-	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : sizeof(T) == 4 ? 14 : 13;
+	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : 14;
 };
 // The plain LDS kernels (one synchronised step at a time) take tiles of up
 // to 2^11 keys.
@@ -357,6 +359,18 @@ __global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_reg(
 // between chunks (2 for 2^14 keys at 32 per thread, against 5 for the
 // register-tail kernel's grouped LDS steps).
 
+// Padded LDS index of the thread's key m at stride 2^A: the position's three
+// parts (hi << (A + LR), m << A, lo < 2^A) have disjoint bits, so bpad splits
+// over their sum and the part of m is a compile-time offset.
+template <int A, int LR>
+__device__ __forceinline__ u32 layout_base(u32 tid) {
+	return bpad((tid >> A) << (A + LR)) + bpad(tid & ((1u << A) - 1u));
+}
+template <int A>
+__device__ __forceinline__ constexpr u32 layout_off(int m) {
+	return bpad(static_cast<u32>(m) << A);
+}
+
 // Half-cleaners of distances 2^EHI .. 2^ELO (relative to the thread's stride).
 template <typename T, int R, int EHI, int ELO>
 __device__ __forceinline__ void reg_halves(T (&x)[R]) {
@@ -372,18 +386,157 @@ __device__ __forceinline__ void half_chunks(T (&x)[R], T* s_t) {
 	constexpr int THI = LOGT - C * LR;
 	if constexpr (THI >= 1) {
 		constexpr int A = THI - LR > 0 ? THI - LR : 0;
-		const u32 tid = threadIdx.x;
-		const u32 lo = tid & ((1u << A) - 1u);
-		const u32 hi = tid >> A;
+		T* const b = s_t + layout_base<A, LR>(threadIdx.x);
 #pragma unroll
-		for (int m = 0; m < R; ++m)
-			x[m] = s_t[bpad((hi << (A + LR)) + (static_cast<u32>(m) << A) + lo)];
+		for (int m = 0; m < R; ++m) x[m] = b[layout_off<A>(m)];
 		reg_halves<T, R, THI - 1 - A, 0>(x);
 #pragma unroll
-		for (int m = 0; m < R; ++m)
-			s_t[bpad((hi << (A + LR)) + (static_cast<u32>(m) << A) + lo)] = x[m];
+		for (int m = 0; m < R; ++m) b[layout_off<A>(m)] = x[m];
 		__syncthreads();
 		half_chunks<T, R, LR, LOGT, C + 1>(x, s_t);
+	}
+}
+
+// ---- tile sort in registers ----------------------------------------------
+// Stages 1..LOGT on one tile, in the direction form of the network: at stage
+// S every comparator puts the smaller key first where bit S of its lower
+// position is clear and last where it is set, so each block of 2^S keys
+// comes out ascending or descending by that bit and two neighbours form the
+// bitonic input of stage S + 1; bit LOGT of a tile position is always clear,
+// so the last stage sorts the tile ascending, as the global stages expect.
+// Any sorting network gives the same keys, so the output matches the flip
+// form of the network that the global steps use.  Steps go in chunks of up
+// to LR in the layouts of bitonic_tile_half, with one LDS round trip between
+// chunks (22 for 2^14 keys at 32 per thread, against 27 for the grouped LDS
+// steps of bitonic_tile_reg), every index and direction known at compile
+// time except a thread-uniform one, which is folded into the keys (x ^ ~0
+// reverses the order) while they sit in registers.
+
+// Half-cleaner of distance H register slots whose direction is bit SB of the
+// slot index (SB < 0: ascending).
+template <typename T, int R, int H, int SB>
+__device__ __forceinline__ void reg_half_dir(T (&x)[R]) {
+#pragma unroll
+	for (int j = 0; j < R; ++j)
+		if (!(j & H)) {
+			if (SB >= 0 && ((j >> (SB < 0 ? 0 : SB)) & 1)) cmpswap(x[j | H], x[j]);
+			else cmpswap(x[j], x[j | H]);
+		}
+}
+
+template <typename T, int R, int EHI, int SB>
+__device__ __forceinline__ void reg_halves_dir(T (&x)[R]) {
+	if constexpr (EHI >= 0) {
+		reg_half_dir<T, R, (1 << EHI), SB>(x);
+		reg_halves_dir<T, R, EHI - 1, SB>(x);
+	}
+}
+
+// Stage S, steps TT..A+1 at stride 2^A (A = max(0, TT - LR)), then the rest of
+// the stage and the stages after it.  The keys are in LDS on entry and on exit.
+template <typename T, int R, int LR, int LOGT, int S, int TT>
+__device__ __forceinline__ void sort_chunks(T (&x)[R], T* s_t) {
+	if constexpr (S <= LOGT) {
+		if constexpr (TT < 1) {
+			sort_chunks<T, R, LR, LOGT, S + 1, S + 1>(x, s_t);
+		} else {
+			constexpr int A = TT - LR > 0 ? TT - LR : 0;
+			constexpr int SB = S - A < LR ? S - A : -1;
+			const u32 tid = threadIdx.x;
+			T* const b = s_t + layout_base<A, LR>(tid);
+			T dm = T(0);
+			if constexpr (SB < 0) dm = (((tid >> A) >> (S - A - LR)) & 1u) ? static_cast<T>(~T(0)) : T(0);
+#pragma unroll
+			for (int m = 0; m < R; ++m) x[m] = static_cast<T>(b[layout_off<A>(m)] ^ dm);
+			reg_halves_dir<T, R, TT - 1 - A, SB>(x);
+#pragma unroll
+			for (int m = 0; m < R; ++m) b[layout_off<A>(m)] = static_cast<T>(x[m] ^ dm);
+			__syncthreads();
+			sort_chunks<T, R, LR, LOGT, S, A>(x, s_t);
+		}
+	}
+}
+
+// Stages S..LR in the thread-contiguous layout, without leaving registers.
+template <typename T, int R, int LR, int S>
+__device__ __forceinline__ void reg_stages(T (&x)[R], T dm_last) {
+	if constexpr (S <= LR) {
+		if constexpr (S < LR) {
+			reg_halves_dir<T, R, S - 1, S>(x);
+		} else {
+#pragma unroll
+			for (int m = 0; m < R; ++m) x[m] ^= dm_last;
+			reg_halves_dir<T, R, S - 1, -1>(x);
+#pragma unroll
+			for (int m = 0; m < R; ++m) x[m] ^= dm_last;
+		}
+		reg_stages<T, R, LR, S + 1>(x, dm_last);
+	}
+}
+
+// Waves per SIMD (the second launch bound) of the blocks per CU that the
+// kernel's LDS allows, at most 4 (128 VGPRs): without the bound the compiler
+// takes up to 256 VGPRs for 32 keys per thread and one 2^14-key block of
+// 4-byte keys fills a CU alone.
+template <typename T, int R, int LOGT>
+constexpr int tile_sort_waves_per_simd() {
+	int b = static_cast<int>(163840u / (bpad(1u << LOGT) * sizeof(T)));
+	const int waves = (1 << LOGT) / R / 64;
+	while (b > 1 && b * waves > 16) --b;
+	const int w = (b < 1 ? 1 : b) * waves / 4;
+	return w < 1 ? 1 : w;
+}
+
+template <typename T, int R, int LOGT>
+__global__ __launch_bounds__((1 << LOGT) / R, (tile_sort_waves_per_simd<T, R, LOGT>())) void bitonic_tile_sort(T* __restrict__ data, u64 n,
+	T xmask) {
+	constexpr u32 NT = (1u << LOGT) / R;
+	constexpr int LR = R == 32 ? 5 : 4;
+	static_assert(R == 16 || R == 32, "16 or 32 keys per thread");
+	static_assert(LOGT >= 2 * LR, "at least two chunks");
+	constexpr u32 tile_n = 1u << LOGT;
+	__shared__ T s_t[bpad(tile_n)];
+	const u64 base = static_cast<u64>(blockIdx.x) << LOGT;
+	if (base >= n) return;
+	const u32 tid = threadIdx.x;
+	T* const tile = data + base;
+	const u64 left = n - base;
+	const u32 cnt = left < tile_n ? static_cast<u32>(left) : tile_n;
+	constexpr u32 E = 16 / sizeof(T);
+	const bool vec = cnt == tile_n && (reinterpret_cast<uintptr_t>(tile) & 15) == 0;
+	if (vec) {
+		for (u32 w = tid; w < tile_n / E; w += NT) {
+			const uint4 v = reinterpret_cast<const uint4*>(tile)[w];
+			T e[E];
+			__builtin_memcpy(e, &v, 16);
+#pragma unroll
+			for (u32 i = 0; i < E; ++i) s_t[bpad(w * E + i)] = static_cast<T>(e[i] ^ xmask);
+		}
+	} else {
+		for (u32 i = tid; i < tile_n; i += NT)
+			s_t[bpad(i)] = i < cnt ? static_cast<T>(tile[i] ^ xmask) : static_cast<T>(~T(0));
+	}
+	__syncthreads();
+	T x[R];
+#pragma unroll
+	for (int m = 0; m < R; ++m) x[m] = s_t[bpad(tid * R + m)];
+	// Stage LR's direction is bit LR of the position, bit 0 of the thread.
+	reg_stages<T, R, LR, 1>(x, (tid & 1u) ? static_cast<T>(~T(0)) : T(0));
+#pragma unroll
+	for (int m = 0; m < R; ++m) s_t[bpad(tid * R + m)] = x[m];
+	__syncthreads();
+	sort_chunks<T, R, LR, LOGT, LR + 1, LR + 1>(x, s_t);
+	if (vec) {
+		for (u32 w = tid; w < tile_n / E; w += NT) {
+			T e[E];
+#pragma unroll
+			for (u32 i = 0; i < E; ++i) e[i] = static_cast<T>(s_t[bpad(w * E + i)] ^ xmask);
+			uint4 v;
+			__builtin_memcpy(&v, e, 16);
+			reinterpret_cast<uint4*>(tile)[w] = v;
+		}
+	} else {
+		for (u32 i = tid; i < cnt; i += NT) tile[i] = static_cast<T>(s_t[bpad(i)] ^ xmask);
 	}
 }
 
@@ -437,9 +590,9 @@ __global__ __launch_bounds__((1 << LOGT) / R) void bitonic_tile_half(T* __restri
 }
 
 // Keys per thread held in registers by the tile kernels: 0 runs the plain
-// LDS kernel (tiles of 2^11), 16 or 32 the register-tail kernel; the default
-// is 32 for 4-byte keys and 16 otherwise (see TileMax).
-constexpr int default_regkeys(size_t key_bytes) { return key_bytes == 4 ? 32 : 16; }
+// LDS kernel (tiles of 2^11), 16 or 32 the register kernels; the default is
+// 16 for 8-byte keys and 32 otherwise (see TileMax).
+constexpr int default_regkeys(size_t key_bytes) { return key_bytes <= 4 ? 32 : 16; }
 // Global steps fused per launch when the maxps option is not given: 5 (2^5
 // keys per thread) for keys of up to 4 bytes, 4 for 8-byte keys (measured on
 // 2^24 keys: u32 1.34 -> 1.29 ms with 5; 6 steps, and 5 for u64, were slower).
@@ -499,9 +652,11 @@ bool launch_group(CCLQueue* q, T* d, u64 n, u64 npo2, u32 s, u32 t, u32 Qn, T xm
 	return L.done(err);
 }
 
-// Stages s_first..s_last inside tiles of 2^k keys: the register-tail kernel
-// for k >= 5 (16 keys per thread in 512-thread blocks or 32 in 256), the
-// plain LDS kernels below that.
+// Stages s_first..s_last inside tiles of 2^k keys: at the default tiles
+// (2^12..2^14 keys, 16 or 32 per thread) the tile sort and the local passes
+// keep every step in registers (bitonic_tile_sort, bitonic_tile_half); other
+// tiles of k >= 5 take the register-tail kernel, smaller ones the plain LDS
+// kernels.
 template <typename T>
 bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_first,
 	u32 s_last, T xm, const char* nm, GError** err) {
@@ -509,6 +664,20 @@ bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_f
 	const int r = regkeys_of(bd, sizeof(T));
 	const u32 lt = k < kTileLogMin ? kTileLogMin : k;  // LDS tile of the instance
 	Launch L(q, nm);
+	if (s_first == 1 && s_last == k && k == lt && r != 0) {
+		// The tile sort: every step in registers.
+#define CLO_TILE_SORT(R_, LT_)                                                                   \
+	bitonic_tile_sort<T, R_, LT_><<<grid, (1u << LT_) / R_, 0, q->stream>>>(d, n, xm)
+		bool done = true;
+		if (r == 16 && lt == 12) CLO_TILE_SORT(16, 12);
+		else if (r == 16 && lt == 13) CLO_TILE_SORT(16, 13);
+		else if (r == 16 && lt == 14) CLO_TILE_SORT(16, 14);
+		else if (r == 32 && lt == 13) CLO_TILE_SORT(32, 13);
+		else if (r == 32 && lt == 14) CLO_TILE_SORT(32, 14);
+		else done = false;
+#undef CLO_TILE_SORT
+		if (done) return L.done(err);
+	}
 	if (s_first > k && s_first == s_last && k == lt && r != 0) {
 		// A local pass (half-cleaners only): every step in registers.
 #define CLO_TILE_HALF(R_, LT_)                                                                   \
