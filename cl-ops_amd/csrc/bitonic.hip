@@ -38,11 +38,12 @@ constexpr u32 tile_log_cap(size_t key_bytes) { return key_bytes >= 8 ? 14u : 15u
 // sort and local passes in registers): 4-byte keys 2^14 keys with 32 per
 // thread (0.750 ms; 2^14 x 16: 0.785, 2^13 x 32: 0.841, 2^15 x 32: 1.246),
 // 8-byte keys 2^12 with 16 per thread (1.850 ms; 2^13 x 16: 1.908, 2^13 x 32:
-// 2.082, 2^14 x 16: 2.167), 2-byte keys 2^14 with 32 (0.606 ms; 2^13 x 16:
-// 0.707, 2^14 x 16: 0.672), and 1-byte keys as 2-byte ones.
+// 2.082, 2^14 x 16: 2.167; 2^15 x 32 for 4-byte keys: 0.758), 2-byte keys
+// 2^15 with 32 (0.566 ms; 2^14 x 32: 0.606, 2^13 x 16: 0.707), 1-byte keys
+// 2^15 with 32 (0.515 ms; 2^14 x 32: 0.556, 2^13 x 16: 0.670).
 template <typename T>
 struct TileMax {
-	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : 14;
+	static constexpr u32 kLog = sizeof(T) >= 8 ? 12 : sizeof(T) == 4 ? 14 : 15;
 };
 // The plain LDS kernels (one synchronised step at a time) take tiles of up
 // to 2^11 keys.
@@ -653,7 +654,7 @@ bool launch_group(CCLQueue* q, T* d, u64 n, u64 npo2, u32 s, u32 t, u32 Qn, T xm
 }
 
 // Stages s_first..s_last inside tiles of 2^k keys: at the default tiles
-// (2^12..2^14 keys, 16 or 32 per thread) the tile sort and the local passes
+// (2^12..2^15 keys, 16 or 32 per thread) the tile sort and the local passes
 // keep every step in registers (bitonic_tile_sort, bitonic_tile_half); other
 // tiles of k >= 5 take the register-tail kernel, smaller ones the plain LDS
 // kernels.
@@ -674,7 +675,10 @@ bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_f
 		else if (r == 16 && lt == 14) CLO_TILE_SORT(16, 14);
 		else if (r == 32 && lt == 13) CLO_TILE_SORT(32, 13);
 		else if (r == 32 && lt == 14) CLO_TILE_SORT(32, 14);
-		else done = false;
+		else if constexpr (sizeof(T) < 8) {
+			if (r == 32 && lt == 15) CLO_TILE_SORT(32, 15);
+			else done = false;
+		} else done = false;
 #undef CLO_TILE_SORT
 		if (done) return L.done(err);
 	}
@@ -688,7 +692,10 @@ bool launch_tile(CCLQueue* q, const BitonicData* bd, T* d, u64 n, u32 k, u32 s_f
 		else if (r == 16 && lt == 14) CLO_TILE_HALF(16, 14);
 		else if (r == 32 && lt == 13) CLO_TILE_HALF(32, 13);
 		else if (r == 32 && lt == 14) CLO_TILE_HALF(32, 14);
-		else done = false;
+		else if constexpr (sizeof(T) < 8) {
+			if (r == 32 && lt == 15) CLO_TILE_HALF(32, 15);
+			else done = false;
+		} else done = false;
 #undef CLO_TILE_HALF
 		if (done) return L.done(err);
 	}

@@ -6,9 +6,9 @@
  * Options: minps=<1..4>, maxps=<1..4> (register steps fused per global
  * sweep; minps <= maxps), maxsfs=<n> (largest stage-finish fused in LDS, as
  * log2 of the span: at most 15, 14 for 8-byte keys; default 12 for 8-byte
- * keys, 14 otherwise), regkeys=0|16|32 (keys per thread of the LDS tile
- * kernels; default 16 for 8-byte keys, 32 otherwise).  They change the launch
- * strategy, never the result.
+ * keys, 14 for 4-byte keys, 15 otherwise), regkeys=0|16|32 (keys per thread
+ * of the LDS tile kernels; default 16 for 8-byte keys, 32 otherwise).  They
+ * change the launch strategy, never the result.
  */
 #ifndef CLO_SORT_ABITONIC_H_
 #define CLO_SORT_ABITONIC_H_

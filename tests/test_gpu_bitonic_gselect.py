@@ -60,7 +60,7 @@ def test_abitonic_options(clops, oracle, ctx, opts):
 
 
 @pytest.mark.parametrize("regkeys", [0, 16, 32])
-@pytest.mark.parametrize("maxsfs", [5, 9, 11, 12, 13, 14])
+@pytest.mark.parametrize("maxsfs", [5, 9, 11, 12, 13, 14, 15])
 @pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint32, np.int64])
 def test_abitonic_tile_kernels(clops, oracle, ctx, regkeys, maxsfs, dtype):
     """Every tile kernel (plain LDS steps, 16 or 32 register keys per
